@@ -487,7 +487,9 @@ class Oracle:
         if self.chain_first is not None:
             s.nchain = self.chain_first.size
             s.chain_first, s.chain_len = _i(self.chain_first), _i(self.chain_len)
-            s.bond_k, s.bond_p, s.bond_eq = float(self.bond[0]), float(self.bond[1]), float(self.bond[2])
+            # namelist k is divided by p at setup (Src/potential.F90:3561)
+            kin, pw, eq = float(self.bond[0]), int(self.bond[1]), float(self.bond[2])
+            s.bond_k, s.bond_p, s.bond_eq = kin / pw, float(pw), eq
         forceo = np.zeros(3 * self.np)
         e = Energy()
         rc = lib().orc_utotal(C.byref(s), _d(ro), _d(forceo), C.byref(e))

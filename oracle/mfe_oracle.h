@@ -167,7 +167,9 @@ void orc_ewald_surf(const double boxlen[3], int na, const double *r,
 
 /* ---- bonds (Src/energy.F90:5335-5400) ---- */
 /* Chains are runs of consecutive particles: chain c spans particles
- * chain_first[c] .. chain_first[c]+chain_len[c]-1 (1-based). */
+ * chain_first[c] .. chain_first[c]+chain_len[c]-1 (1-based).  k is the
+ * stored bond%k, i.e. the namelist value divided by p (IOPotChain,
+ * Src/potential.F90:3561), so u = (k_in/p)(r-eq)**p. */
 void orc_ubond(const orc_box *b, const double *ro, int nchain,
                const int *chain_first, const int *chain_len, double k,
                double p, double eq, double *force, double *u_bond,

@@ -11,7 +11,7 @@
 !                           r2umin(1:nptpt), rcut, ualpha, EpsiFourPi
 !   <project>.probe_f     : (only if <project>.probe_pos exists: np*3 reals
 !                           giving ro, already inside the box) the state
-!                           after SetVList + UTotal at those positions:
+!                           after SetVList (or SetCellList) + UTotal there:
 !                           np, maxnneigh, force(3,np), forceo(3,np),
 !                           u%tot, u%twob(0:nptpt), u%rec, u%bond, virial,
 !                           nneighpn(1:np), jpnlist(1:maxnneigh,1:np)
@@ -29,6 +29,7 @@ program RefProbe
    use NListModule, only: maxnneigh
    use PotentialModule, only: nugrid
    use CardinalBSplineModule
+   use CellListModule, only: SetCellList
    implicit none
 
    integer(4) :: ios, ufile, nused, narg, i, iord, kmax, iat, jat, idum
@@ -83,6 +84,7 @@ program RefProbe
       close(ufile)
       call SetAtomProp(1, np, .false.)
       if (lvlist) call SetVList
+      if (lCellList) call SetCellList
       call UTotal(iSimulationStep)
       open(newunit=ufile, file=trim(proj)//'.probe_f', access='stream', form='unformatted', status='replace')
       write(ufile) np, maxnneigh

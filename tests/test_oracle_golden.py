@@ -1,0 +1,161 @@
+"""The FP64 oracle against golden vectors produced by the MOLSIM reference.
+
+The fixtures in tests/golden/ were written by tests/golden/make_golden.py
+from the amdflang build of the unmodified reference (tables, UTotal
+forces/energies/virial, Verlet lists, MD trajectory, Erf2,
+CardinalBSpline).  Every comparison here is bitwise.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def load(golden_dir, name):
+    return np.load(os.path.join(golden_dir, name))
+
+
+# ------------------------------------------------------------------ math
+def test_erf2_bitwise(golden_dir):
+    z = load(golden_dir, "math.npz")
+    got = O.erf2(z["args"])
+    assert np.array_equal(got, z["erf2"])
+
+
+@pytest.mark.parametrize("order", range(3, 9))
+def test_cardinal_bspline_bitwise(golden_dir, order):
+    z = load(golden_dir, "math.npz")
+    for i, x in enumerate(z["args"]):
+        k, v, d = O.cardinal_bspline(order, x)
+        assert k == z[f"kmax{order}"][i]
+        assert np.array_equal(v, z[f"val{order}"][i])
+        assert np.array_equal(d, z[f"der{order}"][i])
+
+
+def test_pot_1_6_12_bitwise(golden_dir):
+    """Pot_1_6_12('calc') and CalcEwaldUfac on 600 radii, 3 type pairs."""
+    z = load(golden_dir, "math.npz")
+    efp = O.epsi0_four_pi() / 78.4
+    pairs = [(0, 0), (0, 1), (1, 1)]
+    for i, r in enumerate(z["pot_r"]):
+        for k, (it, jt) in enumerate(pairs):
+            u = O.pot_1_6_12([1.0, -1.0], [4.0, 4.0], [2.4777, 2.4777], it, jt, r,
+                             epsi_four_pi=efp, alpha=float(z["pot_alpha"]))
+            assert np.array_equal(u, z["pot_1_6_12"][i, k]), (r, it, jt)
+
+
+# ---------------------------------------------------------------- tables
+@pytest.mark.parametrize("name,rcut", [("c1_std.npz", 15.0), ("tables_c2.npz", 18.0),
+                                       ("tables_c3.npz", 12.0)])
+def test_tables_bitwise(golden_dir, name, rcut):
+    z = load(golden_dir, name)
+    efp = float(z["epsi_four_pi"])
+    assert efp == O.epsi0_four_pi() / 78.4
+    t = O.build_tables(z["zat"], z["sigat"], z["epsat"], rcut=rcut,
+                       epsi_four_pi=efp, alpha=float(z["alpha"]))
+    assert np.array_equal(t.iubuflow, z["iubuflow"])
+    assert np.array_equal(t.nugrid, z["nugrid"])
+    assert np.array_equal(t.r2umin, z["r2umin"])
+    assert t.ubuf.size == z["ubuf"].size
+    assert np.array_equal(t.ubuf, z["ubuf"])
+
+
+# ---------------------------------------------------------------- UTotal
+def _c1_oracle(z, nlist="vlist", lsurf=False):
+    efp = float(z["epsi_four_pi"])
+    t = O.build_tables(z["zat"], z["sigat"], z["epsat"], rcut=15.0,
+                       epsi_four_pi=efp, alpha=0.2)
+    iptpn = np.repeat([1, 2], z["nppt"])
+    az = np.repeat(z["zat"], z["nppt"])
+    return O.Oracle(50.0, iptpn, az, t, 15.0, ewald="std", alpha=0.2,
+                    epsi_four_pi=efp, ncut=8, lsurf=lsurf, nlist=nlist,
+                    drnlist=2.0)
+
+
+def test_utotal_vlist_std_ewald_bitwise(golden_dir):
+    z = load(golden_dir, "c1_std.npz")
+    orc = _c1_oracle(z)
+    pos = z["probe_pos"]
+    orc.set_list(pos)
+    assert np.array_equal(orc.nn, z["ref_nneighpn"])
+    flat = np.concatenate([orc.jl[i, :orc.nn[i]] for i in range(orc.np)]).astype("<i4")
+    assert hashlib.sha256(flat.tobytes()).hexdigest() == str(z["ref_jpnlist_sha256"])
+    f, e = orc.utotal(pos)
+    assert np.array_equal(f, z["ref_forceo"])
+    assert e["tot"] == z["ref_u_tot"]
+    assert e["twob0"] == z["ref_u_twob"][0]
+    assert np.array_equal(e["twob"], z["ref_u_twob"][1:])
+    assert e["rec"] == z["ref_u_rec"]
+    assert e["virial"] == z["ref_virial"]
+
+
+def test_vlist_cells_equals_bruteforce(golden_dir):
+    z = load(golden_dir, "c1_std.npz")
+    pos = z["probe_pos"]
+    nn1, jl1 = O.set_vlist_md(50.0, pos, 17.0, 1000)
+    nn2, jl2 = O.set_vlist_md(50.0, pos, 17.0, 1000, cells=True)
+    assert np.array_equal(nn1, nn2)
+    assert np.array_equal(jl1, jl2)
+
+
+def test_utotal_celllist_surface_bitwise(golden_dir):
+    z = load(golden_dir, "c1_std.npz")
+    zc = load(golden_dir, "c1_cell_surf.npz")
+    orc = _c1_oracle(z, nlist="cell", lsurf=True)
+    pos = zc["probe_pos"]
+    orc.set_list(pos)
+    f, e = orc.utotal(pos)
+    assert np.array_equal(f, zc["ref_forceo"])
+    assert e["tot"] == zc["ref_u_tot"]
+    assert e["rec"] == zc["ref_u_rec"]
+    assert e["virial"] == zc["ref_virial"]
+
+
+def test_utotal_chains_bonds_bitwise(golden_dir):
+    z = load(golden_dir, "chains.npz")
+    efp = float(z["epsi_four_pi"])
+    t = O.build_tables(z["zat"], z["sigat"], z["epsat"], rcut=20.0,
+                       epsi_four_pi=efp, alpha=float(z["alpha"]))
+    assert np.array_equal(t.ubuf, z["ubuf"])
+    iptpn = np.repeat([1, 2], z["nppt"])
+    az = np.repeat(z["zat"], z["nppt"])
+    nper = int(z["nper"])
+    chains = [(1 + c * nper, nper) for c in range(int(z["nchain"]))]
+    orc = O.Oracle(60.0, iptpn, az, t, 20.0, ewald="std", alpha=float(z["alpha"]),
+                   epsi_four_pi=efp, ncut=5, chains=chains, bond=tuple(z["bond"]))
+    pos = z["probe_pos"]
+    orc.set_list(pos)
+    f, e = orc.utotal(pos)
+    assert e["bond"] == z["ref_u_bond"]
+    assert np.array_equal(f, z["ref_forceo"])
+    assert e["tot"] == z["ref_u_tot"]
+    assert e["virial"] == z["ref_virial"]
+
+
+def test_velver_trajectory_bitwise(golden_dir):
+    """Five VelVer steps (Src/md.F90:252-410) with the skin-test rebuilds.
+
+    rodd starts at zero: MDDriver(iReadInput) zeroes it and nothing evaluates
+    forces before the first step (Src/md.F90:130-149), so the first kick
+    uses only the new acceleration."""
+    z = load(golden_dir, "c1_std.npz")
+    orc = _c1_oracle(z)
+    massip = 1.0 / np.repeat(z["masses"], z["nppt"])
+    ro = z["traj_pos"][0].copy().reshape(-1)
+    rod = z["traj_vel"][0].copy().reshape(-1)
+    orc.set_list(ro)
+    rodd = np.zeros_like(ro)
+    drosum = np.zeros_like(ro)
+    dt = float(z["tstep"])
+    for k in range(1, z["traj_pos"].shape[0]):
+        drostep = O.velver_move(50.0, dt, ro, rod, rodd)
+        f, _ = orc.utotal(ro)
+        O.velver_kick(dt, massip, f.reshape(-1), rod, rodd)
+        assert np.array_equal(ro.reshape(-1, 3), z["traj_pos"][k]), k
+        assert np.array_equal(f, z["traj_forceo"][k]), k
+        assert np.array_equal(rod.reshape(-1, 3), z["traj_vel"][k]), k
+        if O.skin_test(drosum, drostep, 2.0):
+            orc.set_list(ro)
