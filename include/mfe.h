@@ -1,0 +1,198 @@
+/*
+ * mfe.h -- C ABI of the MI355X force engine for MOLSIM's nonbonded path.
+ *
+ * MOLSIM has no plugin interface for this path: its seam is a set of
+ * Fortran subroutines acting on module state.  Each entry point below
+ * replaces one of them and takes MOLSIM's own arrays by pointer, in its own
+ * layout (column-major ro(3,np) == xyz-interleaved, 1-based type and
+ * table indices), so a Fortran host can bind them with ISO_C_BINDING.
+ *
+ * Conventions
+ *  - Every function returns an int status: MFE_OK (0) or an MFE_E_* code;
+ *    mfe_last_error() returns the message of the last failure.
+ *  - Inputs are copied to device memory; no host pointer is kept after a
+ *    call returns.  Calls are synchronous with respect to the host.
+ *  - One engine owns one HIP stream and is not thread-safe: call it from a
+ *    single host thread, as MOLSIM (single-threaded) does.
+ *  - Energies are in kJ/mol and forces follow UTotal: mfe_compute returns
+ *    forceo = force * (sclene/scllen)/sclfor (Src/energy.F90:405-418).
+ */
+#ifndef MFE_H
+#define MFE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MFE_ABI_VERSION 1
+
+/* status codes: the abort conditions of the reference path */
+enum {
+  MFE_OK = 0,
+  MFE_E_BAD_PARAM = 1,       /* argument out of range / inconsistent     */
+  MFE_E_OVERLAP_R2UMIN = 2,  /* r2 < r2umin: StopUTwoBodyA, Src/energy.F90:478 */
+  MFE_E_TABLE_RANGE = 3,     /* ibuf > nbuf, Src/energy.F90:485            */
+  MFE_E_NEIGH_CAPACITY = 4,  /* nneighpn > maxnneigh, Src/nlist.F90:752    */
+  MFE_E_MESH_ODD = 5,        /* nmesh odd, Src/energy.F90:6891             */
+  MFE_E_HIP = 6,             /* HIP runtime failure                        */
+  MFE_E_STATE = 7,           /* call order: a required set_* is missing    */
+  MFE_E_TABLE_BUILD = 8,     /* PotTwoBodyTab3 did not converge            */
+  MFE_E_NOMEM = 9            /* device allocation failed                   */
+};
+
+enum { MFE_PREC_FP64 = 0, MFE_PREC_MIXED = 1 };
+enum { MFE_EWALD_NONE = 0, MFE_EWALD_STD = 1, MFE_EWALD_SPME = 2 };
+/* pair-set semantics: MFE_NL_VLIST = Verlet list of SetVListMD + UTwoBodyA
+   (pairs with r2 <= rcut2); MFE_NL_CELL = UTwoBodyACellList (r2 < rcut2) */
+enum { MFE_NL_VLIST = 0, MFE_NL_CELL = 1 };
+/* mfe_compute flags */
+enum { MFE_FORCES = 1u, MFE_ENERGY = 2u, MFE_VIRIAL = 4u };
+
+#define MFE_MAX_TYPES 8
+#define MFE_MAX_PAIRS (MFE_MAX_TYPES * (MFE_MAX_TYPES + 1) / 2)
+
+typedef struct mfe_engine mfe_engine;
+
+typedef struct {
+  int32_t precision; /* MFE_PREC_FP64 or MFE_PREC_MIXED                      */
+  int32_t device;    /* HIP device ordinal                                    */
+  int32_t max_np;    /* capacity in particles                                 */
+  int32_t reserved;
+} mfe_config;
+
+/* u%tot, u%twob(0:nptpt), u%rec, u%bond and virial of UTotal */
+typedef struct {
+  double tot;
+  double twob0;
+  double rec;
+  double bond;
+  double virial;
+  int32_t nptpt;
+  int32_t rebuilds; /* neighbour-list rebuilds since the last reset */
+  double twob[MFE_MAX_PAIRS];
+} mfe_energy;
+
+/* Pair-potential specification for the (1,6,12) potential, the arguments
+   of IOPotTwoBody that Pot_1_6_12 reads (Src/potential.F90:307-392). */
+typedef struct {
+  int32_t npt;
+  const double *zat;   /* [npt] */
+  const double *sigat; /* [npt] */
+  const double *epsat; /* [npt] */
+  double epsi_four_pi; /* Epsi0FourPi/relpermitt */
+  double rcut;
+  double alpha;        /* ualpha (used when lewald) */
+  int32_t lewald;
+  int32_t lljcut;
+  double ljrcut, ljushift;
+  double r2uminin, utoltab, ftoltab, umaxtab, fmaxtab, rumaxfac;
+} mfe_pot_1_6_12;
+
+/* ---- library ---------------------------------------------------------- */
+int mfe_abi_version(void);
+/* Fills spec with IOPotTwoBody's defaults (Src/potential.F90:340-392,411). */
+int mfe_pot_defaults(mfe_pot_1_6_12 *spec);
+/* PotTwoBodyTab1/2/3 + SetUBuffer + CheckUBuffer for Pot_1_6_12 with
+   monoatomic particle types (Src/potential.F90:868-1258, 1349-1425): fills
+   ubuf (capacity cap doubles) and, per type pair iptjpt (Set_iptpt order),
+   iubuflow (1-based), nugrid, r2umin.  *nbuf_used receives the length. */
+int mfe_build_tables_1_6_12(const mfe_pot_1_6_12 *spec, double *ubuf,
+                            int32_t cap, int32_t *nbuf_used,
+                            int32_t *iubuflow, int32_t *nugrid,
+                            double *r2umin);
+
+/* ---- engine lifetime (replaces MolModule allocation) ------------------- */
+int mfe_create(const mfe_config *cfg, mfe_engine **out);
+int mfe_destroy(mfe_engine *e);
+const char *mfe_last_error(const mfe_engine *e);
+
+/* ---- setup (replaces IOSystem box setup, Particle, IOPotTwoBody,
+        EwaldSetup, IONList) ------------------------------------------- */
+/* Orthorhombic periodic box, txbc='xyz' (Src/molaux.F90:326-335). */
+int mfe_set_box(mfe_engine *e, const double boxlen[3]);
+/* np particles (monoatomic), npt types: iptpn[np] 1-based, az[np]
+   charges, massipt[npt] inverse masses (Src/particle.F90:2189-2278). */
+int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt,
+                      const int32_t *iptpn, const double *az,
+                      const double *massipt);
+/* Potential table in the reference's 12-double interval format
+   (Src/potential.F90:1181-1210): ubuf[nbuf], iubuflow[nptpt] (1-based),
+   r2atat[nptpt], r2umin[nptpt]; rcut is the real-space cut-off. */
+int mfe_set_tables(mfe_engine *e, const double *ubuf, int32_t nbuf,
+                   const int32_t *iubuflow, const double *r2atat,
+                   const double *r2umin, double rcut);
+/* Ewald sum (EwaldSetup, Src/energy.F90:6601-7046): kind MFE_EWALD_*,
+   alpha = ualpha, order and mesh[3] for SPME, ncut (+ ncut_cube) for the
+   standard sum, lsurf the surface term, epsi_four_pi = EpsiFourPi. */
+int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
+                  const int32_t mesh[3], int32_t ncut, int32_t ncut_cube,
+                  int32_t lsurf, double epsi_four_pi);
+/* Neighbour list (IONList, Src/nlist.F90:184-333): kind MFE_NL_*,
+   drnlist the skin, maxnneigh the per-particle capacity (0 = automatic). */
+int mfe_set_neighbor(mfe_engine *e, int32_t kind, double drnlist,
+                     int32_t maxnneigh);
+/* Chains for UBond (Src/energy.F90:5335-5400): chain c holds particles
+   chain_first[c] .. chain_first[c]+chain_len[c]-1 (1-based); k is the
+   namelist force constant (divided by p here, as IOPotChain does at
+   Src/potential.F90:3561), p the integer power, eq the bond length. */
+int mfe_set_chains(mfe_engine *e, int32_t nchain, const int32_t *chain_first,
+                   const int32_t *chain_len, double k, int32_t p, double eq);
+
+/* ---- state ------------------------------------------------------------ */
+/* ro(3,np), positions inside the box (Src/mol.F90:1087). */
+int mfe_upload_positions(mfe_engine *e, const double *ro);
+/* rod(3,np) velocities and rodd(3,np) accelerations (NULL = zero). */
+int mfe_upload_velocities(mfe_engine *e, const double *rod,
+                          const double *rodd);
+int mfe_download_state(mfe_engine *e, double *ro, double *rod, double *rodd);
+
+/* ---- the hot path ----------------------------------------------------- */
+/* SetVList / InitCellList+SetCellList at the current positions. */
+int mfe_build_list(mfe_engine *e);
+/* UTotal (Src/energy.F90:280-434) at the current positions with the
+   current list: forceo(3,np) (may be NULL) and energies (may be NULL). */
+int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo,
+                mfe_energy *energy);
+/* nsteps of MDDriver(iSimulationStep) + NList(iSimulationStep): VelVer
+   (Src/md.F90:252-410) with the skin-test rebuild (Src/nlist.F90:597-616),
+   all on the device.  flags select which UTotal terms are reduced each
+   step (forces are always computed); energies of the last step are kept
+   for mfe_get_energy. */
+int mfe_step_md(mfe_engine *e, int32_t nsteps, double tstep, uint32_t flags);
+/* nsteps of BDStep (Src/bd.F90:120-155) + UTotal + NList: fac1/fac2 per
+   type, noise[nsteps][np][3] unit Gaussian deviates in the reference's
+   draw order (ip outer, xyz inner).  The reference's first BD step uses
+   the forces of the preceding UTotal; call mfe_compute first. */
+int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1,
+                const double *fac2, const double *noise, uint32_t flags);
+int mfe_get_energy(mfe_engine *e, mfe_energy *energy);
+/* The Verlet list in SetVListMD's form (Src/nlist.F90:724-759): for each
+   ip the partners jp it owns under the checkerboard rule, ascending.
+   nneighpn[np]; jpnlist[np][maxnneigh] row-major (row ip-1).  */
+int mfe_get_pairs(mfe_engine *e, int32_t *nneighpn, int32_t *jpnlist,
+                  int32_t maxnneigh);
+
+/* ---- measurement ------------------------------------------------------ */
+enum {
+  MFE_STAGE_PAIR = 0,   /* table pair forces                               */
+  MFE_STAGE_SPREAD = 1, /* binning + charge spreading                     */
+  MFE_STAGE_FFT_X = 2,  /* r2c along x (forward)                          */
+  MFE_STAGE_FFT_Y = 3,  /* complex along y (forward)                      */
+  MFE_STAGE_CONV = 4,   /* z forward + influence/energy/virial + z inverse */
+  MFE_STAGE_GATHER = 5, /* force back-interpolation                       */
+  MFE_STAGE_STEP = 6,   /* one whole MD step                              */
+  MFE_STAGE_NLIST = 7,  /* neighbour-list rebuild                         */
+  MFE_STAGE_IFFT = 8,   /* y inverse + c2r along x                        */
+  MFE_NSTAGES = 9
+};
+/* Runs one stage nrep times on the engine's stream, bracketed by HIP
+   events on that stream, and returns the mean time per run in ms. */
+int mfe_time_stage(mfe_engine *e, int32_t stage, int32_t nrep, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MFE_H */

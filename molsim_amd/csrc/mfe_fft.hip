@@ -1,0 +1,431 @@
+// mfe_fft.hip -- the 3D FFT of SPMFFTRec (Src/energy.F90:6525-6587) on the
+// mesh of UEwaldRecSPM, in FFTW's unnormalised r2c/c2r convention:
+// real mesh [s3][s2][s1] (Fortran Qmesh(s1,s2,s3)) <-> complex mesh
+// [s3][s2][s1/2+1].
+//
+// Five passes, each reading and writing the mesh once:
+//   x forward  : r2c of every x row through a complex FFT of half length
+//   y forward  : complex FFT along y for tiles of kx columns
+//   conv       : z forward, Green function (meshfac) with the FP64 energy
+//                and virial sums of the reference's CalcGIF loop, z inverse
+//                -- the z transform pair is fused around the pointwise
+//                product, so the mesh is not written between them
+//   y inverse, x inverse (c2r)
+// Each 1D transform is a mixed-radix Stockham FFT staged in LDS.
+#include <hip/hip_runtime.h>
+
+#include "mfe_internal.h"
+
+namespace mfe {
+namespace {
+
+template <typename R>
+__device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> cadd(cplx<R> a, cplx<R> b) {
+  return {a.x + b.x, a.y + b.y};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> csub(cplx<R> a, cplx<R> b) {
+  return {a.x - b.x, a.y - b.y};
+}
+template <typename R>
+__device__ __forceinline__ cplx<R> conjc(cplx<R> a) {
+  return {a.x, -a.y};
+}
+
+constexpr int kMaxRadix = 32;
+
+// R-point DFT of v[0..RAD-1] in registers: V[q] = sum_j v[j] w^(j q), with
+// w = exp(-+2 pi i / RAD) taken from the length-n table tw (RAD divides n).
+template <typename R, int RAD>
+__device__ __forceinline__ void small_dft(cplx<R> *v, bool inv,
+                                          const cplx<R> *tw, int n) {
+  if constexpr (RAD == 2) {
+    cplx<R> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (RAD == 4) {
+    cplx<R> a = v[0], b = v[1], c = v[2], d = v[3];
+    cplx<R> s0 = cadd(a, c), s1 = csub(a, c), s2 = cadd(b, d), s3 = csub(b, d);
+    // forward: -i * s3 ; inverse: +i * s3
+    cplx<R> js3 = inv ? cplx<R>{-s3.y, s3.x} : cplx<R>{s3.y, -s3.x};
+    v[0] = cadd(s0, s2);
+    v[2] = csub(s0, s2);
+    v[1] = cadd(s1, js3);
+    v[3] = csub(s1, js3);
+  } else {
+    cplx<R> out[RAD];
+    const int step = n / RAD;
+#pragma unroll
+    for (int qq = 0; qq < RAD; ++qq) {
+      cplx<R> acc = v[0];
+#pragma unroll
+      for (int j = 1; j < RAD; ++j) {
+        cplx<R> w = tw[((j * qq) % RAD) * step];
+        if (inv) w.y = -w.y;
+        acc = cadd(acc, cmul(v[j], w));
+      }
+      out[qq] = acc;
+    }
+#pragma unroll
+    for (int qq = 0; qq < RAD; ++qq) v[qq] = out[qq];
+  }
+}
+
+// one Stockham stage of radix RAD over `batch` sequences (Govindaraju et
+// al., SC'08): j-th butterfly reads in[j + q*n/RAD], twiddles by
+// w^(k q) with k = j mod Ns, and writes out[(j/Ns)*Ns*RAD + k + q*Ns]
+template <typename R, int RAD>
+__device__ __forceinline__ void stockham_stage(const cplx<R> *a, cplx<R> *b,
+                                               int n, int Ns, int batch,
+                                               int ld, const cplx<R> *tw,
+                                               bool inv) {
+  const int nbf = n / RAD;
+  const int total = nbf * batch;
+  const int tstep = n / (Ns * RAD);
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int sq = t / nbf, j = t - sq * nbf;
+    const cplx<R> *in = a + sq * ld;
+    cplx<R> *out = b + sq * ld;
+    const int k = j % Ns;
+    cplx<R> v[RAD];
+#pragma unroll
+    for (int q = 0; q < RAD; ++q) v[q] = in[j + q * nbf];
+#pragma unroll
+    for (int q = 1; q < RAD; ++q) {
+      cplx<R> w = tw[(k * q * tstep) % n];
+      if (inv) w.y = -w.y;
+      v[q] = cmul(v[q], w);
+    }
+    small_dft<R, RAD>(v, inv, tw, n);
+    const int base = (j / Ns) * Ns * RAD + k;
+#pragma unroll
+    for (int q = 0; q < RAD; ++q) out[base + q * Ns] = v[q];
+  }
+}
+
+// generic prime radix up to kMaxRadix (meshes with large prime factors)
+template <typename R>
+__device__ void stockham_stage_generic(const cplx<R> *a, cplx<R> *b, int n,
+                                       int Ns, int rad, int batch, int ld,
+                                       const cplx<R> *tw, bool inv) {
+  const int nbf = n / rad;
+  const int total = nbf * batch;
+  const int tstep = n / (Ns * rad);
+  const int step = n / rad;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int sq = t / nbf, j = t - sq * nbf;
+    const cplx<R> *in = a + sq * ld;
+    cplx<R> *out = b + sq * ld;
+    const int k = j % Ns;
+    const int base = (j / Ns) * Ns * rad + k;
+    for (int qq = 0; qq < rad; ++qq) {
+      cplx<R> acc = {R(0), R(0)};
+      for (int q = 0; q < rad; ++q) {
+        cplx<R> x = in[j + q * nbf];
+        cplx<R> w = tw[((k * q * tstep) + ((q * qq) % rad) * step) % n];
+        if (inv) w.y = -w.y;
+        acc = cadd(acc, cmul(x, w));
+      }
+      out[base + qq * Ns] = acc;
+    }
+  }
+}
+
+// Batched Stockham FFT staged in LDS: `batch` sequences of length p.n at
+// a[s*ld + k]; b is scratch of the same shape.  Returns the buffer holding
+// the result in natural order.  All threads of the block take part.
+template <typename R>
+__device__ cplx<R> *lds_fft(cplx<R> *a, cplx<R> *b, const FftPlan &p,
+                            int batch, int ld, const cplx<R> *tw, bool inv) {
+  const int n = p.n;
+  int Ns = 1;
+  for (int st = 0; st < p.nstages; ++st) {
+    const int r = p.radix[st];
+    switch (r) {
+      case 4: stockham_stage<R, 4>(a, b, n, Ns, batch, ld, tw, inv); break;
+      case 2: stockham_stage<R, 2>(a, b, n, Ns, batch, ld, tw, inv); break;
+      case 3: stockham_stage<R, 3>(a, b, n, Ns, batch, ld, tw, inv); break;
+      case 5: stockham_stage<R, 5>(a, b, n, Ns, batch, ld, tw, inv); break;
+      case 7: stockham_stage<R, 7>(a, b, n, Ns, batch, ld, tw, inv); break;
+      default: stockham_stage_generic<R>(a, b, n, Ns, r, batch, ld, tw, inv); break;
+    }
+    __syncthreads();
+    cplx<R> *tmp = a;
+    a = b;
+    b = tmp;
+    Ns *= r;
+  }
+  return a;
+}
+
+// ----------------------------------------------------------------- x pass
+// r2c of rows of length s1 = 2m: z[k] = x[2k] + i x[2k+1], Z = FFT_m(z),
+// X[k] = (Z[k] + conj(Z[m-k]))/2 - i/2 (Z[k] - conj(Z[m-k])) W^k.
+template <typename R>
+__global__ void __launch_bounds__(256) k_fft_x_fwd(
+    const R *__restrict__ real, cplx<R> *__restrict__ cm, int s1, int nrows,
+    int rows_per_blk, FftPlan ph, const cplx<R> *__restrict__ tw_half,
+    const cplx<R> *__restrict__ tw_full) {
+  extern __shared__ char smem[];
+  const int m = s1 / 2, hx = m + 1, ld = m + 1;
+  cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
+  cplx<R> *b = a + rows_per_blk * ld;
+  const int row0 = blockIdx.x * rows_per_blk;
+  const int nr = min(rows_per_blk, nrows - row0);
+  R *ar = reinterpret_cast<R *>(a);
+  for (int e = threadIdx.x; e < nr * s1; e += blockDim.x) {
+    const int r = e / s1, x = e - r * s1;
+    ar[2 * (r * ld) + x] = real[(size_t)(row0 + r) * s1 + x];
+  }
+  __syncthreads();
+  cplx<R> *z = lds_fft(a, b, ph, nr, ld, tw_half, false);
+  for (int e = threadIdx.x; e < nr * hx; e += blockDim.x) {
+    const int r = e / hx, k = e - r * hx;
+    cplx<R> zk = z[r * ld + (k % m)];
+    cplx<R> zm = conjc(z[r * ld + ((m - k) % m)]);
+    cplx<R> ev = {(zk.x + zm.x) * R(0.5), (zk.y + zm.y) * R(0.5)};
+    cplx<R> d = csub(zk, zm);
+    cplx<R> od = {d.y * R(0.5), -d.x * R(0.5)};
+    cplx<R> X = cadd(ev, cmul(tw_full[k], od));
+    cm[(size_t)(row0 + r) * hx + k] = X;
+  }
+}
+
+// c2r of rows: Re(X[0]), Re(X[m]) only (FFTW's reading of a Hermitian row);
+// Z[k] = (X[k] + conj(X[m-k])) + i (X[k] - conj(X[m-k])) W^-k, z = IFFT_m(Z),
+// x[2j] = Re z[j], x[2j+1] = Im z[j] (unnormalised).
+template <typename R>
+__global__ void __launch_bounds__(256) k_fft_x_inv(
+    const cplx<R> *__restrict__ cm, R *__restrict__ real, int s1, int nrows,
+    int rows_per_blk, FftPlan ph, const cplx<R> *__restrict__ tw_half,
+    const cplx<R> *__restrict__ tw_full) {
+  extern __shared__ char smem[];
+  const int m = s1 / 2, hx = m + 1, ld = m + 1;
+  cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
+  cplx<R> *b = a + rows_per_blk * ld;
+  const int row0 = blockIdx.x * rows_per_blk;
+  const int nr = min(rows_per_blk, nrows - row0);
+  for (int e = threadIdx.x; e < nr * hx; e += blockDim.x) {
+    const int r = e / hx, k = e - r * hx;
+    cplx<R> v = cm[(size_t)(row0 + r) * hx + k];
+    if (k == 0 || k == m) v.y = R(0);
+    b[r * ld + k] = v;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nr * m; e += blockDim.x) {
+    const int r = e / m, k = e - r * m;
+    cplx<R> xk = b[r * ld + k];
+    cplx<R> xm = conjc(b[r * ld + (m - k)]);
+    cplx<R> A = cadd(xk, xm), D = csub(xk, xm);
+    cplx<R> Dw = cmul(D, conjc(tw_full[k]));
+    a[r * ld + k] = {A.x - Dw.y, A.y + Dw.x};
+  }
+  __syncthreads();
+  cplx<R> *z = lds_fft(a, b, ph, nr, ld, tw_half, true);
+  for (int e = threadIdx.x; e < nr * s1; e += blockDim.x) {
+    const int r = e / s1, x = e - r * s1;
+    cplx<R> v = z[r * ld + (x >> 1)];
+    real[(size_t)(row0 + r) * s1 + x] = (x & 1) ? v.y : v.x;
+  }
+}
+
+// ----------------------------------------------------------------- y pass
+template <typename R>
+__global__ void __launch_bounds__(256) k_fft_y(cplx<R> *__restrict__ cm,
+                                               int s2, int hx, int T,
+                                               FftPlan py,
+                                               const cplx<R> *__restrict__ tw,
+                                               int inv) {
+  extern __shared__ char smem[];
+  const int ld = s2 + 1;
+  cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
+  cplx<R> *b = a + T * ld;
+  const int z = blockIdx.y;
+  const int kx0 = blockIdx.x * T;
+  const int nt = min(T, hx - kx0);
+  cplx<R> *base = cm + (size_t)z * s2 * hx + kx0;
+  for (int e = threadIdx.x; e < nt * s2; e += blockDim.x) {
+    const int y = e / nt, c = e - y * nt;
+    a[c * ld + y] = base[(size_t)y * hx + c];
+  }
+  __syncthreads();
+  cplx<R> *res = lds_fft(a, b, py, nt, ld, tw, inv != 0);
+  for (int e = threadIdx.x; e < nt * s2; e += blockDim.x) {
+    const int y = e / nt, c = e - y * nt;
+    base[(size_t)y * hx + c] = res[c * ld + y];
+  }
+}
+
+// ----------------------------------------------- z pass with the convolution
+// For every stored element (kx, ky, kz) the reference's CalcGIF loop counts
+// |F|^2 * meshfac(nx,ny,nz) once, halved when kx is 0 or s1/2 (its
+// energyfac halvings along y and z only undo double visits), with
+// nx = kx, ny = min(ky, s2-ky), nz = min(kz, s3-kz); the virial uses
+// (1 + 2 k^2 expf) on top (Src/energy.F90:6553-6578, 6994-7018).
+template <typename R>
+__global__ void __launch_bounds__(256) k_fft_conv(
+    cplx<R> *__restrict__ cm, int s1, int s2, int s3, int hx, int T,
+    FftPlan pz, const cplx<R> *__restrict__ tw, SpmeAxes ax, int energy,
+    double *__restrict__ part) {
+  extern __shared__ char smem[];
+  const int ld = s3 + 1;
+  cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
+  cplx<R> *b = a + T * ld;
+  __shared__ double red[2][256];
+  const int y = blockIdx.y;
+  const int kx0 = blockIdx.x * T;
+  const int nt = min(T, hx - kx0);
+  const size_t zstride = (size_t)s2 * hx;
+  cplx<R> *base = cm + (size_t)y * hx + kx0;
+  for (int e = threadIdx.x; e < nt * s3; e += blockDim.x) {
+    const int z = e / nt, c = e - z * nt;
+    a[c * ld + z] = base[z * zstride + c];
+  }
+  __syncthreads();
+  cplx<R> *f = lds_fft(a, b, pz, nt, ld, tw, false);
+  const int ny = min(y, s2 - y);
+  const double gy = ax.g[1][ny], kky = ax.kk[1][ny];
+  double esum = 0.0, vsum = 0.0;
+  for (int e = threadIdx.x; e < nt * s3; e += blockDim.x) {
+    const int c = e / s3, z = e - c * s3;
+    const int nx = kx0 + c, nz = min(z, s3 - z);
+    const double m2 = ax.kk[0][nx] + kky + ax.kk[2][nz];
+    double mf = 0.0;
+    if (m2 > 0.0) mf = ax.fac0 * ax.g[0][nx] * gy * ax.g[2][nz] / m2;
+    cplx<R> v = f[c * ld + z];
+    if (energy) {
+      const double q2 = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+      const double w = (nx == 0 || 2 * nx == s1) ? 0.5 : 1.0;
+      esum += w * q2 * mf;
+      vsum -= w * q2 * (1.0 + 2.0 * m2 * ax.expf) * mf;
+    }
+    const R mfr = (R)mf;
+    f[c * ld + z] = {v.x * mfr, v.y * mfr};
+  }
+  __syncthreads();
+  cplx<R> *g = (f == a) ? b : a;
+  cplx<R> *res = lds_fft(f, g, pz, nt, ld, tw, true);
+  for (int e = threadIdx.x; e < nt * s3; e += blockDim.x) {
+    const int z = e / nt, c = e - z * nt;
+    base[z * zstride + c] = res[c * ld + z];
+  }
+  if (energy) {
+    red[0][threadIdx.x] = esum;
+    red[1][threadIdx.x] = vsum;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        red[0][threadIdx.x] += red[0][threadIdx.x + w];
+        red[1][threadIdx.x] += red[1][threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const int blk = blockIdx.y * gridDim.x + blockIdx.x;
+      part[2 * blk] = red[0][0];
+      part[2 * blk + 1] = red[1][0];
+    }
+  }
+}
+
+inline int tile_width(int hx, int target) {
+  const int ntiles = (hx + target - 1) / target;
+  return (hx + ntiles - 1) / ntiles;
+}
+
+template <typename R>
+int rows_per_block(int s1) {
+  // two LDS buffers of rows*(s1/2+1) complex values, about 32 KB in all
+  const int per_row = 2 * (s1 / 2 + 1) * (int)sizeof(cplx<R>);
+  int rows = (32 * 1024) / per_row;
+  return rows < 1 ? 1 : (rows > 16 ? 16 : rows);
+}
+
+template <typename R>
+int tile_target(int n) {
+  const int per_col = 2 * (n + 1) * (int)sizeof(cplx<R>);
+  int t = (32 * 1024) / per_col;
+  return t < 1 ? 1 : (t > 16 ? 16 : t);
+}
+
+template <typename R>
+void x_fwd(mfe_engine *e) {
+  const int s1 = e->s[0], nrows = e->s[1] * e->s[2];
+  const int rpb = rows_per_block<R>(s1);
+  const size_t lds = 2 * (size_t)rpb * (s1 / 2 + 1) * sizeof(cplx<R>);
+  hipLaunchKernelGGL(k_fft_x_fwd<R>, dim3((nrows + rpb - 1) / rpb), dim3(256),
+                     lds, e->stream, (const R *)e->mesh_r, (cplx<R> *)e->mesh_c,
+                     s1, nrows, rpb, e->plan_half, (const cplx<R> *)e->tw_half,
+                     (const cplx<R> *)e->tw[0]);
+}
+
+template <typename R>
+void x_inv(mfe_engine *e) {
+  const int s1 = e->s[0], nrows = e->s[1] * e->s[2];
+  const int rpb = rows_per_block<R>(s1);
+  const size_t lds = 2 * (size_t)rpb * (s1 / 2 + 1) * sizeof(cplx<R>);
+  hipLaunchKernelGGL(k_fft_x_inv<R>, dim3((nrows + rpb - 1) / rpb), dim3(256),
+                     lds, e->stream, (const cplx<R> *)e->mesh_c, (R *)e->mesh_r,
+                     s1, nrows, rpb, e->plan_half, (const cplx<R> *)e->tw_half,
+                     (const cplx<R> *)e->tw[0]);
+}
+
+template <typename R>
+void y_pass(mfe_engine *e, int inv) {
+  const int s2 = e->s[1];
+  const int T = tile_width(e->hx, tile_target<R>(s2));
+  const size_t lds = 2 * (size_t)T * (s2 + 1) * sizeof(cplx<R>);
+  dim3 grid((e->hx + T - 1) / T, e->s[2]);
+  hipLaunchKernelGGL(k_fft_y<R>, grid, dim3(256), lds, e->stream,
+                     (cplx<R> *)e->mesh_c, s2, e->hx, T, e->plan[1],
+                     (const cplx<R> *)e->tw[1], inv);
+}
+
+template <typename R>
+void conv(mfe_engine *e, bool energy) {
+  const int s3 = e->s[2];
+  const int T = tile_width(e->hx, tile_target<R>(s3));
+  const size_t lds = 2 * (size_t)T * (s3 + 1) * sizeof(cplx<R>);
+  dim3 grid((e->hx + T - 1) / T, e->s[1]);
+  const int smax2 = (e->s[0] > e->s[1] ? (e->s[0] > e->s[2] ? e->s[0] : e->s[2])
+                                       : (e->s[1] > e->s[2] ? e->s[1] : e->s[2])) / 2 + 1;
+  SpmeAxes ax;
+  for (int a = 0; a < 3; ++a) {
+    ax.g[a] = e->ax_g + a * smax2;
+    ax.kk[a] = e->ax_kk + a * smax2;
+  }
+  const double Pi = 3.1415926535897932;
+  ax.fac0 = 4.0 * Pi / (e->box.L[0] * e->box.L[1] * e->box.L[2]);
+  ax.expf = -1.0 / (4.0 * e->alpha * e->alpha);
+  hipLaunchKernelGGL(k_fft_conv<R>, grid, dim3(256), lds, e->stream,
+                     (cplx<R> *)e->mesh_c, e->s[0], e->s[1], s3, e->hx, T,
+                     e->plan[2], (const cplx<R> *)e->tw[2], ax, energy ? 1 : 0,
+                     e->part + e->off_conv);
+}
+
+}  // namespace
+
+int conv_blocks(mfe_engine *e) {
+  const int T = e->prec == MFE_PREC_MIXED ? tile_width(e->hx, tile_target<float>(e->s[2]))
+                                          : tile_width(e->hx, tile_target<double>(e->s[2]));
+  return ((e->hx + T - 1) / T) * e->s[1];
+}
+
+void fft_x_forward(mfe_engine *e) {
+  if (e->prec == MFE_PREC_MIXED) x_fwd<float>(e); else x_fwd<double>(e);
+}
+void fft_x_inverse(mfe_engine *e) {
+  if (e->prec == MFE_PREC_MIXED) x_inv<float>(e); else x_inv<double>(e);
+}
+void fft_y(mfe_engine *e, int inverse) {
+  if (e->prec == MFE_PREC_MIXED) y_pass<float>(e, inverse); else y_pass<double>(e, inverse);
+}
+void fft_conv(mfe_engine *e, bool energy) {
+  if (e->prec == MFE_PREC_MIXED) conv<float>(e, energy); else conv<double>(e, energy);
+}
+
+}  // namespace mfe

@@ -1,0 +1,1125 @@
+// mfe_kernels.hip -- device kernels of the force engine except the FFT.
+//
+//   neighbour list ...... SetVListMD / NList skin test (Src/nlist.F90)
+//   pair forces ......... UTwoBodyA / UTwoBodyACellList (Src/energy.F90)
+//   SPME ................ UEwaldRecSPM spreading and gather
+//   standard Ewald ...... UEwaldRecStd
+//   bonds ............... UBond
+//   integrator .......... VelVer (Src/md.F90), BDStep (Src/bd.F90)
+//
+// No global floating-point atomics anywhere: charge spreading owns its
+// output bricks and accumulates in LDS; energies and virials are reduced
+// per block into partial sums that one block adds up in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "mfe_internal.h"
+
+namespace mfe {
+
+int conv_blocks(mfe_engine *e);
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------- helpers
+// PBCr2 for the box (Src/molaux.F90:159-182): one shift per axis, then
+// r2 = dx**2 + dy**2 + dz**2 summed left to right without fusion, so that
+// list membership is decided on the reference's FP64 value.
+__device__ __forceinline__ double pbcr2(double &dx, double &dy, double &dz,
+                                        const Box &b) {
+#pragma clang fp contract(off)
+  if (fabs(dx) > b.L2[0]) dx = dx - copysign(b.L[0], dx);
+  if (fabs(dy) > b.L2[1]) dy = dy - copysign(b.L[1], dy);
+  if (fabs(dz) > b.L2[2]) dz = dz - copysign(b.L[2], dz);
+  double r2 = dx * dx;
+  r2 = r2 + dy * dy;
+  r2 = r2 + dz * dz;
+  return r2;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  T r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double block_max(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void k_fill_int(int *p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_double(double *p, int n, double v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Exclusive scan of count[0..n) into start[0..n], zeroing count for the
+// next use; one block of 1024 threads, each owning a contiguous chunk.
+__global__ void __launch_bounds__(1024) k_scan(int *count, int *start, int n,
+                                               const int *gate) {
+  if (gate && gate[1] == 0) return;
+  __shared__ int sh[1024];
+  const int per = (n + blockDim.x - 1) / blockDim.x;
+  const int lo = threadIdx.x * per, hi = min(n, lo + per);
+  int s = 0;
+  for (int i = lo; i < hi; ++i) s += count[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - s;
+  for (int i = lo; i < hi; ++i) {
+    start[i] = run;
+    run += count[i];
+    count[i] = 0;
+  }
+  if (threadIdx.x == blockDim.x - 1) start[n] = sh[threadIdx.x];
+}
+
+// ------------------------------------------------------- neighbour lists
+// Cells of edge >= rcut + drnlist; a particle's cell is
+// floor((x + L/2) * cinv), clamped to the grid.
+__device__ __forceinline__ int cell_of(const double *r, const CellGeom &g,
+                                       const Box &b, int c[3]) {
+  for (int m = 0; m < 3; ++m) {
+    int k = (int)floor((r[m] + b.L2[m]) * g.cinv[m]);
+    c[m] = min(max(k, 0), g.n[m] - 1);
+  }
+  return c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
+}
+
+__global__ void k_cell_count(const double *__restrict__ ro, int np, CellGeom g,
+                             Box b, int *count, int *cof, int *slot,
+                             const int *gate) {
+  if (gate && gate[1] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int c3[3];
+  const int c = cell_of(ro + 3 * i, g, b, c3);
+  cof[i] = c;
+  slot[i] = atomicAdd(&count[c], 1);
+}
+
+__global__ void k_cell_scatter(int np, const int *cof, const int *slot,
+                               const int *start, int *sorted, const int *gate) {
+  if (gate && gate[1] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  sorted[start[cof[i]] + slot[i]] = i;
+}
+
+// members of each cell in ascending particle index (deterministic lists)
+__global__ void k_cell_sort(int ncells, const int *start, int *sorted,
+                            const int *gate) {
+  if (gate && gate[1] == 0) return;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int lo = start[c], hi = start[c + 1];
+  for (int i = lo + 1; i < hi; ++i) {
+    int v = sorted[i], j = i - 1;
+    while (j >= lo && sorted[j] > v) {
+      sorted[j + 1] = sorted[j];
+      --j;
+    }
+    sorted[j + 1] = v;
+  }
+}
+
+// Full Verlet list: every j != i with r2 < (rcut+drnlist)^2 (SetVListMD's
+// test, Src/nlist.F90:746-750); stored column-major nbr[k*np + i].  The
+// half list the reference keeps is the checkerboard subset of this set.
+__global__ void __launch_bounds__(kBlock) k_nlist_build(
+    const double *__restrict__ ro, int np, CellGeom g, Box b,
+    const int *__restrict__ cof, const int *__restrict__ start,
+    const int *__restrict__ sorted, double rs2, int maxnn, int *nn,
+    int *__restrict__ nbr, int *flags, double *drosum, const int *gate) {
+  if (gate && gate[1] == 0) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
+  const int c = cof[i];
+  const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
+  const int rx = g.n[0] > 1 ? 1 : 0, ry = g.n[1] > 1 ? 1 : 0, rz = g.n[2] > 1 ? 1 : 0;
+  int n = 0;
+  for (int dz = -rz; dz <= rz; ++dz) {
+    const int z = (cz + dz + g.n[2]) % g.n[2];
+    for (int dy = -ry; dy <= ry; ++dy) {
+      const int y = (cy + dy + g.n[1]) % g.n[1];
+      for (int dx = -rx; dx <= rx; ++dx) {
+        const int x = (cx + dx + g.n[0]) % g.n[0];
+        const int cc = x + g.n[0] * (y + g.n[1] * z);
+        for (int p = start[cc]; p < start[cc + 1]; ++p) {
+          const int j = sorted[p];
+          if (j == i) continue;
+          double ddx = xi - ro[3 * j], ddy = yi - ro[3 * j + 1], ddz = zi - ro[3 * j + 2];
+          const double r2 = pbcr2(ddx, ddy, ddz, b);
+          if (r2 < rs2) {
+            if (n < maxnn) nbr[(size_t)n * np + i] = j;
+            ++n;
+          }
+        }
+      }
+    }
+  }
+  // on overflow only the stored maxnn partners are visible to the pair
+  // kernel; the call reports MFE_E_NEIGH_CAPACITY
+  nn[i] = min(n, maxnn);
+  if (n > maxnn) atomicOr(&flags[0], ERR_CAPACITY);
+  if (drosum) {
+    drosum[3 * i] = 0.0;
+    drosum[3 * i + 1] = 0.0;
+    drosum[3 * i + 2] = 0.0;
+  }
+  if (i == 0 && gate) flags[2] += 1;
+}
+
+// ------------------------------------------------------------ pair forces
+// First interval (in storage order) with r2 >= zlow: zlow decreases along
+// the table, so this is a binary search for the same interval the
+// reference's linear walk stops at (Src/energy.F90:482-486).
+__device__ __forceinline__ int find_interval(const double *zl, int n, double r2) {
+  int lo = 0, hi = n - 1;
+  if (!(r2 >= zl[hi])) return -1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (r2 >= zl[mid]) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+struct KahanF {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return (double)s - (double)c; }
+};
+
+// Each thread owns particle i and walks its full list, so forces are
+// accumulated without atomics; every pair is therefore seen twice and its
+// energy and virial are counted with weight 1/2.  semantics: 0 = vlist
+// (r2 <= rcut2 kept), 1 = cell list (r2 < rcut2 kept).
+template <bool MIXED>
+__global__ void __launch_bounds__(kBlock) k_pair(
+    const double *__restrict__ ro, const int *__restrict__ type, int np,
+    const int *__restrict__ nn, const int *__restrict__ nbr, Box b,
+    TableDev t, int semantics, int energy, double *__restrict__ force,
+    double *__restrict__ part, int *flags) {
+  __shared__ double sh[kBlock];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // energy per partner type (static indexing keeps it in registers)
+  double ej[MFE_MAX_TYPES];
+#pragma unroll
+  for (int k = 0; k < MFE_MAX_TYPES; ++k) ej[k] = 0.0;
+  double vir = 0.0;
+  int ti = 0;
+  if (i < np) {
+    const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
+    ti = type[i];
+    KahanF fxf, fyf, fzf;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int n = nn[i];
+    for (int k = 0; k < n; ++k) {
+      const int j = nbr[(size_t)k * np + i];
+      double dx = xi - ro[3 * j], dy = yi - ro[3 * j + 1], dz = zi - ro[3 * j + 2];
+      const double r2 = pbcr2(dx, dy, dz, b);
+      if (semantics == 0 ? (r2 > t.rcut2) : !(r2 < t.rcut2)) continue;
+      const int tj = type[j];
+      const int p = t.pairidx[ti * t.npt + tj];
+      double usum, fsum;
+      if (r2 < t.r2atat[p]) {
+        usum = 1e10;
+        fsum = 1e10;
+      } else if (r2 < t.r2umin[p]) {
+        atomicOr(&flags[0], ERR_OVERLAP);
+        continue;
+      } else {
+        const int o = t.off[p];
+        const int iv = find_interval(t.zlow + o, t.nint[p], r2);
+        if (iv < 0) {
+          atomicOr(&flags[0], ERR_TABLE);
+          continue;
+        }
+        const double d = r2 - t.zlow[o + iv];
+        if (MIXED) {
+          const float *cf = t.coef_f + (size_t)(o + iv) * 11;
+          const float df = (float)d;
+          const float fs = cf[6] + df * (cf[7] + df * (cf[8] + df * (cf[9] + df * cf[10])));
+          fsum = fs;
+          if (energy) {
+            const float us = cf[0] + df * (cf[1] + df * (cf[2] + df * (cf[3] + df * (cf[4] + df * cf[5]))));
+            usum = us;
+          } else {
+            usum = 0.0;
+          }
+        } else {
+          const double *cf = t.coef + (size_t)(o + iv) * 11;
+          fsum = cf[6] + d * (cf[7] + d * (cf[8] + d * (cf[9] + d * cf[10])));
+          usum = cf[0] + d * (cf[1] + d * (cf[2] + d * (cf[3] + d * (cf[4] + d * cf[5]))));
+        }
+      }
+      if (MIXED) {
+        const float fs = (float)fsum;
+        fxf.add(fs * (float)dx);
+        fyf.add(fs * (float)dy);
+        fzf.add(fs * (float)dz);
+      } else {
+        fx += fsum * dx;
+        fy += fsum * dy;
+        fz += fsum * dz;
+      }
+      if (energy) {
+#pragma unroll
+        for (int k = 0; k < MFE_MAX_TYPES; ++k) ej[k] += (k == tj) ? 0.5 * usum : 0.0;
+        vir -= 0.5 * fsum * r2;
+      }
+    }
+    if (MIXED) {
+      fx = fxf.value();
+      fy = fyf.value();
+      fz = fzf.value();
+    }
+    force[3 * i] = fx;
+    force[3 * i + 1] = fy;
+    force[3 * i + 2] = fz;
+  }
+  if (energy) {
+    double *out = part + (size_t)blockIdx.x * (t.nptpt + 1);
+    for (int k = 0; k < t.nptpt; ++k) {
+      double v = 0.0;
+      if (i < np) {
+#pragma unroll
+        for (int u = 0; u < MFE_MAX_TYPES; ++u)
+          if (u < t.npt && t.pairidx[ti * t.npt + u] == k) v += ej[u];
+      }
+      double s = block_sum(v, sh);
+      if (threadIdx.x == 0) out[k] = s;
+    }
+    double s = block_sum(vir, sh);
+    if (threadIdx.x == 0) out[t.nptpt] = s;
+  }
+}
+
+// ------------------------------------------------------------------ SPME
+// CardinalBSpline (Src/mollib.F90:1629-1683) at fraction x in [0,1):
+// v[i] is the weight of grid point kmax - i, d[i] its derivative.
+template <typename R, int ORDER>
+__device__ __forceinline__ void bspline(R x, R *v, R *d) {
+  R last[ORDER];
+  v[0] = x;
+  v[1] = R(1) - x;
+#pragma unroll
+  for (int i = 2; i < ORDER; ++i) v[i] = R(0);
+#pragma unroll
+  for (int i = 0; i < ORDER; ++i) last[i] = R(0);
+#pragma unroll
+  for (int n = 3; n <= ORDER; ++n) {
+#pragma unroll
+    for (int i = 0; i < n - 1; ++i) last[i] = v[i];
+    v[0] = x * last[0] / R(n - 1);
+#pragma unroll
+    for (int i = 1; i <= n - 2; ++i) {
+      const R a = R(i) + x;
+      v[i] = (a * last[i] + (R(n) - a) * last[i - 1]) / R(n - 1);
+    }
+    v[n - 1] = (R(1) - x) * last[n - 2] / R(n - 1);
+  }
+  if (d) {
+    d[0] = last[0];
+#pragma unroll
+    for (int i = 1; i <= ORDER - 2; ++i) d[i] = last[i] - last[i - 1];
+    d[ORDER - 1] = -last[ORDER - 2];
+  }
+}
+
+struct SpmeGeom {
+  int s[3], B[3], nb[3], order;
+  double dkdr[3];
+};
+
+__device__ __forceinline__ void mesh_coords(const double *r, const SpmeGeom &g,
+                                            int mm[3], double fr[3]) {
+  for (int m = 0; m < 3; ++m) {
+    const double u = g.dkdr[m] * r[m];
+    const double f = floor(u);
+    fr[m] = u - f;
+    int k = (int)f % g.s[m];
+    mm[m] = k < 0 ? k + g.s[m] : k;
+  }
+}
+
+__global__ void k_spme_count(const double *__restrict__ ro, int np, SpmeGeom g,
+                             int *bcount, int *bid, int *bslot) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int mm[3];
+  double fr[3];
+  mesh_coords(ro + 3 * i, g, mm, fr);
+  const int id = mm[0] / g.B[0] + g.nb[0] * (mm[1] / g.B[1] + g.nb[1] * (mm[2] / g.B[2]));
+  bid[i] = id;
+  bslot[i] = atomicAdd(&bcount[id], 1);
+}
+
+// particle records in brick order: fraction (x, y, z) and charge, plus the
+// packed meshmax (10 bits per axis) and the particle index
+template <typename R>
+__global__ void k_spme_scatter(const double *__restrict__ ro,
+                               const double *__restrict__ q, int np, SpmeGeom g,
+                               const int *bid, const int *bslot,
+                               const int *bstart, R *frac, int2 *meta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  int mm[3];
+  double fr[3];
+  mesh_coords(ro + 3 * i, g, mm, fr);
+  const int pos = bstart[bid[i]] + bslot[i];
+  frac[4 * pos] = (R)fr[0];
+  frac[4 * pos + 1] = (R)fr[1];
+  frac[4 * pos + 2] = (R)fr[2];
+  frac[4 * pos + 3] = (R)q[i];
+  meta[pos] = make_int2(mm[0] | (mm[1] << 10) | (mm[2] << 20), i);
+}
+
+// LDS accumulator of the spreading: exact 64-bit fixed point (2^-32) for
+// the mixed path, so the sum does not depend on the order of the adds;
+// FP64 LDS atomics for the FP64 path.
+template <typename R>
+struct SpreadAcc;
+template <>
+struct SpreadAcc<float> {
+  typedef unsigned long long T;
+  static __device__ __forceinline__ void add(T *p, float v) {
+    atomicAdd(p, (unsigned long long)(long long)__float2ll_rn(v * 4294967296.0f));
+  }
+  static __device__ __forceinline__ float get(T v) {
+    return (float)((double)(long long)v * (1.0 / 4294967296.0));
+  }
+};
+template <>
+struct SpreadAcc<double> {
+  typedef double T;
+  static __device__ __forceinline__ void add(T *p, double v) { atomicAdd(p, v); }
+  static __device__ __forceinline__ double get(T v) { return v; }
+};
+
+constexpr int kMaxOrder = 8;
+template <typename R> struct Chunk { static constexpr int n = 256; };
+template <> struct Chunk<double> { static constexpr int n = 128; };
+
+// One block owns one brick of B0 x B1 x B2 mesh points and is the only
+// writer of it.  A particle with meshmax m adds to points m, m-1, ...,
+// m-order+1 (Src/energy.F90:1232-1244), so a brick receives from the
+// particles binned in itself and in the next brick along each axis
+// (B >= order-1).  Work items are (particle, iz, iy) rows of order adds.
+template <typename R, int ORDER>
+__global__ void __launch_bounds__(kBlock) k_spread(
+    const R *__restrict__ frac, const int2 *__restrict__ meta,
+    const int *__restrict__ bstart, SpmeGeom g, R *__restrict__ mesh) {
+  typedef typename SpreadAcc<R>::T Acc;
+  constexpr int kChunk = Chunk<R>::n;
+  constexpr int order = ORDER;
+  extern __shared__ char smem[];
+  Acc *tile = reinterpret_cast<Acc *>(smem);
+  __shared__ R w[kChunk][3][ORDER];
+  __shared__ int mmx[kChunk][3];
+  __shared__ R qs[kChunk];
+  __shared__ int seg_lo[8], seg_len[8], seg_acc[9];
+  const int bx = blockIdx.x % g.nb[0];
+  const int by = (blockIdx.x / g.nb[0]) % g.nb[1];
+  const int bz = blockIdx.x / (g.nb[0] * g.nb[1]);
+  const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
+  const int e0 = min(g.B[0], g.s[0] - o0), e1 = min(g.B[1], g.s[1] - o1),
+            e2 = min(g.B[2], g.s[2] - o2);
+  const int ntile = g.B[0] * g.B[1] * g.B[2];
+  for (int k = threadIdx.x; k < ntile; k += blockDim.x) tile[k] = Acc(0);
+  if (threadIdx.x == 0) {
+    int ns = 0, acc = 0;
+    const int ex = g.nb[0] > 1 ? 2 : 1, ey = g.nb[1] > 1 ? 2 : 1, ez = g.nb[2] > 1 ? 2 : 1;
+    for (int dz = 0; dz < ez; ++dz)
+      for (int dy = 0; dy < ey; ++dy)
+        for (int dx = 0; dx < ex; ++dx) {
+          const int cb = (bx + dx) % g.nb[0] +
+                         g.nb[0] * ((by + dy) % g.nb[1] + g.nb[1] * ((bz + dz) % g.nb[2]));
+          seg_lo[ns] = bstart[cb];
+          seg_len[ns] = bstart[cb + 1] - bstart[cb];
+          seg_acc[ns] = acc;
+          acc += seg_len[ns];
+          ++ns;
+        }
+    for (int k = ns; k < 8; ++k) {
+      seg_len[k] = 0;
+      seg_acc[k] = acc;
+    }
+    seg_acc[8] = acc;
+  }
+  __syncthreads();
+  const int ncand = seg_acc[8];
+  for (int c0 = 0; c0 < ncand; c0 += kChunk) {
+    const int nc = min(kChunk, ncand - c0);
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const int gc = c0 + c;
+      int sgi = 0;
+      while (gc >= seg_acc[sgi + 1]) ++sgi;
+      const int rec = seg_lo[sgi] + (gc - seg_acc[sgi]);
+      const int2 md = meta[rec];
+      mmx[c][0] = md.x & 1023;
+      mmx[c][1] = (md.x >> 10) & 1023;
+      mmx[c][2] = (md.x >> 20) & 1023;
+      qs[c] = frac[4 * rec + 3];
+      for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * rec + m], w[c][m], nullptr);
+    }
+    __syncthreads();
+    const int items = nc * order * order;
+    for (int it = threadIdx.x; it < items; it += blockDim.x) {
+      const int c = it / (order * order);
+      const int rem = it - c * order * order;
+      const int iz = rem / order, iy = rem - iz * order;
+      int z = mmx[c][2] - iz;
+      if (z < 0) z += g.s[2];
+      z -= o2;
+      if (z < 0 || z >= e2) continue;
+      int y = mmx[c][1] - iy;
+      if (y < 0) y += g.s[1];
+      y -= o1;
+      if (y < 0 || y >= e1) continue;
+      const R qyz = qs[c] * w[c][2][iz] * w[c][1][iy];
+      Acc *row = tile + (z * g.B[1] + y) * g.B[0];
+      const int mx = mmx[c][0];
+#pragma unroll
+      for (int ix = 0; ix < order; ++ix) {
+        int x = mx - ix;
+        if (x < 0) x += g.s[0];
+        x -= o0;
+        if (x < 0 || x >= e0) continue;
+        SpreadAcc<R>::add(&row[x], qyz * w[c][0][ix]);
+      }
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < e0 * e1 * e2; k += blockDim.x) {
+    const int x = k % e0, y = (k / e0) % e1, z = k / (e0 * e1);
+    mesh[((size_t)(o2 + z) * g.s[1] + (o1 + y)) * g.s[0] + (o0 + x)] =
+        SpreadAcc<R>::get(tile[(z * g.B[1] + y) * g.B[0] + x]);
+  }
+}
+
+// CalcProp (Src/energy.F90:1259-1285): esum over the order^3 stencil of the
+// convolved mesh; force -= q * esum * dkdr * EpsiFourPi.  Threads walk the
+// brick-ordered records, so neighbouring threads read neighbouring mesh.
+template <typename R>
+struct Acc3;
+template <>
+struct Acc3<float> {
+  KahanF a[3];
+  __device__ __forceinline__ void add(int m, float v) { a[m].add(v); }
+  __device__ __forceinline__ double get(int m) const { return a[m].value(); }
+};
+template <>
+struct Acc3<double> {
+  double a[3] = {0.0, 0.0, 0.0};
+  __device__ __forceinline__ void add(int m, double v) { a[m] += v; }
+  __device__ __forceinline__ double get(int m) const { return a[m]; }
+};
+
+template <typename R, int ORDER>
+__global__ void __launch_bounds__(kBlock) k_gather(
+    const R *__restrict__ frac, const int2 *__restrict__ meta, int np,
+    SpmeGeom g, const R *__restrict__ mesh, double efp,
+    double *__restrict__ force) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  constexpr int order = ORDER;
+  R v[3][ORDER], d[3][ORDER];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * t + m], v[m], d[m]);
+  const R q = frac[4 * t + 3];
+  const int2 md = meta[t];
+  const int mx = md.x & 1023, my = (md.x >> 10) & 1023, mz = (md.x >> 20) & 1023;
+  Acc3<R> acc;
+#pragma unroll
+  for (int iz = 0; iz < order; ++iz) {
+    int z = mz - iz;
+    if (z < 0) z += g.s[2];
+#pragma unroll
+    for (int iy = 0; iy < order; ++iy) {
+      int y = my - iy;
+      if (y < 0) y += g.s[1];
+      const R *row = mesh + ((size_t)z * g.s[1] + y) * g.s[0];
+      const R vyz = v[1][iy] * v[2][iz];
+      const R dyz = d[1][iy] * v[2][iz];
+      const R vdz = v[1][iy] * d[2][iz];
+      R sx = R(0), sdx = R(0);
+#pragma unroll
+      for (int ix = 0; ix < order; ++ix) {
+        int x = mx - ix;
+        if (x < 0) x += g.s[0];
+        const R qv = row[x];
+        sx += v[0][ix] * qv;
+        sdx += d[0][ix] * qv;
+      }
+      acc.add(0, sdx * vyz);
+      acc.add(1, sx * dyz);
+      acc.add(2, sx * vdz);
+    }
+  }
+  const int i = md.y;
+  const double qd = (double)q;
+  force[3 * i] -= qd * acc.get(0) * g.dkdr[0] * efp;
+  force[3 * i + 1] -= qd * acc.get(1) * g.dkdr[1] * efp;
+  force[3 * i + 2] -= qd * acc.get(2) * g.dkdr[2] * efp;
+}
+
+// --------------------------------------------------------- standard Ewald
+// UEwaldRecStd (Src/energy.F90:1050-1096) over the k-vectors of its kfac
+// table: one block per k-vector sums the four sign combinations of the
+// structure factor; a second kernel gives each particle its force.
+__global__ void __launch_bounds__(kBlock) k_ewald_sk(
+    const double *__restrict__ ro, const double *__restrict__ q, int np,
+    const double *__restrict__ kv, double *__restrict__ ksum, int energy,
+    double alpha2, double *part) {
+  __shared__ double sh[kBlock];
+  const int kn = blockIdx.x;
+  const double kx = kv[4 * kn], ky = kv[4 * kn + 1], kz = kv[4 * kn + 2];
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < np; i += blockDim.x) {
+    const double x = kx * ro[3 * i], y = ky * ro[3 * i + 1], z = kz * ro[3 * i + 2];
+    double sa, ca;
+    // combinations (-x,-y,+z), (-x,+y,+z), (+x,-y,+z), (+x,+y,+z)
+    sincos(-x - y + z, &sa, &ca);
+    s[0] += q[i] * ca; s[1] += q[i] * sa;
+    sincos(-x + y + z, &sa, &ca);
+    s[2] += q[i] * ca; s[3] += q[i] * sa;
+    sincos(x - y + z, &sa, &ca);
+    s[4] += q[i] * ca; s[5] += q[i] * sa;
+    sincos(x + y + z, &sa, &ca);
+    s[6] += q[i] * ca; s[7] += q[i] * sa;
+  }
+  double tot[8];
+  for (int c = 0; c < 8; ++c) tot[c] = block_sum(s[c], sh);
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < 8; ++c) ksum[8 * kn + c] = tot[c];
+    if (energy) {
+      double ssum = 0.0;
+      for (int c = 0; c < 8; ++c) ssum += tot[c] * tot[c];
+      const double term = kv[4 * kn + 3] * ssum;
+      const double k2 = kx * kx + ky * ky + kz * kz;
+      part[2 * kn] = term;
+      part[2 * kn + 1] = -(1.0 - k2 / (2.0 * alpha2)) * term;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ewald_force(
+    const double *__restrict__ ro, const double *__restrict__ q, int np,
+    const double *__restrict__ kv, const double *__restrict__ ksum, int nk,
+    double efp, double *__restrict__ force) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  for (int kn = 0; kn < nk; ++kn) {
+    const double kx = kv[4 * kn], ky = kv[4 * kn + 1], kz = kv[4 * kn + 2];
+    const double kfac2 = efp * 2.0 * kv[4 * kn + 3];
+    const double *S = ksum + 8 * kn;
+    const double x = kx * xi, y = ky * yi, z = kz * zi;
+    double f[4];
+    const double args[4] = {-x - y + z, -x + y + z, x - y + z, x + y + z};
+    for (int c = 0; c < 4; ++c) {
+      double sa, ca;
+      sincos(args[c], &sa, &ca);
+      // -Re(q e) Im(S) + Im(q e) Re(S)
+      f[c] = q[i] * (-ca * S[2 * c + 1] + sa * S[2 * c]);
+    }
+    fx += kfac2 * kx * (-f[0] - f[1] + f[2] + f[3]);
+    fy += kfac2 * ky * (-f[0] + f[1] - f[2] + f[3]);
+    fz += kfac2 * kz * (f[0] + f[1] + f[2] + f[3]);
+  }
+  force[3 * i] += fx;
+  force[3 * i + 1] += fy;
+  force[3 * i + 2] += fz;
+}
+
+// ------------------------------------------------------------------ bonds
+// UBond (Src/energy.F90:5335-5372): u = k (r - eq)^p with k already
+// divided by p; each particle gathers the forces of its (at most two)
+// bonds, and the bond to the next bead carries the energy.
+__global__ void __launch_bounds__(kBlock) k_bonds(
+    const double *__restrict__ ro, int np, const int *__restrict__ prev,
+    const int *__restrict__ next, Box b, double k, int p, double eq,
+    int energy, double *__restrict__ force, double *part) {
+  __shared__ double sh[kBlock];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ub = 0.0, vir = 0.0;
+  if (i < np) {
+    double f[3] = {0.0, 0.0, 0.0};
+    const int nb2[2] = {next[i], prev[i]};
+    for (int s = 0; s < 2; ++s) {
+      const int j = nb2[s];
+      if (j < 0) continue;
+      // bond (a, c) with a the earlier bead: dx = ro(a) - ro(c)
+      const int a = s == 0 ? i : j, c = s == 0 ? j : i;
+      double dx = ro[3 * a] - ro[3 * c], dy = ro[3 * a + 1] - ro[3 * c + 1],
+             dz = ro[3 * a + 2] - ro[3 * c + 2];
+      const double r2 = pbcr2(dx, dy, dz, b);
+      const double r1 = sqrt(r2);
+      const double term = r1 - eq;
+      double pw = 1.0;
+      for (int e = 1; e < p; ++e) pw *= term;
+      double fac = k * pw;
+      if (s == 0) {
+        ub += fac * term;
+      }
+      fac = -p * fac / r1;
+      const double sg = s == 0 ? 1.0 : -1.0;
+      f[0] += sg * fac * dx;
+      f[1] += sg * fac * dy;
+      f[2] += sg * fac * dz;
+      if (s == 0) vir -= fac * r2;
+    }
+    force[3 * i] += f[0];
+    force[3 * i + 1] += f[1];
+    force[3 * i + 2] += f[2];
+  }
+  if (energy) {
+    double s1 = block_sum(ub, sh);
+    double s2 = block_sum(vir, sh);
+    if (threadIdx.x == 0) {
+      part[2 * blockIdx.x] = s1;
+      part[2 * blockIdx.x + 1] = s2;
+    }
+  }
+}
+
+// ------------------------------------------------------- surface + sums
+__global__ void __launch_bounds__(kBlock) k_surf_sum(
+    const double *__restrict__ ro, const double *__restrict__ q, int np,
+    double *part) {
+  __shared__ double sh[kBlock];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double s[3] = {0, 0, 0};
+  if (i < np)
+    for (int m = 0; m < 3; ++m) s[m] = q[i] * ro[3 * i + m];
+  for (int m = 0; m < 3; ++m) {
+    double t = block_sum(s[m], sh);
+    if (threadIdx.x == 0) part[3 * blockIdx.x + m] = t;
+  }
+}
+
+struct ReduceArgs {
+  int nptpt;
+  int nblk_pair, nblk_conv, nblk_bond, nblk_surf, nk;
+  int off_pair, off_conv, off_bond, off_surf, off_std;
+  int ewald, lsurf, energy;
+  double efp, self, vol;
+};
+
+// Final sums in a fixed order (one block): writes
+// en = [tot, twob0, rec, bond, virial, Sx, Sy, Sz, twob(1..nptpt)].
+__global__ void __launch_bounds__(kBlock) k_reduce(const double *part,
+                                                   ReduceArgs a, double *en) {
+  __shared__ double sh[kBlock];
+  auto region = [&](int off, int nblk, int stride, int comp) {
+    double s = 0.0;
+    for (int b2 = threadIdx.x; b2 < nblk; b2 += blockDim.x) s += part[off + b2 * stride + comp];
+    return block_sum(s, sh);
+  };
+  double S[3] = {0, 0, 0};
+  if (a.lsurf)
+    for (int m = 0; m < 3; ++m) S[m] = region(a.off_surf, a.nblk_surf, 3, m);
+  if (!a.energy) {
+    if (threadIdx.x == 0)
+      for (int m = 0; m < 3; ++m) en[5 + m] = S[m];
+    return;
+  }
+  double twob0 = 0.0, vir = 0.0;
+  for (int k = 0; k < a.nptpt; ++k) {
+    const double u = region(a.off_pair, a.nblk_pair, a.nptpt + 1, k);
+    if (threadIdx.x == 0) en[8 + k] = u;
+    twob0 += u;
+  }
+  vir += region(a.off_pair, a.nblk_pair, a.nptpt + 1, a.nptpt);
+  double urec = 0.0, virrec = 0.0;
+  if (a.ewald == MFE_EWALD_SPME) {
+    urec = region(a.off_conv, a.nblk_conv, 2, 0);
+    virrec = region(a.off_conv, a.nblk_conv, 2, 1);
+  } else if (a.ewald == MFE_EWALD_STD) {
+    urec = region(a.off_std, a.nk, 2, 0);
+    virrec = region(a.off_std, a.nk, 2, 1);
+  }
+  double ubond = 0.0;
+  if (a.nblk_bond > 0) {
+    ubond = region(a.off_bond, a.nblk_bond, 2, 0);
+    vir += region(a.off_bond, a.nblk_bond, 2, 1);
+  }
+  double rec = 0.0;
+  if (a.ewald != MFE_EWALD_NONE) {
+    urec += a.self;
+    if (a.lsurf) {
+      const double fac = 2.0 * 3.1415926535897932 / (3.0 * a.vol);
+      const double term = fac * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
+      urec += term;
+      virrec -= term;
+    }
+    rec = a.efp * urec;
+    vir += a.efp * virrec;
+  }
+  if (threadIdx.x == 0) {
+    en[0] = twob0 + rec + ubond;
+    en[1] = twob0;
+    en[2] = rec;
+    en[3] = ubond;
+    en[4] = vir;
+    for (int m = 0; m < 3; ++m) en[5 + m] = S[m];
+  }
+}
+
+// ------------------------------------------------------------ integrators
+// VelVer positions (Src/md.F90:287-298): drostep = dt v + dt^2/2 a,
+// ro += drostep, PBC with ANINT; the skin bookkeeping of NList
+// (Src/nlist.F90:601-603) is folded in: drosum += drostep, r2 = |drosum|^2
+// and the block maximum of r2 for the second-largest rule.
+// For BD (Src/bd.F90:141-151): drostep = fac1 forceo + fac2 noise.
+__global__ void __launch_bounds__(kBlock) k_move(
+    double *__restrict__ ro, const double *__restrict__ rod,
+    const double *__restrict__ rodd, const double *__restrict__ forceo,
+    const int *__restrict__ type, const double *__restrict__ fac1,
+    const double *__restrict__ fac2, const double *__restrict__ noise, int np,
+    Box b, double dt, int bd, double *__restrict__ drosum, double *r2disp,
+    double *part_max) {
+#pragma clang fp contract(off)
+  __shared__ double sh[kBlock];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double r2 = 0.0;
+  if (i < np) {
+    const double fac = 0.5 * (dt * dt);
+    double d[3];
+    for (int m = 0; m < 3; ++m) {
+      if (bd) {
+        const int t = type[i];
+        d[m] = fac1[t] * forceo[3 * i + m] + fac2[t] * noise[3 * i + m];
+      } else {
+        d[m] = dt * rod[3 * i + m] + fac * rodd[3 * i + m];
+      }
+      double x = ro[3 * i + m] + d[m];
+      if (fabs(x) > b.L2[m]) x = x - b.L[m] * round(x * b.Li[m]);
+      ro[3 * i + m] = x;
+      drosum[3 * i + m] = drosum[3 * i + m] + d[m];
+    }
+    r2 = drosum[3 * i] * drosum[3 * i];
+    r2 = r2 + drosum[3 * i + 1] * drosum[3 * i + 1];
+    r2 = r2 + drosum[3 * i + 2] * drosum[3 * i + 2];
+    r2disp[i] = r2;
+  }
+  const double m = block_max(r2, sh);
+  if (threadIdx.x == 0) part_max[blockIdx.x] = m;
+}
+
+// NList's r2max2 (Src/nlist.F90:602-608) is the largest r2 that is not a
+// new running maximum at the moment it is visited in particle order.  Each
+// block owns the same particle range as in k_move; the running maximum
+// entering it is the max of the earlier blocks' maxima.
+__global__ void __launch_bounds__(kBlock) k_skin_blocks(
+    const double *__restrict__ r2disp, int np, const double *part_max,
+    double *part_cand) {
+  __shared__ double sh[kBlock];
+  double pm = 0.0;
+  for (int b2 = threadIdx.x; b2 < (int)blockIdx.x; b2 += blockDim.x) pm = fmax(pm, part_max[b2]);
+  const double P = block_max(pm, sh);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double v = i < np ? r2disp[i] : 0.0;
+  // inclusive running max in particle order, then shift to exclusive
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    double o = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0.0;
+    __syncthreads();
+    sh[threadIdx.x] = fmax(sh[threadIdx.x], o);
+    __syncthreads();
+  }
+  const double before = fmax(P, threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0.0);
+  __syncthreads();
+  const double cand = (i < np && !(v > before)) ? v : 0.0;
+  const double c = block_max(cand, sh);
+  if (threadIdx.x == 0) part_cand[blockIdx.x] = c;
+}
+
+__global__ void __launch_bounds__(kBlock) k_skin_final(const double *part_max,
+                                                       const double *part_cand,
+                                                       int nblk, double drnlist,
+                                                       int *flags) {
+  __shared__ double sh[kBlock];
+  double a = 0.0, c = 0.0;
+  for (int b2 = threadIdx.x; b2 < nblk; b2 += blockDim.x) {
+    a = fmax(a, part_max[b2]);
+    c = fmax(c, part_cand[b2]);
+  }
+  const double m1 = block_max(a, sh);
+  const double m2 = block_max(c, sh);
+  if (threadIdx.x == 0) flags[1] = (sqrt(m1) + sqrt(m2) > drnlist) ? 1 : 0;
+}
+
+// UTotal's tail (Src/energy.F90:405-418) plus, for MD, GetLinAcc and the
+// velocity half of VelVer (Src/md.F90:348-358):
+//   mode 0: forceo = force*fac        mode 1: + VelVer kick
+__global__ void __launch_bounds__(kBlock) k_finalize(
+    const double *__restrict__ force, double *__restrict__ forceo,
+    const double *__restrict__ q, const double *__restrict__ massip, int np,
+    double ffac, int surf, double surf_fac, const double *en, int mode,
+    double dt, double *__restrict__ rod, double *__restrict__ rodd) {
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  for (int m = 0; m < 3; ++m) {
+    double f = force[3 * i + m];
+    if (surf) f = f - surf_fac * q[i] * en[5 + m];
+    const double fo = f * ffac;
+    forceo[3 * i + m] = fo;
+    if (mode == 1) {
+      const double aold = rodd[3 * i + m];
+      const double anew = massip[i] * fo;
+      rodd[3 * i + m] = anew;
+      rod[3 * i + m] = rod[3 * i + m] + 0.5 * dt * (anew + aold);
+    }
+  }
+}
+
+inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
+
+SpmeGeom spme_geom(const mfe_engine *e) {
+  SpmeGeom g;
+  for (int m = 0; m < 3; ++m) {
+    g.s[m] = e->s[m];
+    g.B[m] = e->B[m];
+    g.nb[m] = e->nb[m];
+    g.dkdr[m] = e->s[m] / e->box.L[m];
+  }
+  g.order = e->order;
+  return g;
+}
+
+TableDev table_dev(const mfe_engine *e) {
+  TableDev t;
+  t.zlow = e->t_zlow;
+  t.coef = e->t_coef;
+  t.coef_f = e->t_coef_f;
+  t.off = e->t_off;
+  t.nint = e->t_nint;
+  t.r2atat = e->t_r2atat;
+  t.r2umin = e->t_r2umin;
+  t.pairidx = e->t_pairidx;
+  t.npt = e->npt;
+  t.nptpt = e->nptpt;
+  t.rcut2 = e->rcut * e->rcut;
+  return t;
+}
+
+}  // namespace
+
+template <typename R, int O>
+const void *k_spread_ptr() {
+  return (const void *)&k_spread<R, O>;
+}
+
+void launch_fill_int(int *p, int n, int v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, v);
+}
+void launch_fill_double(double *p, int n, double v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_double, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, v);
+}
+
+void launch_nlist_rebuild(mfe_engine *e, bool gated) {
+  const int *gate = gated ? e->flags : nullptr;
+  const int np = e->np;
+  const double rs = e->rcut + e->drnlist;
+  hipLaunchKernelGGL(k_cell_count, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+                     e->ro, np, e->cg, e->box, e->c_count, e->c_of, e->c_slot, gate);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, e->stream, e->c_count,
+                     e->c_start, e->cg.ncells, gate);
+  hipLaunchKernelGGL(k_cell_scatter, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+                     np, e->c_of, e->c_slot, e->c_start, e->c_sorted, gate);
+  hipLaunchKernelGGL(k_cell_sort, dim3(nblocks(e->cg.ncells)), dim3(kBlock), 0,
+                     e->stream, e->cg.ncells, e->c_start, e->c_sorted, gate);
+  hipLaunchKernelGGL(k_nlist_build, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+                     e->ro, np, e->cg, e->box, e->c_of, e->c_start, e->c_sorted,
+                     rs * rs, e->maxnn, e->nn, e->nbr, e->flags, e->drosum, gate);
+}
+
+void launch_pair(mfe_engine *e, bool energy) {
+  const int np = e->np;
+  TableDev t = table_dev(e);
+  const int sem = e->nlkind == MFE_NL_CELL ? 1 : 0;
+  if (e->prec == MFE_PREC_MIXED)
+    hipLaunchKernelGGL(k_pair<true>, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+                       e->ro, e->type, np, e->nn, e->nbr, e->box, t, sem,
+                       energy ? 1 : 0, e->force, e->part + e->off_pair, e->flags);
+  else
+    hipLaunchKernelGGL(k_pair<false>, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+                       e->ro, e->type, np, e->nn, e->nbr, e->box, t, sem,
+                       energy ? 1 : 0, e->force, e->part + e->off_pair, e->flags);
+}
+
+template <typename R>
+static void spme_stages(mfe_engine *e, bool energy, int first, int last) {
+  const int np = e->np;
+  SpmeGeom g = spme_geom(e);
+  for (int st = first; st <= last; ++st) {
+    switch (st) {
+      case SP_BIN:
+        hipLaunchKernelGGL(k_spme_count, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+                           e->ro, np, g, e->b_count, e->b_id, e->b_slot);
+        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, e->stream, e->b_count,
+                           e->b_start, e->nbricks, (const int *)nullptr);
+        hipLaunchKernelGGL(k_spme_scatter<R>, dim3(nblocks(np)), dim3(kBlock), 0,
+                           e->stream, e->ro, e->q, np, g, e->b_id, e->b_slot,
+                           e->b_start, (R *)e->rec_frac, e->rec_meta);
+        break;
+      case SP_SPREAD:
+        switch (e->order) {
+#define MFE_SPREAD(O)                                                                 \
+  case O:                                                                             \
+    hipLaunchKernelGGL((k_spread<R, O>), dim3(e->nbricks), dim3(kBlock), e->spread_lds, \
+                       e->stream, (const R *)e->rec_frac, e->rec_meta, e->b_start, g,   \
+                       (R *)e->mesh_r);                                                \
+    break;
+          MFE_SPREAD(3) MFE_SPREAD(4) MFE_SPREAD(5) MFE_SPREAD(6) MFE_SPREAD(7) MFE_SPREAD(8)
+#undef MFE_SPREAD
+        }
+        break;
+      case SP_FFTX: fft_x_forward(e); break;
+      case SP_FFTY: fft_y(e, 0); break;
+      case SP_CONV: fft_conv(e, energy); break;
+      case SP_IFFTY: fft_y(e, 1); break;
+      case SP_IFFTX: fft_x_inverse(e); break;
+      case SP_GATHER:
+        switch (e->order) {
+#define MFE_GATHER(O)                                                                   \
+  case O:                                                                               \
+    hipLaunchKernelGGL((k_gather<R, O>), dim3(nblocks(np)), dim3(kBlock), 0, e->stream, \
+                       (const R *)e->rec_frac, e->rec_meta, np, g, (const R *)e->mesh_r, \
+                       e->efp, e->force);                                               \
+    break;
+          MFE_GATHER(3) MFE_GATHER(4) MFE_GATHER(5) MFE_GATHER(6) MFE_GATHER(7) MFE_GATHER(8)
+#undef MFE_GATHER
+        }
+        break;
+    }
+  }
+}
+
+void launch_spme(mfe_engine *e, bool energy, int first, int last) {
+  if (e->prec == MFE_PREC_MIXED) spme_stages<float>(e, energy, first, last);
+  else spme_stages<double>(e, energy, first, last);
+}
+
+void launch_std_ewald(mfe_engine *e) {
+  hipLaunchKernelGGL(k_ewald_sk, dim3(e->nkvec), dim3(kBlock), 0, e->stream, e->ro,
+                     e->q, e->np, e->k_vec, e->k_sum, 1, e->alpha * e->alpha,
+                     e->part + e->off_std);
+  hipLaunchKernelGGL(k_ewald_force, dim3(nblocks(e->np)), dim3(kBlock), 0, e->stream,
+                     e->ro, e->q, e->np, e->k_vec, e->k_sum, e->nkvec, e->efp, e->force);
+}
+
+void launch_bonds(mfe_engine *e) {
+  hipLaunchKernelGGL(k_bonds, dim3(nblocks(e->np)), dim3(kBlock), 0, e->stream, e->ro,
+                     e->np, e->ch_prev, e->ch_next, e->box, e->bond_k, e->bond_p,
+                     e->bond_eq, 1, e->force, e->part + e->off_bond);
+}
+
+void launch_surf(mfe_engine *e) {
+  hipLaunchKernelGGL(k_surf_sum, dim3(nblocks(e->np)), dim3(kBlock), 0, e->stream, e->ro,
+                     e->q, e->np, e->part + e->off_surf);
+}
+
+void launch_reduce(mfe_engine *e, unsigned flags) {
+  ReduceArgs a;
+  a.nptpt = e->nptpt;
+  a.nblk_pair = e->nblk_pair;
+  a.nblk_conv = e->ewald == MFE_EWALD_SPME ? conv_blocks(e) : 0;
+  a.nblk_bond = e->nchain > 0 ? e->nblk_bond : 0;
+  a.nblk_surf = e->nblk_surf;
+  a.nk = e->nkvec;
+  a.off_pair = e->off_pair;
+  a.off_conv = e->off_conv;
+  a.off_bond = e->off_bond;
+  a.off_surf = e->off_surf;
+  a.off_std = e->off_std;
+  a.ewald = e->ewald;
+  a.lsurf = e->lsurf;
+  a.energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
+  a.efp = e->efp;
+  a.self = e->self_energy;
+  a.vol = e->box.L[0] * e->box.L[1] * e->box.L[2];
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e->stream, e->part, a, e->en);
+}
+
+void launch_finalize(mfe_engine *e, int mode, double dt) {
+  const double sclvel = 1.0e-10 / 1.0e-12, sclacc = sclvel / 1.0e-12;
+  const double ffac = (1.0e+03 / 1.0e-10) / (sclacc * 1.0e-03);
+  const double surf_fac = e->efp * (2.0 * (2.0 * 3.1415926535897932 /
+                                           (3.0 * e->box.L[0] * e->box.L[1] * e->box.L[2])));
+  hipLaunchKernelGGL(k_finalize, dim3(nblocks(e->np)), dim3(kBlock), 0, e->stream,
+                     e->force, e->forceo, e->q, e->massip, e->np, ffac,
+                     (e->lsurf && e->ewald != MFE_EWALD_NONE) ? 1 : 0, surf_fac, e->en,
+                     mode, dt, e->rod, e->rodd);
+}
+
+void launch_move(mfe_engine *e, double dt, int bd, const double *fac1,
+                 const double *fac2, const double *noise) {
+  hipLaunchKernelGGL(k_move, dim3(e->nblk_move), dim3(kBlock), 0, e->stream, e->ro,
+                     e->rod, e->rodd, e->forceo, e->type, fac1, fac2, noise, e->np,
+                     e->box, dt, bd, e->drosum, e->r2disp, e->part + e->off_move);
+}
+
+void launch_skin(mfe_engine *e) {
+  hipLaunchKernelGGL(k_skin_blocks, dim3(e->nblk_move), dim3(kBlock), 0, e->stream,
+                     e->r2disp, e->np, e->part + e->off_move, e->part + e->off_skin);
+  hipLaunchKernelGGL(k_skin_final, dim3(1), dim3(kBlock), 0, e->stream,
+                     e->part + e->off_move, e->part + e->off_skin, e->nblk_move,
+                     e->drnlist, e->flags);
+}
+
+}  // namespace mfe
+
+// Spreading bricks above 64 KB of LDS (meshes whose only divisors >= order-1
+// are large, e.g. 2*23) need the per-kernel dynamic LDS limit raised.
+void mfe_allow_big_spread_lds(int prec, int bytes) {
+#define MFE_ATTR(R, O)                                                       \
+  (void)hipFuncSetAttribute((const void *)mfe::k_spread_ptr<R, O>(),          \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (prec == MFE_PREC_MIXED) {
+    MFE_ATTR(float, 3) MFE_ATTR(float, 4) MFE_ATTR(float, 5)
+    MFE_ATTR(float, 6) MFE_ATTR(float, 7) MFE_ATTR(float, 8)
+  } else {
+    MFE_ATTR(double, 3) MFE_ATTR(double, 4) MFE_ATTR(double, 5)
+    MFE_ATTR(double, 6) MFE_ATTR(double, 7) MFE_ATTR(double, 8)
+  }
+#undef MFE_ATTR
+}

@@ -1,0 +1,261 @@
+"""HIP engine against the oracle and the reference's golden vectors.
+
+Tolerances (stated per result):
+  FP64 path:  forces rel-RMS <= 1e-10 and energies rel <= 1e-11 against the
+              oracle for table pairs and SPME (summation order differs);
+              standard Ewald <= 2e-6, because the reference seeds
+              exp(i k x) in single precision (CMPLX without KIND,
+              Src/energy.F90:1404-1409) while the engine uses FP64 sincos.
+  mixed path: forces rel-RMS <= 1e-5, energies rel <= 1e-6.
+  pair sets:  bit-exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from molsim_amd import engine as E
+from molsim_amd import systems as S
+
+from helpers import efp, oracle_for, owned_pairs, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def golden():
+    here = os.path.join(os.path.dirname(__file__), "golden")
+    return {k: np.load(os.path.join(here, k + ".npz"))
+            for k in ("c1_std", "c1_cell_surf", "chains")}
+
+
+def c1_from_golden(z):
+    s = S.c1()
+    s.ro = z["probe_pos"].copy()
+    return s
+
+
+# ------------------------------------------------------------- pair sets
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_pair_set_matches_reference(golden, prec):
+    z = golden["c1_std"]
+    s = c1_from_golden(z)
+    say("case: pair set c1 golden", prec)
+    eng = S.configure(s, prec)
+    eng.build_list()
+    nn, jl = eng.get_pairs(1024)
+    assert np.array_equal(nn, z["ref_nneighpn"])
+    import hashlib
+    flat = np.concatenate([jl[i, :nn[i]] for i in range(s.np)]).astype("<i4")
+    assert hashlib.sha256(flat.tobytes()).hexdigest() == str(z["ref_jpnlist_sha256"])
+    eng.close()
+
+
+def test_pair_set_bruteforce_c2():
+    s = S.c2()
+    say("case: pair set c2 vs brute force")
+    eng = S.configure(s, "mixed")
+    eng.build_list()
+    nn, jl = eng.get_pairs(2048)
+    nn0, jl0 = O.set_vlist_md(s.boxlen, s.ro, s.rcut + s.drnlist, 2048)
+    assert np.array_equal(nn, nn0)
+    for i in range(s.np):
+        assert np.array_equal(jl[i, :nn[i]], jl0[i, :nn0[i]]), i
+    eng.close()
+
+
+# --------------------------------------------------------- table forces
+@pytest.mark.parametrize("prec,ftol,etol", [("fp64", 1e-12, 1e-12), ("mixed", 1e-5, 1e-6)])
+def test_pair_forces_vs_oracle(prec, ftol, etol):
+    s = S.c1()
+    s.ewald = "none"
+    say("case: table pair forces c1", prec)
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    eng = S.configure(s, prec, tables=tab)
+    f, en = eng.utotal()
+    orc = oracle_for(s, tab, ewald="none")
+    orc.set_list(s.ro)
+    fo, eo = orc.utotal(s.ro)
+    assert rel_rms(f, fo) <= ftol
+    assert abs(en.twob0 - eo["twob0"]) <= etol * abs(eo["twob0"])
+    assert np.allclose(en.twob, eo["twob"], rtol=etol * 10, atol=0)
+    assert abs(en.virial - eo["virial"]) <= etol * abs(eo["virial"])
+    eng.close()
+
+
+@pytest.mark.parametrize("prec,ftol,etol", [("fp64", 2e-6, 1e-7), ("mixed", 1e-5, 1e-6)])
+def test_utotal_std_ewald_vs_reference(golden, prec, ftol, etol):
+    z = golden["c1_std"]
+    s = c1_from_golden(z)
+    say("case: UTotal std Ewald c1 golden", prec)
+    eng = S.configure(s, prec)
+    f, en = eng.utotal()
+    assert rel_rms(f, z["ref_forceo"]) <= ftol
+    assert abs(en.tot - z["ref_u_tot"]) <= etol * abs(z["ref_u_tot"])
+    assert abs(en.rec - z["ref_u_rec"]) <= etol * 10 * abs(z["ref_u_rec"])
+    eng.close()
+
+
+def test_utotal_cell_list_surface_vs_reference(golden):
+    zc = golden["c1_cell_surf"]
+    s = c1_from_golden(golden["c1_std"])
+    s.ro = zc["probe_pos"].copy()
+    s.lsurf = True
+    say("case: UTotal cell list + surface c1 golden")
+    eng = S.configure(s, "fp64", nlist="cell")
+    f, en = eng.utotal()
+    assert rel_rms(f, zc["ref_forceo"]) <= 2e-6
+    assert abs(en.tot - zc["ref_u_tot"]) <= 1e-7 * abs(zc["ref_u_tot"])
+    eng.close()
+
+
+def test_utotal_bonds_vs_reference(golden):
+    z = golden["chains"]
+    s = S.System("chains", 60.0, z["probe_pos"].copy(), np.zeros_like(z["probe_pos"]),
+                 np.repeat([1, 2], z["nppt"]).astype(np.int32), z["zat"], z["sigat"],
+                 z["epsat"], np.array([30.0, 23.0]), rcut=20.0, alpha=float(z["alpha"]),
+                 ewald="std", ncut=5, chains=[(1 + c * 10, 10) for c in range(4)],
+                 bond=(float(z["bond"][0]), 2, float(z["bond"][2])))
+    say("case: UTotal with bonds, golden chains")
+    eng = S.configure(s, "fp64")
+    f, en = eng.utotal()
+    assert abs(en.bond - float(z["ref_u_bond"])) <= 1e-12 * max(abs(float(z["ref_u_bond"])), 1e-3)
+    assert rel_rms(f, z["ref_forceo"]) <= 2e-6
+    assert abs(en.tot - float(z["ref_u_tot"])) <= 1e-7 * abs(float(z["ref_u_tot"]))
+    eng.close()
+
+
+# ----------------------------------------------------------------- SPME
+@pytest.mark.parametrize("prec,ftol,etol", [("fp64", 1e-10, 1e-11), ("mixed", 1e-5, 1e-6)])
+def test_spme_c2_vs_oracle(prec, ftol, etol):
+    s = S.c2()
+    say("case: SPME c2 (N=6000, 64^3, order 4)", prec)
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    eng = S.configure(s, prec, tables=tab)
+    f, en = eng.utotal()
+    orc = oracle_for(s, tab)
+    orc.set_list(s.ro)
+    fo, eo = orc.utotal(s.ro)
+    r = rel_rms(f, fo)
+    say(f"  force rel-RMS {r:.3e}  tot {en.tot:.12e} vs {eo['tot']:.12e}  "
+        f"rec {en.rec:.12e} vs {eo['rec']:.12e}")
+    assert r <= ftol
+    assert abs(en.rec - eo["rec"]) <= etol * abs(eo["rec"])
+    assert abs(en.tot - eo["tot"]) <= etol * abs(eo["tot"])
+    assert abs(en.bond - eo["bond"]) <= 1e-12 * abs(eo["bond"])
+    assert abs(en.virial - eo["virial"]) <= etol * 10 * abs(eo["virial"])
+    eng.close()
+
+
+@pytest.mark.parametrize("mesh,order", [(32, 4), (48, 5), (40, 6), (24, 8), (18, 3), (36, 7)])
+def test_spme_meshes_and_orders(mesh, order):
+    """Mixed-radix meshes (2, 3, 5) and every supported spline order."""
+    s = S.c2()
+    s.mesh, s.order = mesh, order
+    say("case: SPME mesh", mesh, "order", order)
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    eng = S.configure(s, "fp64", tables=tab)
+    f, en = eng.utotal()
+    orc = oracle_for(s, tab)
+    orc.set_list(s.ro)
+    fo, eo = orc.utotal(s.ro)
+    assert rel_rms(f, fo) <= 1e-10
+    assert abs(en.rec - eo["rec"]) <= 1e-11 * abs(eo["rec"])
+    eng.close()
+
+
+# ------------------------------------------------------------------- MD
+def test_velver_vs_reference_trajectory(golden):
+    """Five MD steps from the reference's initial state (rodd = 0)."""
+    z = golden["c1_std"]
+    s = S.c1()
+    s.ro = z["traj_pos"][0].copy()
+    s.rod = z["traj_vel"][0].copy()
+    say("case: VelVer 5 steps vs reference trajectory")
+    eng = S.configure(s, "fp64")
+    eng.build_list()
+    for k in range(1, z["traj_pos"].shape[0]):
+        eng.step_md(1, 0.002)
+        ro, rod, _ = eng.download_state()
+        d = ro - z["traj_pos"][k]
+        d -= 50.0 * np.round(d / 50.0)
+        assert np.abs(d).max() < 1e-9, k
+        assert np.abs(rod - z["traj_vel"][k]).max() < 1e-6 * np.abs(z["traj_vel"][k]).max(), k
+    eng.close()
+
+
+def test_md_rebuilds_match_oracle():
+    """Skin-test decisions (NList, Src/nlist.F90:597-616) over 40 steps."""
+    s = S.c1()
+    s.ewald = "none"
+    say("case: MD rebuild schedule c1, 40 steps")
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    eng = S.configure(s, "fp64", tables=tab)
+    eng.build_list()
+    orc = oracle_for(s, tab, ewald="none")
+    ro = s.ro.reshape(-1).copy()
+    rod = s.rod.reshape(-1).copy()
+    rodd = np.zeros_like(ro)
+    drosum = np.zeros_like(ro)
+    massip = s.massipt[s.iptpn - 1]
+    orc.set_list(ro)
+    nreb = 0
+    for k in range(40):
+        dro = O.velver_move(s.boxlen, 0.002, ro, rod, rodd)
+        f, _ = orc.utotal(ro)
+        O.velver_kick(0.002, massip, f.reshape(-1), rod, rodd)
+        if O.skin_test(drosum, dro, s.drnlist):
+            orc.set_list(ro)
+            nreb += 1
+    eng.step_md(40, 0.002)
+    en = eng.energy()
+    ro_g, _, _ = eng.download_state()
+    d = ro_g.reshape(-1) - ro
+    d -= s.boxlen * np.round(d / s.boxlen)
+    assert en.rebuilds == nreb
+    assert np.abs(d).max() < 1e-8
+    eng.close()
+
+
+# --------------------------------------------------------- error paths
+def test_overlap_is_reported():
+    s = S.c1()
+    s.ewald = "none"
+    s.ro[1] = s.ro[0] + np.array([0.2, 0.0, 0.0])  # r2 = 0.04 < r2umin = 0.1
+    s.ro = s.ro - 50.0 * np.round(s.ro / 50.0)
+    say("case: overlap error")
+    eng = S.configure(s, "fp64")
+    with pytest.raises(E.MfeError) as ei:
+        eng.utotal()
+    assert ei.value.code == 2
+    eng.close()
+
+
+def test_capacity_is_reported():
+    s = S.c1()
+    say("case: neighbour capacity error")
+    eng = S.configure(s, "mixed", maxnneigh=4)
+    with pytest.raises(E.MfeError) as ei:
+        eng.build_list()
+    assert ei.value.code == 4
+    eng.close()
+
+
+def test_odd_mesh_is_rejected():
+    s = S.c2()
+    eng = E.ForceEngine(s.np, "mixed")
+    eng.set_box(s.boxlen)
+    eng.set_particles(s.iptpn, s.az, s.massipt)
+    with pytest.raises(E.MfeError) as ei:
+        eng.set_ewald("spm", s.alpha, 4, (33, 32, 32))
+    assert ei.value.code == 5
+    eng.close()
