@@ -1,0 +1,172 @@
+"""Throughput of the MOLSIM force engine on one MI355X.
+
+Workload (default): 200k ions -- 40k (+1), 40k (+2), 120k (-1) at 0.5 M
+(L = 642.9 A) -- WCA sigma 4 A, SPME 128^3 with order 6, rc = 12 A,
+skin 2 A, velocity Verlet with dt = 2 fs.  A step is one VelVer step of
+MDDriver + NList: positions, UTotal (table pairs, SPME, energy/virial
+reductions), velocities, skin test and, when it fires, the list rebuild.
+
+The mixed-precision path (FP32 pair/mesh arithmetic, FP64 reductions and
+integration) is `value`; the FP64 GPU path is timed the same way and the
+force relative RMS of mixed vs FP64 is reported beside it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from molsim_amd import engine as E  # noqa: E402
+from molsim_amd import systems as S  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+
+
+def algorithmic_bytes(sysm, stage):
+    """Compulsory bytes per launch of each stage in its storage precision
+    (FP32 mixed path): N particles, M = s^3 mesh points, Mc = (s/2+1) s^2
+    complex points; indices and LDS traffic excluded."""
+    n = sysm.np
+    s = sysm.mesh
+    M = s ** 3
+    Mc = (s // 2 + 1) * s * s
+    return {
+        "spread": 16 * n + 4 * M,          # xyz+q in, mesh out
+        "fft_x": 4 * M + 8 * Mc,           # real in, half-complex out
+        "fft_y": 16 * Mc,                  # complex in and out
+        "conv": 16 * Mc,                   # complex in and out (z pair fused)
+        "ifft": 16 * Mc + 8 * Mc + 4 * M,  # y inverse + c2r
+        "gather": 4 * M + 16 * n + 12 * n,  # mesh + xyz,q in, force out
+    }[stage]
+
+
+def cpu_baseline(sysm, nsteps):
+    """The FP64 oracle (serial C restatement of the reference path) on the
+    same system: MD steps with UTotal, timed on one host core."""
+    import oracle as O
+    from molsim_amd import engine as EE
+    efp = EE.epsi0_four_pi() / 78.4
+    tab = EE.build_tables(sysm.zat, sysm.sigat, sysm.epsat, rcut=sysm.rcut,
+                          epsi_four_pi=efp, alpha=sysm.alpha)
+    t = O.Tables(tab.ubuf, tab.iubuflow, tab.nugrid, tab.r2umin, None)
+    orc = O.Oracle(sysm.boxlen, sysm.iptpn, sysm.az, t, sysm.rcut, ewald="spm",
+                   alpha=sysm.alpha, epsi_four_pi=efp, order=sysm.order,
+                   mesh=(sysm.mesh,) * 3, drnlist=sysm.drnlist, maxnneigh=256)
+    ro = sysm.ro.reshape(-1).copy()
+    rod = sysm.rod.reshape(-1).copy()
+    rodd = np.zeros_like(ro)
+    drosum = np.zeros_like(ro)
+    massip = sysm.massipt[sysm.iptpn - 1]
+    orc.set_list(ro)
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        dro = O.velver_move(sysm.boxlen, sysm.tstep, ro, rod, rodd)
+        f, _ = orc.utotal(ro)
+        O.velver_kick(sysm.tstep, massip, f.reshape(-1), rod, rodd)
+        if O.skin_test(drosum, dro, sysm.drnlist):
+            orc.set_list(ro)
+    dt = time.perf_counter() - t0
+    return nsteps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--units", type=int, default=40000, help="salt units (40000 -> 200k ions)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it",
+              file=sys.stderr)
+        sys.exit(2)
+
+    sysm = S.c3(n_units=args.units)
+    flags = E.FORCES | E.ENERGY | E.VIRIAL
+
+    def run(prec):
+        eng = S.configure(sysm, prec)
+        eng.build_list()
+        f0, e0 = eng.utotal()
+        eng.step_md(args.warmup, sysm.tstep, flags)
+        eng.energy()  # synchronises
+        t0 = time.perf_counter()
+        eng.step_md(args.steps, sysm.tstep, flags)
+        en = eng.energy()
+        dt = time.perf_counter() - t0
+        stages = {}
+        if prec == "mixed":
+            for st in ("pair", "spread", "fft_x", "fft_y", "conv", "ifft", "gather", "utotal"):
+                stages[st] = eng.time_stage(st, 20)
+        eng.close()
+        return dict(sps=args.steps / dt, ms=1000 * dt / args.steps, f0=f0, e0=e0,
+                    en=en, stages=stages)
+
+    mixed = run("mixed")
+    fp64 = None if args.no_fp64 else run("fp64")
+
+    # roofline of the dominant SPME kernel (stage times from HIP events on
+    # the engine's stream)
+    spme = {k: v for k, v in mixed["stages"].items() if k not in ("pair", "utotal")}
+    dom = max(spme, key=spme.get)
+    nbytes = algorithmic_bytes(sysm, dom)
+    achieved = nbytes / (spme[dom] * 1e-3) / 1e9
+    stage_report = {k: {"ms": round(v, 5),
+                        "GBs": (round(algorithmic_bytes(sysm, k) / (v * 1e-3) / 1e9, 1)
+                                if k in spme else None)}
+                    for k, v in mixed["stages"].items()}
+
+    out = {
+        "metric": "steps/s at 200k-ion SPME: mixed-precision vs FP64 GPU, force rel-RMS <=1e-5",
+        "value": round(mixed["sps"], 2),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 reductions and integration)",
+        "data": "synthetic (seeded electrolyte, molsim_amd.systems.c3)",
+        "config": {"workload": f"c3: {sysm.np} ions (+1/+2/-1), L={sysm.boxlen:.1f} A, "
+                               f"SPME {sysm.mesh}^3 order {sysm.order}, rc {sysm.rcut} A, "
+                               f"skin {sysm.drnlist} A, VelVer dt {sysm.tstep} ps",
+                   "n_particles": int(sysm.np), "mesh": sysm.mesh, "order": sysm.order},
+        "ns_per_day": round(mixed["sps"] * sysm.tstep * 1e-3 * 86400, 3),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "algorithmic_bytes": nbytes, "traffic": None},
+        "stages": stage_report,
+    }
+    if fp64 is not None:
+        f_m, f_d = mixed["f0"], fp64["f0"]
+        rel = float(np.sqrt(((f_m - f_d) ** 2).sum() / (f_d ** 2).sum()))
+        out["fp64_steps_per_s"] = round(fp64["sps"], 2)
+        out["mixed_over_fp64"] = round(mixed["sps"] / fp64["sps"], 3)
+        out["force_rel_rms_mixed_vs_fp64"] = rel
+        out["energy_rel_mixed_vs_fp64"] = abs(mixed["e0"].tot - fp64["e0"].tot) / abs(fp64["e0"].tot)
+    if not args.no_cpu_baseline:
+        sps, secs = cpu_baseline(sysm, args.cpu_steps)
+        out["cpu_baseline"] = {"value": round(sps, 4), "unit": "steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{args.cpu_steps} MD steps of the same 200k-ion "
+                                         f"system with the FP64 oracle ({secs:.1f} s)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

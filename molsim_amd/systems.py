@@ -115,9 +115,18 @@ def _chains(rng, nchain, nper, bondlen, L, dmin):
     from scipy.spatial import cKDTree
     placed = np.zeros((0, 3))
     out = []
+    def clear(p, others):
+        if others.shape[0] == 0:
+            return True
+        d = _wrap(others - p, L)
+        return bool((np.einsum("ij,ij->i", d, d) >= dmin * dmin).all())
+
     for _ in range(nchain):
         for _try in range(100):
-            chain = [rng.uniform(-L / 2, L / 2, 3)]
+            first = rng.uniform(-L / 2, L / 2, 3)
+            while not clear(first, placed):
+                first = rng.uniform(-L / 2, L / 2, 3)
+            chain = [first]
             ok = True
             for _s in range(1, nper):
                 for _t in range(200):

