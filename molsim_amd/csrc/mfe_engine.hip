@@ -132,6 +132,7 @@ void free_all(mfe_engine *e) {
   dfree(e->t_nint); dfree(e->t_pairidx); dfree(e->t_r2atat); dfree(e->t_r2umin);
   dfree(e->c_count); dfree(e->c_start); dfree(e->c_of); dfree(e->c_slot);
   dfree(e->c_sorted); dfree(e->nn); dfree(e->nbr); dfree(e->flags);
+  dfree(e->ro_s); dfree(e->type_s);
   dfree(e->mesh_r); dfree(e->mesh_c); dfree(e->b_count); dfree(e->b_start);
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
   dfree(e->ax_g); dfree(e->ax_kk);
@@ -148,14 +149,14 @@ void free_all(mfe_engine *e) {
 // partial-sum regions of the reductions (see k_reduce)
 void layout_partials(mfe_engine *e) {
   const int nb = (e->np + 255) / 256;
-  e->nblk_pair = nb;
+  e->nblk_pair = (e->np * mfe::kPairLanes + 255) / 256;
   e->nblk_bond = nb;
   e->nblk_surf = nb;
   e->nblk_move = nb;
   int conv = 0;
   if (e->ewald == MFE_EWALD_SPME && e->hx > 0) conv = mfe::conv_blocks(e);
   int off = 0;
-  e->off_pair = off; off += nb * (e->nptpt + 1);
+  e->off_pair = off; off += e->nblk_pair * (e->nptpt + 1);
   e->off_conv = off; off += 2 * conv;
   e->off_bond = off; off += 2 * nb;
   e->off_surf = off; off += 3 * nb;
@@ -231,22 +232,23 @@ int read_flags(mfe_engine *e) {
   return MFE_OK;
 }
 
-// UTotal's device sequence; mode 0 = forces only, 1 = with VelVer kick
-void launch_utotal(mfe_engine *e, unsigned flags, int mode, double dt) {
+// UTotal's device sequence; mode 0 = forces only, 1 = with VelVer kick;
+// skin: also take the NList skin decision (MD/BD steps)
+void launch_utotal(mfe_engine *e, unsigned flags, int mode, double dt, bool skin = false) {
   const bool energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) != 0;
   mfe::launch_pair(e, energy);
   if (e->ewald == MFE_EWALD_SPME) mfe::launch_spme(e, energy, SP_BIN, SP_GATHER);
   else if (e->ewald == MFE_EWALD_STD) mfe::launch_std_ewald(e);
   if (e->nchain > 0) mfe::launch_bonds(e);
   if (e->lsurf && e->ewald != MFE_EWALD_NONE) mfe::launch_surf(e);
-  mfe::launch_reduce(e, flags);
+  mfe::launch_reduce(e, flags, skin);
   mfe::launch_finalize(e, mode, dt);
 }
 
 void launch_md_step(mfe_engine *e, double dt, unsigned flags) {
   mfe::launch_move(e, dt, 0, nullptr, nullptr, nullptr);
-  launch_utotal(e, flags, 1, dt);
   mfe::launch_skin(e);
+  launch_utotal(e, flags, 1, dt, true);
   mfe::launch_nlist_rebuild(e, true);
 }
 
@@ -356,6 +358,7 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
       dalloc(e->q, (size_t)np); dalloc(e->massip, (size_t)np);
       dalloc(e->c_of, (size_t)np); dalloc(e->c_slot, (size_t)np);
       dalloc(e->c_sorted, (size_t)np); dalloc(e->nn, (size_t)np);
+      dalloc(e->ro_s, 3 * (size_t)np); dalloc(e->type_s, (size_t)np);
       dalloc(e->b_id, (size_t)np); dalloc(e->b_slot, (size_t)np);
       dalloc(e->rec_meta, (size_t)np);
       dfree(e->rec_frac);
@@ -372,6 +375,11 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
     e->h_q.assign(az, az + np);
     e->h_massipt.assign(massipt, massipt + npt);
     upload(e->type, t.data(), (size_t)np, e->stream);
+    {
+      std::vector<int> ident(np);
+      for (int i = 0; i < np; ++i) ident[i] = i;
+      upload(e->c_sorted, ident.data(), (size_t)np, e->stream);
+    }
     upload(e->q, az, (size_t)np, e->stream);
     upload(e->massip, mi.data(), (size_t)np, e->stream);
     mfe::launch_fill_double(e->rod, 3 * np, 0.0, e->stream);
@@ -497,7 +505,11 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
       e->nbricks = e->nb[0] * e->nb[1] * e->nb[2];
       const size_t rsz = e->prec == MFE_PREC_MIXED ? 4 : 8;
       e->spread_lds = (int)(tile * 8);
-      if (e->spread_lds > 96 * 1024) return fail(e, MFE_E_BAD_PARAM, "mesh brick too large");
+      size_t gext = rsz;
+      for (int m = 0; m < 3; ++m) gext *= (size_t)(e->B[m] + order - 1);
+      const int lds_need = (int)std::max<size_t>((size_t)e->spread_lds, gext);
+      if (lds_need > 150 * 1024)
+        return fail(e, MFE_E_BAD_PARAM, "mesh brick too large for LDS (mesh 2q, q prime > 16?)");
       const size_t M = (size_t)e->s[0] * e->s[1] * e->s[2];
       const size_t Mc = (size_t)e->hx * e->s[1] * e->s[2];
       dfree(e->mesh_r);
@@ -543,9 +555,9 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
                                              : make_twiddles<double>(e->s[m], e->stream);
       e->tw_half = e->prec == MFE_PREC_MIXED ? make_twiddles<float>(e->s[0] / 2, e->stream)
                                              : make_twiddles<double>(e->s[0] / 2, e->stream);
-      if (e->spread_lds > 64 * 1024) {
+      if (lds_need > 64 * 1024) {
         // large bricks (meshes 2q with q prime) need the LDS above 64 KB
-        mfe_allow_big_spread_lds(e->prec, e->spread_lds);
+        mfe_allow_big_spread_lds(e->prec, lds_need);
       }
     } else if (kind == MFE_EWALD_STD) {
       if (ncut < 1) return fail(e, MFE_E_BAD_PARAM, "ncut < 1");
@@ -775,8 +787,8 @@ int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1, const double 
     for (int s = 0; s < nsteps && r == MFE_OK; ++s) {
       upload(d_noise, noise + 3 * (size_t)e->np * s, 3 * (size_t)e->np, e->stream);
       mfe::launch_move(e, 0.0, 1, d_f1, d_f2, d_noise);
-      launch_utotal(e, flags, 0, 0.0);
       mfe::launch_skin(e);
+      launch_utotal(e, flags, 0, 0.0, true);
       mfe::launch_nlist_rebuild(e, true);
     }
     fetch_energy(e);
@@ -804,22 +816,26 @@ int mfe_get_pairs(mfe_engine *e, int32_t *nneighpn, int32_t *jpnlist, int32_t ma
   if (!e->list_valid) return fail(e, MFE_E_STATE, "no neighbour list built");
   return guarded(e, [&]() -> int {
     const int np = e->np;
-    std::vector<int> nn(np), nbr((size_t)e->maxnn * np);
+    std::vector<int> nn(np), nbr((size_t)e->maxnn * np), sorted(np);
     MFE_HIP_TRY(hipMemcpyAsync(nn.data(), e->nn, np * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     MFE_HIP_TRY(hipMemcpyAsync(nbr.data(), e->nbr, nbr.size() * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    MFE_HIP_TRY(hipMemcpyAsync(sorted.data(), e->c_sorted, np * sizeof(int), hipMemcpyDeviceToHost, e->stream));
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
-    std::vector<int> row;
-    for (int i = 0; i < np; ++i) {
-      row.clear();
-      const int ip = i + 1;
-      for (int k = 0; k < nn[i]; ++k) {
-        const int jp = nbr[(size_t)k * np + i] + 1;
+    std::vector<std::vector<int>> rows(np);
+    for (int t = 0; t < np; ++t) {
+      const int ip = sorted[t] + 1;
+      auto &row = rows[ip - 1];
+      for (int k = 0; k < nn[t]; ++k) {
+        const int jp = sorted[nbr[(size_t)k * np + t]] + 1;
         const int par = (ip + jp) % 2;
         // checkerboard ownership of SetVListMD (Src/nlist.F90:742-745)
         if (ip < jp && par == 0) continue;
         if (ip > jp && par != 0) continue;
         row.push_back(jp);
       }
+    }
+    for (int i = 0; i < np; ++i) {
+      auto &row = rows[i];
       std::sort(row.begin(), row.end());
       if ((int)row.size() > maxnneigh) return fail(e, MFE_E_NEIGH_CAPACITY, "jpnlist capacity too small");
       nneighpn[i] = (int)row.size();

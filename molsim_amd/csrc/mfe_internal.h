@@ -18,6 +18,9 @@
 
 namespace mfe {
 
+// lanes of the pair kernel that share one particle's neighbour list
+constexpr int kPairLanes = 4;
+
 struct HipError {
   hipError_t code;
   const char *expr;
@@ -129,8 +132,10 @@ struct mfe_engine {
   int maxnn = 0;
   mfe::CellGeom cg{};
   int *c_count = nullptr, *c_start = nullptr, *c_of = nullptr, *c_slot = nullptr;
-  int *c_sorted = nullptr;
-  int *nn = nullptr, *nbr = nullptr;
+  int *c_sorted = nullptr;   // cell order: position t -> particle index
+  double *ro_s = nullptr;    // positions in cell order (refreshed per UTotal)
+  int *type_s = nullptr;
+  int *nn = nullptr, *nbr = nullptr;  // by cell-order position
   int *flags = nullptr;      // [0] error bits, [1] rebuild flag, [2] rebuild count
   int ncells_alloc = 0;
 
@@ -187,7 +192,7 @@ void launch_spme(mfe_engine *e, bool energy, int first_stage, int last_stage);
 void launch_std_ewald(mfe_engine *e);
 void launch_bonds(mfe_engine *e);
 void launch_surf(mfe_engine *e);
-void launch_reduce(mfe_engine *e, unsigned flags);
+void launch_reduce(mfe_engine *e, unsigned flags, bool skin);
 void launch_finalize(mfe_engine *e, int mode, double dt);
 void launch_move(mfe_engine *e, double dt, int bd, const double *fac1,
                  const double *fac2, const double *noise);
@@ -198,6 +203,8 @@ void launch_fill_double(double *p, int n, double v, hipStream_t s);
 int conv_blocks(mfe_engine *e);
 template <typename R, int O>
 const void *k_spread_ptr();
+template <typename R, int O>
+const void *k_gather_ptr();
 
 // FFT passes (mfe_fft.hip)
 void fft_x_forward(mfe_engine *e);

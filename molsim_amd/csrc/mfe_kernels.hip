@@ -12,6 +12,7 @@
 // per block into partial sums that one block adds up in a fixed order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "mfe_internal.h"
@@ -117,28 +118,26 @@ __global__ void k_cell_count(const double *__restrict__ ro, int np, CellGeom g,
                              Box b, int *count, int *cof, int *slot,
                              const int *gate) {
   if (gate && gate[1] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  int c3[3];
-  const int c = cell_of(ro + 3 * i, g, b, c3);
-  cof[i] = c;
-  slot[i] = atomicAdd(&count[c], 1);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+    int c3[3];
+    const int c = cell_of(ro + 3 * i, g, b, c3);
+    cof[i] = c;
+    slot[i] = atomicAdd(&count[c], 1);
+  }
 }
 
 __global__ void k_cell_scatter(int np, const int *cof, const int *slot,
                                const int *start, int *sorted, const int *gate) {
   if (gate && gate[1] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  sorted[start[cof[i]] + slot[i]] = i;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x)
+    sorted[start[cof[i]] + slot[i]] = i;
 }
 
 // members of each cell in ascending particle index (deterministic lists)
 __global__ void k_cell_sort(int ncells, const int *start, int *sorted,
                             const int *gate) {
   if (gate && gate[1] == 0) return;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncells) return;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
   const int lo = start[c], hi = start[c + 1];
   for (int i = lo + 1; i < hi; ++i) {
     int v = sorted[i], j = i - 1;
@@ -148,54 +147,72 @@ __global__ void k_cell_sort(int ncells, const int *start, int *sorted,
     }
     sorted[j + 1] = v;
   }
+  }
 }
 
 // Full Verlet list: every j != i with r2 < (rcut+drnlist)^2 (SetVListMD's
-// test, Src/nlist.F90:746-750); stored column-major nbr[k*np + i].  The
-// half list the reference keeps is the checkerboard subset of this set.
+// test, Src/nlist.F90:746-750).  Threads take particles in cell order
+// (t -> i = sorted[t]) and store partners by their cell-order position, so
+// the pair kernel reads neighbouring positions from neighbouring memory;
+// the list is column-major, nbr[k*np + t].  The half list the reference
+// keeps is the checkerboard subset of this set.
 __global__ void __launch_bounds__(kBlock) k_nlist_build(
     const double *__restrict__ ro, int np, CellGeom g, Box b,
     const int *__restrict__ cof, const int *__restrict__ start,
     const int *__restrict__ sorted, double rs2, int maxnn, int *nn,
     int *__restrict__ nbr, int *flags, double *drosum, const int *gate) {
   if (gate && gate[1] == 0) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
-  const int c = cof[i];
-  const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
-  const int rx = g.n[0] > 1 ? 1 : 0, ry = g.n[1] > 1 ? 1 : 0, rz = g.n[2] > 1 ? 1 : 0;
-  int n = 0;
-  for (int dz = -rz; dz <= rz; ++dz) {
-    const int z = (cz + dz + g.n[2]) % g.n[2];
-    for (int dy = -ry; dy <= ry; ++dy) {
-      const int y = (cy + dy + g.n[1]) % g.n[1];
-      for (int dx = -rx; dx <= rx; ++dx) {
-        const int x = (cx + dx + g.n[0]) % g.n[0];
-        const int cc = x + g.n[0] * (y + g.n[1] * z);
-        for (int p = start[cc]; p < start[cc + 1]; ++p) {
-          const int j = sorted[p];
-          if (j == i) continue;
-          double ddx = xi - ro[3 * j], ddy = yi - ro[3 * j + 1], ddz = zi - ro[3 * j + 2];
-          const double r2 = pbcr2(ddx, ddy, ddz, b);
-          if (r2 < rs2) {
-            if (n < maxnn) nbr[(size_t)n * np + i] = j;
-            ++n;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < np; t += gridDim.x * blockDim.x) {
+    const int i = sorted[t];
+    const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
+    const int c = cof[i];
+    const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
+    const int rx = g.n[0] > 1 ? 1 : 0, ry = g.n[1] > 1 ? 1 : 0, rz = g.n[2] > 1 ? 1 : 0;
+    int n = 0;
+    for (int dz = -rz; dz <= rz; ++dz) {
+      const int z = (cz + dz + g.n[2]) % g.n[2];
+      for (int dy = -ry; dy <= ry; ++dy) {
+        const int y = (cy + dy + g.n[1]) % g.n[1];
+        for (int dx = -rx; dx <= rx; ++dx) {
+          const int x = (cx + dx + g.n[0]) % g.n[0];
+          const int cc = x + g.n[0] * (y + g.n[1] * z);
+          for (int p = start[cc]; p < start[cc + 1]; ++p) {
+            if (p == t) continue;
+            const int j = sorted[p];
+            double ddx = xi - ro[3 * j], ddy = yi - ro[3 * j + 1], ddz = zi - ro[3 * j + 2];
+            const double r2 = pbcr2(ddx, ddy, ddz, b);
+            if (r2 < rs2) {
+              if (n < maxnn) nbr[(size_t)n * np + t] = p;
+              ++n;
+            }
           }
         }
       }
     }
+    // on overflow only the stored maxnn partners are visible to the pair
+    // kernel; the call reports MFE_E_NEIGH_CAPACITY
+    nn[t] = min(n, maxnn);
+    if (n > maxnn) atomicOr(&flags[0], ERR_CAPACITY);
+    if (drosum) {
+      drosum[3 * i] = 0.0;
+      drosum[3 * i + 1] = 0.0;
+      drosum[3 * i + 2] = 0.0;
+    }
   }
-  // on overflow only the stored maxnn partners are visible to the pair
-  // kernel; the call reports MFE_E_NEIGH_CAPACITY
-  nn[i] = min(n, maxnn);
-  if (n > maxnn) atomicOr(&flags[0], ERR_CAPACITY);
-  if (drosum) {
-    drosum[3 * i] = 0.0;
-    drosum[3 * i + 1] = 0.0;
-    drosum[3 * i + 2] = 0.0;
-  }
-  if (i == 0 && gate) flags[2] += 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && gate) flags[2] += 1;
+}
+
+// positions and types in cell order for the pair kernel (every UTotal)
+__global__ void k_sort_pos(const double *__restrict__ ro, const int *__restrict__ type,
+                           const int *__restrict__ sorted, int np,
+                           double *__restrict__ ro_s, int *__restrict__ type_s) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= np) return;
+  const int i = sorted[t];
+  ro_s[3 * t] = ro[3 * i];
+  ro_s[3 * t + 1] = ro[3 * i + 1];
+  ro_s[3 * t + 2] = ro[3 * i + 2];
+  type_s[t] = type[i];
 }
 
 // ------------------------------------------------------------ pair forces
@@ -223,31 +240,35 @@ struct KahanF {
   __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// Each thread owns particle i and walks its full list, so forces are
-// accumulated without atomics; every pair is therefore seen twice and its
-// energy and virial are counted with weight 1/2.  semantics: 0 = vlist
-// (r2 <= rcut2 kept), 1 = cell list (r2 < rcut2 kept).
+// Four lanes share one particle (cell order t) and split its full list
+// k = lane, lane+4, ...; their partial forces are combined by shuffles in
+// FP64 in a fixed order, so forces need no atomics and stay
+// deterministic.  Every pair is seen from both ends, so its energy and
+// virial carry weight 1/2.  semantics: 0 = vlist (r2 <= rcut2 kept),
+// 1 = cell list (r2 < rcut2 kept).
 template <bool MIXED>
 __global__ void __launch_bounds__(kBlock) k_pair(
     const double *__restrict__ ro, const int *__restrict__ type, int np,
-    const int *__restrict__ nn, const int *__restrict__ nbr, Box b,
-    TableDev t, int semantics, int energy, double *__restrict__ force,
-    double *__restrict__ part, int *flags) {
+    const int *__restrict__ nn, const int *__restrict__ nbr,
+    const int *__restrict__ sorted, Box b, TableDev t, int semantics,
+    int energy, double *__restrict__ force, double *__restrict__ part,
+    int *flags) {
   __shared__ double sh[kBlock];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kPairLanes, lane = gid % kPairLanes;
   // energy per partner type (static indexing keeps it in registers)
   double ej[MFE_MAX_TYPES];
 #pragma unroll
   for (int k = 0; k < MFE_MAX_TYPES; ++k) ej[k] = 0.0;
   double vir = 0.0;
   int ti = 0;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
   if (i < np) {
     const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
     ti = type[i];
     KahanF fxf, fyf, fzf;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
     const int n = nn[i];
-    for (int k = 0; k < n; ++k) {
+    for (int k = lane; k < n; k += kPairLanes) {
       const int j = nbr[(size_t)k * np + i];
       double dx = xi - ro[3 * j], dy = yi - ro[3 * j + 1], dz = zi - ro[3 * j + 2];
       const double r2 = pbcr2(dx, dy, dz, b);
@@ -298,7 +319,7 @@ __global__ void __launch_bounds__(kBlock) k_pair(
       }
       if (energy) {
 #pragma unroll
-        for (int k = 0; k < MFE_MAX_TYPES; ++k) ej[k] += (k == tj) ? 0.5 * usum : 0.0;
+        for (int k2 = 0; k2 < MFE_MAX_TYPES; ++k2) ej[k2] += (k2 == tj) ? 0.5 * usum : 0.0;
         vir -= 0.5 * fsum * r2;
       }
     }
@@ -307,9 +328,18 @@ __global__ void __launch_bounds__(kBlock) k_pair(
       fy = fyf.value();
       fz = fzf.value();
     }
-    force[3 * i] = fx;
-    force[3 * i + 1] = fy;
-    force[3 * i + 2] = fz;
+  }
+#pragma unroll
+  for (int w = 1; w < kPairLanes; w <<= 1) {
+    fx += __shfl_xor(fx, w, kPairLanes);
+    fy += __shfl_xor(fy, w, kPairLanes);
+    fz += __shfl_xor(fz, w, kPairLanes);
+  }
+  if (i < np && lane == 0) {
+    const int io = sorted[i];
+    force[3 * io] = fx;
+    force[3 * io + 1] = fy;
+    force[3 * io + 2] = fz;
   }
   if (energy) {
     double *out = part + (size_t)blockIdx.x * (t.nptpt + 1);
@@ -320,11 +350,11 @@ __global__ void __launch_bounds__(kBlock) k_pair(
         for (int u = 0; u < MFE_MAX_TYPES; ++u)
           if (u < t.npt && t.pairidx[ti * t.npt + u] == k) v += ej[u];
       }
-      double s = block_sum(v, sh);
-      if (threadIdx.x == 0) out[k] = s;
+      double s2 = block_sum(v, sh);
+      if (threadIdx.x == 0) out[k] = s2;
     }
-    double s = block_sum(vir, sh);
-    if (threadIdx.x == 0) out[t.nptpt] = s;
+    double s2 = block_sum(vir, sh);
+    if (threadIdx.x == 0) out[t.nptpt] = s2;
   }
 }
 
@@ -344,13 +374,16 @@ __device__ __forceinline__ void bspline(R x, R *v, R *d) {
   for (int n = 3; n <= ORDER; ++n) {
 #pragma unroll
     for (int i = 0; i < n - 1; ++i) last[i] = v[i];
-    v[0] = x * last[0] / R(n - 1);
+    // 1/(n-1) folded at compile time (the reference divides; the results
+    // differ by at most an ulp)
+    const R inv = R(1.0 / (n - 1));
+    v[0] = x * last[0] * inv;
 #pragma unroll
     for (int i = 1; i <= n - 2; ++i) {
       const R a = R(i) + x;
-      v[i] = (a * last[i] + (R(n) - a) * last[i - 1]) / R(n - 1);
+      v[i] = (a * last[i] + (R(n) - a) * last[i - 1]) * inv;
     }
-    v[n - 1] = (R(1) - x) * last[n - 2] / R(n - 1);
+    v[n - 1] = (R(1) - x) * last[n - 2] * inv;
   }
   if (d) {
     d[0] = last[0];
@@ -376,16 +409,40 @@ __device__ __forceinline__ void mesh_coords(const double *r, const SpmeGeom &g,
   }
 }
 
-__global__ void k_spme_count(const double *__restrict__ ro, int np, SpmeGeom g,
+// Brick of every particle, counted with one global atomic per distinct
+// brick of a wavefront: lanes walk the cell-ordered particles, so a
+// wavefront's particles share few bricks (atomics across XCDs are the
+// expensive part of binning).
+__global__ void k_spme_count(const double *__restrict__ ro,
+                             const int *__restrict__ sorted, int np, SpmeGeom g,
                              int *bcount, int *bid, int *bslot) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
-  int mm[3];
-  double fr[3];
-  mesh_coords(ro + 3 * i, g, mm, fr);
-  const int id = mm[0] / g.B[0] + g.nb[0] * (mm[1] / g.B[1] + g.nb[1] * (mm[2] / g.B[2]));
-  bid[i] = id;
-  bslot[i] = atomicAdd(&bcount[id], 1);
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = t < np;
+  int i = 0, id = -1;
+  if (act) {
+    i = sorted[t];
+    int mm[3];
+    double fr[3];
+    mesh_coords(ro + 3 * i, g, mm, fr);
+    id = mm[0] / g.B[0] + g.nb[0] * (mm[1] / g.B[1] + g.nb[1] * (mm[2] / g.B[2]));
+  }
+  const int lane = __lane_id();
+  unsigned long long pending = __ballot(act);
+  int slot = 0;
+  while (pending) {
+    const int leader = __ffsll((long long)pending) - 1;
+    const int lid = __shfl(id, leader);
+    const unsigned long long same = __ballot(act && id == lid) & pending;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&bcount[lid], __popcll(same));
+    base = __shfl(base, leader);
+    if ((same >> lane) & 1ull) slot = base + __popcll(same & ((1ull << lane) - 1ull));
+    pending &= ~same;
+  }
+  if (act) {
+    bid[i] = id;
+    bslot[i] = slot;
+  }
 }
 
 // particle records in brick order: fraction (x, y, z) and charge, plus the
@@ -434,30 +491,53 @@ constexpr int kMaxOrder = 8;
 template <typename R> struct Chunk { static constexpr int n = 256; };
 template <> struct Chunk<double> { static constexpr int n = 128; };
 
-// One block owns one brick of B0 x B1 x B2 mesh points and is the only
-// writer of it.  A particle with meshmax m adds to points m, m-1, ...,
-// m-order+1 (Src/energy.F90:1232-1244), so a brick receives from the
-// particles binned in itself and in the next brick along each axis
-// (B >= order-1).  Work items are (particle, iz, iy) rows of order adds.
+// Stencil indices i whose point m - i falls inside a brick [o, o+e) of an
+// axis: with l = (m - o) mod s, i runs over [max(0, l-e+1), min(order-1, l)]
+// (point m - i sits at brick-local l - i).  An axis with one brick holds
+// the whole periodic stencil: every i, local (l - i) mod s.
+__device__ __forceinline__ void stencil_range(int m, int o, int e, int s, int nb,
+                                              int order, int &l, int &lo,
+                                              int &cnt) {
+  if (nb == 1) {
+    l = m;
+    lo = 0;
+    cnt = order;
+    return;
+  }
+  l = m - o;
+  if (l < 0) l += s;
+  const int a = max(0, l - e + 1), b = min(order - 1, l);
+  lo = a;
+  cnt = max(0, b - a + 1);
+}
+
+// One block owns one brick of B0 x B1 x B2 mesh points and is its only
+// writer.  A particle with meshmax m adds to points m, m-1, ..., m-order+1
+// (Src/energy.F90:1232-1244), so the brick receives from particles binned
+// in itself and in the next brick along each axis (B >= order-1).  For
+// each candidate the block works out the exact stencil planes, rows and
+// columns inside the brick; the planes of all candidates are numbered by a
+// prefix sum and handed out to threads (a plane of rows each), so every
+// issued add lands.
 template <typename R, int ORDER>
 __global__ void __launch_bounds__(kBlock) k_spread(
     const R *__restrict__ frac, const int2 *__restrict__ meta,
     const int *__restrict__ bstart, SpmeGeom g, R *__restrict__ mesh) {
   typedef typename SpreadAcc<R>::T Acc;
   constexpr int kChunk = Chunk<R>::n;
-  constexpr int order = ORDER;
   extern __shared__ char smem[];
   Acc *tile = reinterpret_cast<Acc *>(smem);
   __shared__ R w[kChunk][3][ORDER];
-  __shared__ int mmx[kChunk][3];
+  __shared__ int rng[kChunk][3];   // l | lo << 16 | cnt << 24 per axis
   __shared__ R qs[kChunk];
-  __shared__ int seg_lo[8], seg_len[8], seg_acc[9];
+  __shared__ int pre[kChunk + 1];
+  __shared__ int seg_lo[8], seg_acc[9];
   const int bx = blockIdx.x % g.nb[0];
   const int by = (blockIdx.x / g.nb[0]) % g.nb[1];
   const int bz = blockIdx.x / (g.nb[0] * g.nb[1]);
-  const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
-  const int e0 = min(g.B[0], g.s[0] - o0), e1 = min(g.B[1], g.s[1] - o1),
-            e2 = min(g.B[2], g.s[2] - o2);
+  const int o[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
+  const int e[3] = {min(g.B[0], g.s[0] - o[0]), min(g.B[1], g.s[1] - o[1]),
+                    min(g.B[2], g.s[2] - o[2])};
   const int ntile = g.B[0] * g.B[1] * g.B[2];
   for (int k = threadIdx.x; k < ntile; k += blockDim.x) tile[k] = Acc(0);
   if (threadIdx.x == 0) {
@@ -469,71 +549,95 @@ __global__ void __launch_bounds__(kBlock) k_spread(
           const int cb = (bx + dx) % g.nb[0] +
                          g.nb[0] * ((by + dy) % g.nb[1] + g.nb[1] * ((bz + dz) % g.nb[2]));
           seg_lo[ns] = bstart[cb];
-          seg_len[ns] = bstart[cb + 1] - bstart[cb];
           seg_acc[ns] = acc;
-          acc += seg_len[ns];
+          acc += bstart[cb + 1] - bstart[cb];
           ++ns;
         }
-    for (int k = ns; k < 8; ++k) {
-      seg_len[k] = 0;
-      seg_acc[k] = acc;
-    }
-    seg_acc[8] = acc;
+    for (int k = ns; k < 9; ++k) seg_acc[k] = acc;
   }
   __syncthreads();
   const int ncand = seg_acc[8];
   for (int c0 = 0; c0 < ncand; c0 += kChunk) {
     const int nc = min(kChunk, ncand - c0);
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
-      const int gc = c0 + c;
-      int sgi = 0;
-      while (gc >= seg_acc[sgi + 1]) ++sgi;
-      const int rec = seg_lo[sgi] + (gc - seg_acc[sgi]);
-      const int2 md = meta[rec];
-      mmx[c][0] = md.x & 1023;
-      mmx[c][1] = (md.x >> 10) & 1023;
-      mmx[c][2] = (md.x >> 20) & 1023;
-      qs[c] = frac[4 * rec + 3];
-      for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * rec + m], w[c][m], nullptr);
+    int rows = 0;
+    for (int c = threadIdx.x; c < kChunk; c += blockDim.x) {
+      if (c < nc) {
+        const int gc = c0 + c;
+        int sgi = 0;
+        while (gc >= seg_acc[sgi + 1]) ++sgi;
+        const int rec = seg_lo[sgi] + (gc - seg_acc[sgi]);
+        const int2 md = meta[rec];
+        const int mm[3] = {md.x & 1023, (md.x >> 10) & 1023, (md.x >> 20) & 1023};
+        int cnt[3];
+        for (int m = 0; m < 3; ++m) {
+          int l, lo;
+          stencil_range(mm[m], o[m], e[m], g.s[m], g.nb[m], ORDER, l, lo, cnt[m]);
+          rng[c][m] = l | (lo << 16) | (cnt[m] << 24);
+        }
+        qs[c] = frac[4 * rec + 3];
+        for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * rec + m], w[c][m], nullptr);
+        rows = (cnt[0] > 0 && cnt[1] > 0) ? cnt[2] : 0;  // planes inside the brick
+      }
+      pre[c + 1] = rows;
     }
+    if (threadIdx.x == 0) pre[0] = 0;
     __syncthreads();
-    const int items = nc * order * order;
-    for (int it = threadIdx.x; it < items; it += blockDim.x) {
-      const int c = it / (order * order);
-      const int rem = it - c * order * order;
-      const int iz = rem / order, iy = rem - iz * order;
-      int z = mmx[c][2] - iz;
-      if (z < 0) z += g.s[2];
-      z -= o2;
-      if (z < 0 || z >= e2) continue;
-      int y = mmx[c][1] - iy;
-      if (y < 0) y += g.s[1];
-      y -= o1;
-      if (y < 0 || y >= e1) continue;
-      const R qyz = qs[c] * w[c][2][iz] * w[c][1][iy];
-      Acc *row = tile + (z * g.B[1] + y) * g.B[0];
-      const int mx = mmx[c][0];
+    // inclusive scan of pre[1..kChunk]
+    for (int off = 1; off < kChunk; off <<= 1) {
+      int v = 0;
+      const int k = threadIdx.x + 1;
+      if (k <= kChunk && k > off) v = pre[k - off];
+      __syncthreads();
+      if (k <= kChunk) pre[k] += v;
+      __syncthreads();
+    }
+    const int total = pre[nc];
+    for (int r = threadIdx.x; r < total; r += blockDim.x) {
+      int lo = 0, hi = nc - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= r) lo = mid; else hi = mid - 1;
+      }
+      const int c = lo;
+      const int rx = rng[c][0], ry = rng[c][1], rz = rng[c][2];
+      const int iz = ((rz >> 16) & 255) + (r - pre[c]);
+      int zl = (rz & 0xffff) - iz;
+      if (zl < 0) zl += g.s[2];
+      const R qz = qs[c] * w[c][2][iz];
+      const int ly = ry & 0xffff, y0 = (ry >> 16) & 255, cy = (ry >> 24) & 255;
+      const int lx = rx & 0xffff, x0 = (rx >> 16) & 255, cx = (rx >> 24) & 255;
+      R wx[ORDER];
 #pragma unroll
-      for (int ix = 0; ix < order; ++ix) {
-        int x = mx - ix;
-        if (x < 0) x += g.s[0];
-        x -= o0;
-        if (x < 0 || x >= e0) continue;
-        SpreadAcc<R>::add(&row[x], qyz * w[c][0][ix]);
+      for (int j = 0; j < ORDER; ++j) wx[j] = (j < cx) ? w[c][0][x0 + j] : R(0);
+#pragma unroll
+      for (int jy = 0; jy < ORDER; ++jy) {
+        if (jy < cy) {
+          const int iy = y0 + jy;
+          int yl = ly - iy;
+          if (yl < 0) yl += g.s[1];
+          const R qyz = qz * w[c][1][iy];
+          Acc *row = tile + (zl * g.B[1] + yl) * g.B[0];
+#pragma unroll
+          for (int j = 0; j < ORDER; ++j) {
+            if (j < cx) {
+              int xl = lx - (x0 + j);
+              if (xl < 0) xl += g.s[0];
+              SpreadAcc<R>::add(&row[xl], qyz * wx[j]);
+            }
+          }
+        }
       }
     }
     __syncthreads();
   }
-  for (int k = threadIdx.x; k < e0 * e1 * e2; k += blockDim.x) {
-    const int x = k % e0, y = (k / e0) % e1, z = k / (e0 * e1);
-    mesh[((size_t)(o2 + z) * g.s[1] + (o1 + y)) * g.s[0] + (o0 + x)] =
+  for (int k = threadIdx.x; k < e[0] * e[1] * e[2]; k += blockDim.x) {
+    const int x = k % e[0], y = (k / e[0]) % e[1], z = k / (e[0] * e[1]);
+    mesh[((size_t)(o[2] + z) * g.s[1] + (o[1] + y)) * g.s[0] + (o[0] + x)] =
         SpreadAcc<R>::get(tile[(z * g.B[1] + y) * g.B[0] + x]);
   }
 }
 
-// CalcProp (Src/energy.F90:1259-1285): esum over the order^3 stencil of the
-// convolved mesh; force -= q * esum * dkdr * EpsiFourPi.  Threads walk the
-// brick-ordered records, so neighbouring threads read neighbouring mesh.
+// per-lane stencil sums: compensated FP32 on the mixed path, plain FP64
 template <typename R>
 struct Acc3;
 template <>
@@ -549,52 +653,102 @@ struct Acc3<double> {
   __device__ __forceinline__ double get(int m) const { return a[m]; }
 };
 
+// CalcProp (Src/energy.F90:1259-1285): esum over the order^3 stencil of the
+// convolved mesh; force -= q * esum * dkdr * EpsiFourPi.  One block per
+// brick stages the mesh the brick's particles touch -- points
+// [o - (order-1), o + B) per axis -- in LDS; each particle is taken by 8
+// lanes, lane iz < order summing stencil plane iz (FP32 with compensated
+// row sums on the mixed path), and the planes are combined by shuffles in
+// FP64 in a fixed order.
 template <typename R, int ORDER>
 __global__ void __launch_bounds__(kBlock) k_gather(
-    const R *__restrict__ frac, const int2 *__restrict__ meta, int np,
-    SpmeGeom g, const R *__restrict__ mesh, double efp,
-    double *__restrict__ force) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  constexpr int order = ORDER;
-  R v[3][ORDER], d[3][ORDER];
-#pragma unroll
-  for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * t + m], v[m], d[m]);
-  const R q = frac[4 * t + 3];
-  const int2 md = meta[t];
-  const int mx = md.x & 1023, my = (md.x >> 10) & 1023, mz = (md.x >> 20) & 1023;
-  Acc3<R> acc;
-#pragma unroll
-  for (int iz = 0; iz < order; ++iz) {
-    int z = mz - iz;
+    const R *__restrict__ frac, const int2 *__restrict__ meta,
+    const int *__restrict__ bstart, SpmeGeom g, const R *__restrict__ mesh,
+    double efp, double *__restrict__ force) {
+  extern __shared__ char smem[];
+  R *tile = reinterpret_cast<R *>(smem);
+  const int bx = blockIdx.x % g.nb[0];
+  const int by = (blockIdx.x / g.nb[0]) % g.nb[1];
+  const int bz = blockIdx.x / (g.nb[0] * g.nb[1]);
+  const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
+  const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
+  for (int k = threadIdx.x; k < E0 * E1 * E2; k += blockDim.x) {
+    const int k01 = k / E0;
+    const int xe = k - k01 * E0, ye = k01 % E1, ze = k01 / E1;
+    // region coordinates lie in [-(order-1), s + B): one wrap at most,
+    // or (one-brick axis, B = s) possibly one more below zero
+    int x = o0 - (ORDER - 1) + xe, y = o1 - (ORDER - 1) + ye, z = o2 - (ORDER - 1) + ze;
+    if (x < 0) x += g.s[0];
+    if (x < 0) x += g.s[0];
+    if (x >= g.s[0]) x -= g.s[0];
+    if (y < 0) y += g.s[1];
+    if (y < 0) y += g.s[1];
+    if (y >= g.s[1]) y -= g.s[1];
     if (z < 0) z += g.s[2];
+    if (z < 0) z += g.s[2];
+    if (z >= g.s[2]) z -= g.s[2];
+    tile[k] = mesh[((size_t)z * g.s[1] + y) * g.s[0] + x];
+  }
+  __syncthreads();
+  const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
+  const int span = ((8 * n + blockDim.x - 1) / blockDim.x) * blockDim.x;
+  for (int it = threadIdx.x; it < span; it += blockDim.x) {
+    const int p = it >> 3, iz = it & 7;
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    int idx = -1;
+    R q = R(0);
+    if (p < n && iz < ORDER) {
+      const int t = first + p;
+      R v[3][ORDER], d[3][ORDER];
 #pragma unroll
-    for (int iy = 0; iy < order; ++iy) {
-      int y = my - iy;
-      if (y < 0) y += g.s[1];
-      const R *row = mesh + ((size_t)z * g.s[1] + y) * g.s[0];
-      const R vyz = v[1][iy] * v[2][iz];
-      const R dyz = d[1][iy] * v[2][iz];
-      const R vdz = v[1][iy] * d[2][iz];
-      R sx = R(0), sdx = R(0);
+      for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * t + m], v[m], d[m]);
+      q = frac[4 * t + 3];
+      const int2 md = meta[t];
+      idx = md.y;
+      // brick-local coordinates inside the staged region
+      const int lx = (md.x & 1023) - o0 + ORDER - 1;
+      const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
+      const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
+      R vz = R(0), dz = R(0);
 #pragma unroll
-      for (int ix = 0; ix < order; ++ix) {
-        int x = mx - ix;
-        if (x < 0) x += g.s[0];
-        const R qv = row[x];
-        sx += v[0][ix] * qv;
-        sdx += d[0][ix] * qv;
+      for (int k = 0; k < ORDER; ++k)
+        if (k == iz) {
+          vz = v[2][k];
+          dz = d[2][k];
+        }
+      const R *plane = tile + (size_t)(lz - iz) * E1 * E0;
+      Acc3<R> acc;
+#pragma unroll
+      for (int iy = 0; iy < ORDER; ++iy) {
+        const R *row = plane + (ly - iy) * E0;
+        R sx = R(0), sdx = R(0);
+#pragma unroll
+        for (int ix = 0; ix < ORDER; ++ix) {
+          const R qv = row[lx - ix];
+          sx += v[0][ix] * qv;
+          sdx += d[0][ix] * qv;
+        }
+        acc.add(0, sdx * (v[1][iy] * vz));
+        acc.add(1, sx * (d[1][iy] * vz));
+        acc.add(2, sx * (v[1][iy] * dz));
       }
-      acc.add(0, sdx * vyz);
-      acc.add(1, sx * dyz);
-      acc.add(2, sx * vdz);
+      ex = acc.get(0);
+      ey = acc.get(1);
+      ez = acc.get(2);
+    }
+#pragma unroll
+    for (int w = 1; w < 8; w <<= 1) {
+      ex += __shfl_xor(ex, w, 8);
+      ey += __shfl_xor(ey, w, 8);
+      ez += __shfl_xor(ez, w, 8);
+    }
+    if (iz == 0 && p < n) {
+      const double qd = (double)q;
+      force[3 * idx] -= qd * ex * g.dkdr[0] * efp;
+      force[3 * idx + 1] -= qd * ey * g.dkdr[1] * efp;
+      force[3 * idx + 2] -= qd * ez * g.dkdr[2] * efp;
     }
   }
-  const int i = md.y;
-  const double qd = (double)q;
-  force[3 * i] -= qd * acc.get(0) * g.dkdr[0] * efp;
-  force[3 * i + 1] -= qd * acc.get(1) * g.dkdr[1] * efp;
-  force[3 * i + 2] -= qd * acc.get(2) * g.dkdr[2] * efp;
 }
 
 // --------------------------------------------------------- standard Ewald
@@ -735,50 +889,78 @@ __global__ void __launch_bounds__(kBlock) k_surf_sum(
 
 struct ReduceArgs {
   int nptpt;
-  int nblk_pair, nblk_conv, nblk_bond, nblk_surf, nk;
-  int off_pair, off_conv, off_bond, off_surf, off_std;
-  int ewald, lsurf, energy;
-  double efp, self, vol;
+  int nblk_pair, nblk_conv, nblk_bond, nblk_surf, nk, nblk_move;
+  int off_pair, off_conv, off_bond, off_surf, off_std, off_move, off_skin;
+  int ewald, lsurf, energy, skin;
+  double efp, self, vol, drnlist;
 };
 
-// Final sums in a fixed order (one block): writes
-// en = [tot, twob0, rec, bond, virial, Sx, Sy, Sz, twob(1..nptpt)].
+constexpr int kRedComp = MFE_MAX_PAIRS + 12;
+
+// Final sums in a fixed order (one block, one pass over the partials):
+// writes en = [tot, twob0, rec, bond, virial, Sx, Sy, Sz, twob(1..nptpt)]
+// and, in MD, the skin-test flag of NList (Src/nlist.F90:612-615).
 __global__ void __launch_bounds__(kBlock) k_reduce(const double *part,
-                                                   ReduceArgs a, double *en) {
-  __shared__ double sh[kBlock];
-  auto region = [&](int off, int nblk, int stride, int comp) {
-    double s = 0.0;
-    for (int b2 = threadIdx.x; b2 < nblk; b2 += blockDim.x) s += part[off + b2 * stride + comp];
-    return block_sum(s, sh);
-  };
-  double S[3] = {0, 0, 0};
-  if (a.lsurf)
-    for (int m = 0; m < 3; ++m) S[m] = region(a.off_surf, a.nblk_surf, 3, m);
-  if (!a.energy) {
-    if (threadIdx.x == 0)
-      for (int m = 0; m < 3; ++m) en[5 + m] = S[m];
-    return;
+                                                   ReduceArgs a, double *en,
+                                                   int *flags) {
+  __shared__ double sh[kRedComp][kBlock / 4];
+  // component layout: [0,P) pair energies, P pair virial, P+1..P+2 rec
+  // (u, vir), P+3..P+4 bond (u, vir), P+5..P+7 S, P+8 max r2, P+9 cand
+  const int P = a.nptpt;
+  const int nc = P + 10;
+  double v[kRedComp];
+#pragma unroll
+  for (int c = 0; c < kRedComp; ++c) v[c] = 0.0;
+  const int nrec = a.ewald == MFE_EWALD_SPME ? a.nblk_conv : (a.ewald == MFE_EWALD_STD ? a.nk : 0);
+  const int offrec = a.ewald == MFE_EWALD_SPME ? a.off_conv : a.off_std;
+  for (int b2 = threadIdx.x; b2 < a.nblk_pair && a.energy; b2 += blockDim.x)
+    for (int c = 0; c <= P; ++c) v[c] += part[a.off_pair + b2 * (P + 1) + c];
+  for (int b2 = threadIdx.x; b2 < nrec && a.energy; b2 += blockDim.x) {
+    v[P + 1] += part[offrec + 2 * b2];
+    v[P + 2] += part[offrec + 2 * b2 + 1];
   }
-  double twob0 = 0.0, vir = 0.0;
-  for (int k = 0; k < a.nptpt; ++k) {
-    const double u = region(a.off_pair, a.nblk_pair, a.nptpt + 1, k);
-    if (threadIdx.x == 0) en[8 + k] = u;
-    twob0 += u;
+  for (int b2 = threadIdx.x; b2 < a.nblk_bond && a.energy; b2 += blockDim.x) {
+    v[P + 3] += part[a.off_bond + 2 * b2];
+    v[P + 4] += part[a.off_bond + 2 * b2 + 1];
   }
-  vir += region(a.off_pair, a.nblk_pair, a.nptpt + 1, a.nptpt);
-  double urec = 0.0, virrec = 0.0;
-  if (a.ewald == MFE_EWALD_SPME) {
-    urec = region(a.off_conv, a.nblk_conv, 2, 0);
-    virrec = region(a.off_conv, a.nblk_conv, 2, 1);
-  } else if (a.ewald == MFE_EWALD_STD) {
-    urec = region(a.off_std, a.nk, 2, 0);
-    virrec = region(a.off_std, a.nk, 2, 1);
+  for (int b2 = threadIdx.x; b2 < a.nblk_surf && a.lsurf; b2 += blockDim.x)
+    for (int m = 0; m < 3; ++m) v[P + 5 + m] += part[a.off_surf + 3 * b2 + m];
+  for (int b2 = threadIdx.x; b2 < a.nblk_move && a.skin; b2 += blockDim.x) {
+    v[P + 8] = fmax(v[P + 8], part[a.off_move + b2]);
+    v[P + 9] = fmax(v[P + 9], part[a.off_skin + b2]);
   }
-  double ubond = 0.0;
-  if (a.nblk_bond > 0) {
-    ubond = region(a.off_bond, a.nblk_bond, 2, 0);
-    vir += region(a.off_bond, a.nblk_bond, 2, 1);
+  // fold 256 lanes to 64 in registers through LDS, then a fixed tree
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  for (int g2 = 1; g2 < 4; ++g2) {
+    if (grp == g2)
+      for (int c = 0; c < nc; ++c) sh[c][lane] = v[c];
+    __syncthreads();
+    if (grp == 0)
+      for (int c = 0; c < nc; ++c)
+        v[c] = (c >= P + 8) ? fmax(v[c], sh[c][lane]) : v[c] + sh[c][lane];
+    __syncthreads();
   }
+  if (grp == 0) {
+    for (int w = 32; w > 0; w >>= 1)
+      for (int c = 0; c < nc; ++c) {
+        const double o = __shfl_down(v[c], w, 64);
+        v[c] = (c >= P + 8) ? fmax(v[c], o) : v[c] + o;
+      }
+  }
+  if (threadIdx.x != 0) return;
+  const double S[3] = {v[P + 5], v[P + 6], v[P + 7]};
+  for (int m = 0; m < 3; ++m) en[5 + m] = S[m];
+  if (a.skin) flags[1] = (sqrt(v[P + 8]) + sqrt(v[P + 9]) > a.drnlist) ? 1 : 0;
+  if (!a.energy) return;
+  double twob0 = 0.0;
+  for (int k = 0; k < P; ++k) {
+    en[8 + k] = v[k];
+    twob0 += v[k];
+  }
+  double vir = v[P];
+  double urec = v[P + 1], virrec = v[P + 2];
+  const double ubond = v[P + 3];
+  vir += v[P + 4];
   double rec = 0.0;
   if (a.ewald != MFE_EWALD_NONE) {
     urec += a.self;
@@ -791,14 +973,11 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const double *part,
     rec = a.efp * urec;
     vir += a.efp * virrec;
   }
-  if (threadIdx.x == 0) {
-    en[0] = twob0 + rec + ubond;
-    en[1] = twob0;
-    en[2] = rec;
-    en[3] = ubond;
-    en[4] = vir;
-    for (int m = 0; m < 3; ++m) en[5 + m] = S[m];
-  }
+  en[0] = twob0 + rec + ubond;
+  en[1] = twob0;
+  en[2] = rec;
+  en[3] = ubond;
+  en[4] = vir;
 }
 
 // ------------------------------------------------------------ integrators
@@ -871,21 +1050,6 @@ __global__ void __launch_bounds__(kBlock) k_skin_blocks(
   if (threadIdx.x == 0) part_cand[blockIdx.x] = c;
 }
 
-__global__ void __launch_bounds__(kBlock) k_skin_final(const double *part_max,
-                                                       const double *part_cand,
-                                                       int nblk, double drnlist,
-                                                       int *flags) {
-  __shared__ double sh[kBlock];
-  double a = 0.0, c = 0.0;
-  for (int b2 = threadIdx.x; b2 < nblk; b2 += blockDim.x) {
-    a = fmax(a, part_max[b2]);
-    c = fmax(c, part_cand[b2]);
-  }
-  const double m1 = block_max(a, sh);
-  const double m2 = block_max(c, sh);
-  if (threadIdx.x == 0) flags[1] = (sqrt(m1) + sqrt(m2) > drnlist) ? 1 : 0;
-}
-
 // UTotal's tail (Src/energy.F90:405-418) plus, for MD, GetLinAcc and the
 // velocity half of VelVer (Src/md.F90:348-358):
 //   mode 0: forceo = force*fac        mode 1: + VelVer kick
@@ -947,6 +1111,10 @@ template <typename R, int O>
 const void *k_spread_ptr() {
   return (const void *)&k_spread<R, O>;
 }
+template <typename R, int O>
+const void *k_gather_ptr() {
+  return (const void *)&k_gather<R, O>;
+}
 
 void launch_fill_int(int *p, int n, int v, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, v);
@@ -959,15 +1127,19 @@ void launch_nlist_rebuild(mfe_engine *e, bool gated) {
   const int *gate = gated ? e->flags : nullptr;
   const int np = e->np;
   const double rs = e->rcut + e->drnlist;
-  hipLaunchKernelGGL(k_cell_count, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+  // gated kernels run on small grids (grid-stride loops): when the skin
+  // test does not fire they cost one dispatch of a few hundred wavefronts
+  const int gb = gated ? std::min(nblocks(np), 256) : nblocks(np);
+  const int gc = gated ? std::min(nblocks(e->cg.ncells), 256) : nblocks(e->cg.ncells);
+  hipLaunchKernelGGL(k_cell_count, dim3(gb), dim3(kBlock), 0, e->stream,
                      e->ro, np, e->cg, e->box, e->c_count, e->c_of, e->c_slot, gate);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, e->stream, e->c_count,
                      e->c_start, e->cg.ncells, gate);
-  hipLaunchKernelGGL(k_cell_scatter, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+  hipLaunchKernelGGL(k_cell_scatter, dim3(gb), dim3(kBlock), 0, e->stream,
                      np, e->c_of, e->c_slot, e->c_start, e->c_sorted, gate);
-  hipLaunchKernelGGL(k_cell_sort, dim3(nblocks(e->cg.ncells)), dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(kBlock), 0,
                      e->stream, e->cg.ncells, e->c_start, e->c_sorted, gate);
-  hipLaunchKernelGGL(k_nlist_build, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
+  hipLaunchKernelGGL(k_nlist_build, dim3(gb), dim3(kBlock), 0, e->stream,
                      e->ro, np, e->cg, e->box, e->c_of, e->c_start, e->c_sorted,
                      rs * rs, e->maxnn, e->nn, e->nbr, e->flags, e->drosum, gate);
 }
@@ -976,13 +1148,16 @@ void launch_pair(mfe_engine *e, bool energy) {
   const int np = e->np;
   TableDev t = table_dev(e);
   const int sem = e->nlkind == MFE_NL_CELL ? 1 : 0;
+  hipLaunchKernelGGL(k_sort_pos, dim3(nblocks(np)), dim3(kBlock), 0, e->stream, e->ro,
+                     e->type, e->c_sorted, np, e->ro_s, e->type_s);
+  const int pb = e->nblk_pair;
   if (e->prec == MFE_PREC_MIXED)
-    hipLaunchKernelGGL(k_pair<true>, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
-                       e->ro, e->type, np, e->nn, e->nbr, e->box, t, sem,
+    hipLaunchKernelGGL(k_pair<true>, dim3(pb), dim3(kBlock), 0, e->stream,
+                       e->ro_s, e->type_s, np, e->nn, e->nbr, e->c_sorted, e->box, t, sem,
                        energy ? 1 : 0, e->force, e->part + e->off_pair, e->flags);
   else
-    hipLaunchKernelGGL(k_pair<false>, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
-                       e->ro, e->type, np, e->nn, e->nbr, e->box, t, sem,
+    hipLaunchKernelGGL(k_pair<false>, dim3(pb), dim3(kBlock), 0, e->stream,
+                       e->ro_s, e->type_s, np, e->nn, e->nbr, e->c_sorted, e->box, t, sem,
                        energy ? 1 : 0, e->force, e->part + e->off_pair, e->flags);
 }
 
@@ -994,7 +1169,7 @@ static void spme_stages(mfe_engine *e, bool energy, int first, int last) {
     switch (st) {
       case SP_BIN:
         hipLaunchKernelGGL(k_spme_count, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
-                           e->ro, np, g, e->b_count, e->b_id, e->b_slot);
+                           e->ro, e->c_sorted, np, g, e->b_count, e->b_id, e->b_slot);
         hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, e->stream, e->b_count,
                            e->b_start, e->nbricks, (const int *)nullptr);
         hipLaunchKernelGGL(k_spme_scatter<R>, dim3(nblocks(np)), dim3(kBlock), 0,
@@ -1022,9 +1197,11 @@ static void spme_stages(mfe_engine *e, bool energy, int first, int last) {
         switch (e->order) {
 #define MFE_GATHER(O)                                                                   \
   case O:                                                                               \
-    hipLaunchKernelGGL((k_gather<R, O>), dim3(nblocks(np)), dim3(kBlock), 0, e->stream, \
-                       (const R *)e->rec_frac, e->rec_meta, np, g, (const R *)e->mesh_r, \
-                       e->efp, e->force);                                               \
+    hipLaunchKernelGGL((k_gather<R, O>), dim3(e->nbricks), dim3(kBlock),                \
+                       (size_t)(e->B[0] + O - 1) * (e->B[1] + O - 1) * (e->B[2] + O - 1) *  \
+                           sizeof(R),                                                   \
+                       e->stream, (const R *)e->rec_frac, e->rec_meta, e->b_start, g,   \
+                       (const R *)e->mesh_r, e->efp, e->force);                         \
     break;
           MFE_GATHER(3) MFE_GATHER(4) MFE_GATHER(5) MFE_GATHER(6) MFE_GATHER(7) MFE_GATHER(8)
 #undef MFE_GATHER
@@ -1058,7 +1235,7 @@ void launch_surf(mfe_engine *e) {
                      e->q, e->np, e->part + e->off_surf);
 }
 
-void launch_reduce(mfe_engine *e, unsigned flags) {
+void launch_reduce(mfe_engine *e, unsigned flags, bool skin) {
   ReduceArgs a;
   a.nptpt = e->nptpt;
   a.nblk_pair = e->nblk_pair;
@@ -1066,18 +1243,24 @@ void launch_reduce(mfe_engine *e, unsigned flags) {
   a.nblk_bond = e->nchain > 0 ? e->nblk_bond : 0;
   a.nblk_surf = e->nblk_surf;
   a.nk = e->nkvec;
+  a.nblk_move = e->nblk_move;
   a.off_pair = e->off_pair;
   a.off_conv = e->off_conv;
   a.off_bond = e->off_bond;
   a.off_surf = e->off_surf;
   a.off_std = e->off_std;
+  a.off_move = e->off_move;
+  a.off_skin = e->off_skin;
   a.ewald = e->ewald;
-  a.lsurf = e->lsurf;
+  a.lsurf = (e->lsurf && e->ewald != MFE_EWALD_NONE) ? 1 : 0;
   a.energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
+  a.skin = skin ? 1 : 0;
   a.efp = e->efp;
   a.self = e->self_energy;
   a.vol = e->box.L[0] * e->box.L[1] * e->box.L[2];
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e->stream, e->part, a, e->en);
+  a.drnlist = e->drnlist;
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e->stream, e->part, a, e->en,
+                     e->flags);
 }
 
 void launch_finalize(mfe_engine *e, int mode, double dt) {
@@ -1101,18 +1284,17 @@ void launch_move(mfe_engine *e, double dt, int bd, const double *fac1,
 void launch_skin(mfe_engine *e) {
   hipLaunchKernelGGL(k_skin_blocks, dim3(e->nblk_move), dim3(kBlock), 0, e->stream,
                      e->r2disp, e->np, e->part + e->off_move, e->part + e->off_skin);
-  hipLaunchKernelGGL(k_skin_final, dim3(1), dim3(kBlock), 0, e->stream,
-                     e->part + e->off_move, e->part + e->off_skin, e->nblk_move,
-                     e->drnlist, e->flags);
 }
 
 }  // namespace mfe
 
-// Spreading bricks above 64 KB of LDS (meshes whose only divisors >= order-1
-// are large, e.g. 2*23) need the per-kernel dynamic LDS limit raised.
+// Bricks above 64 KB of LDS (meshes whose only divisors >= order-1 are
+// large, e.g. 2*23) need the per-kernel dynamic LDS limit raised.
 void mfe_allow_big_spread_lds(int prec, int bytes) {
 #define MFE_ATTR(R, O)                                                       \
   (void)hipFuncSetAttribute((const void *)mfe::k_spread_ptr<R, O>(),          \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes); \
+  (void)hipFuncSetAttribute((const void *)mfe::k_gather_ptr<R, O>(),          \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (prec == MFE_PREC_MIXED) {
     MFE_ATTR(float, 3) MFE_ATTR(float, 4) MFE_ATTR(float, 5)
