@@ -78,6 +78,53 @@ def cpu_baseline(sysm, nsteps):
     return nsteps / dt, dt
 
 
+def c4_batched(nrep, steps, tol, seq_sample=16):
+    """Batched BD of nrep replicas (N 500-4000, 0.1-1 M) with the
+    autotuner's plan, against the same replicas stepped one engine at a
+    time: replica-steps per second both ways."""
+    from molsim_amd import autotune as A
+    reps = [S.c4_replica(r) for r in range(nrep)]
+    t0 = time.perf_counter()
+    plan = A.autotune(reps, tol=tol)
+    t_tune = time.perf_counter() - t0
+    tuned = plan.apply(reps)
+    cache = {}
+    fac1, fac2 = 2.0e-7, 0.0316
+    engs = [S.configure(s, "mixed", tables=A._tables(s, cache)) for s in tuned]
+    for r, e in enumerate(engs):
+        e.set_bd(np.full(2, fac1), np.full(2, fac2), 1000 + r)
+    b = E.Batch(engs)
+    b.compute()
+    b.step_bd(10)
+    b.energies()
+    t0 = time.perf_counter()
+    b.step_bd(steps)
+    b.energies()
+    dt = time.perf_counter() - t0
+    b.close()
+    # the same work one engine at a time (a sample, scaled by its N share)
+    idx = np.linspace(0, nrep - 1, seq_sample).round().astype(int)
+    tseq = 0.0
+    for r in idx:
+        e = engs[r]
+        e.step_bd(nsteps=5)
+        e.energy()
+        t1 = time.perf_counter()
+        e.step_bd(nsteps=steps)
+        e.energy()
+        tseq += time.perf_counter() - t1
+    for e in engs:
+        e.close()
+    seq_rate = seq_sample * steps / tseq
+    return {"replicas": nrep, "n_particles": int(sum(s.np for s in reps)),
+            "steps": steps, "batched_replica_steps_per_s": round(nrep * steps / dt, 1),
+            "one_at_a_time_replica_steps_per_s": round(seq_rate, 1),
+            "batching_speedup": round(nrep * steps / dt / seq_rate, 2),
+            "plan": {"alpha": plan.alpha, "order": plan.order, "alpha_h": plan.ah,
+                     "mesh_min": int(min(plan.meshes)), "mesh_max": int(max(plan.meshes)),
+                     "force_rel_rms_tol": tol, "tune_s": round(t_tune, 1)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +134,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the batched-replica BD leg")
+    ap.add_argument("--c4-replicas", type=int, default=512)
+    ap.add_argument("--c4-steps", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it",
@@ -159,6 +209,8 @@ def main():
         out["mixed_over_fp64"] = round(mixed["sps"] / fp64["sps"], 3)
         out["force_rel_rms_mixed_vs_fp64"] = rel
         out["energy_rel_mixed_vs_fp64"] = abs(mixed["e0"].tot - fp64["e0"].tot) / abs(fp64["e0"].tot)
+    if not args.no_c4:
+        out["c4_batched_bd"] = c4_batched(args.c4_replicas, args.c4_steps, 1e-4)
     if not args.no_cpu_baseline:
         sps, secs = cpu_baseline(sysm, args.cpu_steps)
         out["cpu_baseline"] = {"value": round(sps, 4), "unit": "steps/s", "cores": 1,

@@ -148,6 +148,8 @@ int mfe_upload_positions(mfe_engine *e, const double *ro);
 int mfe_upload_velocities(mfe_engine *e, const double *rod,
                           const double *rodd);
 int mfe_download_state(mfe_engine *e, double *ro, double *rod, double *rodd);
+/* forceo(3,np) of the last UTotal (mfe_compute or a step). */
+int mfe_download_forces(mfe_engine *e, double *forceo);
 
 /* ---- the hot path ----------------------------------------------------- */
 /* SetVList / InitCellList+SetCellList at the current positions. */
@@ -163,11 +165,23 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo,
    for mfe_get_energy. */
 int mfe_step_md(mfe_engine *e, int32_t nsteps, double tstep, uint32_t flags);
 /* nsteps of BDStep (Src/bd.F90:120-155) + UTotal + NList: fac1/fac2 per
-   type, noise[nsteps][np][3] unit Gaussian deviates in the reference's
-   draw order (ip outer, xyz inner).  The reference's first BD step uses
+   type (NULL: keep those of mfe_set_bd), noise[nsteps][np][3] unit
+   Gaussian deviates in the reference's draw order (ip outer, xyz inner).
+   With noise == NULL the deviates come from the engine's counter-based
+   stream (see mfe_set_bd) instead of GauRandom's serial LCG
+   (Src/mollib.F90:2739-2766), so replicas draw independent, reproducible
+   noise without a host round trip.  The reference's first BD step uses
    the forces of the preceding UTotal; call mfe_compute first. */
 int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1,
                 const double *fac2, const double *noise, uint32_t flags);
+/* BDStep's per-type factors (fac1 = dt D/kT in force units, fac2 =
+   sqrt(2 D dt)/scllen, Src/bd.F90:120-140) and the seed of the device
+   noise stream: particle i at BD step n draws Philox4x32-10 with counter
+   (i, n, 0, 0) and key seed, and takes three Box-Muller deviates.  Resets
+   the step counter n to 0.  No reference counterpart (replaces the
+   GauRandom seed iseed of nmlSystem). */
+int mfe_set_bd(mfe_engine *e, const double *fac1, const double *fac2,
+               uint64_t seed);
 int mfe_get_energy(mfe_engine *e, mfe_energy *energy);
 /* The Verlet list in SetVListMD's form (Src/nlist.F90:724-759): for each
    ip the partners jp it owns under the checkerboard rule, ascending.
@@ -191,6 +205,61 @@ enum {
 /* Runs one stage nrep times on the engine's stream, bracketed by HIP
    events on that stream, and returns the mean time per run in ms. */
 int mfe_time_stage(mfe_engine *e, int32_t stage, int32_t nrep, double *ms);
+
+/* ---- parameter adjoint ------------------------------------------------ */
+/* No reference counterpart.  The model the adjoint differentiates is the
+   pair potential the tables interpolate -- Pot_1_6_12 with exact erfc
+   (Src/potential.F90:1349-1425, Lorentz-Berthelot mixing sig = (sig_i +
+   sig_j)/2, eps = sqrt(eps_i eps_j), optional WCA cut at ljrcut*sig with
+   shift ljushift*eps) -- plus the optional spline term below, SPME (or no
+   Ewald sum) with its self and surface terms, and the bonds. */
+/* The parameters of that model: spec as for mfe_build_tables_1_6_12; its
+   zat must equal the charges az of mfe_set_particles. */
+int mfe_set_pair_params(mfe_engine *e, const mfe_pot_1_6_12 *spec);
+/* An additive pair term per type pair p (Set_iptpt order), a uniform cubic
+   B-spline in r with nknot coefficients c[p][k] on [r_lo, r_hi) and zero
+   outside: u_p(r) = sum_k c[p][k] B_k(r), B_k the cubic B-spline on knots
+   r_lo + (j-3) h, h = (r_hi - r_lo)/(nknot-3).  Applied in every UTotal
+   (forces and energies); nknot = 0 removes it. */
+int mfe_set_knots(mfe_engine *e, int32_t nknot, double r_lo, double r_hi,
+                  const double *c);
+/* One UTotal at the current positions and the gradient of the model
+   energy: dE/dq[np] (per particle), dE/dzat[npt], dE/dsigat[npt],
+   dE/depsat[npt], dE/dc[nptpt][nknot] and the position adjoint
+   dE/dro(3,np) = -force (kJ/mol and A; any output may be NULL).  energy
+   receives the model energy: exact pair sum + u%rec + u%bond. */
+int mfe_adjoint(mfe_engine *e, double *dE_dq, double *dE_dzat,
+                double *dE_dsigat, double *dE_depsat, double *dE_dknot,
+                double *dE_dro, double *energy);
+
+/* ---- batched replicas ------------------------------------------------- */
+/* No reference counterpart: MOLSIM runs one system per process.  A batch
+   drives nrep engines, each set up as above (own box, particles, tables,
+   list, mesh), as one system: every stage of UTotal, VelVer and BDStep is
+   one launch over all replicas.  The replicas must share device,
+   precision, Ewald kind and SPME order; box, N, mesh, tables and cutoffs
+   are per replica.  Each replica computes exactly what its engine computes
+   alone.  Results stay in the engines (mfe_download_forces,
+   mfe_download_state, mfe_get_energy) or come back together through
+   mfe_batch_get_energies.  The engines must outlive the batch; a set-up
+   call on an engine is picked up at the batch's next call.  On an error
+   mfe_batch_failed_replica names the first failing replica. */
+typedef struct mfe_batch mfe_batch;
+int mfe_batch_create(mfe_engine *const *engines, int32_t nrep, mfe_batch **out);
+int mfe_batch_destroy(mfe_batch *b);
+const char *mfe_batch_last_error(const mfe_batch *b);
+int32_t mfe_batch_failed_replica(const mfe_batch *b);
+/* SetVList for every replica */
+int mfe_batch_build_list(mfe_batch *b);
+/* UTotal for every replica (energies and virials always reduced) */
+int mfe_batch_compute(mfe_batch *b, uint32_t flags);
+/* nsteps of VelVer + NList for every replica, one graph launch per step */
+int mfe_batch_step_md(mfe_batch *b, int32_t nsteps, double tstep, uint32_t flags);
+/* nsteps of BDStep + NList with each replica's device noise stream and
+   factors (mfe_set_bd) */
+int mfe_batch_step_bd(mfe_batch *b, int32_t nsteps, uint32_t flags);
+/* energies of the last UTotal of every replica: out[nrep] */
+int mfe_batch_get_energies(mfe_batch *b, mfe_energy *out);
 
 #ifdef __cplusplus
 }

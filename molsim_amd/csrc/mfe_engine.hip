@@ -131,7 +131,7 @@ void free_all(mfe_engine *e) {
   dfree(e->t_zlow); dfree(e->t_coef); dfree(e->t_coef_f); dfree(e->t_off);
   dfree(e->t_nint); dfree(e->t_pairidx); dfree(e->t_r2atat); dfree(e->t_r2umin);
   dfree(e->c_count); dfree(e->c_start); dfree(e->c_of); dfree(e->c_slot);
-  dfree(e->c_sorted); dfree(e->nn); dfree(e->nbr); dfree(e->flags);
+  dfree(e->c_sorted); dfree(e->c_pos); dfree(e->nn); dfree(e->nbr); dfree(e->flags);
   dfree(e->ro_s); dfree(e->type_s);
   dfree(e->mesh_r); dfree(e->mesh_c); dfree(e->b_count); dfree(e->b_start);
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
@@ -140,6 +140,9 @@ void free_all(mfe_engine *e) {
   dfree(e->tw_half);
   dfree(e->k_vec); dfree(e->k_sum); dfree(e->ch_prev); dfree(e->ch_next);
   dfree(e->part); dfree(e->en);
+  dfree(e->bd_fac1); dfree(e->bd_fac2); dfree(e->bd_noise); dfree(e->d_sys);
+  dfree(e->knot_c); dfree(e->phi_rec); dfree(e->adj_phi); dfree(e->adj_dq);
+  dfree(e->adj_part); dfree(e->adj_out);
   if (e->h_en) (void)hipHostFree(e->h_en);
   if (e->h_flags) (void)hipHostFree(e->h_flags);
   e->h_en = nullptr;
@@ -154,7 +157,13 @@ void layout_partials(mfe_engine *e) {
   e->nblk_surf = nb;
   e->nblk_move = nb;
   int conv = 0;
-  if (e->ewald == MFE_EWALD_SPME && e->hx > 0) conv = mfe::conv_blocks(e);
+  if (e->ewald == MFE_EWALD_SPME && e->hx > 0) {
+    int a, b, c, d, fz_T, fz_nbx;
+    size_t l0, l1, l2;
+    mfe::fft_geometry(e->prec, e->s, &a, &b, &c, &d, &fz_T, &fz_nbx, &l0, &l1, &l2);
+    conv = fz_nbx * e->s[1];
+  }
+  e->nblk_conv = conv;
   int off = 0;
   e->off_pair = off; off += e->nblk_pair * (e->nptpt + 1);
   e->off_conv = off; off += 2 * conv;
@@ -218,39 +227,225 @@ int check_ready(mfe_engine *e) {
 int read_flags(mfe_engine *e) {
   MFE_HIP_TRY(hipMemcpyAsync(e->h_flags, e->flags, 3 * sizeof(int), hipMemcpyDeviceToHost, e->stream));
   MFE_HIP_TRY(hipStreamSynchronize(e->stream));
-  const int err = e->h_flags[0];
-  if (err) {
-    mfe::launch_fill_int(e->flags, 1, 0, e->stream);
-    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
-    if (err & mfe::ERR_CAPACITY) {
-      e->list_valid = false;
-      return fail(e, MFE_E_NEIGH_CAPACITY, "nneighpn > maxnneigh: raise maxnneigh");
+  return mfe_map_flag_error(e, e->h_flags[0]);
+}
+
+inline int nblocks256(int n) { return (n + 255) / 256; }
+
+}  // namespace
+
+// the engine's device view (one replica of the kernels' SysDev array)
+void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
+  v = mfe::SysDev{};
+  v.np = e->np;
+  v.npt = e->npt;
+  v.nptpt = e->nptpt;
+  v.nlsem = e->nlkind == MFE_NL_CELL ? 1 : 0;
+  v.box = e->box;
+  v.ro = e->ro; v.rod = e->rod; v.rodd = e->rodd; v.force = e->force; v.forceo = e->forceo;
+  v.drosum = e->drosum; v.r2disp = e->r2disp; v.type = e->type; v.q = e->q;
+  v.massip = e->massip;
+  mfe::TableDev &t = v.tab;
+  t.zlow = e->t_zlow; t.coef = e->t_coef; t.coef_f = e->t_coef_f; t.off = e->t_off;
+  t.nint = e->t_nint; t.r2atat = e->t_r2atat; t.r2umin = e->t_r2umin; t.pairidx = e->t_pairidx;
+  t.npt = e->npt; t.nptpt = e->nptpt; t.rcut2 = e->rcut * e->rcut;
+  const double rs = e->rcut + e->drnlist;
+  v.rs2 = rs * rs;
+  v.drnlist = e->drnlist;
+  v.maxnn = e->maxnn;
+  v.cg = e->cg;
+  v.c_count = e->c_count; v.c_start = e->c_start; v.c_of = e->c_of; v.c_slot = e->c_slot;
+  v.c_sorted = e->c_sorted; v.c_pos = e->c_pos; v.ro_s = e->ro_s; v.type_s = e->type_s; v.nn = e->nn;
+  v.nbr = e->nbr; v.flags = e->flags;
+  v.ewald = e->ewald;
+  v.lsurf = (e->lsurf && e->ewald != MFE_EWALD_NONE) ? 1 : 0;
+  v.alpha = e->alpha;
+  v.efp = e->efp;
+  v.self_energy = e->self_energy;
+  v.vol = e->box.L[0] * e->box.L[1] * e->box.L[2];
+  if (e->ewald == MFE_EWALD_SPME) {
+    for (int m = 0; m < 3; ++m) {
+      v.sg.s[m] = e->s[m];
+      v.sg.B[m] = e->B[m];
+      v.sg.nb[m] = e->nb[m];
+      v.sg.dkdr[m] = e->s[m] / e->box.L[m];
     }
-    if (err & mfe::ERR_OVERLAP) return fail(e, MFE_E_OVERLAP_R2UMIN, "r2 < r2umin(iptjpt)");
-    if (err & mfe::ERR_TABLE) return fail(e, MFE_E_TABLE_RANGE, "ibuf > nbuf");
+    v.sg.order = e->order;
+    v.hx = e->hx;
+    v.nbricks = e->nbricks;
+    v.mesh_r = e->mesh_r; v.mesh_c = e->mesh_c;
+    v.b_count = e->b_count; v.b_start = e->b_start; v.b_id = e->b_id; v.b_slot = e->b_slot;
+    v.rec_frac = e->rec_frac; v.rec_meta = e->rec_meta;
+    const int smax2 = std::max(e->s[0], std::max(e->s[1], e->s[2])) / 2 + 1;
+    for (int a = 0; a < 3; ++a) {
+      v.ax.g[a] = e->ax_g + a * smax2;
+      v.ax.kk[a] = e->ax_kk + a * smax2;
+    }
+    v.ax.fac0 = 4.0 * kPi / v.vol;
+    v.ax.expf = -1.0 / (4.0 * e->alpha * e->alpha);
+    for (int m = 0; m < 3; ++m) {
+      v.plan[m] = e->plan[m];
+      v.tw[m] = e->tw[m];
+    }
+    v.plan_half = e->plan_half;
+    v.tw_half = e->tw_half;
+    size_t l0, l1, l2;
+    mfe::fft_geometry(e->prec, e->s, &v.fx_rpb, &v.fx_nblk, &v.fy_T, &v.fy_nbx, &v.fz_T,
+                      &v.fz_nbx, &l0, &l1, &l2);
+  }
+  v.k_vec = e->k_vec; v.k_sum = e->k_sum; v.nkvec = e->nkvec;
+  v.nchain = e->nchain; v.bond_p = e->bond_p; v.ch_prev = e->ch_prev; v.ch_next = e->ch_next;
+  v.bond_k = e->bond_k; v.bond_eq = e->bond_eq;
+  v.bd_fac1 = e->bd_fac1; v.bd_fac2 = e->bd_fac2; v.noise = e->bd_noise; v.seed = e->bd_seed;
+  v.nknot = e->nknot;
+  v.knot_r0 = e->knot_r0;
+  v.knot_dinv = e->knot_dinv;
+  v.knot_c = e->knot_c;
+  v.pp = e->pp;
+  v.phi_rec = e->phi_rec;
+  v.adj_phi = e->adj_phi;
+  v.adj_dq = e->adj_dq;
+  v.adj_part = e->adj_part;
+  v.adj_ncol = 1 + 2 * e->npt + e->nptpt * e->nknot;
+  v.part = e->part; v.en = e->en;
+  v.off_pair = e->off_pair; v.off_conv = e->off_conv; v.off_bond = e->off_bond;
+  v.off_surf = e->off_surf; v.off_std = e->off_std; v.off_move = e->off_move;
+  v.off_skin = e->off_skin;
+  v.nblk_np = nblocks256(e->np);
+  v.nblk_pair = e->nblk_pair;
+  v.nblk_conv = e->nblk_conv;
+  // UTotal's force scaling (sclene/scllen)/sclfor and the surface term's
+  // force factor EpsiFourPi*(4 pi/3V) (Src/energy.F90:405-418, 1352-1362)
+  const double sclvel = 1.0e-10 / 1.0e-12, sclacc = sclvel / 1.0e-12;
+  v.ffac = (1.0e+03 / 1.0e-10) / (sclacc * 1.0e-03);
+  v.surf_fac = e->efp * (2.0 * (2.0 * kPi / (3.0 * v.vol)));
+}
+
+// the launch group of a set of engines (maxima of the per-replica grids)
+int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStream_t st,
+                    mfe::Group &g, std::string &err) {
+  g = mfe::Group{};
+  g.R = R;
+  g.d = d;
+  g.stream = st;
+  for (int r = 0; r < R; ++r) {
+    const mfe_engine *e = eng[r];
+    if (r == 0) {
+      g.prec = e->prec;
+      g.ewald = e->ewald;
+      g.order = e->ewald == MFE_EWALD_SPME ? e->order : 0;
+    } else {
+      if (e->prec != g.prec) { err = "replicas differ in precision"; return MFE_E_BAD_PARAM; }
+      if (e->ewald != g.ewald) { err = "replicas differ in Ewald kind"; return MFE_E_BAD_PARAM; }
+      if (g.ewald == MFE_EWALD_SPME && e->order != g.order) {
+        err = "replicas differ in SPME order";
+        return MFE_E_BAD_PARAM;
+      }
+    }
+    if (e->nchain > 0) g.chains = true;
+    if (e->lsurf && e->ewald != MFE_EWALD_NONE) g.surf = true;
+    g.np_blocks = std::max(g.np_blocks, nblocks256(e->np));
+    g.pair_blocks = std::max(g.pair_blocks, e->nblk_pair);
+    g.cell_blocks = std::max(g.cell_blocks, nblocks256(e->cg.ncells));
+    g.kvec = std::max(g.kvec, e->nkvec);
+    if (e->ewald == MFE_EWALD_SPME) {
+      int fx_rpb, fx_nblk, fy_T, fy_nbx, fz_T, fz_nbx;
+      size_t lfx, lfy, lfz;
+      mfe::fft_geometry(e->prec, e->s, &fx_rpb, &fx_nblk, &fy_T, &fy_nbx, &fz_T, &fz_nbx,
+                        &lfx, &lfy, &lfz);
+      g.bricks = std::max(g.bricks, e->nbricks);
+      g.fx_blocks = std::max(g.fx_blocks, fx_nblk);
+      g.fy_x = std::max(g.fy_x, fy_nbx);
+      g.fy_y = std::max(g.fy_y, e->s[2]);
+      g.fz_x = std::max(g.fz_x, fz_nbx);
+      g.fz_y = std::max(g.fz_y, e->s[1]);
+      g.lds_fx = std::max(g.lds_fx, lfx);
+      g.lds_fy = std::max(g.lds_fy, lfy);
+      g.lds_fz = std::max(g.lds_fz, lfz);
+      const size_t rsz = e->prec == MFE_PREC_MIXED ? 4 : 8;
+      size_t gext = rsz;
+      for (int m = 0; m < 3; ++m) gext *= (size_t)(e->B[m] + e->order - 1);
+      g.lds_spread = std::max(g.lds_spread, (size_t)e->spread_lds);
+      g.lds_gather = std::max(g.lds_gather, gext);
+    }
   }
   return MFE_OK;
 }
 
-// UTotal's device sequence; mode 0 = forces only, 1 = with VelVer kick;
-// skin: also take the NList skin decision (MD/BD steps)
-void launch_utotal(mfe_engine *e, unsigned flags, int mode, double dt, bool skin = false) {
-  const bool energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) != 0;
-  mfe::launch_pair(e, energy);
-  if (e->ewald == MFE_EWALD_SPME) mfe::launch_spme(e, energy, SP_BIN, SP_GATHER);
-  else if (e->ewald == MFE_EWALD_STD) mfe::launch_std_ewald(e);
-  if (e->nchain > 0) mfe::launch_bonds(e);
-  if (e->lsurf && e->ewald != MFE_EWALD_NONE) mfe::launch_surf(e);
-  mfe::launch_reduce(e, flags, skin);
-  mfe::launch_finalize(e, mode, dt);
+namespace {
+
+// refresh the engine's device view and its one-replica group
+void sync_view(mfe_engine *e) {
+  if (!e->sys_dirty && e->d_sys) return;
+  if (!e->d_sys) MFE_HIP_TRY(hipMalloc((void **)&e->d_sys, sizeof(mfe::SysDev)));
+  mfe_fill_view(e, e->h_sys);
+  MFE_HIP_TRY(hipMemcpyAsync(e->d_sys, &e->h_sys, sizeof(mfe::SysDev), hipMemcpyHostToDevice,
+                             e->stream));
+  MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+  std::string err;
+  mfe_engine *one[1] = {e};
+  mfe_group_build(one, 1, e->d_sys, e->stream, e->grp, err);
+  e->sys_dirty = false;
 }
 
-void launch_md_step(mfe_engine *e, double dt, unsigned flags) {
-  mfe::launch_move(e, dt, 0, nullptr, nullptr, nullptr);
-  mfe::launch_skin(e);
-  launch_utotal(e, flags, 1, dt, true);
-  mfe::launch_nlist_rebuild(e, true);
+// any set-up change: the device view, the step graphs and batches holding
+// the engine are rebuilt before the next launch
+void touch(mfe_engine *e) {
+  e->graph_dirty = true;
+  e->sys_dirty = true;
+  ++e->version;
 }
+
+}  // namespace
+
+// error bits raised by the kernels -> status (the bits are cleared)
+int mfe_map_flag_error(mfe_engine *e, int err) {
+  if (!err) return MFE_OK;
+  mfe::launch_fill_int(e->flags, 1, 0, e->stream);
+  MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+  if (err & mfe::ERR_CAPACITY) {
+    e->list_valid = false;
+    return fail(e, MFE_E_NEIGH_CAPACITY, "nneighpn > maxnneigh: raise maxnneigh");
+  }
+  if (err & mfe::ERR_OVERLAP) return fail(e, MFE_E_OVERLAP_R2UMIN, "r2 < r2umin(iptjpt)");
+  if (err & mfe::ERR_TABLE) return fail(e, MFE_E_TABLE_RANGE, "ibuf > nbuf");
+  return MFE_OK;
+}
+
+void mfe_sync_view(mfe_engine *e) { sync_view(e); }
+
+// UTotal's device sequence; mode 0 = forces only, 1 = with VelVer kick;
+// skin: also take the NList skin decision (MD/BD steps)
+void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt, bool skin,
+                       bool bd) {
+  const bool energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) != 0;
+  // inside MD/BD steps (skin) k_move has already updated the cell-ordered copy
+  mfe::launch_pair(g, energy, !skin);
+  if (g.ewald == MFE_EWALD_SPME) mfe::launch_spme(g, energy, SP_BIN, SP_GATHER);
+  else if (g.ewald == MFE_EWALD_STD) mfe::launch_std_ewald(g);
+  if (g.chains) mfe::launch_bonds(g);
+  if (g.surf) mfe::launch_surf(g);
+  mfe::launch_reduce(g, flags, skin, bd);
+  mfe::launch_finalize(g, mode, dt);
+}
+
+// one VelVer step with the NList skin test and gated rebuild
+void mfe_launch_md_step(const mfe::Group &g, double dt, unsigned flags) {
+  mfe::launch_move(g, dt, 0);
+  mfe::launch_skin(g);
+  mfe_launch_utotal(g, flags, 1, dt, true, false);
+  mfe::launch_nlist_rebuild(g, true);
+}
+
+// one BDStep (noise: 1 = host-supplied buffer, 2 = counter-based stream)
+void mfe_launch_bd_step(const mfe::Group &g, int noise, unsigned flags) {
+  mfe::launch_move(g, 0.0, noise);
+  mfe::launch_skin(g);
+  mfe_launch_utotal(g, flags, 0, 0.0, true, true);
+  mfe::launch_nlist_rebuild(g, true);
+}
+
+namespace {
 
 void energy_out(mfe_engine *e, mfe_energy *en) {
   std::memset(en, 0, sizeof(*en));
@@ -270,8 +465,6 @@ void fetch_energy(mfe_engine *e) {
 }
 
 }  // namespace
-
-void mfe_allow_big_spread_lds(int prec, int bytes);
 
 // ======================================================================= ABI
 extern "C" {
@@ -293,8 +486,12 @@ int mfe_create(const mfe_config *cfg, mfe_engine **out) {
     dalloc(e->en, 8 + MFE_MAX_PAIRS);
     MFE_HIP_TRY(hipHostMalloc((void **)&e->h_en, (8 + MFE_MAX_PAIRS) * sizeof(double)));
     MFE_HIP_TRY(hipHostMalloc((void **)&e->h_flags, 4 * sizeof(int)));
+    dalloc(e->bd_fac1, MFE_MAX_TYPES);
+    dalloc(e->bd_fac2, MFE_MAX_TYPES);
     mfe::launch_fill_int(e->flags, 4, 0, e->stream);
     mfe::launch_fill_double(e->en, 8 + MFE_MAX_PAIRS, 0.0, e->stream);
+    mfe::launch_fill_double(e->bd_fac1, MFE_MAX_TYPES, 0.0, e->stream);
+    mfe::launch_fill_double(e->bd_fac2, MFE_MAX_TYPES, 0.0, e->stream);
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     return MFE_OK;
   });
@@ -331,7 +528,7 @@ int mfe_set_box(mfe_engine *e, const double boxlen[3]) {
   }
   e->have_box = true;
   e->list_valid = false;
-  e->graph_dirty = true;
+  touch(e);
   return MFE_OK;
 }
 
@@ -357,7 +554,7 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
       dalloc(e->r2disp, (size_t)np); dalloc(e->type, (size_t)np);
       dalloc(e->q, (size_t)np); dalloc(e->massip, (size_t)np);
       dalloc(e->c_of, (size_t)np); dalloc(e->c_slot, (size_t)np);
-      dalloc(e->c_sorted, (size_t)np); dalloc(e->nn, (size_t)np);
+      dalloc(e->c_sorted, (size_t)np); dalloc(e->c_pos, (size_t)np); dalloc(e->nn, (size_t)np);
       dalloc(e->ro_s, 3 * (size_t)np); dalloc(e->type_s, (size_t)np);
       dalloc(e->b_id, (size_t)np); dalloc(e->b_slot, (size_t)np);
       dalloc(e->rec_meta, (size_t)np);
@@ -379,6 +576,8 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
       std::vector<int> ident(np);
       for (int i = 0; i < np; ++i) ident[i] = i;
       upload(e->c_sorted, ident.data(), (size_t)np, e->stream);
+      upload(e->c_pos, ident.data(), (size_t)np, e->stream);
+      MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     }
     upload(e->q, az, (size_t)np, e->stream);
     upload(e->massip, mi.data(), (size_t)np, e->stream);
@@ -390,7 +589,7 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
     e->self_energy = -(e->alpha / std::sqrt(kPi)) * q2;
     e->have_particles = true;
     e->list_valid = false;
-    e->graph_dirty = true;
+    touch(e);
     layout_partials(e);
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     return MFE_OK;
@@ -451,7 +650,7 @@ int mfe_set_tables(mfe_engine *e, const double *ubuf, int32_t nbuf, const int32_
     e->rcut = rcut;
     e->have_tables = true;
     e->list_valid = false;
-    e->graph_dirty = true;
+    touch(e);
     return MFE_OK;
   });
 }
@@ -595,7 +794,7 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
     layout_partials(e);
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     e->have_ewald = true;
-    e->graph_dirty = true;
+    touch(e);
     return MFE_OK;
   });
 }
@@ -618,7 +817,7 @@ int mfe_set_neighbor(mfe_engine *e, int32_t kind, double drnlist, int32_t maxnne
     int rc = prepare_neighbor(e);
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     e->list_valid = false;
-    e->graph_dirty = true;
+    touch(e);
     return rc;
   });
 }
@@ -648,7 +847,7 @@ int mfe_set_chains(mfe_engine *e, int32_t nchain, const int32_t *chain_first,
     e->bond_k = k / p;  // IOPotChain, Src/potential.F90:3561
     e->bond_p = p;
     e->bond_eq = eq;
-    e->graph_dirty = true;
+    touch(e);
     return MFE_OK;
   });
 }
@@ -694,13 +893,25 @@ int mfe_download_state(mfe_engine *e, double *ro, double *rod, double *rodd) {
   });
 }
 
+int mfe_download_forces(mfe_engine *e, double *forceo) {
+  if (!e || !forceo) return MFE_E_BAD_PARAM;
+  if (!e->have_particles) return fail(e, MFE_E_STATE, "mfe_set_particles first");
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipMemcpyAsync(forceo, e->forceo, 3 * (size_t)e->np * sizeof(double),
+                               hipMemcpyDeviceToHost, e->stream));
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    return MFE_OK;
+  });
+}
+
 int mfe_build_list(mfe_engine *e) {
   if (!e) return MFE_E_BAD_PARAM;
   int rc = check_ready(e);
   if (rc) return rc;
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
-    mfe::launch_nlist_rebuild(e, false);
+    sync_view(e);
+    mfe::launch_nlist_rebuild(e->grp, false);
     int r = read_flags(e);
     e->list_valid = (r == MFE_OK);
     return r;
@@ -720,7 +931,8 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo, mfe_energy *energ
     MFE_HIP_TRY(hipSetDevice(e->device));
     unsigned f = flags;
     if (energy) f |= MFE_ENERGY | MFE_VIRIAL;
-    launch_utotal(e, f, 0, 0.0);
+    sync_view(e);
+    mfe_launch_utotal(e->grp, f, 0, 0.0, false, false);
     if (forceo)
       MFE_HIP_TRY(hipMemcpyAsync(forceo, e->forceo, 3 * (size_t)e->np * sizeof(double),
                                  hipMemcpyDeviceToHost, e->stream));
@@ -745,16 +957,19 @@ int mfe_step_md(mfe_engine *e, int32_t nsteps, double tstep, uint32_t flags) {
     MFE_HIP_TRY(hipSetDevice(e->device));
     // one step is captured once into a graph and replayed; the rebuild
     // kernels read the device-side skin-test flag, so the graph is static
-    if (e->graph_dirty || !e->md_graph || e->md_graph_dt != tstep || e->md_graph_flags != flags) {
+    sync_view(e);
+    if (e->graph_dirty || !e->md_graph || e->md_graph_kind != 1 || e->md_graph_dt != tstep ||
+        e->md_graph_flags != flags) {
       if (e->md_graph) (void)hipGraphExecDestroy(e->md_graph);
       e->md_graph = nullptr;
       hipGraph_t g;
       MFE_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      launch_md_step(e, tstep, flags);
+      mfe_launch_md_step(e->grp, tstep, flags);
       MFE_HIP_TRY(hipStreamEndCapture(e->stream, &g));
       hipError_t ie = hipGraphInstantiate(&e->md_graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       MFE_HIP_TRY(ie);
+      e->md_graph_kind = 1;
       e->md_graph_dt = tstep;
       e->md_graph_flags = flags;
       e->graph_dirty = false;
@@ -765,9 +980,28 @@ int mfe_step_md(mfe_engine *e, int32_t nsteps, double tstep, uint32_t flags) {
   });
 }
 
+int mfe_set_bd(mfe_engine *e, const double *fac1, const double *fac2, uint64_t seed) {
+  if (!e || !fac1 || !fac2) return MFE_E_BAD_PARAM;
+  if (!e->have_particles) return fail(e, MFE_E_STATE, "mfe_set_particles first");
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    upload(e->bd_fac1, fac1, (size_t)e->npt, e->stream);
+    upload(e->bd_fac2, fac2, (size_t)e->npt, e->stream);
+    // restart the counter-based stream: step counter flags[3] = 0
+    mfe::launch_fill_int(e->flags + 3, 1, 0, e->stream);
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->bd_seed = seed;
+    e->have_bd = true;
+    touch(e);
+    return MFE_OK;
+  });
+}
+
 int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1, const double *fac2,
                 const double *noise, uint32_t flags) {
-  if (!e || nsteps < 0 || !fac1 || !fac2 || !noise) return MFE_E_BAD_PARAM;
+  if (!e || nsteps < 0) return MFE_E_BAD_PARAM;
+  if ((fac1 == nullptr) != (fac2 == nullptr)) return fail(e, MFE_E_BAD_PARAM, "fac1 and fac2 go together");
+  if (!fac1 && !e->have_bd) return fail(e, MFE_E_STATE, "BD factors not set (mfe_set_bd)");
   int rc = check_ready(e);
   if (rc) return rc;
   if (!e->have_ewald) return fail(e, MFE_E_STATE, "mfe_set_ewald not called");
@@ -777,26 +1011,154 @@ int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1, const double 
   }
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
-    double *d_f1 = nullptr, *d_f2 = nullptr, *d_noise = nullptr;
-    dalloc(d_f1, (size_t)e->npt);
-    dalloc(d_f2, (size_t)e->npt);
-    dalloc(d_noise, 3 * (size_t)e->np);
-    upload(d_f1, fac1, (size_t)e->npt, e->stream);
-    upload(d_f2, fac2, (size_t)e->npt, e->stream);
-    int r = MFE_OK;
-    for (int s = 0; s < nsteps && r == MFE_OK; ++s) {
-      upload(d_noise, noise + 3 * (size_t)e->np * s, 3 * (size_t)e->np, e->stream);
-      mfe::launch_move(e, 0.0, 1, d_f1, d_f2, d_noise);
-      mfe::launch_skin(e);
-      launch_utotal(e, flags, 0, 0.0, true);
-      mfe::launch_nlist_rebuild(e, true);
+    if (fac1) {
+      upload(e->bd_fac1, fac1, (size_t)e->npt, e->stream);
+      upload(e->bd_fac2, fac2, (size_t)e->npt, e->stream);
+      e->have_bd = true;
+    }
+    if (noise && !e->bd_noise) {
+      dalloc(e->bd_noise, 3 * (size_t)e->max_np);
+      e->sys_dirty = true;
+      e->graph_dirty = true;
+      ++e->version;
+    }
+    sync_view(e);
+    if (noise) {
+      // BDStep with the host's deviates (one upload per step)
+      for (int s2 = 0; s2 < nsteps; ++s2) {
+        upload(e->bd_noise, noise + 3 * (size_t)e->np * s2, 3 * (size_t)e->np, e->stream);
+        mfe_launch_bd_step(e->grp, 1, flags);
+      }
+    } else {
+      // counter-based deviates on the device: the step is a static graph
+      if (e->graph_dirty || !e->md_graph || e->md_graph_kind != 2 || e->md_graph_flags != flags) {
+        if (e->md_graph) (void)hipGraphExecDestroy(e->md_graph);
+        e->md_graph = nullptr;
+        hipGraph_t g;
+        MFE_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        mfe_launch_bd_step(e->grp, 2, flags);
+        MFE_HIP_TRY(hipStreamEndCapture(e->stream, &g));
+        hipError_t ie = hipGraphInstantiate(&e->md_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        MFE_HIP_TRY(ie);
+        e->md_graph_kind = 2;
+        e->md_graph_flags = flags;
+        e->graph_dirty = false;
+      }
+      for (int s2 = 0; s2 < nsteps; ++s2) MFE_HIP_TRY(hipGraphLaunch(e->md_graph, e->stream));
     }
     fetch_energy(e);
-    r = read_flags(e);
-    dfree(d_f1);
-    dfree(d_f2);
-    dfree(d_noise);
-    return r;
+    return read_flags(e);
+  });
+}
+
+int mfe_set_pair_params(mfe_engine *e, const mfe_pot_1_6_12 *spec) {
+  if (!e || !spec || !spec->zat || !spec->sigat || !spec->epsat) return MFE_E_BAD_PARAM;
+  if (!e->have_particles) return fail(e, MFE_E_STATE, "mfe_set_particles first");
+  if (spec->npt != e->npt) return fail(e, MFE_E_BAD_PARAM, "spec npt differs from the particles' npt");
+  for (int i = 0; i < e->np; ++i)
+    if (e->h_q[i] != spec->zat[e->h_type[i]])
+      return fail(e, MFE_E_BAD_PARAM, "az differs from zat(iptpn): the adjoint needs q = zat");
+  mfe::PairParams &pp = e->pp;
+  pp = mfe::PairParams{};
+  for (int t = 0; t < e->npt; ++t) {
+    pp.zat[t] = spec->zat[t];
+    pp.sigat[t] = spec->sigat[t];
+    pp.epsat[t] = spec->epsat[t];
+  }
+  pp.efp = spec->epsi_four_pi;
+  pp.alpha = spec->alpha;
+  pp.lewald = spec->lewald ? 1 : 0;
+  pp.lljcut = spec->lljcut ? 1 : 0;
+  pp.ljrcut = spec->ljrcut;
+  pp.ljushift = spec->ljushift;
+  e->have_pp = true;
+  touch(e);
+  return MFE_OK;
+}
+
+int mfe_set_knots(mfe_engine *e, int32_t nknot, double r_lo, double r_hi, const double *c) {
+  if (!e) return MFE_E_BAD_PARAM;
+  if (!e->have_particles) return fail(e, MFE_E_STATE, "mfe_set_particles first");
+  if (nknot == 0) {
+    e->nknot = 0;
+    touch(e);
+    return MFE_OK;
+  }
+  if (nknot < 4 || nknot > 1024 || !c || !(r_hi > r_lo) || !(r_lo >= 0.0))
+    return fail(e, MFE_E_BAD_PARAM, "knots: need 4 <= nknot <= 1024, 0 <= r_lo < r_hi, coefficients");
+  return guarded(e, [&]() -> int {
+    const size_t n = (size_t)e->nptpt * nknot;
+    if (e->nknot != nknot || !e->knot_c) dalloc(e->knot_c, n);
+    upload(e->knot_c, c, n, e->stream);
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->nknot = nknot;
+    e->knot_r0 = r_lo;
+    e->knot_dinv = (nknot - 3) / (r_hi - r_lo);
+    touch(e);
+    return MFE_OK;
+  });
+}
+
+int mfe_adjoint(mfe_engine *e, double *dE_dq, double *dE_dzat, double *dE_dsigat,
+                double *dE_depsat, double *dE_dknot, double *dE_dro, double *energy) {
+  if (!e) return MFE_E_BAD_PARAM;
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (!e->have_ewald) return fail(e, MFE_E_STATE, "mfe_set_ewald not called");
+  if (!e->have_pp) return fail(e, MFE_E_STATE, "mfe_set_pair_params not called");
+  if (e->ewald == MFE_EWALD_STD)
+    return fail(e, MFE_E_BAD_PARAM, "the adjoint covers SPME and no-Ewald systems");
+  if (!e->list_valid) {
+    rc = mfe_build_list(e);
+    if (rc) return rc;
+  }
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    const int ncol = 1 + 2 * e->npt + e->nptpt * e->nknot;
+    const int nb = (e->np + 255) / 256;
+    if (!e->adj_phi || e->adj_cap < nb * ncol) {
+      dalloc(e->phi_rec, (size_t)e->max_np);
+      dalloc(e->adj_phi, (size_t)e->max_np);
+      dalloc(e->adj_dq, (size_t)e->max_np);
+      dalloc(e->adj_part, (size_t)nb * ncol);
+      dalloc(e->adj_out, (size_t)ncol);
+      e->adj_cap = nb * ncol;
+      touch(e);
+    }
+    sync_view(e);
+    const mfe::Group &G = e->grp;
+    const unsigned f = MFE_FORCES | MFE_ENERGY | MFE_VIRIAL;
+    mfe::launch_pair(G, true);
+    if (G.ewald == MFE_EWALD_SPME) mfe::launch_spme(G, true, SP_BIN, SP_GATHER, true);
+    if (G.chains) mfe::launch_bonds(G);
+    if (G.surf) mfe::launch_surf(G);
+    mfe::launch_reduce(G, f, false, false);
+    mfe::launch_finalize(G, 0, 0.0);
+    mfe::launch_adjoint(G, ncol, e->adj_out);
+    std::vector<double> out(ncol), dq(e->np), fo(3 * (size_t)e->np);
+    MFE_HIP_TRY(hipMemcpyAsync(out.data(), e->adj_out, ncol * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    MFE_HIP_TRY(hipMemcpyAsync(dq.data(), e->adj_dq, e->np * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    MFE_HIP_TRY(hipMemcpyAsync(fo.data(), e->forceo, fo.size() * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    fetch_energy(e);
+    int r = read_flags(e);
+    if (r) return r;
+    if (dE_dq) std::copy(dq.begin(), dq.end(), dE_dq);
+    if (dE_dzat) {
+      for (int t = 0; t < e->npt; ++t) dE_dzat[t] = 0.0;
+      for (int i = 0; i < e->np; ++i) dE_dzat[e->h_type[i]] += dq[i];
+    }
+    if (dE_dsigat) for (int t = 0; t < e->npt; ++t) dE_dsigat[t] = out[1 + t];
+    if (dE_depsat) for (int t = 0; t < e->npt; ++t) dE_depsat[t] = out[1 + e->npt + t];
+    if (dE_dknot) std::copy(out.begin() + 1 + 2 * e->npt, out.end(), dE_dknot);
+    if (dE_dro) {
+      // position adjoint: dE/dr = -force (UTotal's force before the unit scaling)
+      const double ffac = e->h_sys.ffac;
+      for (size_t k = 0; k < fo.size(); ++k) dE_dro[k] = -fo[k] / ffac;
+    }
+    // the model energy: exact pair sum + reciprocal (with self/surface) + bonds
+    if (energy) *energy = out[0] + e->h_en[2] + e->h_en[3];
+    return MFE_OK;
   });
 }
 
@@ -859,22 +1221,24 @@ int mfe_time_stage(mfe_engine *e, int32_t stage, int32_t nrep, double *ms) {
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
     // one full evaluation first so every stage has valid inputs
-    launch_utotal(e, MFE_ENERGY, 0, 0.0);
+    sync_view(e);
+    const mfe::Group &G = e->grp;
+    mfe_launch_utotal(G, MFE_ENERGY, 0, 0.0, false, false);
     hipEvent_t a, b;
     MFE_HIP_TRY(hipEventCreate(&a));
     MFE_HIP_TRY(hipEventCreate(&b));
     MFE_HIP_TRY(hipEventRecord(a, e->stream));
     for (int r = 0; r < nrep; ++r) {
       switch (stage) {
-        case MFE_STAGE_PAIR: mfe::launch_pair(e, true); break;
-        case MFE_STAGE_SPREAD: mfe::launch_spme(e, true, SP_BIN, SP_SPREAD); break;
-        case MFE_STAGE_FFT_X: mfe::launch_spme(e, true, SP_FFTX, SP_FFTX); break;
-        case MFE_STAGE_FFT_Y: mfe::launch_spme(e, true, SP_FFTY, SP_FFTY); break;
-        case MFE_STAGE_CONV: mfe::launch_spme(e, true, SP_CONV, SP_CONV); break;
-        case MFE_STAGE_IFFT: mfe::launch_spme(e, true, SP_IFFTY, SP_IFFTX); break;
-        case MFE_STAGE_GATHER: mfe::launch_spme(e, true, SP_GATHER, SP_GATHER); break;
-        case MFE_STAGE_NLIST: mfe::launch_nlist_rebuild(e, false); break;
-        case MFE_STAGE_STEP: launch_utotal(e, MFE_ENERGY, 0, 0.0); break;
+        case MFE_STAGE_PAIR: mfe::launch_pair(G, true); break;
+        case MFE_STAGE_SPREAD: mfe::launch_spme(G, true, SP_BIN, SP_SPREAD); break;
+        case MFE_STAGE_FFT_X: mfe::launch_spme(G, true, SP_FFTX, SP_FFTX); break;
+        case MFE_STAGE_FFT_Y: mfe::launch_spme(G, true, SP_FFTY, SP_FFTY); break;
+        case MFE_STAGE_CONV: mfe::launch_spme(G, true, SP_CONV, SP_CONV); break;
+        case MFE_STAGE_IFFT: mfe::launch_spme(G, true, SP_IFFTY, SP_IFFTX); break;
+        case MFE_STAGE_GATHER: mfe::launch_spme(G, true, SP_GATHER, SP_GATHER); break;
+        case MFE_STAGE_NLIST: mfe::launch_nlist_rebuild(G, false); break;
+        case MFE_STAGE_STEP: mfe_launch_utotal(G, MFE_ENERGY, 0, 0.0, false, false); break;
       }
     }
     MFE_HIP_TRY(hipEventRecord(b, e->stream));
@@ -885,7 +1249,7 @@ int mfe_time_stage(mfe_engine *e, int32_t stage, int32_t nrep, double *ms) {
     (void)hipEventDestroy(b);
     *ms = (double)t / nrep;
     // leave a consistent state (the stage runs may have transformed meshes)
-    launch_utotal(e, MFE_ENERGY, 0, 0.0);
+    mfe_launch_utotal(G, MFE_ENERGY, 0, 0.0, false, false);
     return read_flags(e);
   });
 }

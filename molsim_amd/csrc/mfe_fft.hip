@@ -166,10 +166,15 @@ __device__ cplx<R> *lds_fft(cplx<R> *a, cplx<R> *b, const FftPlan &p,
 // r2c of rows of length s1 = 2m: z[k] = x[2k] + i x[2k+1], Z = FFT_m(z),
 // X[k] = (Z[k] + conj(Z[m-k]))/2 - i/2 (Z[k] - conj(Z[m-k])) W^k.
 template <typename R>
-__global__ void __launch_bounds__(256) k_fft_x_fwd(
-    const R *__restrict__ real, cplx<R> *__restrict__ cm, int s1, int nrows,
-    int rows_per_blk, FftPlan ph, const cplx<R> *__restrict__ tw_half,
-    const cplx<R> *__restrict__ tw_full) {
+__global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.fx_nblk) return;
+  const R *__restrict__ real = (const R *)V.mesh_r;
+  cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
+  const int s1 = V.sg.s[0], nrows = V.sg.s[1] * V.sg.s[2], rows_per_blk = V.fx_rpb;
+  const FftPlan &ph = V.plan_half;
+  const cplx<R> *__restrict__ tw_half = (const cplx<R> *)V.tw_half;
+  const cplx<R> *__restrict__ tw_full = (const cplx<R> *)V.tw[0];
   extern __shared__ char smem[];
   const int m = s1 / 2, hx = m + 1, ld = m + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
@@ -199,10 +204,15 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(
 // Z[k] = (X[k] + conj(X[m-k])) + i (X[k] - conj(X[m-k])) W^-k, z = IFFT_m(Z),
 // x[2j] = Re z[j], x[2j+1] = Im z[j] (unnormalised).
 template <typename R>
-__global__ void __launch_bounds__(256) k_fft_x_inv(
-    const cplx<R> *__restrict__ cm, R *__restrict__ real, int s1, int nrows,
-    int rows_per_blk, FftPlan ph, const cplx<R> *__restrict__ tw_half,
-    const cplx<R> *__restrict__ tw_full) {
+__global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.fx_nblk) return;
+  const cplx<R> *__restrict__ cm = (const cplx<R> *)V.mesh_c;
+  R *__restrict__ real = (R *)V.mesh_r;
+  const int s1 = V.sg.s[0], nrows = V.sg.s[1] * V.sg.s[2], rows_per_blk = V.fx_rpb;
+  const FftPlan &ph = V.plan_half;
+  const cplx<R> *__restrict__ tw_half = (const cplx<R> *)V.tw_half;
+  const cplx<R> *__restrict__ tw_full = (const cplx<R> *)V.tw[0];
   extern __shared__ char smem[];
   const int m = s1 / 2, hx = m + 1, ld = m + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
@@ -235,11 +245,13 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(
 
 // ----------------------------------------------------------------- y pass
 template <typename R>
-__global__ void __launch_bounds__(256) k_fft_y(cplx<R> *__restrict__ cm,
-                                               int s2, int hx, int T,
-                                               FftPlan py,
-                                               const cplx<R> *__restrict__ tw,
-                                               int inv) {
+__global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, int inv) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.fy_nbx || (int)blockIdx.y >= V.sg.s[2]) return;
+  cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
+  const int s2 = V.sg.s[1], hx = V.hx, T = V.fy_T;
+  const FftPlan &py = V.plan[1];
+  const cplx<R> *__restrict__ tw = (const cplx<R> *)V.tw[1];
   extern __shared__ char smem[];
   const int ld = s2 + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
@@ -267,10 +279,15 @@ __global__ void __launch_bounds__(256) k_fft_y(cplx<R> *__restrict__ cm,
 // nx = kx, ny = min(ky, s2-ky), nz = min(kz, s3-kz); the virial uses
 // (1 + 2 k^2 expf) on top (Src/energy.F90:6553-6578, 6994-7018).
 template <typename R>
-__global__ void __launch_bounds__(256) k_fft_conv(
-    cplx<R> *__restrict__ cm, int s1, int s2, int s3, int hx, int T,
-    FftPlan pz, const cplx<R> *__restrict__ tw, SpmeAxes ax, int energy,
-    double *__restrict__ part) {
+__global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys, int energy) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.fz_nbx || (int)blockIdx.y >= V.sg.s[1]) return;
+  cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
+  const int s1 = V.sg.s[0], s2 = V.sg.s[1], s3 = V.sg.s[2], hx = V.hx, T = V.fz_T;
+  const FftPlan &pz = V.plan[2];
+  const cplx<R> *__restrict__ tw = (const cplx<R> *)V.tw[2];
+  const SpmeAxes &ax = V.ax;
+  double *__restrict__ part = V.part + V.off_conv;
   extern __shared__ char smem[];
   const int ld = s3 + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
@@ -325,9 +342,9 @@ __global__ void __launch_bounds__(256) k_fft_conv(
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      const int blk = blockIdx.y * gridDim.x + blockIdx.x;
-      part[2 * blk] = red[0][0];
-      part[2 * blk + 1] = red[1][0];
+      const int blk = blockIdx.y * V.fz_nbx + blockIdx.x;
+      part[blk] = red[0][0];
+      part[V.nblk_conv + blk] = red[1][0];
     }
   }
 }
@@ -352,80 +369,55 @@ int tile_target(int n) {
   return t < 1 ? 1 : (t > 16 ? 16 : t);
 }
 
-template <typename R>
-void x_fwd(mfe_engine *e) {
-  const int s1 = e->s[0], nrows = e->s[1] * e->s[2];
-  const int rpb = rows_per_block<R>(s1);
-  const size_t lds = 2 * (size_t)rpb * (s1 / 2 + 1) * sizeof(cplx<R>);
-  hipLaunchKernelGGL(k_fft_x_fwd<R>, dim3((nrows + rpb - 1) / rpb), dim3(256),
-                     lds, e->stream, (const R *)e->mesh_r, (cplx<R> *)e->mesh_c,
-                     s1, nrows, rpb, e->plan_half, (const cplx<R> *)e->tw_half,
-                     (const cplx<R> *)e->tw[0]);
-}
-
-template <typename R>
-void x_inv(mfe_engine *e) {
-  const int s1 = e->s[0], nrows = e->s[1] * e->s[2];
-  const int rpb = rows_per_block<R>(s1);
-  const size_t lds = 2 * (size_t)rpb * (s1 / 2 + 1) * sizeof(cplx<R>);
-  hipLaunchKernelGGL(k_fft_x_inv<R>, dim3((nrows + rpb - 1) / rpb), dim3(256),
-                     lds, e->stream, (const cplx<R> *)e->mesh_c, (R *)e->mesh_r,
-                     s1, nrows, rpb, e->plan_half, (const cplx<R> *)e->tw_half,
-                     (const cplx<R> *)e->tw[0]);
-}
-
-template <typename R>
-void y_pass(mfe_engine *e, int inv) {
-  const int s2 = e->s[1];
-  const int T = tile_width(e->hx, tile_target<R>(s2));
-  const size_t lds = 2 * (size_t)T * (s2 + 1) * sizeof(cplx<R>);
-  dim3 grid((e->hx + T - 1) / T, e->s[2]);
-  hipLaunchKernelGGL(k_fft_y<R>, grid, dim3(256), lds, e->stream,
-                     (cplx<R> *)e->mesh_c, s2, e->hx, T, e->plan[1],
-                     (const cplx<R> *)e->tw[1], inv);
-}
-
-template <typename R>
-void conv(mfe_engine *e, bool energy) {
-  const int s3 = e->s[2];
-  const int T = tile_width(e->hx, tile_target<R>(s3));
-  const size_t lds = 2 * (size_t)T * (s3 + 1) * sizeof(cplx<R>);
-  dim3 grid((e->hx + T - 1) / T, e->s[1]);
-  const int smax2 = (e->s[0] > e->s[1] ? (e->s[0] > e->s[2] ? e->s[0] : e->s[2])
-                                       : (e->s[1] > e->s[2] ? e->s[1] : e->s[2])) / 2 + 1;
-  SpmeAxes ax;
-  for (int a = 0; a < 3; ++a) {
-    ax.g[a] = e->ax_g + a * smax2;
-    ax.kk[a] = e->ax_kk + a * smax2;
-  }
-  const double Pi = 3.1415926535897932;
-  ax.fac0 = 4.0 * Pi / (e->box.L[0] * e->box.L[1] * e->box.L[2]);
-  ax.expf = -1.0 / (4.0 * e->alpha * e->alpha);
-  hipLaunchKernelGGL(k_fft_conv<R>, grid, dim3(256), lds, e->stream,
-                     (cplx<R> *)e->mesh_c, e->s[0], e->s[1], s3, e->hx, T,
-                     e->plan[2], (const cplx<R> *)e->tw[2], ax, energy ? 1 : 0,
-                     e->part + e->off_conv);
-}
-
 }  // namespace
 
-int conv_blocks(mfe_engine *e) {
-  const int T = e->prec == MFE_PREC_MIXED ? tile_width(e->hx, tile_target<float>(e->s[2]))
-                                          : tile_width(e->hx, tile_target<double>(e->s[2]));
-  return ((e->hx + T - 1) / T) * e->s[1];
+void fft_geometry(int prec, const int s[3], int *fx_rpb, int *fx_nblk, int *fy_T, int *fy_nbx,
+                  int *fz_T, int *fz_nbx, size_t *lds_fx, size_t *lds_fy, size_t *lds_fz) {
+  const bool f = prec == MFE_PREC_MIXED;
+  const size_t csz = f ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  const int hx = s[0] / 2 + 1;
+  const int rpb = f ? rows_per_block<float>(s[0]) : rows_per_block<double>(s[0]);
+  *fx_rpb = rpb;
+  *fx_nblk = (s[1] * s[2] + rpb - 1) / rpb;
+  *lds_fx = 2 * (size_t)rpb * (s[0] / 2 + 1) * csz;
+  const int Ty = tile_width(hx, f ? tile_target<float>(s[1]) : tile_target<double>(s[1]));
+  *fy_T = Ty;
+  *fy_nbx = (hx + Ty - 1) / Ty;
+  *lds_fy = 2 * (size_t)Ty * (s[1] + 1) * csz;
+  const int Tz = tile_width(hx, f ? tile_target<float>(s[2]) : tile_target<double>(s[2]));
+  *fz_T = Tz;
+  *fz_nbx = (hx + Tz - 1) / Tz;
+  *lds_fz = 2 * (size_t)Tz * (s[2] + 1) * csz;
 }
 
-void fft_x_forward(mfe_engine *e) {
-  if (e->prec == MFE_PREC_MIXED) x_fwd<float>(e); else x_fwd<double>(e);
+void fft_x_forward(const Group &g) {
+  const dim3 gr(g.fx_blocks, 1, g.R);
+  if (g.prec == MFE_PREC_MIXED)
+    hipLaunchKernelGGL(k_fft_x_fwd<float>, gr, dim3(256), g.lds_fx, g.stream, g.d);
+  else
+    hipLaunchKernelGGL(k_fft_x_fwd<double>, gr, dim3(256), g.lds_fx, g.stream, g.d);
 }
-void fft_x_inverse(mfe_engine *e) {
-  if (e->prec == MFE_PREC_MIXED) x_inv<float>(e); else x_inv<double>(e);
+void fft_x_inverse(const Group &g) {
+  const dim3 gr(g.fx_blocks, 1, g.R);
+  if (g.prec == MFE_PREC_MIXED)
+    hipLaunchKernelGGL(k_fft_x_inv<float>, gr, dim3(256), g.lds_fx, g.stream, g.d);
+  else
+    hipLaunchKernelGGL(k_fft_x_inv<double>, gr, dim3(256), g.lds_fx, g.stream, g.d);
 }
-void fft_y(mfe_engine *e, int inverse) {
-  if (e->prec == MFE_PREC_MIXED) y_pass<float>(e, inverse); else y_pass<double>(e, inverse);
+void fft_y(const Group &g, int inverse) {
+  const dim3 gr(g.fy_x, g.fy_y, g.R);
+  if (g.prec == MFE_PREC_MIXED)
+    hipLaunchKernelGGL(k_fft_y<float>, gr, dim3(256), g.lds_fy, g.stream, g.d, inverse);
+  else
+    hipLaunchKernelGGL(k_fft_y<double>, gr, dim3(256), g.lds_fy, g.stream, g.d, inverse);
 }
-void fft_conv(mfe_engine *e, bool energy) {
-  if (e->prec == MFE_PREC_MIXED) conv<float>(e, energy); else conv<double>(e, energy);
+void fft_conv(const Group &g, bool energy) {
+  const dim3 gr(g.fz_x, g.fz_y, g.R);
+  if (g.prec == MFE_PREC_MIXED)
+    hipLaunchKernelGGL(k_fft_conv<float>, gr, dim3(256), g.lds_fz, g.stream, g.d, energy ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_fft_conv<double>, gr, dim3(256), g.lds_fz, g.stream, g.d,
+                       energy ? 1 : 0);
 }
 
 }  // namespace mfe

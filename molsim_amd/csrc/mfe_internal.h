@@ -87,7 +87,103 @@ struct SpmeAxes {
   double expf;   // -1/(4 alpha^2)
 };
 
-struct Partials;  // forward
+// Parameters of Pot_1_6_12 (Src/potential.F90:1349-1425) that the adjoint
+// differentiates: per-type charge, sigma and epsilon, Lorentz-Berthelot
+// mixing, optional Ewald damping and WCA cut.
+struct PairParams {
+  double zat[MFE_MAX_TYPES], sigat[MFE_MAX_TYPES], epsat[MFE_MAX_TYPES];
+  double efp, alpha, ljrcut, ljushift;
+  int lewald, lljcut;
+};
+
+// SPME mesh geometry: mesh s, spreading bricks B (nb per axis), dkdr = s/L.
+struct SpmeGeom {
+  int s[3], B[3], nb[3], order;
+  double dkdr[3];
+};
+
+// Device-side view of one engine (one replica): every kernel reads the
+// view of its replica, blockIdx.z, so a single engine and a batch of
+// replicas run the same kernels -- a single engine is a batch of one.
+// Per-replica block counts bound the work of each launch, whose grid is
+// the maximum over the replicas.
+struct SysDev {
+  int np, npt, nptpt, nlsem;   // nlsem: 0 = vlist (r2 <= rcut2), 1 = cell list
+  Box box;
+  double *ro, *rod, *rodd, *force, *forceo, *drosum, *r2disp;
+  int *type;
+  double *q, *massip;
+  TableDev tab;
+  // neighbour list
+  double rs2, drnlist;
+  int maxnn;
+  CellGeom cg;
+  int *c_count, *c_start, *c_of, *c_slot, *c_sorted;
+  int *c_pos;          // inverse of c_sorted: particle -> cell-order position
+  double *ro_s;
+  int *type_s, *nn, *nbr;
+  int *flags;        // [0] error bits, [1] rebuild flag, [2] rebuilds, [3] BD step
+  // Ewald
+  int ewald, lsurf;
+  double alpha, efp, self_energy, vol;
+  // SPME
+  SpmeGeom sg;
+  int hx, nbricks;
+  void *mesh_r, *mesh_c;
+  int *b_count, *b_start, *b_id, *b_slot;
+  void *rec_frac;
+  int2 *rec_meta;
+  SpmeAxes ax;
+  FftPlan plan[3], plan_half;
+  const void *tw[3], *tw_half;
+  int fx_rpb, fx_nblk;   // x pass: rows per block, blocks
+  int fy_T, fy_nbx;      // y pass: kx tile, tiles (grid y = s3)
+  int fz_T, fz_nbx;      // conv pass: kx tile, tiles (grid y = s2)
+  // standard Ewald
+  const double *k_vec;
+  double *k_sum;
+  int nkvec;
+  // chains
+  int nchain, bond_p;
+  const int *ch_prev, *ch_next;
+  double bond_k, bond_eq;
+  // Brownian dynamics: fac1/fac2 per type, noise (host-supplied) or the
+  // counter-based generator keyed by seed
+  const double *bd_fac1, *bd_fac2;
+  const double *noise;
+  unsigned long long seed;
+  // additive cubic B-spline pair term on uniform knots in r (per pair
+  // type, nknot coefficients, r in [knot_r0, knot_r0 + (nknot-3)/knot_dinv))
+  int nknot;
+  double knot_r0, knot_dinv;
+  const double *knot_c;
+  // adjoint
+  PairParams pp;
+  double *phi_rec;    // efp * reciprocal potential per particle (gather)
+  double *adj_phi;    // real-space dE/dq per particle
+  double *adj_dq;     // dE/dq per particle
+  double *adj_part;   // per-block partials [blocks][adj_ncol]
+  int adj_ncol;
+  // reductions and output scaling
+  double *part, *en;
+  int off_pair, off_conv, off_bond, off_surf, off_std, off_move, off_skin;
+  int nblk_np, nblk_pair, nblk_conv;
+  double ffac, surf_fac;
+};
+
+// Replicas launched together: same device, precision, Ewald kind and
+// SPME order (a template parameter of the stencil kernels).
+struct Group {
+  int R = 0;
+  const SysDev *d = nullptr;  // device array [R]
+  hipStream_t stream = nullptr;
+  int prec = 0, ewald = 0, order = 0;
+  bool chains = false, surf = false, cells = false;
+  // maxima over the replicas of each launch's per-replica block counts
+  int np_blocks = 0, pair_blocks = 0, cell_blocks = 0, bricks = 0, kvec = 0;
+  int fx_blocks = 0, fy_x = 0, fy_y = 0, fz_x = 0, fz_y = 0;
+  size_t lds_spread = 0, lds_gather = 0, lds_fx = 0, lds_fy = 0, lds_fz = 0;
+};
 
 }  // namespace mfe
 
@@ -133,6 +229,7 @@ struct mfe_engine {
   mfe::CellGeom cg{};
   int *c_count = nullptr, *c_start = nullptr, *c_of = nullptr, *c_slot = nullptr;
   int *c_sorted = nullptr;   // cell order: position t -> particle index
+  int *c_pos = nullptr;      // particle index -> cell-order position
   double *ro_s = nullptr;    // positions in cell order (refreshed per UTotal)
   int *type_s = nullptr;
   int *nn = nullptr, *nbr = nullptr;  // by cell-order position
@@ -177,41 +274,92 @@ struct mfe_engine {
   int off_move = 0, off_skin = 0;
   int nblk_pair = 0, nblk_conv = 0, nblk_bond = 0, nblk_surf = 0, nblk_move = 0;
 
+  // additive spline pair term (knots) and adjoint state
+  int nknot = 0;
+  double knot_r0 = 0.0, knot_dinv = 0.0;
+  double *knot_c = nullptr;  // [nptpt][nknot]
+  mfe::PairParams pp{};
+  bool have_pp = false;
+  double *phi_rec = nullptr, *adj_phi = nullptr, *adj_dq = nullptr;
+  double *adj_part = nullptr, *adj_out = nullptr;
+  int adj_cap = 0;
+
+  // Brownian dynamics
+  double *bd_fac1 = nullptr, *bd_fac2 = nullptr;  // [MFE_MAX_TYPES]
+  double *bd_noise = nullptr;                     // [3 np] host-supplied noise
+  unsigned long long bd_seed = 0;
+  bool have_bd = false;
+
+  // device view (one replica) and its launch group
+  mfe::SysDev h_sys{};
+  mfe::SysDev *d_sys = nullptr;
+  mfe::Group grp{};
+  bool sys_dirty = true;
+  unsigned version = 1;      // bumped whenever the view changes
+
   // MD bookkeeping
   hipGraphExec_t md_graph = nullptr;
   double md_graph_dt = -1.0;
   unsigned md_graph_flags = 0;
+  int md_graph_kind = 0;     // 1 = MD, 2 = BD
   bool graph_dirty = true;
 };
 
 namespace mfe {
-// launchers (mfe_kernels.hip / mfe_fft.hip)
-void launch_nlist_rebuild(mfe_engine *e, bool gated);
-void launch_pair(mfe_engine *e, bool energy);
-void launch_spme(mfe_engine *e, bool energy, int first_stage, int last_stage);
-void launch_std_ewald(mfe_engine *e);
-void launch_bonds(mfe_engine *e);
-void launch_surf(mfe_engine *e);
-void launch_reduce(mfe_engine *e, unsigned flags, bool skin);
-void launch_finalize(mfe_engine *e, int mode, double dt);
-void launch_move(mfe_engine *e, double dt, int bd, const double *fac1,
-                 const double *fac2, const double *noise);
-void launch_skin(mfe_engine *e);
+// launchers (mfe_kernels.hip / mfe_fft.hip); every launch covers all
+// replicas of the group
+void launch_nlist_rebuild(const Group &g, bool gated);
+// sort_pos: refresh the cell-ordered positions first (not needed inside MD
+// and BD steps, whose k_move keeps them current)
+void launch_pair(const Group &g, bool energy, bool sort_pos = true);
+void launch_spme(const Group &g, bool energy, int first_stage, int last_stage,
+                 bool potential = false);
+// adjoint passes: real-space pair sums, per-particle dE/dq, column sums
+void launch_adjoint(const Group &g, int ncol, double *out);
+void launch_std_ewald(const Group &g);
+void launch_bonds(const Group &g);
+void launch_surf(const Group &g);
+void launch_reduce(const Group &g, unsigned flags, bool skin, bool bd);
+void launch_finalize(const Group &g, int mode, double dt);
+void launch_move(const Group &g, double dt, int bd);
+void launch_skin(const Group &g);
+void launch_collect(const Group &g, int *flags, double *en, int nen);
 void launch_fill_int(int *p, int n, int v, hipStream_t s);
 void launch_fill_double(double *p, int n, double v, hipStream_t s);
 
-int conv_blocks(mfe_engine *e);
+// per-replica launch geometry of the FFT passes (mfe_fft.hip)
+void fft_geometry(int prec, const int s[3], int *fx_rpb, int *fx_nblk, int *fy_T,
+                  int *fy_nbx, int *fz_T, int *fz_nbx, size_t *lds_fx, size_t *lds_fy,
+                  size_t *lds_fz);
 template <typename R, int O>
 const void *k_spread_ptr();
 template <typename R, int O>
 const void *k_gather_ptr();
+template <typename R, int O>
+const void *k_gather_pot_ptr();
 
 // FFT passes (mfe_fft.hip)
-void fft_x_forward(mfe_engine *e);
-void fft_y(mfe_engine *e, int inverse);
-void fft_conv(mfe_engine *e, bool energy);
-void fft_x_inverse(mfe_engine *e);
+void fft_x_forward(const Group &g);
+void fft_y(const Group &g, int inverse);
+void fft_conv(const Group &g, bool energy);
+void fft_x_inverse(const Group &g);
+
+// Philox4x32-10 (Salmon et al., SC'11) on the host, for tests of the
+// device generator
+void philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 }  // namespace mfe
+
+// groups and step sequences (mfe_engine.hip)
+int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStream_t st,
+                    mfe::Group &g, std::string &err);
+void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v);
+void mfe_sync_view(mfe_engine *e);   // throws HipError
+int mfe_map_flag_error(mfe_engine *e, int bits);
+void mfe_allow_big_spread_lds(int prec, int bytes);
+void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt, bool skin,
+                       bool bd);
+void mfe_launch_md_step(const mfe::Group &g, double dt, unsigned flags);
+void mfe_launch_bd_step(const mfe::Group &g, int noise, unsigned flags);
 
 // sizes of SPME stages (stage indices used by launch_spme)
 enum { SP_BIN = 0, SP_SPREAD = 1, SP_FFTX = 2, SP_FFTY = 3, SP_CONV = 4,

@@ -18,9 +18,6 @@
 #include "mfe_internal.h"
 
 namespace mfe {
-
-int conv_blocks(mfe_engine *e);
-
 namespace {
 
 constexpr int kBlock = 256;
@@ -76,30 +73,39 @@ __global__ void k_fill_double(double *p, int n, double v) {
 }
 
 // Exclusive scan of count[0..n) into start[0..n], zeroing count for the
-// next use; one block of 1024 threads, each owning a contiguous chunk.
-__global__ void __launch_bounds__(1024) k_scan(int *count, int *start, int n,
-                                               const int *gate) {
-  if (gate && gate[1] == 0) return;
-  __shared__ int sh[1024];
+// next use; one block of 1024 threads per replica, each thread owning a
+// contiguous chunk; the chunk sums are scanned inside each wavefront with
+// shuffles and across the 16 wavefronts through LDS (one barrier).
+// which: 0 = cells of the neighbour list, 1 = SPME bricks.
+__global__ void __launch_bounds__(1024) k_scan(const SysDev *__restrict__ sys,
+                                               int which, int gated) {
+  const SysDev &V = sys[blockIdx.z];
+  if (gated && V.flags[1] == 0) return;
+  int *count = which ? V.b_count : V.c_count;
+  int *start = which ? V.b_start : V.c_start;
+  const int n = which ? V.nbricks : V.cg.ncells;
+  __shared__ int wsum[16];
   const int per = (n + blockDim.x - 1) / blockDim.x;
   const int lo = threadIdx.x * per, hi = min(n, lo + per);
   int s = 0;
   for (int i = lo; i < hi; ++i) s += count[i];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-    int v = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += v;
-    __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int x = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  int run = sh[threadIdx.x] - s;
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  int wpre = 0;
+  for (int w = 0; w < wave; ++w) wpre += wsum[w];
+  int run = wpre + x - s;
   for (int i = lo; i < hi; ++i) {
     start[i] = run;
     run += count[i];
     count[i] = 0;
   }
-  if (threadIdx.x == blockDim.x - 1) start[n] = sh[threadIdx.x];
+  if (threadIdx.x == blockDim.x - 1) start[n] = wpre + x;
 }
 
 // ------------------------------------------------------- neighbour lists
@@ -114,29 +120,34 @@ __device__ __forceinline__ int cell_of(const double *r, const CellGeom &g,
   return c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
 }
 
-__global__ void k_cell_count(const double *__restrict__ ro, int np, CellGeom g,
-                             Box b, int *count, int *cof, int *slot,
-                             const int *gate) {
-  if (gate && gate[1] == 0) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x) {
+__global__ void k_cell_count(const SysDev *__restrict__ sys, int gated) {
+  const SysDev &V = sys[blockIdx.z];
+  if (gated && V.flags[1] == 0) return;
+  const double *__restrict__ ro = V.ro;
+  const CellGeom g = V.cg;
+  const Box b = V.box;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V.np; i += gridDim.x * blockDim.x) {
     int c3[3];
     const int c = cell_of(ro + 3 * i, g, b, c3);
-    cof[i] = c;
-    slot[i] = atomicAdd(&count[c], 1);
+    V.c_of[i] = c;
+    V.c_slot[i] = atomicAdd(&V.c_count[c], 1);
   }
 }
 
-__global__ void k_cell_scatter(int np, const int *cof, const int *slot,
-                               const int *start, int *sorted, const int *gate) {
-  if (gate && gate[1] == 0) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np; i += gridDim.x * blockDim.x)
-    sorted[start[cof[i]] + slot[i]] = i;
+__global__ void k_cell_scatter(const SysDev *__restrict__ sys, int gated) {
+  const SysDev &V = sys[blockIdx.z];
+  if (gated && V.flags[1] == 0) return;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V.np; i += gridDim.x * blockDim.x)
+    V.c_sorted[V.c_start[V.c_of[i]] + V.c_slot[i]] = i;
 }
 
 // members of each cell in ascending particle index (deterministic lists)
-__global__ void k_cell_sort(int ncells, const int *start, int *sorted,
-                            const int *gate) {
-  if (gate && gate[1] == 0) return;
+__global__ void k_cell_sort(const SysDev *__restrict__ sys, int gated) {
+  const SysDev &V = sys[blockIdx.z];
+  if (gated && V.flags[1] == 0) return;
+  const int ncells = V.cg.ncells;
+  const int *start = V.c_start;
+  int *sorted = V.c_sorted;
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
   const int lo = start[c], hi = start[c + 1];
   for (int i = lo + 1; i < hi; ++i) {
@@ -156,12 +167,23 @@ __global__ void k_cell_sort(int ncells, const int *start, int *sorted,
 // the pair kernel reads neighbouring positions from neighbouring memory;
 // the list is column-major, nbr[k*np + t].  The half list the reference
 // keeps is the checkerboard subset of this set.
-__global__ void __launch_bounds__(kBlock) k_nlist_build(
-    const double *__restrict__ ro, int np, CellGeom g, Box b,
-    const int *__restrict__ cof, const int *__restrict__ start,
-    const int *__restrict__ sorted, double rs2, int maxnn, int *nn,
-    int *__restrict__ nbr, int *flags, double *drosum, const int *gate) {
-  if (gate && gate[1] == 0) return;
+__global__ void __launch_bounds__(kBlock) k_nlist_build(const SysDev *__restrict__ sys,
+                                                        int gated) {
+  const SysDev &V = sys[blockIdx.z];
+  if (gated && V.flags[1] == 0) return;
+  const double *__restrict__ ro = V.ro;
+  const int np = V.np;
+  const CellGeom g = V.cg;
+  const Box b = V.box;
+  const int *__restrict__ cof = V.c_of;
+  const int *__restrict__ start = V.c_start;
+  const int *__restrict__ sorted = V.c_sorted;
+  const double rs2 = V.rs2;
+  const int maxnn = V.maxnn;
+  int *__restrict__ nn = V.nn;
+  int *__restrict__ nbr = V.nbr;
+  int *flags = V.flags;
+  double *drosum = V.drosum;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < np; t += gridDim.x * blockDim.x) {
     const int i = sorted[t];
     const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
@@ -193,26 +215,54 @@ __global__ void __launch_bounds__(kBlock) k_nlist_build(
     // kernel; the call reports MFE_E_NEIGH_CAPACITY
     nn[t] = min(n, maxnn);
     if (n > maxnn) atomicOr(&flags[0], ERR_CAPACITY);
+    // cell-ordered positions and types for the pair kernel; k_move keeps
+    // them current through c_pos until the next rebuild
+    V.c_pos[i] = t;
+    V.ro_s[3 * t] = xi;
+    V.ro_s[3 * t + 1] = yi;
+    V.ro_s[3 * t + 2] = zi;
+    V.type_s[t] = V.type[i];
     if (drosum) {
       drosum[3 * i] = 0.0;
       drosum[3 * i + 1] = 0.0;
       drosum[3 * i + 2] = 0.0;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && gate) flags[2] += 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && gated) flags[2] += 1;
 }
 
 // positions and types in cell order for the pair kernel (every UTotal)
-__global__ void k_sort_pos(const double *__restrict__ ro, const int *__restrict__ type,
-                           const int *__restrict__ sorted, int np,
-                           double *__restrict__ ro_s, int *__restrict__ type_s) {
+__global__ void k_sort_pos(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  const int i = sorted[t];
-  ro_s[3 * t] = ro[3 * i];
-  ro_s[3 * t + 1] = ro[3 * i + 1];
-  ro_s[3 * t + 2] = ro[3 * i + 2];
-  type_s[t] = type[i];
+  if (t >= V.np) return;
+  const int i = V.c_sorted[t];
+  V.ro_s[3 * t] = V.ro[3 * i];
+  V.ro_s[3 * t + 1] = V.ro[3 * i + 1];
+  V.ro_s[3 * t + 2] = V.ro[3 * i + 2];
+  V.type_s[t] = V.type[i];
+}
+
+// ------------------------------------------------- spline pair term
+// Uniform cubic B-spline on knots r0 + k/dinv: on interval i (fraction f)
+// the weights of coefficients i..i+3 are (1-f)^3/6, (3f^3-6f^2+4)/6,
+// (-3f^3+3f^2+3f+1)/6, f^3/6 (de Boor, A Practical Guide to Splines, ch. IX),
+// and their derivatives in f.
+__device__ __forceinline__ bool knot_weights(double r, double r0, double dinv, int nknot,
+                                             int &i, double w[4], double dw[4]) {
+  const double x = (r - r0) * dinv;
+  if (!(x >= 0.0) || !(x < (double)(nknot - 3))) return false;
+  i = (int)x;
+  const double f = x - i, f2 = f * f, f3 = f2 * f, g = 1.0 - f;
+  w[0] = g * g * g / 6.0;
+  w[1] = (3.0 * f3 - 6.0 * f2 + 4.0) / 6.0;
+  w[2] = (-3.0 * f3 + 3.0 * f2 + 3.0 * f + 1.0) / 6.0;
+  w[3] = f3 / 6.0;
+  dw[0] = -0.5 * g * g;
+  dw[1] = 1.5 * f2 - 2.0 * f;
+  dw[2] = -1.5 * f2 + f + 0.5;
+  dw[3] = 0.5 * f2;
+  return true;
 }
 
 // ------------------------------------------------------------ pair forces
@@ -247,12 +297,22 @@ struct KahanF {
 // virial carry weight 1/2.  semantics: 0 = vlist (r2 <= rcut2 kept),
 // 1 = cell list (r2 < rcut2 kept).
 template <bool MIXED>
-__global__ void __launch_bounds__(kBlock) k_pair(
-    const double *__restrict__ ro, const int *__restrict__ type, int np,
-    const int *__restrict__ nn, const int *__restrict__ nbr,
-    const int *__restrict__ sorted, Box b, TableDev t, int semantics,
-    int energy, double *__restrict__ force, double *__restrict__ part,
-    int *flags) {
+__global__ void __launch_bounds__(kBlock) k_pair(const SysDev *__restrict__ sys,
+                                                 int energy) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nblk_pair) return;
+  const double *__restrict__ ro = V.ro_s;
+  const int *__restrict__ type = V.type_s;
+  const int np = V.np;
+  const int *__restrict__ nn = V.nn;
+  const int *__restrict__ nbr = V.nbr;
+  const int *__restrict__ sorted = V.c_sorted;
+  const Box b = V.box;
+  const TableDev t = V.tab;
+  const int semantics = V.nlsem;
+  double *__restrict__ force = V.force;
+  double *__restrict__ part = V.part + V.off_pair;
+  int *flags = V.flags;
   __shared__ double sh[kBlock];
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gid / kPairLanes, lane = gid % kPairLanes;
@@ -306,6 +366,20 @@ __global__ void __launch_bounds__(kBlock) k_pair(
           fsum = cf[6] + d * (cf[7] + d * (cf[8] + d * (cf[9] + d * cf[10])));
           usum = cf[0] + d * (cf[1] + d * (cf[2] + d * (cf[3] + d * (cf[4] + d * cf[5]))));
         }
+        if (V.nknot) {
+          // additive spline term: u += sum c w, fsum -= u'/r
+          const double r = sqrt(r2);
+          int ki;
+          double w[4], dw[4];
+          if (knot_weights(r, V.knot_r0, V.knot_dinv, V.nknot, ki, w, dw)) {
+            const double *c = V.knot_c + (size_t)p * V.nknot + ki;
+            const double du = c[0] * w[0] + c[1] * w[1] + c[2] * w[2] + c[3] * w[3];
+            const double dup = (c[0] * dw[0] + c[1] * dw[1] + c[2] * dw[2] + c[3] * dw[3]) *
+                               V.knot_dinv;
+            usum += du;
+            fsum -= dup / r;
+          }
+        }
       }
       if (MIXED) {
         const float fs = (float)fsum;
@@ -342,7 +416,9 @@ __global__ void __launch_bounds__(kBlock) k_pair(
     force[3 * io + 2] = fz;
   }
   if (energy) {
-    double *out = part + (size_t)blockIdx.x * (t.nptpt + 1);
+    // component-major partials: component k of block b at k * nblk_pair + b
+    double *out = part + blockIdx.x;
+    const size_t cs = (size_t)V.nblk_pair;
     for (int k = 0; k < t.nptpt; ++k) {
       double v = 0.0;
       if (i < np) {
@@ -351,10 +427,10 @@ __global__ void __launch_bounds__(kBlock) k_pair(
           if (u < t.npt && t.pairidx[ti * t.npt + u] == k) v += ej[u];
       }
       double s2 = block_sum(v, sh);
-      if (threadIdx.x == 0) out[k] = s2;
+      if (threadIdx.x == 0) out[k * cs] = s2;
     }
     double s2 = block_sum(vir, sh);
-    if (threadIdx.x == 0) out[t.nptpt] = s2;
+    if (threadIdx.x == 0) out[t.nptpt * cs] = s2;
   }
 }
 
@@ -393,11 +469,6 @@ __device__ __forceinline__ void bspline(R x, R *v, R *d) {
   }
 }
 
-struct SpmeGeom {
-  int s[3], B[3], nb[3], order;
-  double dkdr[3];
-};
-
 __device__ __forceinline__ void mesh_coords(const double *r, const SpmeGeom &g,
                                             int mm[3], double fr[3]) {
   for (int m = 0; m < 3; ++m) {
@@ -413,9 +484,14 @@ __device__ __forceinline__ void mesh_coords(const double *r, const SpmeGeom &g,
 // brick of a wavefront: lanes walk the cell-ordered particles, so a
 // wavefront's particles share few bricks (atomics across XCDs are the
 // expensive part of binning).
-__global__ void k_spme_count(const double *__restrict__ ro,
-                             const int *__restrict__ sorted, int np, SpmeGeom g,
-                             int *bcount, int *bid, int *bslot) {
+__global__ void k_spme_count(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nblk_np) return;
+  const double *__restrict__ ro = V.ro;
+  const int *__restrict__ sorted = V.c_sorted;
+  const int np = V.np;
+  const SpmeGeom g = V.sg;
+  int *bcount = V.b_count, *bid = V.b_id, *bslot = V.b_slot;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const bool act = t < np;
   int i = 0, id = -1;
@@ -448,12 +524,16 @@ __global__ void k_spme_count(const double *__restrict__ ro,
 // particle records in brick order: fraction (x, y, z) and charge, plus the
 // packed meshmax (10 bits per axis) and the particle index
 template <typename R>
-__global__ void k_spme_scatter(const double *__restrict__ ro,
-                               const double *__restrict__ q, int np, SpmeGeom g,
-                               const int *bid, const int *bslot,
-                               const int *bstart, R *frac, int2 *meta) {
+__global__ void k_spme_scatter(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  const double *__restrict__ ro = V.ro;
+  const double *__restrict__ q = V.q;
+  const SpmeGeom g = V.sg;
+  const int *bid = V.b_id, *bslot = V.b_slot, *bstart = V.b_start;
+  R *frac = (R *)V.rec_frac;
+  int2 *meta = V.rec_meta;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
+  if (i >= V.np) return;
   int mm[3];
   double fr[3];
   mesh_coords(ro + 3 * i, g, mm, fr);
@@ -520,9 +600,14 @@ __device__ __forceinline__ void stencil_range(int m, int o, int e, int s, int nb
 // prefix sum and handed out to threads (a plane of rows each), so every
 // issued add lands.
 template <typename R, int ORDER>
-__global__ void __launch_bounds__(kBlock) k_spread(
-    const R *__restrict__ frac, const int2 *__restrict__ meta,
-    const int *__restrict__ bstart, SpmeGeom g, R *__restrict__ mesh) {
+__global__ void __launch_bounds__(kBlock) k_spread(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nbricks) return;
+  const R *__restrict__ frac = (const R *)V.rec_frac;
+  const int2 *__restrict__ meta = V.rec_meta;
+  const int *__restrict__ bstart = V.b_start;
+  const SpmeGeom g = V.sg;
+  R *__restrict__ mesh = (R *)V.mesh_r;
   typedef typename SpreadAcc<R>::T Acc;
   constexpr int kChunk = Chunk<R>::n;
   extern __shared__ char smem[];
@@ -660,11 +745,17 @@ struct Acc3<double> {
 // lanes, lane iz < order summing stencil plane iz (FP32 with compensated
 // row sums on the mixed path), and the planes are combined by shuffles in
 // FP64 in a fixed order.
-template <typename R, int ORDER>
-__global__ void __launch_bounds__(kBlock) k_gather(
-    const R *__restrict__ frac, const int2 *__restrict__ meta,
-    const int *__restrict__ bstart, SpmeGeom g, const R *__restrict__ mesh,
-    double efp, double *__restrict__ force) {
+template <typename R, int ORDER, bool POT>
+__global__ void __launch_bounds__(kBlock) k_gather(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nbricks) return;
+  const R *__restrict__ frac = (const R *)V.rec_frac;
+  const int2 *__restrict__ meta = V.rec_meta;
+  const int *__restrict__ bstart = V.b_start;
+  const SpmeGeom g = V.sg;
+  const R *__restrict__ mesh = (const R *)V.mesh_r;
+  const double efp = V.efp;
+  double *__restrict__ force = V.force;
   extern __shared__ char smem[];
   R *tile = reinterpret_cast<R *>(smem);
   const int bx = blockIdx.x % g.nb[0];
@@ -694,7 +785,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(
   const int span = ((8 * n + blockDim.x - 1) / blockDim.x) * blockDim.x;
   for (int it = threadIdx.x; it < span; it += blockDim.x) {
     const int p = it >> 3, iz = it & 7;
-    double ex = 0.0, ey = 0.0, ez = 0.0;
+    double ex = 0.0, ey = 0.0, ez = 0.0, ep = 0.0;
     int idx = -1;
     R q = R(0);
     if (p < n && iz < ORDER) {
@@ -718,6 +809,7 @@ __global__ void __launch_bounds__(kBlock) k_gather(
         }
       const R *plane = tile + (size_t)(lz - iz) * E1 * E0;
       Acc3<R> acc;
+      double pacc = 0.0;
 #pragma unroll
       for (int iy = 0; iy < ORDER; ++iy) {
         const R *row = plane + (ly - iy) * E0;
@@ -731,22 +823,27 @@ __global__ void __launch_bounds__(kBlock) k_gather(
         acc.add(0, sdx * (v[1][iy] * vz));
         acc.add(1, sx * (d[1][iy] * vz));
         acc.add(2, sx * (v[1][iy] * dz));
+        if (POT) pacc += (double)sx * (double)(v[1][iy] * vz);
       }
       ex = acc.get(0);
       ey = acc.get(1);
       ez = acc.get(2);
+      ep = pacc;
     }
 #pragma unroll
     for (int w = 1; w < 8; w <<= 1) {
       ex += __shfl_xor(ex, w, 8);
       ey += __shfl_xor(ey, w, 8);
       ez += __shfl_xor(ez, w, 8);
+      if (POT) ep += __shfl_xor(ep, w, 8);
     }
     if (iz == 0 && p < n) {
       const double qd = (double)q;
       force[3 * idx] -= qd * ex * g.dkdr[0] * efp;
       force[3 * idx + 1] -= qd * ey * g.dkdr[1] * efp;
       force[3 * idx + 2] -= qd * ez * g.dkdr[2] * efp;
+      // dE_rec/dq = efp * sum_stencil theta * (G * Q) (the convolved mesh)
+      if (POT) V.phi_rec[idx] = efp * ep;
     }
   }
 }
@@ -755,10 +852,17 @@ __global__ void __launch_bounds__(kBlock) k_gather(
 // UEwaldRecStd (Src/energy.F90:1050-1096) over the k-vectors of its kfac
 // table: one block per k-vector sums the four sign combinations of the
 // structure factor; a second kernel gives each particle its force.
-__global__ void __launch_bounds__(kBlock) k_ewald_sk(
-    const double *__restrict__ ro, const double *__restrict__ q, int np,
-    const double *__restrict__ kv, double *__restrict__ ksum, int energy,
-    double alpha2, double *part) {
+__global__ void __launch_bounds__(kBlock) k_ewald_sk(const SysDev *__restrict__ sys,
+                                                     int energy) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nkvec) return;
+  const double *__restrict__ ro = V.ro;
+  const double *__restrict__ q = V.q;
+  const int np = V.np;
+  const double *__restrict__ kv = V.k_vec;
+  double *__restrict__ ksum = V.k_sum;
+  const double alpha2 = V.alpha * V.alpha;
+  double *part = V.part + V.off_std;
   __shared__ double sh[kBlock];
   const int kn = blockIdx.x;
   const double kx = kv[4 * kn], ky = kv[4 * kn + 1], kz = kv[4 * kn + 2];
@@ -785,18 +889,23 @@ __global__ void __launch_bounds__(kBlock) k_ewald_sk(
       for (int c = 0; c < 8; ++c) ssum += tot[c] * tot[c];
       const double term = kv[4 * kn + 3] * ssum;
       const double k2 = kx * kx + ky * ky + kz * kz;
-      part[2 * kn] = term;
-      part[2 * kn + 1] = -(1.0 - k2 / (2.0 * alpha2)) * term;
+      part[kn] = term;
+      part[V.nkvec + kn] = -(1.0 - k2 / (2.0 * alpha2)) * term;
     }
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_ewald_force(
-    const double *__restrict__ ro, const double *__restrict__ q, int np,
-    const double *__restrict__ kv, const double *__restrict__ ksum, int nk,
-    double efp, double *__restrict__ force) {
+__global__ void __launch_bounds__(kBlock) k_ewald_force(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  const double *__restrict__ ro = V.ro;
+  const double *__restrict__ q = V.q;
+  const double *__restrict__ kv = V.k_vec;
+  const double *__restrict__ ksum = V.k_sum;
+  const int nk = V.nkvec;
+  const double efp = V.efp;
+  double *__restrict__ force = V.force;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
+  if (i >= V.np) return;
   const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
   double fx = 0.0, fy = 0.0, fz = 0.0;
   for (int kn = 0; kn < nk; ++kn) {
@@ -825,10 +934,19 @@ __global__ void __launch_bounds__(kBlock) k_ewald_force(
 // UBond (Src/energy.F90:5335-5372): u = k (r - eq)^p with k already
 // divided by p; each particle gathers the forces of its (at most two)
 // bonds, and the bond to the next bead carries the energy.
-__global__ void __launch_bounds__(kBlock) k_bonds(
-    const double *__restrict__ ro, int np, const int *__restrict__ prev,
-    const int *__restrict__ next, Box b, double k, int p, double eq,
-    int energy, double *__restrict__ force, double *part) {
+__global__ void __launch_bounds__(kBlock) k_bonds(const SysDev *__restrict__ sys,
+                                                  int energy) {
+  const SysDev &V = sys[blockIdx.z];
+  if (V.nchain == 0 || (int)blockIdx.x >= V.nblk_np) return;
+  const double *__restrict__ ro = V.ro;
+  const int np = V.np;
+  const int *__restrict__ prev = V.ch_prev;
+  const int *__restrict__ next = V.ch_next;
+  const Box b = V.box;
+  const double k = V.bond_k, eq = V.bond_eq;
+  const int p = V.bond_p;
+  double *__restrict__ force = V.force;
+  double *part = V.part + V.off_bond;
   __shared__ double sh[kBlock];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double ub = 0.0, vir = 0.0;
@@ -866,16 +984,20 @@ __global__ void __launch_bounds__(kBlock) k_bonds(
     double s1 = block_sum(ub, sh);
     double s2 = block_sum(vir, sh);
     if (threadIdx.x == 0) {
-      part[2 * blockIdx.x] = s1;
-      part[2 * blockIdx.x + 1] = s2;
+      part[blockIdx.x] = s1;
+      part[V.nblk_np + blockIdx.x] = s2;
     }
   }
 }
 
 // ------------------------------------------------------- surface + sums
-__global__ void __launch_bounds__(kBlock) k_surf_sum(
-    const double *__restrict__ ro, const double *__restrict__ q, int np,
-    double *part) {
+__global__ void __launch_bounds__(kBlock) k_surf_sum(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  if (!V.lsurf || (int)blockIdx.x >= V.nblk_np) return;
+  const double *__restrict__ ro = V.ro;
+  const double *__restrict__ q = V.q;
+  const int np = V.np;
+  double *part = V.part + V.off_surf;
   __shared__ double sh[kBlock];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double s[3] = {0, 0, 0};
@@ -883,75 +1005,81 @@ __global__ void __launch_bounds__(kBlock) k_surf_sum(
     for (int m = 0; m < 3; ++m) s[m] = q[i] * ro[3 * i + m];
   for (int m = 0; m < 3; ++m) {
     double t = block_sum(s[m], sh);
-    if (threadIdx.x == 0) part[3 * blockIdx.x + m] = t;
+    if (threadIdx.x == 0) part[m * V.nblk_np + blockIdx.x] = t;
   }
 }
-
-struct ReduceArgs {
-  int nptpt;
-  int nblk_pair, nblk_conv, nblk_bond, nblk_surf, nk, nblk_move;
-  int off_pair, off_conv, off_bond, off_surf, off_std, off_move, off_skin;
-  int ewald, lsurf, energy, skin;
-  double efp, self, vol, drnlist;
-};
 
 constexpr int kRedComp = MFE_MAX_PAIRS + 12;
 
 // Final sums in a fixed order (one block, one pass over the partials):
 // writes en = [tot, twob0, rec, bond, virial, Sx, Sy, Sz, twob(1..nptpt)]
 // and, in MD, the skin-test flag of NList (Src/nlist.F90:612-615).
-__global__ void __launch_bounds__(kBlock) k_reduce(const double *part,
-                                                   ReduceArgs a, double *en,
-                                                   int *flags) {
-  __shared__ double sh[kRedComp][kBlock / 4];
+// bd: also advance the replica's Brownian-dynamics step counter.
+// Each wavefront sums whole components (partials are component-major),
+// four independent accumulators per lane, then a shuffle tree; the fixed
+// order keeps the result deterministic.
+__global__ void __launch_bounds__(kBlock) k_reduce(const SysDev *__restrict__ sys,
+                                                   int energy, int skin, int bd) {
+  const SysDev &V = sys[blockIdx.z];
+  double *en = V.en;
+  int *flags = V.flags;
+  const int P = V.nptpt;
+  const int nc = P + 10;
+  const int nb = V.nblk_np;
+  const int nrec = V.ewald == MFE_EWALD_SPME ? V.nblk_conv
+                                             : (V.ewald == MFE_EWALD_STD ? V.nkvec : 0);
+  const int offrec = V.ewald == MFE_EWALD_SPME ? V.off_conv : V.off_std;
+  __shared__ double red[kRedComp];
+  if (bd && threadIdx.x == 0) flags[3] += 1;
   // component layout: [0,P) pair energies, P pair virial, P+1..P+2 rec
   // (u, vir), P+3..P+4 bond (u, vir), P+5..P+7 S, P+8 max r2, P+9 cand
-  const int P = a.nptpt;
-  const int nc = P + 10;
-  double v[kRedComp];
-#pragma unroll
-  for (int c = 0; c < kRedComp; ++c) v[c] = 0.0;
-  const int nrec = a.ewald == MFE_EWALD_SPME ? a.nblk_conv : (a.ewald == MFE_EWALD_STD ? a.nk : 0);
-  const int offrec = a.ewald == MFE_EWALD_SPME ? a.off_conv : a.off_std;
-  for (int b2 = threadIdx.x; b2 < a.nblk_pair && a.energy; b2 += blockDim.x)
-    for (int c = 0; c <= P; ++c) v[c] += part[a.off_pair + b2 * (P + 1) + c];
-  for (int b2 = threadIdx.x; b2 < nrec && a.energy; b2 += blockDim.x) {
-    v[P + 1] += part[offrec + 2 * b2];
-    v[P + 2] += part[offrec + 2 * b2 + 1];
-  }
-  for (int b2 = threadIdx.x; b2 < a.nblk_bond && a.energy; b2 += blockDim.x) {
-    v[P + 3] += part[a.off_bond + 2 * b2];
-    v[P + 4] += part[a.off_bond + 2 * b2 + 1];
-  }
-  for (int b2 = threadIdx.x; b2 < a.nblk_surf && a.lsurf; b2 += blockDim.x)
-    for (int m = 0; m < 3; ++m) v[P + 5 + m] += part[a.off_surf + 3 * b2 + m];
-  for (int b2 = threadIdx.x; b2 < a.nblk_move && a.skin; b2 += blockDim.x) {
-    v[P + 8] = fmax(v[P + 8], part[a.off_move + b2]);
-    v[P + 9] = fmax(v[P + 9], part[a.off_skin + b2]);
-  }
-  // fold 256 lanes to 64 in registers through LDS, then a fixed tree
-  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-  for (int g2 = 1; g2 < 4; ++g2) {
-    if (grp == g2)
-      for (int c = 0; c < nc; ++c) sh[c][lane] = v[c];
-    __syncthreads();
-    if (grp == 0)
-      for (int c = 0; c < nc; ++c)
-        v[c] = (c >= P + 8) ? fmax(v[c], sh[c][lane]) : v[c] + sh[c][lane];
-    __syncthreads();
-  }
-  if (grp == 0) {
-    for (int w = 32; w > 0; w >>= 1)
-      for (int c = 0; c < nc; ++c) {
-        const double o = __shfl_down(v[c], w, 64);
-        v[c] = (c >= P + 8) ? fmax(v[c], o) : v[c] + o;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int c = wave; c < nc; c += kBlock / 64) {
+    const double *src;
+    int n;
+    bool mx = false;
+    if (c <= P) {
+      src = V.part + V.off_pair + (size_t)c * V.nblk_pair;
+      n = energy ? V.nblk_pair : 0;
+    } else if (c <= P + 2) {
+      src = V.part + offrec + (size_t)(c - P - 1) * nrec;
+      n = energy ? nrec : 0;
+    } else if (c <= P + 4) {
+      src = V.part + V.off_bond + (size_t)(c - P - 3) * nb;
+      n = (energy && V.nchain > 0) ? nb : 0;
+    } else if (c <= P + 7) {
+      src = V.part + V.off_surf + (size_t)(c - P - 5) * nb;
+      n = V.lsurf ? nb : 0;
+    } else {
+      src = V.part + (c == P + 8 ? V.off_move : V.off_skin);
+      n = skin ? nb : 0;
+      mx = true;
+    }
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int b2 = lane;
+    for (; b2 + 192 < n; b2 += 256) {
+      const double a0 = src[b2], a1 = src[b2 + 64], a2 = src[b2 + 128], a3 = src[b2 + 192];
+      if (mx) {
+        s0 = fmax(s0, a0); s1 = fmax(s1, a1); s2 = fmax(s2, a2); s3 = fmax(s3, a3);
+      } else {
+        s0 += a0; s1 += a1; s2 += a2; s3 += a3;
       }
+    }
+    for (; b2 < n; b2 += 64) s0 = mx ? fmax(s0, src[b2]) : s0 + src[b2];
+    double sv = mx ? fmax(fmax(s0, s1), fmax(s2, s3)) : (s0 + s1) + (s2 + s3);
+    for (int w = 32; w > 0; w >>= 1) {
+      const double o = __shfl_down(sv, w, 64);
+      sv = mx ? fmax(sv, o) : sv + o;
+    }
+    if (lane == 0) red[c] = sv;
   }
+  __syncthreads();
   if (threadIdx.x != 0) return;
+  const double *v = red;
   const double S[3] = {v[P + 5], v[P + 6], v[P + 7]};
   for (int m = 0; m < 3; ++m) en[5 + m] = S[m];
-  if (a.skin) flags[1] = (sqrt(v[P + 8]) + sqrt(v[P + 9]) > a.drnlist) ? 1 : 0;
-  if (!a.energy) return;
+  if (skin) flags[1] = (sqrt(v[P + 8]) + sqrt(v[P + 9]) > V.drnlist) ? 1 : 0;
+  if (!energy) return;
   double twob0 = 0.0;
   for (int k = 0; k < P; ++k) {
     en[8 + k] = v[k];
@@ -962,16 +1090,16 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const double *part,
   const double ubond = v[P + 3];
   vir += v[P + 4];
   double rec = 0.0;
-  if (a.ewald != MFE_EWALD_NONE) {
-    urec += a.self;
-    if (a.lsurf) {
-      const double fac = 2.0 * 3.1415926535897932 / (3.0 * a.vol);
+  if (V.ewald != MFE_EWALD_NONE) {
+    urec += V.self_energy;
+    if (V.lsurf) {
+      const double fac = 2.0 * 3.1415926535897932 / (3.0 * V.vol);
       const double term = fac * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
       urec += term;
       virrec -= term;
     }
-    rec = a.efp * urec;
-    vir += a.efp * virrec;
+    rec = V.efp * urec;
+    vir += V.efp * virrec;
   }
   en[0] = twob0 + rec + ubond;
   en[1] = twob0;
@@ -980,36 +1108,210 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const double *part,
   en[4] = vir;
 }
 
+// --------------------------------------------------------------- adjoint
+// Gradient of the pair model (Pot_1_6_12, Src/potential.F90:1349-1425,
+// with exact erfc, plus the spline term) with respect to its parameters,
+// summed over the same pair set as the pair kernel.  Each pair is seen from
+// both ends: the visit (i, j) adds
+//   dE/dq_i        q_j efp erfc(alpha r)/r                       (full)
+//   dE/dsig[t_i]   1/2 dLJ/dsig_ij            (sig_ij = (sig_i+sig_j)/2)
+//   dE/deps[t_i]   dLJ/deps_ij eps_j/(2 eps_ij)   (eps_ij = sqrt(eps_i eps_j))
+//   dE/dc[p][k]    1/2 spline weight of coefficient k
+// and 1/2 of the pair's model energy.  Per-block column partials:
+// [E, dsig(npt), deps(npt), dc(nptpt*nknot)].
+__global__ void __launch_bounds__(kBlock) k_adjoint(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nblk_np) return;
+  extern __shared__ double sk[];
+  __shared__ double sh[kBlock];
+  const int nk = V.nptpt * V.nknot;
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) sk[k] = 0.0;
+  __syncthreads();
+  const PairParams &pp = V.pp;
+  const TableDev t = V.tab;
+  const Box b = V.box;
+  const int np = V.np;
+  const int tt = blockIdx.x * blockDim.x + threadIdx.x;
+  double eu = 0.0, gs = 0.0, ge = 0.0;
+  int ti = -1;
+  if (tt < np) {
+    const int i = V.c_sorted[tt];
+    const double xi = V.ro_s[3 * tt], yi = V.ro_s[3 * tt + 1], zi = V.ro_s[3 * tt + 2];
+    ti = V.type_s[tt];
+    const double qi = pp.zat[ti];
+    double phi = 0.0;
+    const int n = V.nn[tt];
+    for (int k = 0; k < n; ++k) {
+      const int pj = V.nbr[(size_t)k * np + tt];
+      double dx = xi - V.ro_s[3 * pj], dy = yi - V.ro_s[3 * pj + 1], dz = zi - V.ro_s[3 * pj + 2];
+      const double r2 = pbcr2(dx, dy, dz, b);
+      if (V.nlsem == 0 ? (r2 > t.rcut2) : !(r2 < t.rcut2)) continue;
+      const int tj = V.type_s[pj];
+      const double r = sqrt(r2);
+      const double qj = pp.zat[tj];
+      const double c0 = pp.lewald ? erfc(pp.alpha * r) : 1.0;
+      phi += qj * pp.efp * c0 / r;
+      eu += 0.5 * qi * qj * pp.efp * c0 / r;
+      const double sig = 0.5 * (pp.sigat[ti] + pp.sigat[tj]);
+      const double eps = sqrt(pp.epsat[ti] * pp.epsat[tj]);
+      if (!pp.lljcut || r < pp.ljrcut * sig) {
+        const double x = sig / r, x2 = x * x, s6 = x2 * x2 * x2, s12 = s6 * s6;
+        const double sh_ = pp.lljcut ? pp.ljushift : 0.0;
+        eu += 0.5 * (4.0 * eps * (s12 - s6) + sh_ * eps);
+        gs += 0.5 * 4.0 * eps * (12.0 * s12 - 6.0 * s6) / sig;
+        if (eps > 0.0) ge += (4.0 * (s12 - s6) + sh_) * pp.epsat[tj] / (2.0 * eps);
+      }
+      if (V.nknot) {
+        int ki;
+        double w[4], dw[4];
+        if (knot_weights(r, V.knot_r0, V.knot_dinv, V.nknot, ki, w, dw)) {
+          const int pidx = t.pairidx[ti * t.npt + tj];
+          const double *c = V.knot_c + (size_t)pidx * V.nknot + ki;
+          eu += 0.5 * (c[0] * w[0] + c[1] * w[1] + c[2] * w[2] + c[3] * w[3]);
+          for (int m = 0; m < 4; ++m) atomicAdd(&sk[pidx * V.nknot + ki + m], 0.5 * w[m]);
+        }
+      }
+    }
+    V.adj_phi[i] = phi;
+  }
+  double *out = V.adj_part + (size_t)blockIdx.x * V.adj_ncol;
+  const double se = block_sum(eu, sh);
+  if (threadIdx.x == 0) out[0] = se;
+  for (int u = 0; u < V.npt; ++u) {
+    const double a = block_sum(ti == u ? gs : 0.0, sh);
+    const double c = block_sum(ti == u ? ge : 0.0, sh);
+    if (threadIdx.x == 0) {
+      out[1 + u] = a;
+      out[1 + V.npt + u] = c;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) out[1 + 2 * V.npt + k] = sk[k];
+}
+
+// dE/dq_i = real-space potential + efp * reciprocal potential
+//           - 2 efp alpha/sqrt(pi) q_i (UEwaldSelf) + surface term
+__global__ void __launch_bounds__(kBlock) k_adjoint_q(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V.np) return;
+  double d = V.adj_phi[i];
+  if (V.ewald == MFE_EWALD_SPME) {
+    d += V.phi_rec[i];
+    d -= 2.0 * V.efp * (V.alpha / 1.7724538509055160273) * V.q[i];
+  }
+  if (V.lsurf) {
+    // efp (2 pi/3V) |sum q r|^2 -> efp (4 pi/3V) (sum q r) . r_i
+    const double f = V.efp * 4.0 * 3.1415926535897932 / (3.0 * V.vol);
+    d += f * (V.en[5] * V.ro[3 * i] + V.en[6] * V.ro[3 * i + 1] + V.en[7] * V.ro[3 * i + 2]);
+  }
+  V.adj_dq[i] = d;
+}
+
+// column sums of the adjoint partials in block order
+__global__ void __launch_bounds__(kBlock) k_adjoint_sum(const SysDev *__restrict__ sys,
+                                                        double *out) {
+  const SysDev &V = sys[blockIdx.z];
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= V.adj_ncol) return;
+  double s = 0.0;
+  for (int b2 = 0; b2 < V.nblk_np; ++b2) s += V.adj_part[(size_t)b2 * V.adj_ncol + c];
+  out[(size_t)blockIdx.z * V.adj_ncol + c] = s;
+}
+
+// ------------------------------------------------- counter-based noise
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+// easy as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with
+// the round keys bumped by the Weyl constants.
+__host__ __device__ __forceinline__ void philox_round(uint32_t c[4], const uint32_t k[2]) {
+  const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+  const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+  const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+  const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+  const uint32_t n0 = hi1 ^ c[1] ^ k[0], n2 = hi0 ^ c[3] ^ k[1];
+  c[0] = n0;
+  c[1] = lo1;
+  c[2] = n2;
+  c[3] = lo0;
+}
+
+__host__ __device__ __forceinline__ void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+  uint32_t k[2] = {k0, k1};
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    philox_round(c, k);
+    k[0] += 0x9E3779B9u;
+    k[1] += 0xBB67AE85u;
+  }
+}
+
+// three standard normal deviates for particle i at step n: Box-Muller on
+// the uniforms (x + 1/2) 2^-32 of the first two pairs of Philox words
+__device__ __forceinline__ void gauss3(unsigned long long seed, unsigned i, unsigned n,
+                                       double g[3]) {
+  uint32_t c[4] = {i, n, 0u, 0u};
+  philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+  const double s = 2.3283064365386963e-10;  // 2^-32
+  const double u0 = ((double)c[0] + 0.5) * s, u1 = ((double)c[1] + 0.5) * s;
+  const double u2 = ((double)c[2] + 0.5) * s, u3 = ((double)c[3] + 0.5) * s;
+  const double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
+  double sn, cs;
+  sincospi(2.0 * u1, &sn, &cs);
+  g[0] = r0 * cs;
+  g[1] = r0 * sn;
+  g[2] = r1 * cospi(2.0 * u3);
+}
+
 // ------------------------------------------------------------ integrators
 // VelVer positions (Src/md.F90:287-298): drostep = dt v + dt^2/2 a,
 // ro += drostep, PBC with ANINT; the skin bookkeeping of NList
 // (Src/nlist.F90:601-603) is folded in: drosum += drostep, r2 = |drosum|^2
 // and the block maximum of r2 for the second-largest rule.
-// For BD (Src/bd.F90:141-151): drostep = fac1 forceo + fac2 noise.
-__global__ void __launch_bounds__(kBlock) k_move(
-    double *__restrict__ ro, const double *__restrict__ rod,
-    const double *__restrict__ rodd, const double *__restrict__ forceo,
-    const int *__restrict__ type, const double *__restrict__ fac1,
-    const double *__restrict__ fac2, const double *__restrict__ noise, int np,
-    Box b, double dt, int bd, double *__restrict__ drosum, double *r2disp,
-    double *part_max) {
+// For BD (Src/bd.F90:141-151): drostep = fac1 forceo + fac2 noise, with
+// the noise supplied by the host (bd = 1) or drawn from the replica's
+// counter-based stream (bd = 2): particle i at BD step n takes
+// Philox4x32-10(counter (i, n, 0, 0), key seed) and three Box-Muller
+// deviates of it.
+__global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
+                                                 double dt, int bd) {
 #pragma clang fp contract(off)
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nblk_np) return;
+  double *__restrict__ ro = V.ro;
+  const double *__restrict__ rod = V.rod;
+  const double *__restrict__ rodd = V.rodd;
+  const double *__restrict__ forceo = V.forceo;
+  const int *__restrict__ type = V.type;
+  const double *__restrict__ fac1 = V.bd_fac1;
+  const double *__restrict__ fac2 = V.bd_fac2;
+  const int np = V.np;
+  const Box b = V.box;
+  double *__restrict__ drosum = V.drosum;
+  double *r2disp = V.r2disp;
+  double *part_max = V.part + V.off_move;
   __shared__ double sh[kBlock];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double r2 = 0.0;
   if (i < np) {
     const double fac = 0.5 * (dt * dt);
     double d[3];
+    double gn[3] = {0.0, 0.0, 0.0};
+    if (bd == 1) {
+      for (int m = 0; m < 3; ++m) gn[m] = V.noise[3 * i + m];
+    } else if (bd == 2) {
+      gauss3(V.seed, (unsigned)i, (unsigned)V.flags[3], gn);
+    }
     for (int m = 0; m < 3; ++m) {
       if (bd) {
         const int t = type[i];
-        d[m] = fac1[t] * forceo[3 * i + m] + fac2[t] * noise[3 * i + m];
+        d[m] = fac1[t] * forceo[3 * i + m] + fac2[t] * gn[m];
       } else {
         d[m] = dt * rod[3 * i + m] + fac * rodd[3 * i + m];
       }
       double x = ro[3 * i + m] + d[m];
       if (fabs(x) > b.L2[m]) x = x - b.L[m] * round(x * b.Li[m]);
       ro[3 * i + m] = x;
+      V.ro_s[3 * V.c_pos[i] + m] = x;
       drosum[3 * i + m] = drosum[3 * i + m] + d[m];
     }
     r2 = drosum[3 * i] * drosum[3 * i];
@@ -1025,9 +1327,13 @@ __global__ void __launch_bounds__(kBlock) k_move(
 // new running maximum at the moment it is visited in particle order.  Each
 // block owns the same particle range as in k_move; the running maximum
 // entering it is the max of the earlier blocks' maxima.
-__global__ void __launch_bounds__(kBlock) k_skin_blocks(
-    const double *__restrict__ r2disp, int np, const double *part_max,
-    double *part_cand) {
+__global__ void __launch_bounds__(kBlock) k_skin_blocks(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nblk_np) return;
+  const double *__restrict__ r2disp = V.r2disp;
+  const int np = V.np;
+  const double *part_max = V.part + V.off_move;
+  double *part_cand = V.part + V.off_skin;
   __shared__ double sh[kBlock];
   double pm = 0.0;
   for (int b2 = threadIdx.x; b2 < (int)blockIdx.x; b2 += blockDim.x) pm = fmax(pm, part_max[b2]);
@@ -1053,14 +1359,21 @@ __global__ void __launch_bounds__(kBlock) k_skin_blocks(
 // UTotal's tail (Src/energy.F90:405-418) plus, for MD, GetLinAcc and the
 // velocity half of VelVer (Src/md.F90:348-358):
 //   mode 0: forceo = force*fac        mode 1: + VelVer kick
-__global__ void __launch_bounds__(kBlock) k_finalize(
-    const double *__restrict__ force, double *__restrict__ forceo,
-    const double *__restrict__ q, const double *__restrict__ massip, int np,
-    double ffac, int surf, double surf_fac, const double *en, int mode,
-    double dt, double *__restrict__ rod, double *__restrict__ rodd) {
+__global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ sys,
+                                                     int mode, double dt) {
 #pragma clang fp contract(off)
+  const SysDev &V = sys[blockIdx.z];
+  const double *__restrict__ force = V.force;
+  double *__restrict__ forceo = V.forceo;
+  const double *__restrict__ q = V.q;
+  const double *__restrict__ massip = V.massip;
+  const double ffac = V.ffac, surf_fac = V.surf_fac;
+  const int surf = V.lsurf;
+  const double *en = V.en;
+  double *__restrict__ rod = V.rod;
+  double *__restrict__ rodd = V.rodd;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= np) return;
+  if (i >= V.np) return;
   for (int m = 0; m < 3; ++m) {
     double f = force[3 * i + m];
     if (surf) f = f - surf_fac * q[i] * en[5 + m];
@@ -1077,33 +1390,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(
 
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
 
-SpmeGeom spme_geom(const mfe_engine *e) {
-  SpmeGeom g;
-  for (int m = 0; m < 3; ++m) {
-    g.s[m] = e->s[m];
-    g.B[m] = e->B[m];
-    g.nb[m] = e->nb[m];
-    g.dkdr[m] = e->s[m] / e->box.L[m];
-  }
-  g.order = e->order;
-  return g;
-}
-
-TableDev table_dev(const mfe_engine *e) {
-  TableDev t;
-  t.zlow = e->t_zlow;
-  t.coef = e->t_coef;
-  t.coef_f = e->t_coef_f;
-  t.off = e->t_off;
-  t.nint = e->t_nint;
-  t.r2atat = e->t_r2atat;
-  t.r2umin = e->t_r2umin;
-  t.pairidx = e->t_pairidx;
-  t.npt = e->npt;
-  t.nptpt = e->nptpt;
-  t.rcut2 = e->rcut * e->rcut;
-  return t;
-}
+inline dim3 grid(int x, const Group &g, int y = 1) { return dim3(x > 0 ? x : 1, y, g.R); }
 
 }  // namespace
 
@@ -1113,7 +1400,16 @@ const void *k_spread_ptr() {
 }
 template <typename R, int O>
 const void *k_gather_ptr() {
-  return (const void *)&k_gather<R, O>;
+  return (const void *)&k_gather<R, O, false>;
+}
+template <typename R, int O>
+const void *k_gather_pot_ptr() {
+  return (const void *)&k_gather<R, O, true>;
+}
+
+void philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  for (int k = 0; k < 4; ++k) out[k] = ctr[k];
+  philox10(out, key[0], key[1]);
 }
 
 void launch_fill_int(int *p, int n, int v, hipStream_t s) {
@@ -1123,85 +1419,66 @@ void launch_fill_double(double *p, int n, double v, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_fill_double, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n, v);
 }
 
-void launch_nlist_rebuild(mfe_engine *e, bool gated) {
-  const int *gate = gated ? e->flags : nullptr;
-  const int np = e->np;
-  const double rs = e->rcut + e->drnlist;
+void launch_nlist_rebuild(const Group &g, bool gated) {
   // gated kernels run on small grids (grid-stride loops): when the skin
   // test does not fire they cost one dispatch of a few hundred wavefronts
-  const int gb = gated ? std::min(nblocks(np), 256) : nblocks(np);
-  const int gc = gated ? std::min(nblocks(e->cg.ncells), 256) : nblocks(e->cg.ncells);
-  hipLaunchKernelGGL(k_cell_count, dim3(gb), dim3(kBlock), 0, e->stream,
-                     e->ro, np, e->cg, e->box, e->c_count, e->c_of, e->c_slot, gate);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, e->stream, e->c_count,
-                     e->c_start, e->cg.ncells, gate);
-  hipLaunchKernelGGL(k_cell_scatter, dim3(gb), dim3(kBlock), 0, e->stream,
-                     np, e->c_of, e->c_slot, e->c_start, e->c_sorted, gate);
-  hipLaunchKernelGGL(k_cell_sort, dim3(gc), dim3(kBlock), 0,
-                     e->stream, e->cg.ncells, e->c_start, e->c_sorted, gate);
-  hipLaunchKernelGGL(k_nlist_build, dim3(gb), dim3(kBlock), 0, e->stream,
-                     e->ro, np, e->cg, e->box, e->c_of, e->c_start, e->c_sorted,
-                     rs * rs, e->maxnn, e->nn, e->nbr, e->flags, e->drosum, gate);
+  const int gb = gated ? std::min(g.np_blocks, 256) : g.np_blocks;
+  const int gc = gated ? std::min(g.cell_blocks, 256) : g.cell_blocks;
+  const int gt = gated ? 1 : 0;
+  hipLaunchKernelGGL(k_cell_count, grid(gb, g), dim3(kBlock), 0, g.stream, g.d, gt);
+  hipLaunchKernelGGL(k_scan, grid(1, g), dim3(1024), 0, g.stream, g.d, 0, gt);
+  hipLaunchKernelGGL(k_cell_scatter, grid(gb, g), dim3(kBlock), 0, g.stream, g.d, gt);
+  hipLaunchKernelGGL(k_cell_sort, grid(gc, g), dim3(kBlock), 0, g.stream, g.d, gt);
+  hipLaunchKernelGGL(k_nlist_build, grid(gb, g), dim3(kBlock), 0, g.stream, g.d, gt);
 }
 
-void launch_pair(mfe_engine *e, bool energy) {
-  const int np = e->np;
-  TableDev t = table_dev(e);
-  const int sem = e->nlkind == MFE_NL_CELL ? 1 : 0;
-  hipLaunchKernelGGL(k_sort_pos, dim3(nblocks(np)), dim3(kBlock), 0, e->stream, e->ro,
-                     e->type, e->c_sorted, np, e->ro_s, e->type_s);
-  const int pb = e->nblk_pair;
-  if (e->prec == MFE_PREC_MIXED)
-    hipLaunchKernelGGL(k_pair<true>, dim3(pb), dim3(kBlock), 0, e->stream,
-                       e->ro_s, e->type_s, np, e->nn, e->nbr, e->c_sorted, e->box, t, sem,
-                       energy ? 1 : 0, e->force, e->part + e->off_pair, e->flags);
+void launch_pair(const Group &g, bool energy, bool sort_pos) {
+  if (sort_pos)
+    hipLaunchKernelGGL(k_sort_pos, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
+  if (g.prec == MFE_PREC_MIXED)
+    hipLaunchKernelGGL(k_pair<true>, grid(g.pair_blocks, g), dim3(kBlock), 0, g.stream, g.d,
+                       energy ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_pair<false>, dim3(pb), dim3(kBlock), 0, e->stream,
-                       e->ro_s, e->type_s, np, e->nn, e->nbr, e->c_sorted, e->box, t, sem,
-                       energy ? 1 : 0, e->force, e->part + e->off_pair, e->flags);
+    hipLaunchKernelGGL(k_pair<false>, grid(g.pair_blocks, g), dim3(kBlock), 0, g.stream, g.d,
+                       energy ? 1 : 0);
 }
 
 template <typename R>
-static void spme_stages(mfe_engine *e, bool energy, int first, int last) {
-  const int np = e->np;
-  SpmeGeom g = spme_geom(e);
+static void spme_stages(const Group &g, bool energy, int first, int last, bool pot) {
   for (int st = first; st <= last; ++st) {
     switch (st) {
       case SP_BIN:
-        hipLaunchKernelGGL(k_spme_count, dim3(nblocks(np)), dim3(kBlock), 0, e->stream,
-                           e->ro, e->c_sorted, np, g, e->b_count, e->b_id, e->b_slot);
-        hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, e->stream, e->b_count,
-                           e->b_start, e->nbricks, (const int *)nullptr);
-        hipLaunchKernelGGL(k_spme_scatter<R>, dim3(nblocks(np)), dim3(kBlock), 0,
-                           e->stream, e->ro, e->q, np, g, e->b_id, e->b_slot,
-                           e->b_start, (R *)e->rec_frac, e->rec_meta);
+        hipLaunchKernelGGL(k_spme_count, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
+        hipLaunchKernelGGL(k_scan, grid(1, g), dim3(1024), 0, g.stream, g.d, 1, 0);
+        hipLaunchKernelGGL(k_spme_scatter<R>, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream,
+                           g.d);
         break;
       case SP_SPREAD:
-        switch (e->order) {
-#define MFE_SPREAD(O)                                                                 \
-  case O:                                                                             \
-    hipLaunchKernelGGL((k_spread<R, O>), dim3(e->nbricks), dim3(kBlock), e->spread_lds, \
-                       e->stream, (const R *)e->rec_frac, e->rec_meta, e->b_start, g,   \
-                       (R *)e->mesh_r);                                                \
+        switch (g.order) {
+#define MFE_SPREAD(O)                                                                   \
+  case O:                                                                               \
+    hipLaunchKernelGGL((k_spread<R, O>), grid(g.bricks, g), dim3(kBlock), g.lds_spread, \
+                       g.stream, g.d);                                                  \
     break;
           MFE_SPREAD(3) MFE_SPREAD(4) MFE_SPREAD(5) MFE_SPREAD(6) MFE_SPREAD(7) MFE_SPREAD(8)
 #undef MFE_SPREAD
         }
         break;
-      case SP_FFTX: fft_x_forward(e); break;
-      case SP_FFTY: fft_y(e, 0); break;
-      case SP_CONV: fft_conv(e, energy); break;
-      case SP_IFFTY: fft_y(e, 1); break;
-      case SP_IFFTX: fft_x_inverse(e); break;
+      case SP_FFTX: fft_x_forward(g); break;
+      case SP_FFTY: fft_y(g, 0); break;
+      case SP_CONV: fft_conv(g, energy); break;
+      case SP_IFFTY: fft_y(g, 1); break;
+      case SP_IFFTX: fft_x_inverse(g); break;
       case SP_GATHER:
-        switch (e->order) {
-#define MFE_GATHER(O)                                                                   \
-  case O:                                                                               \
-    hipLaunchKernelGGL((k_gather<R, O>), dim3(e->nbricks), dim3(kBlock),                \
-                       (size_t)(e->B[0] + O - 1) * (e->B[1] + O - 1) * (e->B[2] + O - 1) *  \
-                           sizeof(R),                                                   \
-                       e->stream, (const R *)e->rec_frac, e->rec_meta, e->b_start, g,   \
-                       (const R *)e->mesh_r, e->efp, e->force);                         \
+        switch (g.order) {
+#define MFE_GATHER(O)                                                                     \
+  case O:                                                                                 \
+    if (pot)                                                                              \
+      hipLaunchKernelGGL((k_gather<R, O, true>), grid(g.bricks, g), dim3(kBlock),         \
+                         g.lds_gather, g.stream, g.d);                                    \
+    else                                                                                  \
+      hipLaunchKernelGGL((k_gather<R, O, false>), grid(g.bricks, g), dim3(kBlock),        \
+                         g.lds_gather, g.stream, g.d);                                    \
     break;
           MFE_GATHER(3) MFE_GATHER(4) MFE_GATHER(5) MFE_GATHER(6) MFE_GATHER(7) MFE_GATHER(8)
 #undef MFE_GATHER
@@ -1211,79 +1488,61 @@ static void spme_stages(mfe_engine *e, bool energy, int first, int last) {
   }
 }
 
-void launch_spme(mfe_engine *e, bool energy, int first, int last) {
-  if (e->prec == MFE_PREC_MIXED) spme_stages<float>(e, energy, first, last);
-  else spme_stages<double>(e, energy, first, last);
+void launch_spme(const Group &g, bool energy, int first, int last, bool pot) {
+  if (g.prec == MFE_PREC_MIXED) spme_stages<float>(g, energy, first, last, pot);
+  else spme_stages<double>(g, energy, first, last, pot);
 }
 
-void launch_std_ewald(mfe_engine *e) {
-  hipLaunchKernelGGL(k_ewald_sk, dim3(e->nkvec), dim3(kBlock), 0, e->stream, e->ro,
-                     e->q, e->np, e->k_vec, e->k_sum, 1, e->alpha * e->alpha,
-                     e->part + e->off_std);
-  hipLaunchKernelGGL(k_ewald_force, dim3(nblocks(e->np)), dim3(kBlock), 0, e->stream,
-                     e->ro, e->q, e->np, e->k_vec, e->k_sum, e->nkvec, e->efp, e->force);
+void launch_std_ewald(const Group &g) {
+  hipLaunchKernelGGL(k_ewald_sk, grid(g.kvec, g), dim3(kBlock), 0, g.stream, g.d, 1);
+  hipLaunchKernelGGL(k_ewald_force, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
 }
 
-void launch_bonds(mfe_engine *e) {
-  hipLaunchKernelGGL(k_bonds, dim3(nblocks(e->np)), dim3(kBlock), 0, e->stream, e->ro,
-                     e->np, e->ch_prev, e->ch_next, e->box, e->bond_k, e->bond_p,
-                     e->bond_eq, 1, e->force, e->part + e->off_bond);
+void launch_bonds(const Group &g) {
+  hipLaunchKernelGGL(k_bonds, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d, 1);
 }
 
-void launch_surf(mfe_engine *e) {
-  hipLaunchKernelGGL(k_surf_sum, dim3(nblocks(e->np)), dim3(kBlock), 0, e->stream, e->ro,
-                     e->q, e->np, e->part + e->off_surf);
+void launch_surf(const Group &g) {
+  hipLaunchKernelGGL(k_surf_sum, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
 }
 
-void launch_reduce(mfe_engine *e, unsigned flags, bool skin) {
-  ReduceArgs a;
-  a.nptpt = e->nptpt;
-  a.nblk_pair = e->nblk_pair;
-  a.nblk_conv = e->ewald == MFE_EWALD_SPME ? conv_blocks(e) : 0;
-  a.nblk_bond = e->nchain > 0 ? e->nblk_bond : 0;
-  a.nblk_surf = e->nblk_surf;
-  a.nk = e->nkvec;
-  a.nblk_move = e->nblk_move;
-  a.off_pair = e->off_pair;
-  a.off_conv = e->off_conv;
-  a.off_bond = e->off_bond;
-  a.off_surf = e->off_surf;
-  a.off_std = e->off_std;
-  a.off_move = e->off_move;
-  a.off_skin = e->off_skin;
-  a.ewald = e->ewald;
-  a.lsurf = (e->lsurf && e->ewald != MFE_EWALD_NONE) ? 1 : 0;
-  a.energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
-  a.skin = skin ? 1 : 0;
-  a.efp = e->efp;
-  a.self = e->self_energy;
-  a.vol = e->box.L[0] * e->box.L[1] * e->box.L[2];
-  a.drnlist = e->drnlist;
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, e->stream, e->part, a, e->en,
-                     e->flags);
+void launch_reduce(const Group &g, unsigned flags, bool skin, bool bd) {
+  const int energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
+  hipLaunchKernelGGL(k_reduce, grid(1, g), dim3(kBlock), 0, g.stream, g.d, energy,
+                     skin ? 1 : 0, bd ? 1 : 0);
 }
 
-void launch_finalize(mfe_engine *e, int mode, double dt) {
-  const double sclvel = 1.0e-10 / 1.0e-12, sclacc = sclvel / 1.0e-12;
-  const double ffac = (1.0e+03 / 1.0e-10) / (sclacc * 1.0e-03);
-  const double surf_fac = e->efp * (2.0 * (2.0 * 3.1415926535897932 /
-                                           (3.0 * e->box.L[0] * e->box.L[1] * e->box.L[2])));
-  hipLaunchKernelGGL(k_finalize, dim3(nblocks(e->np)), dim3(kBlock), 0, e->stream,
-                     e->force, e->forceo, e->q, e->massip, e->np, ffac,
-                     (e->lsurf && e->ewald != MFE_EWALD_NONE) ? 1 : 0, surf_fac, e->en,
-                     mode, dt, e->rod, e->rodd);
+void launch_finalize(const Group &g, int mode, double dt) {
+  hipLaunchKernelGGL(k_finalize, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d, mode,
+                     dt);
 }
 
-void launch_move(mfe_engine *e, double dt, int bd, const double *fac1,
-                 const double *fac2, const double *noise) {
-  hipLaunchKernelGGL(k_move, dim3(e->nblk_move), dim3(kBlock), 0, e->stream, e->ro,
-                     e->rod, e->rodd, e->forceo, e->type, fac1, fac2, noise, e->np,
-                     e->box, dt, bd, e->drosum, e->r2disp, e->part + e->off_move);
+void launch_move(const Group &g, double dt, int bd) {
+  hipLaunchKernelGGL(k_move, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d, dt, bd);
 }
 
-void launch_skin(mfe_engine *e) {
-  hipLaunchKernelGGL(k_skin_blocks, dim3(e->nblk_move), dim3(kBlock), 0, e->stream,
-                     e->r2disp, e->np, e->part + e->off_move, e->part + e->off_skin);
+// flags[0..3] and en[0..nen) of every replica into one array each
+__global__ void k_collect(const SysDev *__restrict__ sys, int *fl, double *en, int nen) {
+  const SysDev &V = sys[blockIdx.z];
+  for (int k = threadIdx.x; k < 4; k += blockDim.x) fl[4 * blockIdx.z + k] = V.flags[k];
+  for (int k = threadIdx.x; k < nen; k += blockDim.x) en[(size_t)nen * blockIdx.z + k] = V.en[k];
+}
+
+void launch_collect(const Group &g, int *fl, double *en, int nen) {
+  hipLaunchKernelGGL(k_collect, grid(1, g), dim3(64), 0, g.stream, g.d, fl, en, nen);
+}
+
+void launch_adjoint(const Group &g, int ncol, double *out) {
+  // LDS of the knot accumulators: the largest replica's nptpt*nknot doubles
+  const size_t lds = (size_t)std::max(ncol - 1, 1) * sizeof(double);
+  hipLaunchKernelGGL(k_adjoint, grid(g.np_blocks, g), dim3(kBlock), lds, g.stream, g.d);
+  hipLaunchKernelGGL(k_adjoint_q, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
+  hipLaunchKernelGGL(k_adjoint_sum, grid(nblocks(ncol), g), dim3(kBlock), 0, g.stream, g.d,
+                     out);
+}
+
+void launch_skin(const Group &g) {
+  hipLaunchKernelGGL(k_skin_blocks, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
 }
 
 }  // namespace mfe
@@ -1295,6 +1554,8 @@ void mfe_allow_big_spread_lds(int prec, int bytes) {
   (void)hipFuncSetAttribute((const void *)mfe::k_spread_ptr<R, O>(),          \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes); \
   (void)hipFuncSetAttribute((const void *)mfe::k_gather_ptr<R, O>(),          \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes); \
+  (void)hipFuncSetAttribute((const void *)mfe::k_gather_pot_ptr<R, O>(),      \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (prec == MFE_PREC_MIXED) {
     MFE_ATTR(float, 3) MFE_ATTR(float, 4) MFE_ATTR(float, 5)

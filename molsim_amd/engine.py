@@ -72,7 +72,11 @@ SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_set_neighbor", "mfe_set_chains", "mfe_upload_positions",
            "mfe_upload_velocities", "mfe_download_state", "mfe_build_list",
            "mfe_compute", "mfe_step_md", "mfe_step_bd", "mfe_get_energy",
-           "mfe_get_pairs", "mfe_time_stage"]
+           "mfe_get_pairs", "mfe_time_stage", "mfe_set_bd", "mfe_download_forces",
+           "mfe_batch_create", "mfe_batch_destroy", "mfe_batch_last_error",
+           "mfe_batch_failed_replica", "mfe_batch_build_list", "mfe_batch_compute",
+           "mfe_batch_step_md", "mfe_batch_step_bd", "mfe_batch_get_energies",
+           "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint"]
 
 _lib = None
 
@@ -110,6 +114,22 @@ def lib():
         L.mfe_get_energy.argtypes = [C.c_void_p, C.POINTER(EnergyStruct)]
         L.mfe_get_pairs.argtypes = [C.c_void_p, _ip, _ip, C.c_int32]
         L.mfe_time_stage.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp]
+        L.mfe_set_bd.argtypes = [C.c_void_p, _dp, _dp, C.c_uint64]
+        L.mfe_download_forces.argtypes = [C.c_void_p, _dp]
+        L.mfe_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_void_p)]
+        L.mfe_batch_destroy.argtypes = [C.c_void_p]
+        L.mfe_batch_last_error.argtypes = [C.c_void_p]
+        L.mfe_batch_last_error.restype = C.c_char_p
+        L.mfe_batch_failed_replica.argtypes = [C.c_void_p]
+        L.mfe_batch_failed_replica.restype = C.c_int32
+        L.mfe_batch_build_list.argtypes = [C.c_void_p]
+        L.mfe_batch_compute.argtypes = [C.c_void_p, C.c_uint32]
+        L.mfe_batch_step_md.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_uint32]
+        L.mfe_batch_step_bd.argtypes = [C.c_void_p, C.c_int32, C.c_uint32]
+        L.mfe_batch_get_energies.argtypes = [C.c_void_p, C.POINTER(EnergyStruct)]
+        L.mfe_set_pair_params.argtypes = [C.c_void_p, C.POINTER(PotSpec)]
+        L.mfe_set_knots.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, _dp]
+        L.mfe_adjoint.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -147,9 +167,9 @@ def pot_defaults() -> PotSpec:
     return s
 
 
-def build_tables(zat, sigat, epsat, *, rcut, epsi_four_pi, alpha=0.0,
-                 lewald=True, lljcut=True, **overrides) -> Tables:
-    """Tables of the (1,6,12) potential for monoatomic types."""
+def pot_spec(zat, sigat, epsat, *, rcut, epsi_four_pi, alpha=0.0, lewald=True,
+             lljcut=True, **overrides):
+    """(spec, kept arrays): IOPotTwoBody's (1,6,12) specification."""
     zat, sigat, epsat = _f64(zat), _f64(sigat), _f64(epsat)
     s = pot_defaults()
     s.npt = zat.size
@@ -158,6 +178,14 @@ def build_tables(zat, sigat, epsat, *, rcut, epsi_four_pi, alpha=0.0,
     s.lewald, s.lljcut = int(lewald), int(lljcut)
     for k, v in overrides.items():
         setattr(s, k, v)
+    return s, (zat, sigat, epsat)
+
+
+def build_tables(zat, sigat, epsat, *, rcut, epsi_four_pi, alpha=0.0,
+                 lewald=True, lljcut=True, **overrides) -> Tables:
+    """Tables of the (1,6,12) potential for monoatomic types."""
+    s, _keep = pot_spec(zat, sigat, epsat, rcut=rcut, epsi_four_pi=epsi_four_pi, alpha=alpha,
+                        lewald=lewald, lljcut=lljcut, **overrides)
     nptpt = s.npt * (s.npt + 1) // 2
     cap = 200000
     ubuf = np.zeros(cap)
@@ -306,11 +334,69 @@ class ForceEngine:
     def step_md(self, nsteps, tstep, flags=FORCES | ENERGY | VIRIAL):
         self._check(self.L.mfe_step_md(self.h, int(nsteps), float(tstep), flags), "mfe_step_md")
 
-    def step_bd(self, fac1, fac2, noise, flags=FORCES | ENERGY | VIRIAL):
-        noise = _f64(noise)
-        nsteps = noise.size // (3 * self.np)
-        self._check(self.L.mfe_step_bd(self.h, nsteps, _d(_f64(fac1)), _d(_f64(fac2)),
-                                       _d(noise), flags), "mfe_step_bd")
+    def set_bd(self, fac1, fac2, seed=0):
+        """BDStep factors per type and the seed of the device noise stream."""
+        self._check(self.L.mfe_set_bd(self.h, _d(_f64(fac1)), _d(_f64(fac2)), int(seed)),
+                    "mfe_set_bd")
+
+    def step_bd(self, fac1=None, fac2=None, noise=None, nsteps=None,
+                flags=FORCES | ENERGY | VIRIAL):
+        """BDStep x nsteps: with host noise[nsteps, np, 3], or (noise None)
+        the device's counter-based stream for nsteps steps."""
+        f1 = None if fac1 is None else _f64(fac1)
+        f2 = None if fac2 is None else _f64(fac2)
+        if noise is not None:
+            noise = _f64(noise)
+            nsteps = noise.size // (3 * self.np)
+        self._check(self.L.mfe_step_bd(self.h, int(nsteps),
+                                       _d(f1) if f1 is not None else None,
+                                       _d(f2) if f2 is not None else None,
+                                       _d(noise) if noise is not None else None, flags),
+                    "mfe_step_bd")
+
+    def set_pair_params(self, zat, sigat, epsat, *, rcut, epsi_four_pi, alpha=0.0,
+                        lewald=True, lljcut=True, **overrides):
+        """Parameters of the pair model the adjoint differentiates."""
+        s, keep = pot_spec(zat, sigat, epsat, rcut=rcut, epsi_four_pi=epsi_four_pi,
+                           alpha=alpha, lewald=lewald, lljcut=lljcut, **overrides)
+        self._check(self.L.mfe_set_pair_params(self.h, C.byref(s)), "mfe_set_pair_params")
+        del keep
+        self.nptpt = self.npt * (self.npt + 1) // 2
+
+    def set_knots(self, r_lo, r_hi, c):
+        """Additive cubic B-spline pair term c[nptpt, nknot] on [r_lo, r_hi)
+        (c None removes it)."""
+        if c is None:
+            self._check(self.L.mfe_set_knots(self.h, 0, 0.0, 0.0, None), "mfe_set_knots")
+            self.nknot = 0
+            return
+        c = _f64(c)
+        self.nknot = c.shape[-1]
+        self._check(self.L.mfe_set_knots(self.h, self.nknot, float(r_lo), float(r_hi),
+                                         _d(c.reshape(-1))), "mfe_set_knots")
+
+    def adjoint(self):
+        """Model energy and its gradient: dict(E, dq[np], dzat[npt],
+        dsigat[npt], depsat[npt], dknot[nptpt, nknot], dro[np, 3])."""
+        nk = getattr(self, "nknot", 0)
+        nptpt = self.npt * (self.npt + 1) // 2
+        out = dict(dq=np.zeros(self.np), dzat=np.zeros(self.npt), dsigat=np.zeros(self.npt),
+                   depsat=np.zeros(self.npt), dknot=np.zeros(max(nptpt * nk, 1)),
+                   dro=np.zeros(3 * self.np))
+        e = C.c_double()
+        self._check(self.L.mfe_adjoint(self.h, _d(out["dq"]), _d(out["dzat"]),
+                                       _d(out["dsigat"]), _d(out["depsat"]),
+                                       _d(out["dknot"]), _d(out["dro"]), C.byref(e)),
+                    "mfe_adjoint")
+        out["dknot"] = out["dknot"][:nptpt * nk].reshape(nptpt, nk)
+        out["dro"] = out["dro"].reshape(-1, 3)
+        out["E"] = e.value
+        return out
+
+    def download_forces(self):
+        f = np.zeros(3 * self.np)
+        self._check(self.L.mfe_download_forces(self.h, _d(f)), "mfe_download_forces")
+        return f.reshape(-1, 3)
 
     def energy(self) -> Energies:
         e = EnergyStruct()
@@ -335,3 +421,61 @@ class ForceEngine:
     def _energies(e: EnergyStruct) -> Energies:
         return Energies(e.tot, e.twob0, e.rec, e.bond, e.virial,
                         np.array(e.twob[:e.nptpt]), e.rebuilds)
+
+
+class Batch:
+    """Replicas driven together (mfe_batch_*): every stage is one launch
+    over all replicas.  The engines stay owned by the caller and must
+    outlive the batch."""
+
+    def __init__(self, engines):
+        self.L = lib()
+        self.engines = list(engines)
+        arr = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        rc = self.L.mfe_batch_create(arr, len(self.engines), C.byref(h))
+        if rc != 0:
+            raise MfeError(rc, "mfe_batch_create failed (see stderr)")
+        self.h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.L.mfe_batch_last_error(self.h).decode()
+            err = MfeError(rc, f"{what}: {msg}")
+            err.replica = self.L.mfe_batch_failed_replica(self.h)
+            raise err
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.mfe_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def build_list(self):
+        self._check(self.L.mfe_batch_build_list(self.h), "mfe_batch_build_list")
+
+    def compute(self, flags=FORCES | ENERGY | VIRIAL):
+        self._check(self.L.mfe_batch_compute(self.h, flags), "mfe_batch_compute")
+
+    def step_md(self, nsteps, tstep, flags=FORCES | ENERGY | VIRIAL):
+        self._check(self.L.mfe_batch_step_md(self.h, int(nsteps), float(tstep), flags),
+                    "mfe_batch_step_md")
+
+    def step_bd(self, nsteps, flags=FORCES | ENERGY | VIRIAL):
+        self._check(self.L.mfe_batch_step_bd(self.h, int(nsteps), flags), "mfe_batch_step_bd")
+
+    def energies(self):
+        arr = (EnergyStruct * len(self.engines))()
+        self._check(self.L.mfe_batch_get_energies(self.h, arr), "mfe_batch_get_energies")
+        return [ForceEngine._energies(e) for e in arr]

@@ -147,6 +147,14 @@ def _declare(L):
     L.orc_velver_kick.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp]
     L.orc_bd_step.argtypes = [C.POINTER(Box), C.c_int, _ip, _dp, _dp, _dp, _dp,
                               _dp, _dp]
+    L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                    C.POINTER(C.c_uint32)]
+    L.orc_bd_noise.argtypes = [C.c_uint64, C.c_int, C.c_uint32, _dp]
+    L.orc_model_pair_energy.restype = C.c_double
+    L.orc_model_pair_energy.argtypes = [C.POINTER(Box), C.c_int, _ip, _dp, C.c_int, _dp, _dp,
+                                        _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.c_double, C.c_double, C.c_double, C.c_int,
+                                        C.c_double, C.c_double, _dp, _dp]
     L.orc_gaurandom.restype = C.c_double
     L.orc_gaurandom.argtypes = [C.POINTER(C.c_int32), C.POINTER(GauState)]
 
@@ -513,6 +521,47 @@ def velver_kick(tstep, massip, forceo, rod, rodd):
     roddo = np.zeros_like(rodd)
     lib().orc_velver_kick(rod.size // 3, float(tstep), _d(f64(massip)),
                           _d(f64(forceo)), _d(rod), _d(rodd), _d(roddo))
+
+
+def philox4x32_10(ctr, key):
+    """Philox4x32-10 of one counter (4 words) under one key (2 words)."""
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return [int(v) for v in o]
+
+
+def bd_noise(seed, np_, step):
+    """The engine's device BD noise of one step: [np, 3]."""
+    out = np.zeros(3 * np_)
+    lib().orc_bd_noise(int(seed), int(np_), int(step), _d(out))
+    return out.reshape(-1, 3)
+
+
+def model_pair_energy(boxlen, iptpn, ro, zat, sigat, epsat, efp, alpha, rcut, *,
+                      lewald=True, lljcut=True, ljrcut=None, ljushift=1.0, knots=None):
+    """The pair model of the engine's adjoint (exact erfc, brute force):
+    (energy, spline-term forces [np, 3]).  knots = (r0, r1, c[nptpt, nknot])."""
+    b = make_box(boxlen)
+    ro = f64(ro).reshape(-1)
+    np_ = ro.size // 3
+    if ljrcut is None:
+        # ljrcut = Two**(1.0/6.0) with a default-real exponent (Src/potential.F90)
+        ljrcut = 2.0 ** float(np.float32(1.0) / np.float32(6.0))
+    fk = np.zeros(3 * np_)
+    if knots is None:
+        nk, r0, r1, c = 0, 0.0, 1.0, np.zeros(1)
+    else:
+        r0, r1, c = knots
+        c = f64(c)
+        nk = c.shape[-1]
+    e = lib().orc_model_pair_energy(C.byref(b), np_, _i(i32(iptpn)), _d(ro), len(zat),
+                                    _d(f64(zat)), _d(f64(sigat)), _d(f64(epsat)), float(efp),
+                                    float(alpha), int(lewald), int(lljcut), float(ljrcut),
+                                    float(ljushift), float(rcut), int(nk), float(r0),
+                                    float(r1), _d(c.reshape(-1)), _d(fk))
+    return e, fk.reshape(-1, 3)
 
 
 def bd_step(boxlen, iptpn, fac1, fac2, forceo, noise, ro):

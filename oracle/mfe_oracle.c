@@ -1338,3 +1338,108 @@ double orc_random(int32_t *idum, orc_random_state *st) {
   if (st->iy < 0) st->iy += im;
   return st->am * (double)((im & (st->ix ^ st->iy)) | 1);
 }
+
+/* ---- engine extensions without a reference counterpart ---- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    const uint32_t n3 = (uint32_t)p0;
+    c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+void orc_bd_noise(uint64_t seed, int np, uint32_t step, double *noise) {
+  const double s = 1.0 / 4294967296.0;
+  const double twopi = 6.283185307179586476925286766559;
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  for (int i = 0; i < np; ++i) {
+    const uint32_t ctr[4] = {(uint32_t)i, step, 0u, 0u};
+    uint32_t x[4];
+    orc_philox4x32_10(ctr, key, x);
+    const double u0 = ((double)x[0] + 0.5) * s, u1 = ((double)x[1] + 0.5) * s;
+    const double u2 = ((double)x[2] + 0.5) * s, u3 = ((double)x[3] + 0.5) * s;
+    const double r0 = sqrt(-2.0 * log(u0)), r1 = sqrt(-2.0 * log(u2));
+    noise[3 * i] = r0 * cos(twopi * u1);
+    noise[3 * i + 1] = r0 * sin(twopi * u1);
+    noise[3 * i + 2] = r1 * cos(twopi * u3);
+  }
+}
+
+int orc_knot_weights(double r, double r0, double dinv, int nknot, int *i, double w[4],
+                     double dw[4]) {
+  const double x = (r - r0) * dinv;
+  if (!(x >= 0.0) || !(x < (double)(nknot - 3))) return 0;
+  *i = (int)floor(x);
+  const double f = x - *i, g = 1.0 - f;
+  /* de Boor's uniform cubic B-spline segments */
+  w[0] = g * g * g / 6.0;
+  w[1] = (3.0 * f * f * f - 6.0 * f * f + 4.0) / 6.0;
+  w[2] = (-3.0 * f * f * f + 3.0 * f * f + 3.0 * f + 1.0) / 6.0;
+  w[3] = f * f * f / 6.0;
+  dw[0] = -0.5 * g * g;
+  dw[1] = 1.5 * f * f - 2.0 * f;
+  dw[2] = -1.5 * f * f + f + 0.5;
+  dw[3] = 0.5 * f * f;
+  return 1;
+}
+
+double orc_model_pair_energy(const orc_box *b, int np, const int *iptpn, const double *ro,
+                             int npt, const double *zat, const double *sigat,
+                             const double *epsat, double efp, double alpha, int lewald,
+                             int lljcut, double ljrcut, double ljushift, double rcut,
+                             int nknot, double r0, double r1, const double *c,
+                             double *fknot) {
+  const double dinv = nknot > 3 ? (nknot - 3) / (r1 - r0) : 0.0;
+  double e = 0.0;
+  if (fknot)
+    for (int k = 0; k < 3 * np; ++k) fknot[k] = 0.0;
+  for (int i = 0; i < np; ++i) {
+    const int ti = iptpn[i] - 1;
+    for (int j = i + 1; j < np; ++j) {
+      const int tj = iptpn[j] - 1;
+      double dx = ro[3 * i] - ro[3 * j], dy = ro[3 * i + 1] - ro[3 * j + 1],
+             dz = ro[3 * i + 2] - ro[3 * j + 2];
+      double r2;
+      orc_pbcr2(b, &dx, &dy, &dz, &r2);
+      if (r2 > rcut * rcut) continue;
+      const double r = sqrt(r2);
+      const double c0 = lewald ? 1.0 - orc_erf2(alpha * r) : 1.0;
+      e += zat[ti] * zat[tj] * efp * c0 / r;
+      const double sig = 0.5 * (sigat[ti] + sigat[tj]);
+      const double eps = sqrt(epsat[ti] * epsat[tj]);
+      if (!lljcut || r < ljrcut * sig) {
+        const double x = sig / r, s6 = x * x * x * x * x * x;
+        e += 4.0 * eps * (s6 * s6 - s6) + (lljcut ? ljushift * eps : 0.0);
+      }
+      if (nknot > 3) {
+        int ki;
+        double w[4], dw[4];
+        if (orc_knot_weights(r, r0, dinv, nknot, &ki, w, dw)) {
+          /* Set_iptpt order: (1,1), (1,2), ..., (1,npt), (2,2), ... */
+          const int a = ti < tj ? ti : tj, bb = ti < tj ? tj : ti;
+          const int p = a * npt - a * (a - 1) / 2 + (bb - a);
+          const double *cp = c + (size_t)p * nknot + ki;
+          e += cp[0] * w[0] + cp[1] * w[1] + cp[2] * w[2] + cp[3] * w[3];
+          if (fknot) {
+            const double dup = (cp[0] * dw[0] + cp[1] * dw[1] + cp[2] * dw[2] + cp[3] * dw[3]) * dinv;
+            const double fr = -dup / r;
+            fknot[3 * i] += fr * dx; fknot[3 * i + 1] += fr * dy; fknot[3 * i + 2] += fr * dz;
+            fknot[3 * j] -= fr * dx; fknot[3 * j + 1] -= fr * dy; fknot[3 * j + 2] -= fr * dz;
+          }
+        }
+      }
+    }
+  }
+  return e;
+}

@@ -223,6 +223,31 @@ void orc_bd_step(const orc_box *b, int np, const int *iptpn,
 /* ---- random numbers (Src/mollib.F90:2689-2766) ---- */
 typedef struct { int flag; double save; } orc_gau_state;
 double orc_gaurandom(int32_t *iseed, orc_gau_state *st);
+
+/* ---- engine extensions without a reference counterpart ---- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the generator of the
+   engine's per-replica BD noise; pinned by the Random123 known-answer
+   vectors. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* noise[3*np] of BD step n: particle i draws Philox4x32-10 with counter
+   (i, n, 0, 0) and key (seed lo, seed hi); uniforms u_k = (x_k + 1/2) 2^-32;
+   g0, g1 = sqrt(-2 ln u0) (cos, sin)(2 pi u1), g2 = sqrt(-2 ln u2) cos(2 pi u3) */
+void orc_bd_noise(uint64_t seed, int np, uint32_t step, double *noise);
+/* Uniform cubic B-spline of the engine's additive pair term: weights and
+   f-derivatives of coefficients i..i+3 at r (0 outside [r0, r0+(nknot-3)/dinv)). */
+int orc_knot_weights(double r, double r0, double dinv, int nknot, int *i, double w[4],
+                     double dw[4]);
+/* The pair model the engine's adjoint differentiates, by brute force over
+   all pairs with r2 <= rcut^2 (minimum image): Pot_1_6_12 with Erf2 for the
+   Ewald damping (Src/potential.F90:1349-1425) plus the spline term with
+   coefficients c[nptpt][nknot] (Set_iptpt pair order).  Returns the energy;
+   fknot (3*np, may be NULL) receives the spline term's forces. */
+double orc_model_pair_energy(const orc_box *b, int np, const int *iptpn, const double *ro,
+                             int npt, const double *zat, const double *sigat,
+                             const double *epsat, double efp, double alpha, int lewald,
+                             int lljcut, double ljrcut, double ljushift, double rcut,
+                             int nknot, double r0, double r1, const double *c,
+                             double *fknot);
 typedef struct { int32_t ix, iy; double am; } orc_random_state;
 double orc_random(int32_t *idum, orc_random_state *st);
 

@@ -90,3 +90,20 @@ def test_fortran_binding_compiles(tmp_path):
     r = subprocess.run(["amdflang", "-c", mod, "-o", str(tmp_path / "m.o")],
                        cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_autotune_mesh_rule():
+    from molsim_amd import autotune as A
+    assert A.even_ceil(15.2) == 16 and A.even_ceil(16.0) == 16 and A.even_ceil(17.0) == 18
+    # alpha*h <= target: s >= alpha L / ah, even, at least 16
+    assert [n for n in range(16, 50, 2) if A.smooth(n)] == [16, 18, 20, 24, 28, 30, 32, 36,
+                                                             40, 42, 48]
+    for L, a, ah in [(116.1, 0.25, 1.0), (298.7, 0.25, 1.3), (74.6, 0.3, 2.0),
+                     (300.0, 0.3, 0.45)]:
+        s = A.mesh_for(L, a, ah)
+        assert s % 2 == 0 and s >= 16 and a * L / s <= ah + 1e-12 and A.smooth(s)
+        # no smaller even 7-smooth mesh reaches the spacing
+        assert all(a * L / m > ah for m in range(16, s, 2) if A.smooth(m))
+    sysm = S.c4_replica(0)
+    t = A.with_params(sysm, 0.2, 5, 30)
+    assert (t.alpha, t.order, t.mesh) == (0.2, 5, 30) and abs(t.rcut - 15.0) < 1e-12

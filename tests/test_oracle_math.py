@@ -201,3 +201,116 @@ def test_skin_test_second_max_rule():
     drostep2 = drostep[[3, 4, 5, 0, 1, 2]]  # larger first: r2max2 = 3
     assert O.skin_test(drosum, drostep2, 2.5) is True
     assert not drosum.any()
+
+
+# ---- engine extensions without a reference counterpart -------------------
+def test_philox_known_answers():
+    """Philox4x32-10 against the Random123 known-answer vectors (counter,
+    key -> output) for the all-zero, all-ones and pi-digit inputs."""
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+        ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+        ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+         [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+    ]
+    for ctr, key, want in kat:
+        assert O.philox4x32_10(ctr, key) == want
+
+
+def test_bd_noise_is_standard_normal_and_counter_based():
+    n0 = O.bd_noise(12345, 100000, 0)
+    # moments of N(0, 1) within 5 standard errors
+    se = 1.0 / np.sqrt(n0.shape[0])
+    assert np.all(np.abs(n0.mean(0)) < 5 * se)
+    assert np.all(np.abs(n0.var(0) - 1.0) < 5 * np.sqrt(2.0) * se)
+    assert abs(np.corrcoef(n0[:, 0], n0[:, 1])[0, 1]) < 5 * se
+    # Kolmogorov-Smirnov distance to the normal CDF
+    from scipy.stats import kstest
+    assert kstest(n0[:, 2], "norm").pvalue > 1e-4
+    # counter-based: a particle's draw depends only on (seed, i, step)
+    sub = O.bd_noise(12345, 1000, 0)
+    assert np.array_equal(sub, n0[:1000])
+    assert not np.allclose(O.bd_noise(12345, 1000, 1), sub)
+    assert not np.allclose(O.bd_noise(12346, 1000, 0), sub)
+
+
+def test_knot_weights_closed_forms():
+    """Uniform cubic B-spline segments: partition of unity, linear
+    reproduction (coefficients c_k = a + b k give a + b (x + 1) on segment
+    coordinates x = (r - r0)/h), derivatives by central differences, zero
+    outside [r0, r0 + (nknot - 3) h)."""
+    import ctypes as C
+    lib = O.lib()
+    lib.orc_knot_weights.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int,
+                                     C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+    r0, r1, nk = 3.0, 12.0, 16
+    dinv = (nk - 3) / (r1 - r0)
+
+    def wts(r):
+        i = C.c_int()
+        w = (C.c_double * 4)()
+        dw = (C.c_double * 4)()
+        ok = lib.orc_knot_weights(r, r0, dinv, nk, C.byref(i), w, dw)
+        return ok, i.value, np.array(w[:]), np.array(dw[:])
+
+    rng = np.random.default_rng(3)
+    for r in rng.uniform(r0, r1, 200):
+        ok, i, w, dw = wts(r)
+        assert ok and 0 <= i <= nk - 4
+        assert abs(w.sum() - 1.0) < 1e-15 and abs(dw.sum()) < 1e-15
+        x = (r - r0) * dinv
+        a, b = 0.7, -1.3
+        c = a + b * (i + np.arange(4))
+        assert abs((c * w).sum() - (a + b * (x + 1.0))) < 1e-13
+        h = 1e-6
+        _, i2, wp, _ = wts(r + h)
+        _, i3, wm, _ = wts(r - h)
+        if i2 == i and i3 == i:
+            assert np.allclose((wp - wm) / (2 * h), dw * dinv, atol=1e-8)
+    assert wts(r0 - 1e-9)[0] == 0 and wts(r1)[0] == 0
+
+
+def test_model_pair_energy_two_particles():
+    """Two ions: E = q1 q2 efp erfc(alpha r)/r + 4 eps (s^12 - s^6) + eps
+    inside the WCA cut, 0 LJ beyond; Lorentz-Berthelot mixing; minimum
+    image; pairs beyond rc excluded."""
+    L = 40.0
+    efp = 138.935485 / 78.4
+    zat, sig, eps = [1.0, -2.0], [3.0, 5.0], [1.5, 0.6]
+    alpha, rc = 0.3, 12.0
+    s, e = 0.5 * (sig[0] + sig[1]), math.sqrt(eps[0] * eps[1])
+    ljrcut = 2.0 ** float(np.float32(1.0) / np.float32(6.0))
+    for r in [3.7, 4.4, 4.6, 9.0, 11.9, 12.5]:
+        # across the periodic boundary: x = -L/2 + 0.1 and x = L/2 + 0.1 - r
+        ro = np.array([[-L / 2 + 0.1, 1.0, -2.0], [L / 2 + 0.1 - r, 1.0, -2.0]])
+        got, _ = O.model_pair_energy(L, [1, 2], ro, zat, sig, eps, efp, alpha, rc)
+        want = 0.0
+        if r <= rc:
+            want = zat[0] * zat[1] * efp * math.erfc(alpha * r) / r
+            if r < ljrcut * s:
+                x6 = (s / r) ** 6
+                want += 4 * e * (x6 * x6 - x6) + e
+        assert abs(got - want) <= 1e-13 * max(1.0, abs(want)), r
+
+
+def test_model_spline_forces_are_minus_gradient():
+    rng = np.random.default_rng(11)
+    L, n = 30.0, 40
+    ro = rng.uniform(-L / 2, L / 2, (n, 3))
+    iptpn = rng.integers(1, 3, n)
+    c = rng.normal(size=(3, 12))
+    knots = (2.0, 10.0, c)
+    z = [0.0, 0.0]
+    e0, fk = O.model_pair_energy(L, iptpn, ro, z, [1.0, 1.0], [0.0, 0.0], 1.0, 0.3, 10.0,
+                                 lljcut=False, knots=knots)
+    h = 1e-6
+    for i, m in [(0, 0), (5, 1), (17, 2), (31, 0)]:
+        rp, rm = ro.copy(), ro.copy()
+        rp[i, m] += h
+        rm[i, m] -= h
+        ep, _ = O.model_pair_energy(L, iptpn, rp, z, [1.0, 1.0], [0.0, 0.0], 1.0, 0.3, 10.0,
+                                    lljcut=False, knots=knots)
+        em, _ = O.model_pair_energy(L, iptpn, rm, z, [1.0, 1.0], [0.0, 0.0], 1.0, 0.3, 10.0,
+                                    lljcut=False, knots=knots)
+        assert abs(-(ep - em) / (2 * h) - fk[i, m]) < 1e-6 * max(1.0, abs(fk[i, m]))
