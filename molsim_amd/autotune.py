@@ -33,9 +33,11 @@ import numpy as np
 from . import engine as E
 from . import systems as S
 
-ALPHAS = (0.20, 0.25, 0.30)
+# alpha*rc = 3: alpha 0.1 -> rc 30 A.  Boxes up to ~320 A on meshes <= 48
+# (h >= 6.7 A) need small alpha for alpha*h <~ 1
+ALPHAS = (0.10, 0.125, 0.15, 0.20, 0.25)
 ORDERS = (4, 5, 6)
-AH_LADDER = (2.0, 1.8, 1.6, 1.45, 1.3, 1.15, 1.0, 0.9, 0.8, 0.7, 0.6)
+AH_LADDER = (2.0, 1.8, 1.6, 1.45, 1.3, 1.15, 1.0, 0.9, 0.8, 0.7, 0.6, 0.5)
 MESH_MIN, MESH_MAX = 16, 48
 
 
@@ -94,7 +96,8 @@ def rel_rms(a, b):
 
 def calibrate(sample, alpha, order, tol, cache, precision="mixed"):
     """Largest alpha*h on the ladder at which every sampled replica's force
-    rel-RMS against the FP64 order-8 reference is <= tol (None if none)."""
+    rel-RMS against the FP64 order-8 reference is <= tol (None if none),
+    and the measured error of every rung tried (largest over the sample)."""
     refs = []
     for s in sample:
         s_ref = with_params(s, alpha, 8, min(256, mesh_for(s.boxlen, alpha, 0.45)))
@@ -147,6 +150,9 @@ class Plan:
     rcut: float
     sec_per_step: float
     table: dict          # (alpha, order) -> {"ah", "errors", "sec_per_step"}
+    error: float = 0.0   # measured force rel-RMS at the chosen alpha*h (sample max)
+    met: bool = True     # False: no candidate reached the tolerance; the
+                         # most accurate feasible one was taken
 
     def apply(self, systems):
         return [with_params(s, self.alpha, self.order, m) for s, m in zip(systems, self.meshes)]
@@ -164,11 +170,15 @@ def autotune(systems, tol=1e-4, alphas=ALPHAS, orders=ORDERS, n_sample=4, n_time
     tsample = [systems[i] for i in sorted(set(tidx))]
     table = {}
     best = None
+    fallback = None     # (error, alpha, order, ah): most accurate feasible rung
     for a in alphas:
         for p in orders:
             ah, errs = calibrate(sample, a, p, tol, cache, precision)
             entry = {"ah": ah, "errors": errs, "sec_per_step": None}
             table[(a, p)] = entry
+            for ah_k, e_k in errs.items():
+                if fallback is None or e_k < fallback[0]:
+                    fallback = (e_k, a, p, ah_k)
             if ah is None:
                 continue
             meshes = [mesh_for(s.boxlen, a, ah) for s in systems]
@@ -179,9 +189,14 @@ def autotune(systems, tol=1e-4, alphas=ALPHAS, orders=ORDERS, n_sample=4, n_time
             entry["sec_per_step"] = sec
             if best is None or sec < best.sec_per_step:
                 best = Plan(a, p, ah, meshes, min(3.0 / a, 0.45 * min(s.boxlen for s in systems)),
-                            sec, table)
+                            sec, table, error=errs[ah])
     if best is None:
-        raise RuntimeError(f"no (alpha, order) reaches force rel-RMS {tol} with meshes "
-                           f"<= {MESH_MAX}")
+        if fallback is None:
+            raise RuntimeError(f"no (alpha, order) fits meshes <= {MESH_MAX}")
+        e, a, p, ah = fallback
+        meshes = [mesh_for(s.boxlen, a, ah) for s in systems]
+        tm = [with_params(s, a, p, mesh_for(s.boxlen, a, ah)) for s in tsample]
+        best = Plan(a, p, ah, meshes, min(3.0 / a, 0.45 * min(s.boxlen for s in systems)),
+                    time_bd(tm, time_steps, cache, precision), table, error=e, met=False)
     best.table = table
     return best

@@ -242,21 +242,21 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   v.nptpt = e->nptpt;
   v.nlsem = e->nlkind == MFE_NL_CELL ? 1 : 0;
   v.box = e->box;
-  v.ro = e->ro; v.rod = e->rod; v.rodd = e->rodd; v.force = e->force; v.forceo = e->forceo;
-  v.drosum = e->drosum; v.r2disp = e->r2disp; v.type = e->type; v.q = e->q;
-  v.massip = e->massip;
+  MFE_SETP(v.ro, e->ro); MFE_SETP(v.rod, e->rod); MFE_SETP(v.rodd, e->rodd); MFE_SETP(v.force, e->force); MFE_SETP(v.forceo, e->forceo);
+  MFE_SETP(v.drosum, e->drosum); MFE_SETP(v.r2disp, e->r2disp); MFE_SETP(v.type, e->type); MFE_SETP(v.q, e->q);
+  MFE_SETP(v.massip, e->massip);
   mfe::TableDev &t = v.tab;
-  t.zlow = e->t_zlow; t.coef = e->t_coef; t.coef_f = e->t_coef_f; t.off = e->t_off;
-  t.nint = e->t_nint; t.r2atat = e->t_r2atat; t.r2umin = e->t_r2umin; t.pairidx = e->t_pairidx;
+  MFE_SETP(t.zlow, e->t_zlow); MFE_SETP(t.coef, e->t_coef); MFE_SETP(t.coef_f, e->t_coef_f); MFE_SETP(t.off, e->t_off);
+  MFE_SETP(t.nint, e->t_nint); MFE_SETP(t.r2atat, e->t_r2atat); MFE_SETP(t.r2umin, e->t_r2umin); MFE_SETP(t.pairidx, e->t_pairidx);
   t.npt = e->npt; t.nptpt = e->nptpt; t.rcut2 = e->rcut * e->rcut;
   const double rs = e->rcut + e->drnlist;
   v.rs2 = rs * rs;
   v.drnlist = e->drnlist;
   v.maxnn = e->maxnn;
   v.cg = e->cg;
-  v.c_count = e->c_count; v.c_start = e->c_start; v.c_of = e->c_of; v.c_slot = e->c_slot;
-  v.c_sorted = e->c_sorted; v.c_pos = e->c_pos; v.ro_s = e->ro_s; v.type_s = e->type_s; v.nn = e->nn;
-  v.nbr = e->nbr; v.flags = e->flags;
+  MFE_SETP(v.c_count, e->c_count); MFE_SETP(v.c_start, e->c_start); MFE_SETP(v.c_of, e->c_of); MFE_SETP(v.c_slot, e->c_slot);
+  MFE_SETP(v.c_sorted, e->c_sorted); MFE_SETP(v.c_pos, e->c_pos); MFE_SETP(v.ro_s, e->ro_s); MFE_SETP(v.type_s, e->type_s); MFE_SETP(v.nn, e->nn);
+  MFE_SETP(v.nbr, e->nbr); MFE_SETP(v.flags, e->flags);
   v.ewald = e->ewald;
   v.lsurf = (e->lsurf && e->ewald != MFE_EWALD_NONE) ? 1 : 0;
   v.alpha = e->alpha;
@@ -273,41 +273,41 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
     v.sg.order = e->order;
     v.hx = e->hx;
     v.nbricks = e->nbricks;
-    v.mesh_r = e->mesh_r; v.mesh_c = e->mesh_c;
-    v.b_count = e->b_count; v.b_start = e->b_start; v.b_id = e->b_id; v.b_slot = e->b_slot;
-    v.rec_frac = e->rec_frac; v.rec_meta = e->rec_meta;
+    MFE_SETP(v.mesh_r, e->mesh_r); MFE_SETP(v.mesh_c, e->mesh_c);
+    MFE_SETP(v.b_count, e->b_count); MFE_SETP(v.b_start, e->b_start); MFE_SETP(v.b_id, e->b_id); MFE_SETP(v.b_slot, e->b_slot);
+    MFE_SETP(v.rec_frac, e->rec_frac); MFE_SETP(v.rec_meta, e->rec_meta);
     const int smax2 = std::max(e->s[0], std::max(e->s[1], e->s[2])) / 2 + 1;
     for (int a = 0; a < 3; ++a) {
-      v.ax.g[a] = e->ax_g + a * smax2;
-      v.ax.kk[a] = e->ax_kk + a * smax2;
+      MFE_SETP(v.ax.g[a], e->ax_g + a * smax2);
+      MFE_SETP(v.ax.kk[a], e->ax_kk + a * smax2);
     }
     v.ax.fac0 = 4.0 * kPi / v.vol;
     v.ax.expf = -1.0 / (4.0 * e->alpha * e->alpha);
     for (int m = 0; m < 3; ++m) {
       v.plan[m] = e->plan[m];
-      v.tw[m] = e->tw[m];
+      MFE_SETP(v.tw[m], e->tw[m]);
     }
     v.plan_half = e->plan_half;
-    v.tw_half = e->tw_half;
+    MFE_SETP(v.tw_half, e->tw_half);
     size_t l0, l1, l2;
     mfe::fft_geometry(e->prec, e->s, &v.fx_rpb, &v.fx_nblk, &v.fy_T, &v.fy_nbx, &v.fz_T,
                       &v.fz_nbx, &l0, &l1, &l2);
   }
-  v.k_vec = e->k_vec; v.k_sum = e->k_sum; v.nkvec = e->nkvec;
-  v.nchain = e->nchain; v.bond_p = e->bond_p; v.ch_prev = e->ch_prev; v.ch_next = e->ch_next;
+  MFE_SETP(v.k_vec, e->k_vec); MFE_SETP(v.k_sum, e->k_sum); v.nkvec = e->nkvec;
+  v.nchain = e->nchain; v.bond_p = e->bond_p; MFE_SETP(v.ch_prev, e->ch_prev); MFE_SETP(v.ch_next, e->ch_next);
   v.bond_k = e->bond_k; v.bond_eq = e->bond_eq;
-  v.bd_fac1 = e->bd_fac1; v.bd_fac2 = e->bd_fac2; v.noise = e->bd_noise; v.seed = e->bd_seed;
+  MFE_SETP(v.bd_fac1, e->bd_fac1); MFE_SETP(v.bd_fac2, e->bd_fac2); MFE_SETP(v.noise, e->bd_noise); v.seed = e->bd_seed;
   v.nknot = e->nknot;
   v.knot_r0 = e->knot_r0;
   v.knot_dinv = e->knot_dinv;
-  v.knot_c = e->knot_c;
+  MFE_SETP(v.knot_c, e->knot_c);
   v.pp = e->pp;
-  v.phi_rec = e->phi_rec;
-  v.adj_phi = e->adj_phi;
-  v.adj_dq = e->adj_dq;
-  v.adj_part = e->adj_part;
+  MFE_SETP(v.phi_rec, e->phi_rec);
+  MFE_SETP(v.adj_phi, e->adj_phi);
+  MFE_SETP(v.adj_dq, e->adj_dq);
+  MFE_SETP(v.adj_part, e->adj_part);
   v.adj_ncol = 1 + 2 * e->npt + e->nptpt * e->nknot;
-  v.part = e->part; v.en = e->en;
+  MFE_SETP(v.part, e->part); MFE_SETP(v.en, e->en);
   v.off_pair = e->off_pair; v.off_conv = e->off_conv; v.off_bond = e->off_bond;
   v.off_surf = e->off_surf; v.off_std = e->off_std; v.off_move = e->off_move;
   v.off_skin = e->off_skin;

@@ -109,7 +109,7 @@ __device__ __forceinline__ void stockham_stage(const cplx<R> *a, cplx<R> *b,
 
 // generic prime radix up to kMaxRadix (meshes with large prime factors)
 template <typename R>
-__device__ void stockham_stage_generic(const cplx<R> *a, cplx<R> *b, int n,
+__device__ __forceinline__ void stockham_stage_generic(const cplx<R> *a, cplx<R> *b, int n,
                                        int Ns, int rad, int batch, int ld,
                                        const cplx<R> *tw, bool inv) {
   const int nbf = n / rad;
@@ -139,7 +139,7 @@ __device__ void stockham_stage_generic(const cplx<R> *a, cplx<R> *b, int n,
 // a[s*ld + k]; b is scratch of the same shape.  Returns the buffer holding
 // the result in natural order.  All threads of the block take part.
 template <typename R>
-__device__ cplx<R> *lds_fft(cplx<R> *a, cplx<R> *b, const FftPlan &p,
+__device__ __forceinline__ cplx<R> *lds_fft(cplx<R> *a, cplx<R> *b, const FftPlan &p,
                             int batch, int ld, const cplx<R> *tw, bool inv) {
   const int n = p.n;
   int Ns = 1;

@@ -6,9 +6,23 @@
 
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "mfe.h"
+
+// Pointers that kernels read out of a device view (SysDev) are global
+// memory, but the compiler cannot know that for a pointer loaded from
+// memory and would emit flat loads and stores (which also wait on the LDS
+// counter).  On the device pass they are declared in the global address
+// space; the host stores plain addresses (same 64-bit layout) through
+// MFE_SETP.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MFE_DPTR(T) __attribute__((address_space(1))) T *
+#else
+#define MFE_DPTR(T) T *
+#endif
+#define MFE_SETP(dst, src) ((dst) = (std::remove_reference_t<decltype(dst)>)(src))
 
 #define MFE_HIP_TRY(expr)                                                    \
   do {                                                                       \
@@ -58,14 +72,14 @@ struct CellGeom {
 // start at index off[p]; zlow[off+k] (FP64, decreasing in k) and the 11
 // coefficients coef[(off+k)*11 + c] (usum: c = 0..5, fsum: c = 6..10).
 struct TableDev {
-  const double *zlow;
-  const double *coef;
-  const float *coef_f;
-  const int *off;
-  const int *nint;
-  const double *r2atat;
-  const double *r2umin;
-  const int *pairidx;  // [npt*npt] -> pair index 0..nptpt-1
+  MFE_DPTR(const double) zlow;
+  MFE_DPTR(const double) coef;
+  MFE_DPTR(const float) coef_f;
+  MFE_DPTR(const int) off;
+  MFE_DPTR(const int) nint;
+  MFE_DPTR(const double) r2atat;
+  MFE_DPTR(const double) r2umin;
+  MFE_DPTR(const int) pairidx;  // [npt*npt] -> pair index 0..nptpt-1
   int npt, nptpt;
   double rcut2;
 };
@@ -81,8 +95,8 @@ struct FftPlan {
 // kk[n] = k_n^2 for n = 0..s/2, so that
 // meshfac(nx,ny,nz) = fac0 * gx*gy*gz / (kkx+kky+kkz)  (Essmann eq. 3.9).
 struct SpmeAxes {
-  const double *g[3];
-  const double *kk[3];
+  MFE_DPTR(const double) g[3];
+  MFE_DPTR(const double) kk[3];
   double fac0;   // 4*pi/V
   double expf;   // -1/(4 alpha^2)
 };
@@ -110,62 +124,83 @@ struct SpmeGeom {
 struct SysDev {
   int np, npt, nptpt, nlsem;   // nlsem: 0 = vlist (r2 <= rcut2), 1 = cell list
   Box box;
-  double *ro, *rod, *rodd, *force, *forceo, *drosum, *r2disp;
-  int *type;
-  double *q, *massip;
+  MFE_DPTR(double) ro;
+  MFE_DPTR(double) rod;
+  MFE_DPTR(double) rodd;
+  MFE_DPTR(double) force;
+  MFE_DPTR(double) forceo;
+  MFE_DPTR(double) drosum;
+  MFE_DPTR(double) r2disp;
+  MFE_DPTR(int) type;
+  MFE_DPTR(double) q;
+  MFE_DPTR(double) massip;
   TableDev tab;
   // neighbour list
   double rs2, drnlist;
   int maxnn;
   CellGeom cg;
-  int *c_count, *c_start, *c_of, *c_slot, *c_sorted;
-  int *c_pos;          // inverse of c_sorted: particle -> cell-order position
-  double *ro_s;
-  int *type_s, *nn, *nbr;
-  int *flags;        // [0] error bits, [1] rebuild flag, [2] rebuilds, [3] BD step
+  MFE_DPTR(int) c_count;
+  MFE_DPTR(int) c_start;
+  MFE_DPTR(int) c_of;
+  MFE_DPTR(int) c_slot;
+  MFE_DPTR(int) c_sorted;
+  MFE_DPTR(int) c_pos;          // inverse of c_sorted: particle -> cell-order position
+  MFE_DPTR(double) ro_s;
+  MFE_DPTR(int) type_s;
+  MFE_DPTR(int) nn;
+  MFE_DPTR(int) nbr;
+  MFE_DPTR(int) flags;        // [0] error bits, [1] rebuild flag, [2] rebuilds, [3] BD step
   // Ewald
   int ewald, lsurf;
   double alpha, efp, self_energy, vol;
   // SPME
   SpmeGeom sg;
   int hx, nbricks;
-  void *mesh_r, *mesh_c;
-  int *b_count, *b_start, *b_id, *b_slot;
-  void *rec_frac;
-  int2 *rec_meta;
+  MFE_DPTR(void) mesh_r;
+  MFE_DPTR(void) mesh_c;
+  MFE_DPTR(int) b_count;
+  MFE_DPTR(int) b_start;
+  MFE_DPTR(int) b_id;
+  MFE_DPTR(int) b_slot;
+  MFE_DPTR(void) rec_frac;
+  MFE_DPTR(int2) rec_meta;
   SpmeAxes ax;
   FftPlan plan[3], plan_half;
-  const void *tw[3], *tw_half;
+  MFE_DPTR(const void) tw[3];
+  MFE_DPTR(const void) tw_half;
   int fx_rpb, fx_nblk;   // x pass: rows per block, blocks
   int fy_T, fy_nbx;      // y pass: kx tile, tiles (grid y = s3)
   int fz_T, fz_nbx;      // conv pass: kx tile, tiles (grid y = s2)
   // standard Ewald
-  const double *k_vec;
-  double *k_sum;
+  MFE_DPTR(const double) k_vec;
+  MFE_DPTR(double) k_sum;
   int nkvec;
   // chains
   int nchain, bond_p;
-  const int *ch_prev, *ch_next;
+  MFE_DPTR(const int) ch_prev;
+  MFE_DPTR(const int) ch_next;
   double bond_k, bond_eq;
   // Brownian dynamics: fac1/fac2 per type, noise (host-supplied) or the
   // counter-based generator keyed by seed
-  const double *bd_fac1, *bd_fac2;
-  const double *noise;
+  MFE_DPTR(const double) bd_fac1;
+  MFE_DPTR(const double) bd_fac2;
+  MFE_DPTR(const double) noise;
   unsigned long long seed;
   // additive cubic B-spline pair term on uniform knots in r (per pair
   // type, nknot coefficients, r in [knot_r0, knot_r0 + (nknot-3)/knot_dinv))
   int nknot;
   double knot_r0, knot_dinv;
-  const double *knot_c;
+  MFE_DPTR(const double) knot_c;
   // adjoint
   PairParams pp;
-  double *phi_rec;    // efp * reciprocal potential per particle (gather)
-  double *adj_phi;    // real-space dE/dq per particle
-  double *adj_dq;     // dE/dq per particle
-  double *adj_part;   // per-block partials [blocks][adj_ncol]
+  MFE_DPTR(double) phi_rec;    // efp * reciprocal potential per particle (gather)
+  MFE_DPTR(double) adj_phi;    // real-space dE/dq per particle
+  MFE_DPTR(double) adj_dq;     // dE/dq per particle
+  MFE_DPTR(double) adj_part;   // per-block partials [blocks][adj_ncol]
   int adj_ncol;
   // reductions and output scaling
-  double *part, *en;
+  MFE_DPTR(double) part;
+  MFE_DPTR(double) en;
   int off_pair, off_conv, off_bond, off_surf, off_std, off_move, off_skin;
   int nblk_np, nblk_pair, nblk_conv;
   double ffac, surf_fac;
