@@ -135,6 +135,7 @@ void free_all(mfe_engine *e) {
   dfree(e->ro_s); dfree(e->type_s);
   dfree(e->mesh_r); dfree(e->mesh_c); dfree(e->b_count); dfree(e->b_start);
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
+  dfree(e->spread_tiles);
   dfree(e->ax_g); dfree(e->ax_kk);
   for (auto &t : e->tw) dfree(t);
   dfree(e->tw_half);
@@ -248,7 +249,7 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   mfe::TableDev &t = v.tab;
   MFE_SETP(t.zlow, e->t_zlow); MFE_SETP(t.coef, e->t_coef); MFE_SETP(t.coef_f, e->t_coef_f); MFE_SETP(t.off, e->t_off);
   MFE_SETP(t.nint, e->t_nint); MFE_SETP(t.r2atat, e->t_r2atat); MFE_SETP(t.r2umin, e->t_r2umin); MFE_SETP(t.pairidx, e->t_pairidx);
-  t.npt = e->npt; t.nptpt = e->nptpt; t.rcut2 = e->rcut * e->rcut;
+  t.npt = e->npt; t.nptpt = e->nptpt; t.rcut2 = e->rcut * e->rcut; t.nzl = e->t_nzl;
   const double rs = e->rcut + e->drnlist;
   v.rs2 = rs * rs;
   v.drnlist = e->drnlist;
@@ -276,6 +277,7 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
     MFE_SETP(v.mesh_r, e->mesh_r); MFE_SETP(v.mesh_c, e->mesh_c);
     MFE_SETP(v.b_count, e->b_count); MFE_SETP(v.b_start, e->b_start); MFE_SETP(v.b_id, e->b_id); MFE_SETP(v.b_slot, e->b_slot);
     MFE_SETP(v.rec_frac, e->rec_frac); MFE_SETP(v.rec_meta, e->rec_meta);
+    MFE_SETP(v.spread_tiles, e->spread_tiles);
     const int smax2 = std::max(e->s[0], std::max(e->s[1], e->s[2])) / 2 + 1;
     for (int a = 0; a < 3; ++a) {
       MFE_SETP(v.ax.g[a], e->ax_g + a * smax2);
@@ -354,6 +356,7 @@ int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStre
       mfe::fft_geometry(e->prec, e->s, &fx_rpb, &fx_nblk, &fy_T, &fy_nbx, &fz_T, &fz_nbx,
                         &lfx, &lfy, &lfz);
       g.bricks = std::max(g.bricks, e->nbricks);
+      g.mesh_blocks = std::max(g.mesh_blocks, (e->s[0] * e->s[1] * e->s[2] + 255) / 256);
       g.fx_blocks = std::max(g.fx_blocks, fx_nblk);
       g.fy_x = std::max(g.fy_x, fy_nbx);
       g.fy_y = std::max(g.fy_y, e->s[2]);
@@ -635,6 +638,7 @@ int mfe_set_tables(mfe_engine *e, const double *ubuf, int32_t nbuf, const int32_
     int k = 0;
     for (int i = 0; i < e->npt; ++i)
       for (int j = i; j < e->npt; ++j, ++k) pidx[i * e->npt + j] = pidx[j * e->npt + i] = k;
+    e->t_nzl = (int)zl.size();
     dalloc(e->t_zlow, zl.size()); dalloc(e->t_coef, cf.size()); dalloc(e->t_coef_f, cf.size());
     dalloc(e->t_off, (size_t)P); dalloc(e->t_nint, (size_t)P); dalloc(e->t_pairidx, pidx.size());
     dalloc(e->t_r2atat, (size_t)P); dalloc(e->t_r2umin, (size_t)P);
@@ -692,22 +696,27 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
         e->plan[m] = make_plan(e->s[m]);
         if (!plan_ok(e->plan[m])) return fail(e, MFE_E_BAD_PARAM, "mesh has a prime factor > 32");
       }
-      // bricks owned by the spreading blocks: B >= order - 1 divides s
-      size_t tile = 1;
+      // bricks of the spreading and gather blocks: B divides s; 16 when s
+      // allows (fewer halo points per brick), else the smallest divisor >= 8
+      size_t ext = 1;
       for (int m = 0; m < 3; ++m) {
-        int b = smallest_divisor_at_least(e->s[m], std::max(order - 1, 8));
+        int b = (e->s[m] % 16 == 0) ? 16 : smallest_divisor_at_least(e->s[m], 8);
         if (b > 16) b = smallest_divisor_at_least(e->s[m], order - 1);
         e->B[m] = b;
         e->nb[m] = e->s[m] / b;
-        tile *= (size_t)b;
+        ext *= (size_t)(b + order - 1);
       }
       e->nbricks = e->nb[0] * e->nb[1] * e->nb[2];
       const size_t rsz = e->prec == MFE_PREC_MIXED ? 4 : 8;
-      e->spread_lds = (int)(tile * 8);
-      size_t gext = rsz;
-      for (int m = 0; m < 3; ++m) gext *= (size_t)(e->B[m] + order - 1);
+      // spreading tile: 64-bit fixed point (mixed) or FP64 per extended point
+      e->spread_lds = (int)(ext * 8);
+      const size_t gext = ext * rsz;
       const int lds_need = (int)std::max<size_t>((size_t)e->spread_lds, gext);
-      if (lds_need > 150 * 1024)
+      // k_spread also holds a chunk of candidate weights in static LDS:
+      // 256 (FP32) or 128 (FP64) x (3 order + 1) values and 256 ints
+      const size_t chunk = e->prec == MFE_PREC_MIXED ? 256 : 128;
+      const size_t lds_static = chunk * (3 * order + 1) * rsz + chunk * 4 + 256;
+      if ((size_t)lds_need + lds_static > 160 * 1024)
         return fail(e, MFE_E_BAD_PARAM, "mesh brick too large for LDS (mesh 2q, q prime > 16?)");
       const size_t M = (size_t)e->s[0] * e->s[1] * e->s[2];
       const size_t Mc = (size_t)e->hx * e->s[1] * e->s[2];
@@ -715,6 +724,9 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
       dfree(e->mesh_c);
       MFE_HIP_TRY(hipMalloc(&e->mesh_r, M * rsz));
       MFE_HIP_TRY(hipMalloc(&e->mesh_c, Mc * 2 * rsz));
+      dfree(e->spread_tiles);
+      e->spread_tiles_bytes = (size_t)e->nbricks * ext * rsz;
+      MFE_HIP_TRY(hipMalloc(&e->spread_tiles, e->spread_tiles_bytes));
       dalloc(e->b_count, (size_t)e->nbricks);
       dalloc(e->b_start, (size_t)e->nbricks + 1);
       mfe::launch_fill_int(e->b_count, e->nbricks, 0, e->stream);
@@ -754,7 +766,7 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
                                              : make_twiddles<double>(e->s[m], e->stream);
       e->tw_half = e->prec == MFE_PREC_MIXED ? make_twiddles<float>(e->s[0] / 2, e->stream)
                                              : make_twiddles<double>(e->s[0] / 2, e->stream);
-      if (lds_need > 64 * 1024) {
+      if ((size_t)lds_need + lds_static > 64 * 1024) {
         // large bricks (meshes 2q with q prime) need the LDS above 64 KB
         mfe_allow_big_spread_lds(e->prec, lds_need);
       }

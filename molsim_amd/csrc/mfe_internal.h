@@ -81,6 +81,7 @@ struct TableDev {
   MFE_DPTR(const double) r2umin;
   MFE_DPTR(const int) pairidx;  // [npt*npt] -> pair index 0..nptpt-1
   int npt, nptpt;
+  int nzl;             // intervals over all pairs (length of zlow)
   double rcut2;
 };
 
@@ -164,6 +165,7 @@ struct SysDev {
   MFE_DPTR(int) b_slot;
   MFE_DPTR(void) rec_frac;
   MFE_DPTR(int2) rec_meta;
+  MFE_DPTR(void) spread_tiles;   // [nbricks][(B+order-1)^3] spreading tiles
   SpmeAxes ax;
   FftPlan plan[3], plan_half;
   MFE_DPTR(const void) tw[3];
@@ -216,6 +218,7 @@ struct Group {
   bool chains = false, surf = false, cells = false;
   // maxima over the replicas of each launch's per-replica block counts
   int np_blocks = 0, pair_blocks = 0, cell_blocks = 0, bricks = 0, kvec = 0;
+  int mesh_blocks = 0;   // blocks of 256 over the real mesh
   int fx_blocks = 0, fy_x = 0, fy_y = 0, fz_x = 0, fz_y = 0;
   size_t lds_spread = 0, lds_gather = 0, lds_fx = 0, lds_fy = 0, lds_fz = 0;
 };
@@ -255,6 +258,7 @@ struct mfe_engine {
   float *t_coef_f = nullptr;
   int *t_off = nullptr, *t_nint = nullptr, *t_pairidx = nullptr;
   double *t_r2atat = nullptr, *t_r2umin = nullptr;
+  int t_nzl = 0;
   double rcut = 0.0;
 
   // neighbour list
@@ -282,6 +286,8 @@ struct mfe_engine {
   void *mesh_c = nullptr;    // complex mesh [s3][s2][hx]
   int *b_count = nullptr, *b_start = nullptr, *b_id = nullptr, *b_slot = nullptr;
   void *rec_frac = nullptr;  // per sorted particle: (fx, fy, fz, q)
+  void *spread_tiles = nullptr;  // per brick: extended spreading tile
+  size_t spread_tiles_bytes = 0;
   int2 *rec_meta = nullptr;  // per sorted particle: (packed meshmax, index)
   double *ax_g = nullptr, *ax_kk = nullptr;  // [3][smax/2+1]
   void *tw[3] = {nullptr, nullptr, nullptr};     // twiddles of length s[a]

@@ -266,10 +266,14 @@ __device__ __forceinline__ bool knot_weights(double r, double r0, double dinv, i
 }
 
 // ------------------------------------------------------------ pair forces
+// table intervals (all type pairs) the pair kernel keeps in LDS; larger
+// tables are searched in global memory
+constexpr int kPairLdsIntervals = 2048;
 // First interval (in storage order) with r2 >= zlow: zlow decreases along
 // the table, so this is a binary search for the same interval the
 // reference's linear walk stops at (Src/energy.F90:482-486).
-__device__ __forceinline__ int find_interval(const double *zl, int n, double r2) {
+template <typename P>
+__device__ __forceinline__ int find_interval(P zl, int n, double r2) {
   int lo = 0, hi = n - 1;
   if (!(r2 >= zl[hi])) return -1;
   while (lo < hi) {
@@ -314,6 +318,23 @@ __global__ void __launch_bounds__(kBlock) k_pair(const SysDev *__restrict__ sys,
   double *__restrict__ part = V.part + V.off_pair;
   int *flags = V.flags;
   __shared__ double sh[kBlock];
+  // the interval bounds of every table and the per-pair constants in LDS:
+  // the interval search then runs on LDS instead of chained global loads
+  __shared__ double zl_s[kPairLdsIntervals];
+  __shared__ double r2at_s[MFE_MAX_PAIRS], r2um_s[MFE_MAX_PAIRS];
+  __shared__ int off_s[MFE_MAX_PAIRS], nint_s[MFE_MAX_PAIRS];
+  __shared__ int pidx_s[MFE_MAX_TYPES * MFE_MAX_TYPES];
+  const bool lds_tab = t.nzl <= kPairLdsIntervals;
+  if (lds_tab)
+    for (int k = threadIdx.x; k < t.nzl; k += blockDim.x) zl_s[k] = t.zlow[k];
+  for (int k = threadIdx.x; k < t.nptpt; k += blockDim.x) {
+    r2at_s[k] = t.r2atat[k];
+    r2um_s[k] = t.r2umin[k];
+    off_s[k] = t.off[k];
+    nint_s[k] = t.nint[k];
+  }
+  for (int k = threadIdx.x; k < t.npt * t.npt; k += blockDim.x) pidx_s[k] = t.pairidx[k];
+  __syncthreads();
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gid / kPairLanes, lane = gid % kPairLanes;
   // energy per partner type (static indexing keeps it in registers)
@@ -334,22 +355,23 @@ __global__ void __launch_bounds__(kBlock) k_pair(const SysDev *__restrict__ sys,
       const double r2 = pbcr2(dx, dy, dz, b);
       if (semantics == 0 ? (r2 > t.rcut2) : !(r2 < t.rcut2)) continue;
       const int tj = type[j];
-      const int p = t.pairidx[ti * t.npt + tj];
+      const int p = pidx_s[ti * t.npt + tj];
       double usum, fsum;
-      if (r2 < t.r2atat[p]) {
+      if (r2 < r2at_s[p]) {
         usum = 1e10;
         fsum = 1e10;
-      } else if (r2 < t.r2umin[p]) {
+      } else if (r2 < r2um_s[p]) {
         atomicOr(&flags[0], ERR_OVERLAP);
         continue;
       } else {
-        const int o = t.off[p];
-        const int iv = find_interval(t.zlow + o, t.nint[p], r2);
+        const int o = off_s[p];
+        const int iv = lds_tab ? find_interval(zl_s + o, nint_s[p], r2)
+                               : find_interval(t.zlow + o, nint_s[p], r2);
         if (iv < 0) {
           atomicOr(&flags[0], ERR_TABLE);
           continue;
         }
-        const double d = r2 - t.zlow[o + iv];
+        const double d = r2 - (lds_tab ? zl_s[o + iv] : t.zlow[o + iv]);
         if (MIXED) {
           const float *cf = t.coef_f + (size_t)(o + iv) * 11;
           const float df = (float)d;
@@ -424,7 +446,7 @@ __global__ void __launch_bounds__(kBlock) k_pair(const SysDev *__restrict__ sys,
       if (i < np) {
 #pragma unroll
         for (int u = 0; u < MFE_MAX_TYPES; ++u)
-          if (u < t.npt && t.pairidx[ti * t.npt + u] == k) v += ej[u];
+          if (u < t.npt && pidx_s[ti * t.npt + u] == k) v += ej[u];
       }
       double s2 = block_sum(v, sh);
       if (threadIdx.x == 0) out[k * cs] = s2;
@@ -480,21 +502,28 @@ __device__ __forceinline__ void mesh_coords(const double *r, const SpmeGeom &g,
   }
 }
 
-// Brick of every particle, counted with one global atomic per distinct
-// brick of a wavefront: lanes walk the cell-ordered particles, so a
-// wavefront's particles share few bricks (atomics across XCDs are the
-// expensive part of binning).
-__global__ void k_spme_count(const SysDev *__restrict__ sys) {
+// Brick of every particle and its slot within the brick.  A block takes
+// 1024 consecutive particles of the cell order -- a compact piece of the
+// box, spanning a few dozen bricks -- counts them in an LDS histogram and
+// then adds each non-empty bin to the global count with one atomic, so
+// global atomics scale with the bricks a block touches, not with its
+// particles.  Meshes with more bricks than the histogram holds fall back to
+// one global atomic per particle.
+constexpr int kCountThreads = 1024;
+constexpr int kCountBins = 8192;
+
+__global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
-  if ((int)blockIdx.x >= V.nblk_np) return;
+  const int np = V.np;
+  if ((int)blockIdx.x * kCountThreads >= np) return;
   const double *__restrict__ ro = V.ro;
   const int *__restrict__ sorted = V.c_sorted;
-  const int np = V.np;
   const SpmeGeom g = V.sg;
   int *bcount = V.b_count, *bid = V.b_id, *bslot = V.b_slot;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int hist[kCountBins];
+  const int t = blockIdx.x * kCountThreads + threadIdx.x;
   const bool act = t < np;
-  int i = 0, id = -1;
+  int i = 0, id = 0;
   if (act) {
     i = sorted[t];
     int mm[3];
@@ -502,22 +531,26 @@ __global__ void k_spme_count(const SysDev *__restrict__ sys) {
     mesh_coords(ro + 3 * i, g, mm, fr);
     id = mm[0] / g.B[0] + g.nb[0] * (mm[1] / g.B[1] + g.nb[1] * (mm[2] / g.B[2]));
   }
-  const int lane = __lane_id();
-  unsigned long long pending = __ballot(act);
-  int slot = 0;
-  while (pending) {
-    const int leader = __ffsll((long long)pending) - 1;
-    const int lid = __shfl(id, leader);
-    const unsigned long long same = __ballot(act && id == lid) & pending;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&bcount[lid], __popcll(same));
-    base = __shfl(base, leader);
-    if ((same >> lane) & 1ull) slot = base + __popcll(same & ((1ull << lane) - 1ull));
-    pending &= ~same;
+  if (V.nbricks > kCountBins) {
+    if (act) {
+      bid[i] = id;
+      bslot[i] = atomicAdd(&bcount[id], 1);
+    }
+    return;
   }
+  for (int k = threadIdx.x; k < V.nbricks; k += kCountThreads) hist[k] = 0;
+  __syncthreads();
+  const int rank = act ? atomicAdd(&hist[id], 1) : 0;
+  __syncthreads();
+  // one global atomic per non-empty bin; its old value is the bin's base
+  for (int k = threadIdx.x; k < V.nbricks; k += kCountThreads) {
+    const int c = hist[k];
+    if (c) hist[k] = atomicAdd(&bcount[k], c);
+  }
+  __syncthreads();
   if (act) {
     bid[i] = id;
-    bslot[i] = slot;
+    bslot[i] = hist[id] + rank;
   }
 }
 
@@ -568,37 +601,19 @@ struct SpreadAcc<double> {
 };
 
 constexpr int kMaxOrder = 8;
-template <typename R> struct Chunk { static constexpr int n = 256; };
-template <> struct Chunk<double> { static constexpr int n = 128; };
 
-// Stencil indices i whose point m - i falls inside a brick [o, o+e) of an
-// axis: with l = (m - o) mod s, i runs over [max(0, l-e+1), min(order-1, l)]
-// (point m - i sits at brick-local l - i).  An axis with one brick holds
-// the whole periodic stencil: every i, local (l - i) mod s.
-__device__ __forceinline__ void stencil_range(int m, int o, int e, int s, int nb,
-                                              int order, int &l, int &lo,
-                                              int &cnt) {
-  if (nb == 1) {
-    l = m;
-    lo = 0;
-    cnt = order;
-    return;
-  }
-  l = m - o;
-  if (l < 0) l += s;
-  const int a = max(0, l - e + 1), b = min(order - 1, l);
-  lo = a;
-  cnt = max(0, b - a + 1);
-}
+// Spreading in two passes, each particle once and no global atomics.
+// k_spread: one block per brick takes the brick's own particles and adds
+// their whole stencil -- points m, m-1, ..., m-order+1 along each axis
+// from meshmax m (Src/energy.F90:1232-1244) -- into an LDS tile of the
+// brick extended by order-1 points on the low side of each axis, then
+// writes the tile to a scratch array.  k_spread_sum: every mesh point adds
+// its value in its own brick's tile to the low-side halo values of the next
+// brick along each axis (at most 8 tiles), in a fixed order.  Work is
+// handed out as (particle, stencil plane) items, order planes per particle.
+template <typename R> struct SpreadChunk { static constexpr int n = 256; };
+template <> struct SpreadChunk<double> { static constexpr int n = 128; };
 
-// One block owns one brick of B0 x B1 x B2 mesh points and is its only
-// writer.  A particle with meshmax m adds to points m, m-1, ..., m-order+1
-// (Src/energy.F90:1232-1244), so the brick receives from particles binned
-// in itself and in the next brick along each axis (B >= order-1).  For
-// each candidate the block works out the exact stencil planes, rows and
-// columns inside the brick; the planes of all candidates are numbered by a
-// prefix sum and handed out to threads (a plane of rows each), so every
-// issued add lands.
 template <typename R, int ORDER>
 __global__ void __launch_bounds__(kBlock) k_spread(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
@@ -607,119 +622,90 @@ __global__ void __launch_bounds__(kBlock) k_spread(const SysDev *__restrict__ sy
   const int2 *__restrict__ meta = V.rec_meta;
   const int *__restrict__ bstart = V.b_start;
   const SpmeGeom g = V.sg;
-  R *__restrict__ mesh = (R *)V.mesh_r;
   typedef typename SpreadAcc<R>::T Acc;
-  constexpr int kChunk = Chunk<R>::n;
+  constexpr int kSpreadChunk = SpreadChunk<R>::n;
   extern __shared__ char smem[];
   Acc *tile = reinterpret_cast<Acc *>(smem);
-  __shared__ R w[kChunk][3][ORDER];
-  __shared__ int rng[kChunk][3];   // l | lo << 16 | cnt << 24 per axis
-  __shared__ R qs[kChunk];
-  __shared__ int pre[kChunk + 1];
-  __shared__ int seg_lo[8], seg_acc[9];
+  __shared__ R w[kSpreadChunk][3][ORDER];
+  __shared__ R qs[kSpreadChunk];
+  __shared__ int loc[kSpreadChunk];
   const int bx = blockIdx.x % g.nb[0];
   const int by = (blockIdx.x / g.nb[0]) % g.nb[1];
   const int bz = blockIdx.x / (g.nb[0] * g.nb[1]);
-  const int o[3] = {bx * g.B[0], by * g.B[1], bz * g.B[2]};
-  const int e[3] = {min(g.B[0], g.s[0] - o[0]), min(g.B[1], g.s[1] - o[1]),
-                    min(g.B[2], g.s[2] - o[2])};
-  const int ntile = g.B[0] * g.B[1] * g.B[2];
+  const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
+  const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
+  const int ntile = E0 * E1 * E2;
   for (int k = threadIdx.x; k < ntile; k += blockDim.x) tile[k] = Acc(0);
-  if (threadIdx.x == 0) {
-    int ns = 0, acc = 0;
-    const int ex = g.nb[0] > 1 ? 2 : 1, ey = g.nb[1] > 1 ? 2 : 1, ez = g.nb[2] > 1 ? 2 : 1;
-    for (int dz = 0; dz < ez; ++dz)
-      for (int dy = 0; dy < ey; ++dy)
-        for (int dx = 0; dx < ex; ++dx) {
-          const int cb = (bx + dx) % g.nb[0] +
-                         g.nb[0] * ((by + dy) % g.nb[1] + g.nb[1] * ((bz + dz) % g.nb[2]));
-          seg_lo[ns] = bstart[cb];
-          seg_acc[ns] = acc;
-          acc += bstart[cb + 1] - bstart[cb];
-          ++ns;
-        }
-    for (int k = ns; k < 9; ++k) seg_acc[k] = acc;
-  }
-  __syncthreads();
-  const int ncand = seg_acc[8];
-  for (int c0 = 0; c0 < ncand; c0 += kChunk) {
-    const int nc = min(kChunk, ncand - c0);
-    int rows = 0;
-    for (int c = threadIdx.x; c < kChunk; c += blockDim.x) {
-      if (c < nc) {
-        const int gc = c0 + c;
-        int sgi = 0;
-        while (gc >= seg_acc[sgi + 1]) ++sgi;
-        const int rec = seg_lo[sgi] + (gc - seg_acc[sgi]);
-        const int2 md = meta[rec];
-        const int mm[3] = {md.x & 1023, (md.x >> 10) & 1023, (md.x >> 20) & 1023};
-        int cnt[3];
-        for (int m = 0; m < 3; ++m) {
-          int l, lo;
-          stencil_range(mm[m], o[m], e[m], g.s[m], g.nb[m], ORDER, l, lo, cnt[m]);
-          rng[c][m] = l | (lo << 16) | (cnt[m] << 24);
-        }
-        qs[c] = frac[4 * rec + 3];
-        for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * rec + m], w[c][m], nullptr);
-        rows = (cnt[0] > 0 && cnt[1] > 0) ? cnt[2] : 0;  // planes inside the brick
-      }
-      pre[c + 1] = rows;
+  const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
+  for (int c0 = 0; c0 < n; c0 += kSpreadChunk) {
+    const int nc = min(kSpreadChunk, n - c0);
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const int rec = first + c0 + c;
+      const int2 md = meta[rec];
+      // meshmax in extended-tile coordinates: [order-1, B+order-2]
+      const int lx = (md.x & 1023) - o0 + ORDER - 1;
+      const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
+      const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
+      loc[c] = lx | (ly << 10) | (lz << 20);
+      qs[c] = frac[4 * rec + 3];
+      for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * rec + m], w[c][m], nullptr);
     }
-    if (threadIdx.x == 0) pre[0] = 0;
     __syncthreads();
-    // inclusive scan of pre[1..kChunk]
-    for (int off = 1; off < kChunk; off <<= 1) {
-      int v = 0;
-      const int k = threadIdx.x + 1;
-      if (k <= kChunk && k > off) v = pre[k - off];
-      __syncthreads();
-      if (k <= kChunk) pre[k] += v;
-      __syncthreads();
-    }
-    const int total = pre[nc];
-    for (int r = threadIdx.x; r < total; r += blockDim.x) {
-      int lo = 0, hi = nc - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= r) lo = mid; else hi = mid - 1;
-      }
-      const int c = lo;
-      const int rx = rng[c][0], ry = rng[c][1], rz = rng[c][2];
-      const int iz = ((rz >> 16) & 255) + (r - pre[c]);
-      int zl = (rz & 0xffff) - iz;
-      if (zl < 0) zl += g.s[2];
+    for (int r = threadIdx.x; r < nc * ORDER; r += blockDim.x) {
+      const int c = r / ORDER, iz = r - c * ORDER;
+      const int lc = loc[c];
+      const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = (lc >> 20) & 1023;
       const R qz = qs[c] * w[c][2][iz];
-      const int ly = ry & 0xffff, y0 = (ry >> 16) & 255, cy = (ry >> 24) & 255;
-      const int lx = rx & 0xffff, x0 = (rx >> 16) & 255, cx = (rx >> 24) & 255;
       R wx[ORDER];
 #pragma unroll
-      for (int j = 0; j < ORDER; ++j) wx[j] = (j < cx) ? w[c][0][x0 + j] : R(0);
+      for (int j = 0; j < ORDER; ++j) wx[j] = w[c][0][j];
+      Acc *plane = tile + (size_t)(lz - iz) * E1 * E0;
 #pragma unroll
       for (int jy = 0; jy < ORDER; ++jy) {
-        if (jy < cy) {
-          const int iy = y0 + jy;
-          int yl = ly - iy;
-          if (yl < 0) yl += g.s[1];
-          const R qyz = qz * w[c][1][iy];
-          Acc *row = tile + (zl * g.B[1] + yl) * g.B[0];
+        const R qyz = qz * w[c][1][jy];
+        Acc *row = plane + (ly - jy) * E0 + lx;
 #pragma unroll
-          for (int j = 0; j < ORDER; ++j) {
-            if (j < cx) {
-              int xl = lx - (x0 + j);
-              if (xl < 0) xl += g.s[0];
-              SpreadAcc<R>::add(&row[xl], qyz * wx[j]);
-            }
-          }
-        }
+        for (int j = 0; j < ORDER; ++j) SpreadAcc<R>::add(&row[-j], qyz * wx[j]);
       }
     }
     __syncthreads();
   }
-  for (int k = threadIdx.x; k < e[0] * e[1] * e[2]; k += blockDim.x) {
-    const int x = k % e[0], y = (k / e[0]) % e[1], z = k / (e[0] * e[1]);
-    mesh[((size_t)(o[2] + z) * g.s[1] + (o[1] + y)) * g.s[0] + (o[0] + x)] =
-        SpreadAcc<R>::get(tile[(z * g.B[1] + y) * g.B[0] + x]);
+  R *__restrict__ out = (R *)V.spread_tiles + (size_t)blockIdx.x * ntile;
+  for (int k = threadIdx.x; k < ntile; k += blockDim.x) out[k] = SpreadAcc<R>::get(tile[k]);
+}
+
+// mesh point = own brick's tile + low-side halos of the next bricks
+template <typename R, int ORDER>
+__global__ void __launch_bounds__(kBlock) k_spread_sum(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  const SpmeGeom g = V.sg;
+  const int M = g.s[0] * g.s[1] * g.s[2];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= M) return;
+  const R *__restrict__ tiles = (const R *)V.spread_tiles;
+  const int x = k % g.s[0], y = (k / g.s[0]) % g.s[1], z = k / (g.s[0] * g.s[1]);
+  const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
+  const int bc[3] = {x / g.B[0], y / g.B[1], z / g.B[2]};
+  const int l[3] = {x - bc[0] * g.B[0], y - bc[1] * g.B[1], z - bc[2] * g.B[2]};
+  R acc = R(0);
+  for (int dz = 0; dz < 2; ++dz) {
+    if (dz && l[2] < g.B[2] - (ORDER - 1)) break;
+    const int tz = dz ? (bc[2] + 1) % g.nb[2] : bc[2];
+    const int ez = l[2] + ORDER - 1 - dz * g.B[2];
+    for (int dy = 0; dy < 2; ++dy) {
+      if (dy && l[1] < g.B[1] - (ORDER - 1)) break;
+      const int ty = dy ? (bc[1] + 1) % g.nb[1] : bc[1];
+      const int ey = l[1] + ORDER - 1 - dy * g.B[1];
+      for (int dx = 0; dx < 2; ++dx) {
+        if (dx && l[0] < g.B[0] - (ORDER - 1)) break;
+        const int tx = dx ? (bc[0] + 1) % g.nb[0] : bc[0];
+        const int ex = l[0] + ORDER - 1 - dx * g.B[0];
+        const int t = tx + g.nb[0] * (ty + g.nb[1] * tz);
+        acc += tiles[(size_t)t * (E0 * E1 * E2) + (ez * E1 + ey) * E0 + ex];
+      }
+    }
   }
+  ((R *)V.mesh_r)[k] = acc;
 }
 
 // per-lane stencil sums: compensated FP32 on the mixed path, plain FP64
@@ -745,8 +731,10 @@ struct Acc3<double> {
 // lanes, lane iz < order summing stencil plane iz (FP32 with compensated
 // row sums on the mixed path), and the planes are combined by shuffles in
 // FP64 in a fixed order.
+constexpr int kGatherLanes = kBlock;
+
 template <typename R, int ORDER, bool POT>
-__global__ void __launch_bounds__(kBlock) k_gather(const SysDev *__restrict__ sys) {
+__global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
   if ((int)blockIdx.x >= V.nbricks) return;
   const R *__restrict__ frac = (const R *)V.rec_frac;
@@ -763,22 +751,39 @@ __global__ void __launch_bounds__(kBlock) k_gather(const SysDev *__restrict__ sy
   const int bz = blockIdx.x / (g.nb[0] * g.nb[1]);
   const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
   const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
-  for (int k = threadIdx.x; k < E0 * E1 * E2; k += blockDim.x) {
-    const int k01 = k / E0;
-    const int xe = k - k01 * E0, ye = k01 % E1, ze = k01 / E1;
-    // region coordinates lie in [-(order-1), s + B): one wrap at most,
-    // or (one-brick axis, B = s) possibly one more below zero
-    int x = o0 - (ORDER - 1) + xe, y = o1 - (ORDER - 1) + ye, z = o2 - (ORDER - 1) + ze;
-    if (x < 0) x += g.s[0];
-    if (x < 0) x += g.s[0];
-    if (x >= g.s[0]) x -= g.s[0];
-    if (y < 0) y += g.s[1];
-    if (y < 0) y += g.s[1];
-    if (y >= g.s[1]) y -= g.s[1];
-    if (z < 0) z += g.s[2];
-    if (z < 0) z += g.s[2];
-    if (z >= g.s[2]) z -= g.s[2];
-    tile[k] = mesh[((size_t)z * g.s[1] + y) * g.s[0] + x];
+  const int nstage = E0 * E1 * E2;
+  // staged in batches of 8 loads per lane, so the loads of a batch are in
+  // flight together
+  constexpr int kU = 8;
+  for (int k0 = threadIdx.x; k0 < nstage; k0 += kU * blockDim.x) {
+    R val[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int k = k0 + u * blockDim.x;
+      val[u] = R(0);
+      if (k < nstage) {
+        const int k01 = k / E0;
+        const int xe = k - k01 * E0, ye = k01 % E1, ze = k01 / E1;
+        // region coordinates lie in [-(order-1), s + B): one wrap at most,
+        // or (one-brick axis, B = s) possibly one more below zero
+        int x = o0 - (ORDER - 1) + xe, y = o1 - (ORDER - 1) + ye, z = o2 - (ORDER - 1) + ze;
+        if (x < 0) x += g.s[0];
+        if (x < 0) x += g.s[0];
+        if (x >= g.s[0]) x -= g.s[0];
+        if (y < 0) y += g.s[1];
+        if (y < 0) y += g.s[1];
+        if (y >= g.s[1]) y -= g.s[1];
+        if (z < 0) z += g.s[2];
+        if (z < 0) z += g.s[2];
+        if (z >= g.s[2]) z -= g.s[2];
+        val[u] = mesh[((size_t)z * g.s[1] + y) * g.s[0] + x];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int k = k0 + u * blockDim.x;
+      if (k < nstage) tile[k] = val[u];
+    }
   }
   __syncthreads();
   const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
@@ -1448,7 +1453,9 @@ static void spme_stages(const Group &g, bool energy, int first, int last, bool p
   for (int st = first; st <= last; ++st) {
     switch (st) {
       case SP_BIN:
-        hipLaunchKernelGGL(k_spme_count, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
+        hipLaunchKernelGGL(k_spme_count,
+                           grid((g.np_blocks * kBlock + kCountThreads - 1) / kCountThreads, g),
+                           dim3(kCountThreads), 0, g.stream, g.d);
         hipLaunchKernelGGL(k_scan, grid(1, g), dim3(1024), 0, g.stream, g.d, 1, 0);
         hipLaunchKernelGGL(k_spme_scatter<R>, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream,
                            g.d);
@@ -1457,7 +1464,9 @@ static void spme_stages(const Group &g, bool energy, int first, int last, bool p
         switch (g.order) {
 #define MFE_SPREAD(O)                                                                   \
   case O:                                                                               \
-    hipLaunchKernelGGL((k_spread<R, O>), grid(g.bricks, g), dim3(kBlock), g.lds_spread, \
+    hipLaunchKernelGGL((k_spread<R, O>), grid(g.bricks, g), dim3(kBlock), g.lds_spread,    \
+                       g.stream, g.d);                                                  \
+    hipLaunchKernelGGL((k_spread_sum<R, O>), grid(g.mesh_blocks, g), dim3(kBlock), 0,   \
                        g.stream, g.d);                                                  \
     break;
           MFE_SPREAD(3) MFE_SPREAD(4) MFE_SPREAD(5) MFE_SPREAD(6) MFE_SPREAD(7) MFE_SPREAD(8)
@@ -1474,10 +1483,10 @@ static void spme_stages(const Group &g, bool energy, int first, int last, bool p
 #define MFE_GATHER(O)                                                                     \
   case O:                                                                                 \
     if (pot)                                                                              \
-      hipLaunchKernelGGL((k_gather<R, O, true>), grid(g.bricks, g), dim3(kBlock),         \
+      hipLaunchKernelGGL((k_gather<R, O, true>), grid(g.bricks, g), dim3(kGatherLanes),   \
                          g.lds_gather, g.stream, g.d);                                    \
     else                                                                                  \
-      hipLaunchKernelGGL((k_gather<R, O, false>), grid(g.bricks, g), dim3(kBlock),        \
+      hipLaunchKernelGGL((k_gather<R, O, false>), grid(g.bricks, g), dim3(kGatherLanes),  \
                          g.lds_gather, g.stream, g.d);                                    \
     break;
           MFE_GATHER(3) MFE_GATHER(4) MFE_GATHER(5) MFE_GATHER(6) MFE_GATHER(7) MFE_GATHER(8)
