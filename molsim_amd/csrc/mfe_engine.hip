@@ -62,6 +62,10 @@ mfe::FftPlan make_plan(int n) {
   mfe::FftPlan p{};
   p.n = n;
   int m = n;
+  while (m % 8 == 0 && p.nstages < 20) {
+    p.radix[p.nstages++] = 8;
+    m /= 8;
+  }
   while (m % 4 == 0 && p.nstages < 20) {
     p.radix[p.nstages++] = 4;
     m /= 4;

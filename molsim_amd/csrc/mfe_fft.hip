@@ -56,6 +56,26 @@ __device__ __forceinline__ void small_dft(cplx<R> *v, bool inv,
     v[2] = csub(s0, s2);
     v[1] = cadd(s1, js3);
     v[3] = csub(s1, js3);
+  } else if constexpr (RAD == 8) {
+    // radix 2 x 4: two 4-point DFTs of the even and odd samples, then the
+    // odd half twiddled by w8^q (w8 = e^{-+i pi/4})
+    const R h = R(0.70710678118654752440);
+    cplx<R> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    small_dft<R, 4>(e, inv, tw, n);
+    small_dft<R, 4>(o, inv, tw, n);
+    // o[1] *= w8, o[2] *= w8^2 = -+i, o[3] *= w8^3
+    const R sg = inv ? R(1) : R(-1);
+    cplx<R> o1 = {h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x)};
+    cplx<R> o2 = {-sg * o[2].y, sg * o[2].x};
+    cplx<R> o3 = {h * (-o[3].x - sg * o[3].y), h * (-o[3].y + sg * o[3].x)};
+    v[0] = cadd(e[0], o[0]);
+    v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);
+    v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);
+    v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);
+    v[7] = csub(e[3], o3);
   } else {
     cplx<R> out[RAD];
     const int step = n / RAD;
@@ -78,30 +98,43 @@ __device__ __forceinline__ void small_dft(cplx<R> *v, bool inv,
 // one Stockham stage of radix RAD over `batch` sequences (Govindaraju et
 // al., SC'08): j-th butterfly reads in[j + q*n/RAD], twiddles by
 // w^(k q) with k = j mod Ns, and writes out[(j/Ns)*Ns*RAD + k + q*Ns]
+// lg_nbf / lg_ns: log2 of n/RAD and of Ns when both are powers of two
+// (shifts replace the divisions), -1 otherwise.  k q tstep < n, so the
+// twiddle index needs no reduction.
 template <typename R, int RAD>
 __device__ __forceinline__ void stockham_stage(const cplx<R> *a, cplx<R> *b,
                                                int n, int Ns, int batch,
                                                int ld, const cplx<R> *tw,
-                                               bool inv) {
+                                               bool inv, int lg_nbf, int lg_ns) {
   const int nbf = n / RAD;
   const int total = nbf * batch;
   const int tstep = n / (Ns * RAD);
   for (int t = threadIdx.x; t < total; t += blockDim.x) {
-    const int sq = t / nbf, j = t - sq * nbf;
+    int sq, j, jb, k;
+    if (lg_nbf >= 0) {
+      sq = t >> lg_nbf;
+      j = t & (nbf - 1);
+      jb = j >> lg_ns;
+      k = j & (Ns - 1);
+    } else {
+      sq = t / nbf;
+      j = t - sq * nbf;
+      jb = j / Ns;
+      k = j - jb * Ns;
+    }
     const cplx<R> *in = a + sq * ld;
     cplx<R> *out = b + sq * ld;
-    const int k = j % Ns;
     cplx<R> v[RAD];
 #pragma unroll
     for (int q = 0; q < RAD; ++q) v[q] = in[j + q * nbf];
 #pragma unroll
     for (int q = 1; q < RAD; ++q) {
-      cplx<R> w = tw[(k * q * tstep) % n];
+      cplx<R> w = tw[k * q * tstep];
       if (inv) w.y = -w.y;
       v[q] = cmul(v[q], w);
     }
     small_dft<R, RAD>(v, inv, tw, n);
-    const int base = (j / Ns) * Ns * RAD + k;
+    const int base = jb * Ns * RAD + k;
 #pragma unroll
     for (int q = 0; q < RAD; ++q) out[base + q * Ns] = v[q];
   }
@@ -143,14 +176,18 @@ __device__ __forceinline__ cplx<R> *lds_fft(cplx<R> *a, cplx<R> *b, const FftPla
                             int batch, int ld, const cplx<R> *tw, bool inv) {
   const int n = p.n;
   int Ns = 1;
+  const bool pow2 = (n & (n - 1)) == 0;
   for (int st = 0; st < p.nstages; ++st) {
     const int r = p.radix[st];
+    const int lg_nbf = pow2 ? __builtin_ctz(n / r) : -1;
+    const int lg_ns = pow2 ? __builtin_ctz(Ns) : -1;
     switch (r) {
-      case 4: stockham_stage<R, 4>(a, b, n, Ns, batch, ld, tw, inv); break;
-      case 2: stockham_stage<R, 2>(a, b, n, Ns, batch, ld, tw, inv); break;
-      case 3: stockham_stage<R, 3>(a, b, n, Ns, batch, ld, tw, inv); break;
-      case 5: stockham_stage<R, 5>(a, b, n, Ns, batch, ld, tw, inv); break;
-      case 7: stockham_stage<R, 7>(a, b, n, Ns, batch, ld, tw, inv); break;
+      case 8: stockham_stage<R, 8>(a, b, n, Ns, batch, ld, tw, inv, lg_nbf, lg_ns); break;
+      case 4: stockham_stage<R, 4>(a, b, n, Ns, batch, ld, tw, inv, lg_nbf, lg_ns); break;
+      case 2: stockham_stage<R, 2>(a, b, n, Ns, batch, ld, tw, inv, lg_nbf, lg_ns); break;
+      case 3: stockham_stage<R, 3>(a, b, n, Ns, batch, ld, tw, inv, lg_nbf, lg_ns); break;
+      case 5: stockham_stage<R, 5>(a, b, n, Ns, batch, ld, tw, inv, lg_nbf, lg_ns); break;
+      case 7: stockham_stage<R, 7>(a, b, n, Ns, batch, ld, tw, inv, lg_nbf, lg_ns); break;
       default: stockham_stage_generic<R>(a, b, n, Ns, r, batch, ld, tw, inv); break;
     }
     __syncthreads();
@@ -331,20 +368,26 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
     base[z * zstride + c] = res[c * ld + z];
   }
   if (energy) {
-    red[0][threadIdx.x] = esum;
-    red[1][threadIdx.x] = vsum;
-    __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-      if ((int)threadIdx.x < w) {
-        red[0][threadIdx.x] += red[0][threadIdx.x + w];
-        red[1][threadIdx.x] += red[1][threadIdx.x + w];
-      }
-      __syncthreads();
+    // shuffle trees in each wavefront, then the wavefronts in order
+    for (int w = 32; w > 0; w >>= 1) {
+      esum += __shfl_down(esum, w, 64);
+      vsum += __shfl_down(vsum, w, 64);
     }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+      red[0][wave] = esum;
+      red[1][wave] = vsum;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
+      double e = red[0][0], v = red[1][0];
+      for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+        e += red[0][k];
+        v += red[1][k];
+      }
       const int blk = blockIdx.y * V.fz_nbx + blockIdx.x;
-      part[blk] = red[0][0];
-      part[V.nblk_conv + blk] = red[1][0];
+      part[blk] = e;
+      part[V.nblk_conv + blk] = v;
     }
   }
 }

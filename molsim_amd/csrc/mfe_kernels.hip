@@ -38,28 +38,29 @@ __device__ __forceinline__ double pbcr2(double &dx, double &dy, double &dz,
   return r2;
 }
 
+// Block-wide sum (max) in a fixed order: a shuffle tree inside each
+// wavefront, then the wavefront results in wavefront order.  Every thread
+// gets the result; sh needs one slot per wavefront.
 template <typename T>
 __device__ __forceinline__ T block_sum(T v, T *sh) {
-  sh[threadIdx.x] = v;
+  for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
-  }
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
   T r = sh[0];
-  __syncthreads();
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r += sh[k];
   return r;
 }
 
 __device__ __forceinline__ double block_max(double v, double *sh) {
-  sh[threadIdx.x] = v;
+  for (int w = 32; w > 0; w >>= 1) v = fmax(v, __shfl_down(v, w, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
-    __syncthreads();
-  }
+  if (lane == 0) sh[wave] = v;
+  __syncthreads();
   double r = sh[0];
-  __syncthreads();
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = fmax(r, sh[k]);
   return r;
 }
 
@@ -1023,8 +1024,10 @@ constexpr int kRedComp = MFE_MAX_PAIRS + 12;
 // Each wavefront sums whole components (partials are component-major),
 // four independent accumulators per lane, then a shuffle tree; the fixed
 // order keeps the result deterministic.
-__global__ void __launch_bounds__(kBlock) k_reduce(const SysDev *__restrict__ sys,
-                                                   int energy, int skin, int bd) {
+constexpr int kReduceThreads = 1024;
+
+__global__ void __launch_bounds__(kReduceThreads) k_reduce(const SysDev *__restrict__ sys,
+                                                           int energy, int skin, int bd) {
   const SysDev &V = sys[blockIdx.z];
   double *en = V.en;
   int *flags = V.flags;
@@ -1039,7 +1042,7 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const SysDev *__restrict__ sy
   // component layout: [0,P) pair energies, P pair virial, P+1..P+2 rec
   // (u, vir), P+3..P+4 bond (u, vir), P+5..P+7 S, P+8 max r2, P+9 cand
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int c = wave; c < nc; c += kBlock / 64) {
+  for (int c = wave; c < nc; c += kReduceThreads / 64) {
     const double *src;
     int n;
     bool mx = false;
@@ -1060,18 +1063,24 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const SysDev *__restrict__ sy
       n = skin ? nb : 0;
       mx = true;
     }
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    // eight accumulators, eight loads in flight per lane
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int b2 = lane;
-    for (; b2 + 192 < n; b2 += 256) {
-      const double a0 = src[b2], a1 = src[b2 + 64], a2 = src[b2 + 128], a3 = src[b2 + 192];
-      if (mx) {
-        s0 = fmax(s0, a0); s1 = fmax(s1, a1); s2 = fmax(s2, a2); s3 = fmax(s3, a3);
-      } else {
-        s0 += a0; s1 += a1; s2 += a2; s3 += a3;
-      }
+    for (; b2 + 7 * 64 < n; b2 += 8 * 64) {
+      double a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = src[b2 + u * 64];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc[u] = mx ? fmax(acc[u], a[u]) : acc[u] + a[u];
     }
-    for (; b2 < n; b2 += 64) s0 = mx ? fmax(s0, src[b2]) : s0 + src[b2];
-    double sv = mx ? fmax(fmax(s0, s1), fmax(s2, s3)) : (s0 + s1) + (s2 + s3);
+    for (; b2 < n; b2 += 64) acc[0] = mx ? fmax(acc[0], src[b2]) : acc[0] + src[b2];
+    double sv;
+    if (mx) {
+      sv = fmax(fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3])),
+                fmax(fmax(acc[4], acc[5]), fmax(acc[6], acc[7])));
+    } else {
+      sv = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    }
     for (int w = 32; w > 0; w >>= 1) {
       const double o = __shfl_down(sv, w, 64);
       sv = mx ? fmax(sv, o) : sv + o;
@@ -1345,17 +1354,22 @@ __global__ void __launch_bounds__(kBlock) k_skin_blocks(const SysDev *__restrict
   const double P = block_max(pm, sh);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const double v = i < np ? r2disp[i] : 0.0;
-  // inclusive running max in particle order, then shift to exclusive
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-    double o = (int)threadIdx.x >= off ? sh[threadIdx.x - off] : 0.0;
-    __syncthreads();
-    sh[threadIdx.x] = fmax(sh[threadIdx.x], o);
-    __syncthreads();
+  // running max in particle order: inclusive max-scan inside each
+  // wavefront by shuffles, then the maxima of the earlier wavefronts
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double inc = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const double y = __shfl_up(inc, off, 64);
+    if (lane >= off) inc = fmax(inc, y);
   }
-  const double before = fmax(P, threadIdx.x > 0 ? sh[threadIdx.x - 1] : 0.0);
+  double excl = __shfl_up(inc, 1, 64);
+  if (lane == 0) excl = 0.0;
+  __shared__ double wmax[kBlock / 64];
+  if (lane == 63) wmax[wave] = inc;
   __syncthreads();
+  double pw = 0.0;
+  for (int k = 0; k < wave; ++k) pw = fmax(pw, wmax[k]);
+  const double before = fmax(P, fmax(pw, excl));
   const double cand = (i < np && !(v > before)) ? v : 0.0;
   const double c = block_max(cand, sh);
   if (threadIdx.x == 0) part_cand[blockIdx.x] = c;
@@ -1517,7 +1531,7 @@ void launch_surf(const Group &g) {
 
 void launch_reduce(const Group &g, unsigned flags, bool skin, bool bd) {
   const int energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
-  hipLaunchKernelGGL(k_reduce, grid(1, g), dim3(kBlock), 0, g.stream, g.d, energy,
+  hipLaunchKernelGGL(k_reduce, grid(1, g), dim3(kReduceThreads), 0, g.stream, g.d, energy,
                      skin ? 1 : 0, bd ? 1 : 0);
 }
 
