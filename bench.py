@@ -31,15 +31,17 @@ from molsim_amd import systems as S  # noqa: E402
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 
-def algorithmic_bytes(sysm, stage):
+def algorithmic_bytes(sysm, stage, npairs=0):
     """Compulsory bytes per launch of each stage in its storage precision
     (FP32 mixed path): N particles, M = s^3 mesh points, Mc = (s/2+1) s^2
-    complex points; indices and LDS traffic excluded."""
+    complex points, P stored list entries; indices and LDS traffic
+    excluded."""
     n = sysm.np
     s = sysm.mesh
     M = s ** 3
     Mc = (s // 2 + 1) * s * s
     return {
+        "pair": 20 * n + 4 * npairs + 12 * n,  # xyz,q,type in, list, force out
         "spread": 16 * n + 4 * M,          # xyz+q in, mesh out
         "fft_x": 4 * M + 8 * Mc,           # real in, half-complex out
         "fft_y": 16 * Mc,                  # complex in and out
@@ -138,6 +140,9 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the batched-replica BD leg")
     ap.add_argument("--c4-replicas", type=int, default=512)
     ap.add_argument("--c4-steps", type=int, default=100)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run a plain device copy of this size (calibrates the "
+                         "FETCH_SIZE/WRITE_SIZE counters of a profiling run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it",
@@ -158,25 +163,30 @@ def main():
         en = eng.energy()
         dt = time.perf_counter() - t0
         stages = {}
+        npairs = 0
         if prec == "mixed":
             for st in ("pair", "spread", "fft_x", "fft_y", "conv", "ifft", "gather", "utotal"):
                 stages[st] = eng.time_stage(st, 20)
+            # stored (full) list entries: twice the half list within rc + skin
+            nn, _ = eng.get_pairs(maxnneigh=256)
+            npairs = 2 * int(nn.sum())
         eng.close()
         return dict(sps=args.steps / dt, ms=1000 * dt / args.steps, f0=f0, e0=e0,
-                    en=en, stages=stages)
+                    en=en, stages=stages, npairs=npairs)
 
     mixed = run("mixed")
     fp64 = None if args.no_fp64 else run("fp64")
 
-    # roofline of the dominant SPME kernel (stage times from HIP events on
-    # the engine's stream)
-    spme = {k: v for k, v in mixed["stages"].items() if k not in ("pair", "utotal")}
-    dom = max(spme, key=spme.get)
-    nbytes = algorithmic_bytes(sysm, dom)
-    achieved = nbytes / (spme[dom] * 1e-3) / 1e9
+    # roofline of the dominant stage (stage times from HIP events on the
+    # engine's stream)
+    npairs = mixed["npairs"]
+    kern = {k: v for k, v in mixed["stages"].items() if k != "utotal"}
+    dom = max(kern, key=kern.get)
+    nbytes = algorithmic_bytes(sysm, dom, npairs)
+    achieved = nbytes / (kern[dom] * 1e-3) / 1e9
     stage_report = {k: {"ms": round(v, 5),
-                        "GBs": (round(algorithmic_bytes(sysm, k) / (v * 1e-3) / 1e9, 1)
-                                if k in spme else None)}
+                        "GBs": (round(algorithmic_bytes(sysm, k, npairs) / (v * 1e-3) / 1e9, 1)
+                                if k in kern else None)}
                     for k, v in mixed["stages"].items()}
 
     out = {
@@ -218,6 +228,15 @@ def main():
                                "kind": "port",
                                "sample": f"{args.cpu_steps} MD steps of the same 200k-ion "
                                          f"system with the FP64 oracle ({secs:.1f} s)"}
+    if args.copy_probe_mb:
+        import torch
+        nbytes_copy = args.copy_probe_mb << 20
+        src = torch.ones(nbytes_copy // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = nbytes_copy
     print(json.dumps(out), flush=True)
 
 

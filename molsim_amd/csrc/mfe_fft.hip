@@ -38,6 +38,27 @@ __device__ __forceinline__ cplx<R> conjc(cplx<R> a) {
 
 constexpr int kMaxRadix = 32;
 
+// e = threadIdx.x + k blockDim.x < n in batches of 8 per thread: the global
+// loads of a batch are all issued before the first is stored to LDS, so a
+// thread waits for one load latency per batch instead of per element
+template <typename Load, typename Store>
+__device__ __forceinline__ void batched_copy(int n, Load load, Store store) {
+  constexpr int kU = 8;
+  for (int e0 = threadIdx.x; e0 < n; e0 += kU * blockDim.x) {
+    decltype(load(0)) v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int e = e0 + u * blockDim.x;
+      if (e < n) v[u] = load(e);
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int e = e0 + u * blockDim.x;
+      if (e < n) store(e, v[u]);
+    }
+  }
+}
+
 // R-point DFT of v[0..RAD-1] in registers: V[q] = sum_j v[j] w^(j q), with
 // w = exp(-+2 pi i / RAD) taken from the length-n table tw (RAD divides n).
 template <typename R, int RAD>
@@ -199,9 +220,44 @@ __device__ __forceinline__ cplx<R> *lds_fft(cplx<R> *a, cplx<R> *b, const FftPla
   return a;
 }
 
+// The spread mesh at point (x, y, z): the point's value in its own brick's
+// spreading tile plus the low-side halo values of the next brick along each
+// axis (at most 8 tiles), added in a fixed order.  Tiles are the bricks
+// extended by order-1 points below (see k_spread).
+template <typename R>
+__device__ __forceinline__ R spread_point(const SysDev &V, const R *__restrict__ tiles, int x,
+                                          int y, int z) {
+  const SpmeGeom &g = V.sg;
+  const int h = g.order - 1;
+  const int E0 = g.B[0] + h, E1 = g.B[1] + h, E2 = g.B[2] + h;
+  const int bc0 = x / g.B[0], bc1 = y / g.B[1], bc2 = z / g.B[2];
+  const int l0 = x - bc0 * g.B[0], l1 = y - bc1 * g.B[1], l2 = z - bc2 * g.B[2];
+  R acc = R(0);
+  for (int dz = 0; dz < 2; ++dz) {
+    if (dz && l2 < g.B[2] - h) break;
+    const int tz = dz ? (bc2 + 1 == g.nb[2] ? 0 : bc2 + 1) : bc2;
+    const int ez = l2 + h - dz * g.B[2];
+    for (int dy = 0; dy < 2; ++dy) {
+      if (dy && l1 < g.B[1] - h) break;
+      const int ty = dy ? (bc1 + 1 == g.nb[1] ? 0 : bc1 + 1) : bc1;
+      const int ey = l1 + h - dy * g.B[1];
+      for (int dx = 0; dx < 2; ++dx) {
+        if (dx && l0 < g.B[0] - h) break;
+        const int tx = dx ? (bc0 + 1 == g.nb[0] ? 0 : bc0 + 1) : bc0;
+        const int ex = l0 + h - dx * g.B[0];
+        const int t = tx + g.nb[0] * (ty + g.nb[1] * tz);
+        acc += tiles[(size_t)t * (E0 * E1 * E2) + (ez * E1 + ey) * E0 + ex];
+      }
+    }
+  }
+  return acc;
+}
+
 // ----------------------------------------------------------------- x pass
 // r2c of rows of length s1 = 2m: z[k] = x[2k] + i x[2k+1], Z = FFT_m(z),
 // X[k] = (Z[k] + conj(Z[m-k]))/2 - i/2 (Z[k] - conj(Z[m-k])) W^k.
+// The rows are read from the spreading tiles (the spread mesh is never
+// stored).
 template <typename R>
 __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
@@ -219,10 +275,19 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
   const int row0 = blockIdx.x * rows_per_blk;
   const int nr = min(rows_per_blk, nrows - row0);
   R *ar = reinterpret_cast<R *>(a);
-  for (int e = threadIdx.x; e < nr * s1; e += blockDim.x) {
-    const int r = e / s1, x = e - r * s1;
-    ar[2 * (r * ld) + x] = real[(size_t)(row0 + r) * s1 + x];
-  }
+  const R *__restrict__ tiles = (const R *)V.spread_tiles;
+  const int s2 = V.sg.s[1];
+  batched_copy(
+      nr * s1,
+      [&](int e) {
+        const int r = e / s1, x = e - r * s1;
+        const int row = row0 + r, y = row % s2, z = row / s2;
+        return spread_point<R>(V, tiles, x, y, z);
+      },
+      [&](int e, R v) {
+        const int r = e / s1, x = e - r * s1;
+        ar[2 * (r * ld) + x] = v;
+      });
   __syncthreads();
   cplx<R> *z = lds_fft(a, b, ph, nr, ld, tw_half, false);
   for (int e = threadIdx.x; e < nr * hx; e += blockDim.x) {
@@ -256,12 +321,13 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
   cplx<R> *b = a + rows_per_blk * ld;
   const int row0 = blockIdx.x * rows_per_blk;
   const int nr = min(rows_per_blk, nrows - row0);
-  for (int e = threadIdx.x; e < nr * hx; e += blockDim.x) {
-    const int r = e / hx, k = e - r * hx;
-    cplx<R> v = cm[(size_t)(row0 + r) * hx + k];
-    if (k == 0 || k == m) v.y = R(0);
-    b[r * ld + k] = v;
-  }
+  batched_copy(
+      nr * hx, [&](int e) { return cm[(size_t)row0 * hx + e]; },
+      [&](int e, cplx<R> v) {
+        const int r = e / hx, k = e - r * hx;
+        if (k == 0 || k == m) v.y = R(0);
+        b[r * ld + k] = v;
+      });
   __syncthreads();
   for (int e = threadIdx.x; e < nr * m; e += blockDim.x) {
     const int r = e / m, k = e - r * m;
@@ -297,10 +363,16 @@ __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, i
   const int kx0 = blockIdx.x * T;
   const int nt = min(T, hx - kx0);
   cplx<R> *base = cm + (size_t)z * s2 * hx + kx0;
-  for (int e = threadIdx.x; e < nt * s2; e += blockDim.x) {
-    const int y = e / nt, c = e - y * nt;
-    a[c * ld + y] = base[(size_t)y * hx + c];
-  }
+  batched_copy(
+      nt * s2,
+      [&](int e) {
+        const int y = e / nt, c = e - y * nt;
+        return base[(size_t)y * hx + c];
+      },
+      [&](int e, cplx<R> v) {
+        const int y = e / nt, c = e - y * nt;
+        a[c * ld + y] = v;
+      });
   __syncthreads();
   cplx<R> *res = lds_fft(a, b, py, nt, ld, tw, inv != 0);
   for (int e = threadIdx.x; e < nt * s2; e += blockDim.x) {
@@ -335,10 +407,16 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   const int nt = min(T, hx - kx0);
   const size_t zstride = (size_t)s2 * hx;
   cplx<R> *base = cm + (size_t)y * hx + kx0;
-  for (int e = threadIdx.x; e < nt * s3; e += blockDim.x) {
-    const int z = e / nt, c = e - z * nt;
-    a[c * ld + z] = base[z * zstride + c];
-  }
+  batched_copy(
+      nt * s3,
+      [&](int e) {
+        const int z = e / nt, c = e - z * nt;
+        return base[z * zstride + c];
+      },
+      [&](int e, cplx<R> v) {
+        const int z = e / nt, c = e - z * nt;
+        a[c * ld + z] = v;
+      });
   __syncthreads();
   cplx<R> *f = lds_fft(a, b, pz, nt, ld, tw, false);
   const int ny = min(y, s2 - y);

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "mfe_internal.h"
 
@@ -295,8 +296,8 @@ struct KahanF {
   __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// Four lanes share one particle (cell order t) and split its full list
-// k = lane, lane+4, ...; their partial forces are combined by shuffles in
+// kPairLanes (8) lanes share one particle (cell order t) and split its
+// full list k = lane, lane+8, ...; their partial forces are combined by shuffles in
 // FP64 in a fixed order, so forces need no atomics and stay
 // deterministic.  Every pair is seen from both ends, so its energy and
 // virial carry weight 1/2.  semantics: 0 = vlist (r2 <= rcut2 kept),
@@ -603,20 +604,24 @@ struct SpreadAcc<double> {
 
 constexpr int kMaxOrder = 8;
 
-// Spreading in two passes, each particle once and no global atomics.
-// k_spread: one block per brick takes the brick's own particles and adds
-// their whole stencil -- points m, m-1, ..., m-order+1 along each axis
-// from meshmax m (Src/energy.F90:1232-1244) -- into an LDS tile of the
-// brick extended by order-1 points on the low side of each axis, then
-// writes the tile to a scratch array.  k_spread_sum: every mesh point adds
-// its value in its own brick's tile to the low-side halo values of the next
-// brick along each axis (at most 8 tiles), in a fixed order.  Work is
-// handed out as (particle, stencil plane) items, order planes per particle.
+// Spreading, each particle once and no global atomics: one block per brick
+// takes the brick's own particles and adds their whole stencil -- points
+// m, m-1, ..., m-order+1 along each axis from meshmax m
+// (Src/energy.F90:1232-1244) -- into an LDS tile of the brick extended by
+// order-1 points on the low side of each axis, then writes the tile to a
+// scratch array.  The forward x FFT reads each mesh point as its own
+// brick's tile value plus the low-side halos of the next bricks (at most 8
+// tiles) in a fixed order (spread_point in mfe_fft.hip).  Work is handed
+// out as (particle, stencil plane) items, order planes per particle.
 template <typename R> struct SpreadChunk { static constexpr int n = 256; };
 template <> struct SpreadChunk<double> { static constexpr int n = 128; };
 
+// 1024 threads share one brick's tile: the tile's LDS allows one block per
+// CU, so the block supplies the wavefronts that hide the load latencies
+constexpr int kSpreadThreads = 1024;
+
 template <typename R, int ORDER>
-__global__ void __launch_bounds__(kBlock) k_spread(const SysDev *__restrict__ sys) {
+__global__ void __launch_bounds__(kSpreadThreads) k_spread(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
   if ((int)blockIdx.x >= V.nbricks) return;
   const R *__restrict__ frac = (const R *)V.rec_frac;
@@ -640,16 +645,20 @@ __global__ void __launch_bounds__(kBlock) k_spread(const SysDev *__restrict__ sy
   const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
   for (int c0 = 0; c0 < n; c0 += kSpreadChunk) {
     const int nc = min(kSpreadChunk, n - c0);
-    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+    // one thread per (particle, axis) B-spline
+    for (int cm = threadIdx.x; cm < 3 * nc; cm += blockDim.x) {
+      const int c = cm / 3, m = cm - 3 * c;
       const int rec = first + c0 + c;
-      const int2 md = meta[rec];
-      // meshmax in extended-tile coordinates: [order-1, B+order-2]
-      const int lx = (md.x & 1023) - o0 + ORDER - 1;
-      const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
-      const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
-      loc[c] = lx | (ly << 10) | (lz << 20);
-      qs[c] = frac[4 * rec + 3];
-      for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * rec + m], w[c][m], nullptr);
+      bspline<R, ORDER>(frac[4 * rec + m], w[c][m], nullptr);
+      if (m == 0) {
+        const int2 md = meta[rec];
+        // meshmax in extended-tile coordinates: [order-1, B+order-2]
+        const int lx = (md.x & 1023) - o0 + ORDER - 1;
+        const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
+        const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
+        loc[c] = lx | (ly << 10) | (lz << 20);
+        qs[c] = frac[4 * rec + 3];
+      }
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nc * ORDER; r += blockDim.x) {
@@ -675,40 +684,6 @@ __global__ void __launch_bounds__(kBlock) k_spread(const SysDev *__restrict__ sy
   for (int k = threadIdx.x; k < ntile; k += blockDim.x) out[k] = SpreadAcc<R>::get(tile[k]);
 }
 
-// mesh point = own brick's tile + low-side halos of the next bricks
-template <typename R, int ORDER>
-__global__ void __launch_bounds__(kBlock) k_spread_sum(const SysDev *__restrict__ sys) {
-  const SysDev &V = sys[blockIdx.z];
-  const SpmeGeom g = V.sg;
-  const int M = g.s[0] * g.s[1] * g.s[2];
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= M) return;
-  const R *__restrict__ tiles = (const R *)V.spread_tiles;
-  const int x = k % g.s[0], y = (k / g.s[0]) % g.s[1], z = k / (g.s[0] * g.s[1]);
-  const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
-  const int bc[3] = {x / g.B[0], y / g.B[1], z / g.B[2]};
-  const int l[3] = {x - bc[0] * g.B[0], y - bc[1] * g.B[1], z - bc[2] * g.B[2]};
-  R acc = R(0);
-  for (int dz = 0; dz < 2; ++dz) {
-    if (dz && l[2] < g.B[2] - (ORDER - 1)) break;
-    const int tz = dz ? (bc[2] + 1) % g.nb[2] : bc[2];
-    const int ez = l[2] + ORDER - 1 - dz * g.B[2];
-    for (int dy = 0; dy < 2; ++dy) {
-      if (dy && l[1] < g.B[1] - (ORDER - 1)) break;
-      const int ty = dy ? (bc[1] + 1) % g.nb[1] : bc[1];
-      const int ey = l[1] + ORDER - 1 - dy * g.B[1];
-      for (int dx = 0; dx < 2; ++dx) {
-        if (dx && l[0] < g.B[0] - (ORDER - 1)) break;
-        const int tx = dx ? (bc[0] + 1) % g.nb[0] : bc[0];
-        const int ex = l[0] + ORDER - 1 - dx * g.B[0];
-        const int t = tx + g.nb[0] * (ty + g.nb[1] * tz);
-        acc += tiles[(size_t)t * (E0 * E1 * E2) + (ez * E1 + ey) * E0 + ex];
-      }
-    }
-  }
-  ((R *)V.mesh_r)[k] = acc;
-}
-
 // per-lane stencil sums: compensated FP32 on the mixed path, plain FP64
 template <typename R>
 struct Acc3;
@@ -732,7 +707,7 @@ struct Acc3<double> {
 // lanes, lane iz < order summing stencil plane iz (FP32 with compensated
 // row sums on the mixed path), and the planes are combined by shuffles in
 // FP64 in a fixed order.
-constexpr int kGatherLanes = kBlock;
+constexpr int kGatherLanes = 512;
 
 template <typename R, int ORDER, bool POT>
 __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restrict__ sys) {
@@ -788,31 +763,44 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
   }
   __syncthreads();
   const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
-  const int span = ((8 * n + blockDim.x - 1) / blockDim.x) * blockDim.x;
-  for (int it = threadIdx.x; it < span; it += blockDim.x) {
-    const int p = it >> 3, iz = it & 7;
+  // particles in chunks of blockDim/8; the B-spline weights of a chunk are
+  // computed once (one thread per particle and axis) into LDS
+  constexpr int kGChunk = kGatherLanes / 8;
+  __shared__ R wv[kGChunk][3][ORDER], wd[kGChunk][3][ORDER];
+  __shared__ R qsh[kGChunk];
+  __shared__ int locs[kGChunk], idxs[kGChunk];
+  for (int p0 = 0; p0 < n; p0 += kGChunk) {
+    const int nc = min(kGChunk, n - p0);
+    if ((int)threadIdx.x < 3 * nc) {
+      const int c = threadIdx.x / 3, m = threadIdx.x - 3 * c;
+      const int t = first + p0 + c;
+      bspline<R, ORDER>(frac[4 * t + m], wv[c][m], wd[c][m]);
+      if (m == 0) {
+        const int2 md = meta[t];
+        // brick-local coordinates inside the staged region
+        const int lx = (md.x & 1023) - o0 + ORDER - 1;
+        const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
+        const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
+        locs[c] = lx | (ly << 10) | (lz << 20);
+        idxs[c] = md.y;
+        qsh[c] = frac[4 * t + 3];
+      }
+    }
+    __syncthreads();
+    const int p = threadIdx.x >> 3, iz = threadIdx.x & 7;
     double ex = 0.0, ey = 0.0, ez = 0.0, ep = 0.0;
-    int idx = -1;
-    R q = R(0);
-    if (p < n && iz < ORDER) {
-      const int t = first + p;
-      R v[3][ORDER], d[3][ORDER];
+    if (p < nc && iz < ORDER) {
+      const int lc = locs[p];
+      const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = (lc >> 20) & 1023;
+      R v[2][ORDER], d[2][ORDER];
 #pragma unroll
-      for (int m = 0; m < 3; ++m) bspline<R, ORDER>(frac[4 * t + m], v[m], d[m]);
-      q = frac[4 * t + 3];
-      const int2 md = meta[t];
-      idx = md.y;
-      // brick-local coordinates inside the staged region
-      const int lx = (md.x & 1023) - o0 + ORDER - 1;
-      const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
-      const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
-      R vz = R(0), dz = R(0);
+      for (int m = 0; m < 2; ++m)
 #pragma unroll
-      for (int k = 0; k < ORDER; ++k)
-        if (k == iz) {
-          vz = v[2][k];
-          dz = d[2][k];
+        for (int k = 0; k < ORDER; ++k) {
+          v[m][k] = wv[p][m][k];
+          d[m][k] = wd[p][m][k];
         }
+      const R vz = wv[p][2][iz], dz = wd[p][2][iz];
       const R *plane = tile + (size_t)(lz - iz) * E1 * E0;
       Acc3<R> acc;
       double pacc = 0.0;
@@ -843,14 +831,16 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       ez += __shfl_xor(ez, w, 8);
       if (POT) ep += __shfl_xor(ep, w, 8);
     }
-    if (iz == 0 && p < n) {
-      const double qd = (double)q;
+    if (iz == 0 && p < nc) {
+      const int idx = idxs[p];
+      const double qd = (double)qsh[p];
       force[3 * idx] -= qd * ex * g.dkdr[0] * efp;
       force[3 * idx + 1] -= qd * ey * g.dkdr[1] * efp;
       force[3 * idx + 2] -= qd * ez * g.dkdr[2] * efp;
       // dE_rec/dq = efp * sum_stencil theta * (G * Q) (the convolved mesh)
       if (POT) V.phi_rec[idx] = efp * ep;
     }
+    __syncthreads();
   }
 }
 
@@ -1478,9 +1468,8 @@ static void spme_stages(const Group &g, bool energy, int first, int last, bool p
         switch (g.order) {
 #define MFE_SPREAD(O)                                                                   \
   case O:                                                                               \
-    hipLaunchKernelGGL((k_spread<R, O>), grid(g.bricks, g), dim3(kBlock), g.lds_spread,    \
-                       g.stream, g.d);                                                  \
-    hipLaunchKernelGGL((k_spread_sum<R, O>), grid(g.mesh_blocks, g), dim3(kBlock), 0,   \
+    hipLaunchKernelGGL((k_spread<R, O>), grid(g.bricks, g), dim3(kSpreadThreads),          \
+                       g.lds_spread,                                                    \
                        g.stream, g.d);                                                  \
     break;
           MFE_SPREAD(3) MFE_SPREAD(4) MFE_SPREAD(5) MFE_SPREAD(6) MFE_SPREAD(7) MFE_SPREAD(8)
