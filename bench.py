@@ -29,6 +29,18 @@ from molsim_amd import engine as E  # noqa: E402
 from molsim_amd import systems as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "c3_kernel_traffic.md")
+
+
+def measured_traffic(stage, n_units):
+    """HBM bytes per launch of a stage's main kernel from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (200k-ion workload only)."""
+    if n_units != 40000 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_json:"):
+            return json.loads(line.split(":", 1)[1]).get(stage)
+    return None
 
 
 def algorithmic_bytes(sysm, stage, npairs=0):
@@ -135,7 +147,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--units", type=int, default=40000, help="salt units (40000 -> 200k ions)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=6)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched-replica BD leg")
     ap.add_argument("--c4-replicas", type=int, default=512)
@@ -164,7 +176,18 @@ def main():
         dt = time.perf_counter() - t0
         stages = {}
         npairs = 0
+        host_rate = None
         if prec == "mixed":
+            # UTotal with the host's buffers: positions in, forces out over
+            # PCIe each call (mfe_upload_positions + mfe_compute)
+            ro_h, _, _ = eng.download_state()
+            eng.upload_positions(ro_h)
+            eng.utotal(flags)
+            t1 = time.perf_counter()
+            for _ in range(20):
+                eng.upload_positions(ro_h)
+                eng.utotal(flags)
+            host_rate = 20 / (time.perf_counter() - t1)
             for st in ("pair", "spread", "fft_x", "fft_y", "conv", "ifft", "gather", "utotal"):
                 stages[st] = eng.time_stage(st, 20)
             # stored (full) list entries: twice the half list within rc + skin
@@ -172,7 +195,7 @@ def main():
             npairs = 2 * int(nn.sum())
         eng.close()
         return dict(sps=args.steps / dt, ms=1000 * dt / args.steps, f0=f0, e0=e0,
-                    en=en, stages=stages, npairs=npairs)
+                    en=en, stages=stages, npairs=npairs, host_rate=host_rate)
 
     mixed = run("mixed")
     fp64 = None if args.no_fp64 else run("fp64")
@@ -210,8 +233,10 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "algorithmic_bytes": nbytes, "traffic": None},
+                     "algorithmic_bytes": nbytes,
+                     "traffic": measured_traffic(dom, args.units)},
         "stages": stage_report,
+        "utotal_host_buffers_per_s": round(mixed["host_rate"], 1),
     }
     if fp64 is not None:
         f_m, f_d = mixed["f0"], fp64["f0"]
@@ -229,13 +254,20 @@ def main():
                                "sample": f"{args.cpu_steps} MD steps of the same 200k-ion "
                                          f"system with the FP64 oracle ({secs:.1f} s)"}
     if args.copy_probe_mb:
-        import torch
+        # three device-to-device copies of a known size through the HIP
+        # runtime (its copy kernel is the counters' calibration reference)
+        import ctypes as C
+        hip = C.CDLL("libamdhip64.so")
         nbytes_copy = args.copy_probe_mb << 20
-        src = torch.ones(nbytes_copy // 4, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
+        a, b = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes_copy)) == 0
+        assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes_copy)) == 0
+        assert hip.hipMemset(a, 1, C.c_size_t(nbytes_copy)) == 0
         for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+            assert hip.hipMemcpy(b, a, C.c_size_t(nbytes_copy), 3) == 0  # device to device
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(a)
+        hip.hipFree(b)
         out["copy_probe_bytes"] = nbytes_copy
     print(json.dumps(out), flush=True)
 

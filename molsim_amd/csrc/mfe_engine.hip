@@ -146,6 +146,7 @@ void free_all(mfe_engine *e) {
   dfree(e->k_vec); dfree(e->k_sum); dfree(e->ch_prev); dfree(e->ch_next);
   dfree(e->part); dfree(e->en);
   dfree(e->bd_fac1); dfree(e->bd_fac2); dfree(e->bd_noise); dfree(e->d_sys);
+  dfree(e->ro_stage);
   dfree(e->knot_c); dfree(e->phi_rec); dfree(e->adj_phi); dfree(e->adj_dq);
   dfree(e->adj_part); dfree(e->adj_out);
   if (e->h_en) (void)hipHostFree(e->h_en);
@@ -877,6 +878,19 @@ int mfe_upload_positions(mfe_engine *e, const double *ro) {
       return fail(e, MFE_E_BAD_PARAM, "particle outside the box [-L/2, L/2]");
   }
   return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    if (e->list_valid && e->have_positions && e->have_neighbor) {
+      // NList(iSimulationStep): displacements since the last upload, the
+      // skin test and, only when it fires, the rebuild -- all on the device
+      if (!e->ro_stage) dalloc(e->ro_stage, 3 * (size_t)e->max_np);
+      upload(e->ro_stage, ro, 3 * (size_t)e->np, e->stream);
+      sync_view(e);
+      mfe::launch_set_positions(e->grp, e->ro_stage);
+      mfe::launch_skin(e->grp);
+      mfe::launch_reduce(e->grp, 0u, true, false);
+      mfe::launch_nlist_rebuild(e->grp, true);
+      return read_flags(e);
+    }
     upload(e->ro, ro, 3 * (size_t)e->np, e->stream);
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     e->have_positions = true;

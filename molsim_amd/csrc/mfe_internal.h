@@ -328,6 +328,7 @@ struct mfe_engine {
   // Brownian dynamics
   double *bd_fac1 = nullptr, *bd_fac2 = nullptr;  // [MFE_MAX_TYPES]
   double *bd_noise = nullptr;                     // [3 np] host-supplied noise
+  double *ro_stage = nullptr;                     // [3 np] positions from the host
   unsigned long long bd_seed = 0;
   bool have_bd = false;
 
@@ -364,6 +365,7 @@ void launch_reduce(const Group &g, unsigned flags, bool skin, bool bd);
 void launch_finalize(const Group &g, int mode, double dt);
 void launch_move(const Group &g, double dt, int bd);
 void launch_skin(const Group &g);
+void launch_set_positions(const Group &g, const double *ro_new);
 void launch_collect(const Group &g, int *flags, double *en, int nen);
 void launch_fill_int(int *p, int n, int v, hipStream_t s);
 void launch_fill_double(double *p, int n, double v, hipStream_t s);

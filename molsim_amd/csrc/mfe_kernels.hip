@@ -1327,6 +1327,38 @@ __global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
   if (threadIdx.x == 0) part_max[blockIdx.x] = m;
 }
 
+// New positions from the host with a valid list: NList's displacement
+// bookkeeping (Src/nlist.F90:601-603) on the step since the last upload,
+// taken as the minimum image of new - old (one shift per axis, as PBCr2),
+// so the skin test that follows decides whether the list is rebuilt.
+__global__ void __launch_bounds__(kBlock) k_set_positions(const SysDev *__restrict__ sys,
+                                                          const double *__restrict__ ro_new) {
+#pragma clang fp contract(off)
+  const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nblk_np) return;
+  __shared__ double sh[kBlock];
+  const Box b = V.box;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double r2 = 0.0;
+  if (i < V.np) {
+    for (int m = 0; m < 3; ++m) {
+      const double x = ro_new[3 * i + m];
+      double d = x - V.ro[3 * i + m];
+      if (fabs(d) > b.L2[m]) d = d - copysign(b.L[m], d);
+      V.ro[3 * i + m] = x;
+      V.ro_s[3 * V.c_pos[i] + m] = x;
+      V.drosum[3 * i + m] = V.drosum[3 * i + m] + d;
+    }
+    const double *ds = V.drosum + 3 * i;
+    r2 = ds[0] * ds[0];
+    r2 = r2 + ds[1] * ds[1];
+    r2 = r2 + ds[2] * ds[2];
+    V.r2disp[i] = r2;
+  }
+  const double mx = block_max(r2, sh);
+  if (threadIdx.x == 0) V.part[V.off_move + blockIdx.x] = mx;
+}
+
 // NList's r2max2 (Src/nlist.F90:602-608) is the largest r2 that is not a
 // new running maximum at the moment it is visited in particle order.  Each
 // block owns the same particle range as in k_move; the running maximum
@@ -1551,6 +1583,11 @@ void launch_adjoint(const Group &g, int ncol, double *out) {
   hipLaunchKernelGGL(k_adjoint_q, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
   hipLaunchKernelGGL(k_adjoint_sum, grid(nblocks(ncol), g), dim3(kBlock), 0, g.stream, g.d,
                      out);
+}
+
+void launch_set_positions(const Group &g, const double *ro_new) {
+  hipLaunchKernelGGL(k_set_positions, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d,
+                     ro_new);
 }
 
 void launch_skin(const Group &g) {

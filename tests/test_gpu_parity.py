@@ -227,6 +227,38 @@ def test_md_rebuilds_match_oracle():
 
 
 # --------------------------------------------------------- error paths
+def test_host_positions_follow_the_skin_rule():
+    """Host-driven UTotal (upload, compute) keeps the list while the NList
+    skin test holds and rebuilds when it fires; forces equal a freshly
+    built engine's at the same positions (the pair set is the same; the
+    partner order of a list built earlier may differ, so FP64 to 1e-13)."""
+    s = S.c2()
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    say("case: c2 host uploads with the skin rule, fp64")
+    eng = S.configure(s, "fp64", tables=tab)
+    eng.utotal()
+    r0 = eng.energy().rebuilds
+    rng = np.random.default_rng(8)
+    ro = s.ro.copy()
+    # rigid shifts (no new close contacts) plus 0.02 A jitter; the 1.1 A
+    # shift moves every particle by more than skin/2 = 1 A
+    for step, shift in enumerate([0.05, 0.05, 1.1, 0.05]):
+        ro = ro + shift * np.array([0.6, 0.64, 0.48]) + 0.02 * rng.uniform(-1, 1, ro.shape)
+        ro -= s.boxlen * np.round(ro / s.boxlen)
+        eng.upload_positions(ro)
+        f, en = eng.utotal()
+        fresh = S.configure(s, "fp64", tables=tab)
+        fresh.upload_positions(ro)
+        f2, en2 = fresh.utotal()
+        fresh.close()
+        say(f"  step {step}: rebuilds {en.rebuilds - r0}")
+        assert rel_rms(f, f2) <= 1e-13 and abs(en.tot - en2.tot) <= 1e-13 * abs(en2.tot)
+    # the small moves keep the list; the 1.1 A shift rebuilds it
+    assert en.rebuilds - r0 == 1
+    eng.close()
+
+
 def test_overlap_is_reported():
     s = S.c1()
     s.ewald = "none"
