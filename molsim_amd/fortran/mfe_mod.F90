@@ -200,6 +200,109 @@ module mfe_mod
          real(c_double), intent(out) :: ms
       end function
 
+      ! BD with the device noise stream: pass c_null_ptr for fac1/fac2
+      ! (factors of mfe_set_bd) and for noise
+      integer(c_int) function mfe_step_bd_ptr(eng, nsteps, fac1, fac2, noise, flags) &
+            bind(c, name='mfe_step_bd')
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: eng, fac1, fac2, noise
+         integer(c_int32_t), value :: nsteps
+         integer(c_int32_t), value :: flags
+      end function
+
+      integer(c_int) function mfe_set_bd(eng, fac1, fac2, seed) bind(c, name='mfe_set_bd')
+         import :: c_int, c_int64_t, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(in) :: fac1(*), fac2(*)
+         integer(c_int64_t), value :: seed
+      end function
+
+      integer(c_int) function mfe_download_forces(eng, forceo) bind(c, name='mfe_download_forces')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: forceo(*)
+      end function
+
+      integer(c_int) function mfe_set_pair_params(eng, spec) bind(c, name='mfe_set_pair_params')
+         import :: c_int, c_ptr, mfe_pot_1_6_12
+         type(c_ptr), value :: eng
+         type(mfe_pot_1_6_12), intent(in) :: spec
+      end function
+
+      integer(c_int) function mfe_set_knots(eng, nknot, r_lo, r_hi, c) bind(c, name='mfe_set_knots')
+         import :: c_int, c_int32_t, c_ptr, c_double
+         type(c_ptr), value :: eng
+         integer(c_int32_t), value :: nknot
+         real(c_double), value :: r_lo, r_hi
+         real(c_double), intent(in) :: c(*)
+      end function
+
+      integer(c_int) function mfe_adjoint(eng, dq, dzat, dsigat, depsat, dknot, dro, energy) &
+            bind(c, name='mfe_adjoint')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: dq(*), dzat(*), dsigat(*), depsat(*), dknot(*), dro(*)
+         real(c_double), intent(out) :: energy
+      end function
+
+      ! batched replicas: engines(nrep) are the handles of set-up engines
+      integer(c_int) function mfe_batch_create(engines, nrep, batch) bind(c, name='mfe_batch_create')
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), intent(in) :: engines(*)
+         integer(c_int32_t), value :: nrep
+         type(c_ptr), intent(out) :: batch
+      end function
+
+      integer(c_int) function mfe_batch_destroy(batch) bind(c, name='mfe_batch_destroy')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function
+
+      type(c_ptr) function mfe_batch_last_error(batch) bind(c, name='mfe_batch_last_error')
+         import :: c_ptr
+         type(c_ptr), value :: batch
+      end function
+
+      integer(c_int32_t) function mfe_batch_failed_replica(batch) &
+            bind(c, name='mfe_batch_failed_replica')
+         import :: c_int32_t, c_ptr
+         type(c_ptr), value :: batch
+      end function
+
+      integer(c_int) function mfe_batch_build_list(batch) bind(c, name='mfe_batch_build_list')
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function
+
+      integer(c_int) function mfe_batch_compute(batch, flags) bind(c, name='mfe_batch_compute')
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int32_t), value :: flags
+      end function
+
+      integer(c_int) function mfe_batch_step_md(batch, nsteps, tstep, flags) &
+            bind(c, name='mfe_batch_step_md')
+         import :: c_int, c_int32_t, c_ptr, c_double
+         type(c_ptr), value :: batch
+         integer(c_int32_t), value :: nsteps
+         real(c_double), value :: tstep
+         integer(c_int32_t), value :: flags
+      end function
+
+      integer(c_int) function mfe_batch_step_bd(batch, nsteps, flags) &
+            bind(c, name='mfe_batch_step_bd')
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int32_t), value :: nsteps, flags
+      end function
+
+      integer(c_int) function mfe_batch_get_energies(batch, energies) &
+            bind(c, name='mfe_batch_get_energies')
+         import :: c_int, c_ptr, mfe_energies
+         type(c_ptr), value :: batch
+         type(mfe_energies), intent(out) :: energies(*)
+      end function
+
    end interface
 
 contains
