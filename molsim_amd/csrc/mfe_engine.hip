@@ -134,6 +134,7 @@ void free_all(mfe_engine *e) {
   dfree(e->drosum); dfree(e->r2disp); dfree(e->type); dfree(e->q); dfree(e->massip);
   dfree(e->t_zlow); dfree(e->t_coef); dfree(e->t_coef_f); dfree(e->t_off);
   dfree(e->t_nint); dfree(e->t_pairidx); dfree(e->t_r2atat); dfree(e->t_r2umin);
+  dfree(e->t_bin_k); dfree(e->t_rec);
   dfree(e->c_count); dfree(e->c_start); dfree(e->c_of); dfree(e->c_slot);
   dfree(e->c_sorted); dfree(e->c_pos); dfree(e->nn); dfree(e->nbr); dfree(e->flags);
   dfree(e->ro_s); dfree(e->type_s);
@@ -158,7 +159,8 @@ void free_all(mfe_engine *e) {
 // partial-sum regions of the reductions (see k_reduce)
 void layout_partials(mfe_engine *e) {
   const int nb = (e->np + 255) / 256;
-  e->nblk_pair = (e->np * mfe::kPairLanes + 255) / 256;
+  e->nblk_pair = std::min(mfe::kPairGridMax,
+                          (e->np * mfe::kPairLanes + mfe::kPairBlock - 1) / mfe::kPairBlock);
   e->nblk_bond = nb;
   e->nblk_surf = nb;
   e->nblk_move = nb;
@@ -254,6 +256,9 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   mfe::TableDev &t = v.tab;
   MFE_SETP(t.zlow, e->t_zlow); MFE_SETP(t.coef, e->t_coef); MFE_SETP(t.coef_f, e->t_coef_f); MFE_SETP(t.off, e->t_off);
   MFE_SETP(t.nint, e->t_nint); MFE_SETP(t.r2atat, e->t_r2atat); MFE_SETP(t.r2umin, e->t_r2umin); MFE_SETP(t.pairidx, e->t_pairidx);
+  MFE_SETP(t.bin_k, e->t_bin_k); MFE_SETP(t.rec, e->t_rec);
+  t.nbin = e->t_nbin;
+  t.lds_mode = e->t_lds_mode;
   t.npt = e->npt; t.nptpt = e->nptpt; t.rcut2 = e->rcut * e->rcut; t.nzl = e->t_nzl;
   const double rs = e->rcut + e->drnlist;
   v.rs2 = rs * rs;
@@ -353,6 +358,9 @@ int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStre
     if (e->lsurf && e->ewald != MFE_EWALD_NONE) g.surf = true;
     g.np_blocks = std::max(g.np_blocks, nblocks256(e->np));
     g.pair_blocks = std::max(g.pair_blocks, e->nblk_pair);
+    g.lds_pair = std::max(g.lds_pair, e->t_lds_bytes);
+    g.pair_npt = std::max(g.pair_npt, e->npt);
+    if (e->t_lds_mode < 2) g.pair_cf_lds = false;
     g.cell_blocks = std::max(g.cell_blocks, nblocks256(e->cg.ncells));
     g.kvec = std::max(g.kvec, e->nkvec);
     if (e->ewald == MFE_EWALD_SPME) {
@@ -636,9 +644,53 @@ int mfe_set_tables(mfe_engine *e, const double *ubuf, int32_t nbuf, const int32_
         const double *u = &ubuf[s0 - 1 + 12 * k];
         zl.push_back(u[0]);
         for (int c = 1; c < 12; ++c) cf.push_back(u[c]);
+        for (int c = 11; c < mfe::kCoefStride; ++c) cf.push_back(0.0);
       }
     }
     std::vector<float> cff(cf.begin(), cf.end());
+    // bins uniform in s = 1/r2, where the tables' intervals are roughly
+    // evenly spaced.  The kernel computes the bin in FP32 from r2 clamped
+    // to [lo, inf) and clamps it into [0, nbin); each bin is built over
+    // its s range widened by kBinMargin bins on both sides (far more than
+    // that rounding), so "interval kb or kb + 1" holds for every r2 the
+    // kernel can put there -- else the bin is marked slow.
+    const int nbin = std::min(mfe::kTableBinsPerPair, mfe::kTableBins / P);
+    constexpr double kBinMargin = 0.05;
+    std::vector<unsigned short> bin_k((size_t)P * nbin);
+    std::vector<mfe::PairRec> rec(P);
+    for (int p = 0; p < P; ++p) {
+      const double *z = zl.data() + off[p];
+      const int n = nint[p];
+      if (n > mfe::kSlowBin - 1)
+        return fail(e, MFE_E_BAD_PARAM, "more than 32767 intervals in one table");
+      const double lo = z[n - 1], hi = std::max(rcut * rcut, z[0]);
+      // first interval k with r2 >= z[k] (r2 >= lo)
+      auto first_at = [&](double r2) {
+        int k = n - 1;
+        while (k > 0 && r2 >= z[k - 1]) --k;
+        return k;
+      };
+      mfe::PairRec &q = rec[p];
+      q.r2at = r2atat[p];
+      q.r2um = r2umin[p];
+      q.lo = lo;
+      q.pad = 0.0;
+      q.off = off[p];
+      q.nint = n;
+      q.s_lo = (float)(1.0 / hi);
+      const double span = 1.0 / lo - (double)q.s_lo;
+      q.s_inv = span > 0.0 ? (float)(nbin / span) : 0.0f;
+      const double w = q.s_inv > 0.0f ? 1.0 / (double)q.s_inv : 0.0;
+      for (int b = 0; b < nbin; ++b) {
+        const double sa = (double)q.s_lo + (b - kBinMargin) * w;
+        const double sb = (double)q.s_lo + (b + 1 + kBinMargin) * w;
+        const double r2a = (b == 0 || !(sa > 0.0)) ? HUGE_VAL : 1.0 / sa;
+        const double r2b = (b == nbin - 1 || !(sb > 0.0)) ? lo : std::max(lo, 1.0 / sb);
+        const int kmin = first_at(r2a), kmax = first_at(r2b);
+        bin_k[(size_t)p * nbin + b] =
+            (unsigned short)(kmax - kmin <= 1 ? kmin : (kmin | mfe::kSlowBin));
+      }
+    }
     std::vector<int> pidx(e->npt * e->npt);
     int k = 0;
     for (int i = 0; i < e->npt; ++i)
@@ -647,6 +699,24 @@ int mfe_set_tables(mfe_engine *e, const double *ubuf, int32_t nbuf, const int32_
     dalloc(e->t_zlow, zl.size()); dalloc(e->t_coef, cf.size()); dalloc(e->t_coef_f, cf.size());
     dalloc(e->t_off, (size_t)P); dalloc(e->t_nint, (size_t)P); dalloc(e->t_pairidx, pidx.size());
     dalloc(e->t_r2atat, (size_t)P); dalloc(e->t_r2umin, (size_t)P);
+    dalloc(e->t_bin_k, bin_k.size()); dalloc(e->t_rec, (size_t)P);
+    upload(e->t_bin_k, bin_k.data(), bin_k.size(), e->stream);
+    upload(e->t_rec, rec.data(), (size_t)P, e->stream);
+    e->t_nbin = nbin;
+    // LDS staging of the pair kernel within its budget
+    const size_t csz = e->prec == MFE_PREC_MIXED ? 4 : 8;
+    const size_t bb = mfe::pair_bin_bytes(P, nbin), zb = mfe::pair_zl_bytes((int)zl.size());
+    const size_t cb = zl.size() * mfe::kCoefStride * csz;
+    e->t_lds_mode = 0;
+    e->t_lds_bytes = bb;
+    if (bb + zb <= mfe::kPairLdsBudget) {
+      e->t_lds_mode = 1;
+      e->t_lds_bytes = bb + zb;
+      if (bb + zb + cb <= mfe::kPairLdsBudget) {
+        e->t_lds_mode = 2;
+        e->t_lds_bytes = bb + zb + cb;
+      }
+    }
     upload(e->t_zlow, zl.data(), zl.size(), e->stream);
     upload(e->t_coef, cf.data(), cf.size(), e->stream);
     upload(e->t_coef_f, cff.data(), cff.size(), e->stream);

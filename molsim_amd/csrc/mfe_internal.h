@@ -71,6 +71,17 @@ struct CellGeom {
 // Table intervals in device layout: for pair p, intervals k = 0..nint[p]-1
 // start at index off[p]; zlow[off+k] (FP64, decreasing in k) and the 11
 // coefficients coef[(off+k)*11 + c] (usum: c = 0..5, fsum: c = 6..10).
+// Per type pair: what the pair kernel needs before the interval search.
+struct alignas(16) PairRec {
+  double r2at, r2um;  // hard core below r2at; overlap error below r2um
+  double lo, pad;     // lowest interval bound zlow(nint - 1)
+  float s_lo, s_inv;  // bin of r2: (1/r2 - s_lo) * s_inv
+  int off, nint;      // the pair's intervals in zlow / coef
+};
+constexpr int kSlowBin = 0x8000;
+// coefficients stored per interval (11 used, padded for 16-byte loads)
+constexpr int kCoefStride = 12;
+
 struct TableDev {
   MFE_DPTR(const double) zlow;
   MFE_DPTR(const double) coef;
@@ -80,10 +91,36 @@ struct TableDev {
   MFE_DPTR(const double) r2atat;
   MFE_DPTR(const double) r2umin;
   MFE_DPTR(const int) pairidx;  // [npt*npt] -> pair index 0..nptpt-1
+  // bins of s = 1/r2 per pair (see PairRec): entry kb means the interval
+  // of any r2 in the bin is kb or kb + 1; kb | kSlowBin means the bin
+  // spans more intervals and the search walks from kb
+  MFE_DPTR(const unsigned short) bin_k;  // [nptpt * nbin]
+  MFE_DPTR(const PairRec) rec;           // [nptpt]
+  int nbin;
+  // what the pair kernel stages in LDS: 0 = the bins, 1 = also the
+  // interval bounds, 2 = also the polynomial coefficients
+  int lds_mode;
   int npt, nptpt;
   int nzl;             // intervals over all pairs (length of zlow)
   double rcut2;
 };
+
+// r2 bins per pair, the total over pairs bounded by kTableBins
+constexpr int kTableBins = 4096;
+constexpr int kTableBinsPerPair = 256;
+// threads per block of the pair kernel and its dynamic LDS budget
+constexpr int kPairBlock = 512;
+// blocks of the persistent pair kernel (two per CU)
+constexpr int kPairGridMax = 512;
+constexpr size_t kPairLdsBudget = 56 * 1024;
+// dynamic LDS of the pair kernel: bins, then zlow, then coefficients,
+// each part starting on 16 bytes
+__host__ __device__ inline size_t pair_bin_bytes(int nptpt, int nbin) {
+  return ((size_t)nptpt * nbin * 2 + 15) / 16 * 16;
+}
+__host__ __device__ inline size_t pair_zl_bytes(int nzl) {
+  return ((size_t)nzl * 8 + 15) / 16 * 16;
+}
 
 // FFT plan of one transform length (mixed radix Stockham).
 struct FftPlan {
@@ -221,6 +258,9 @@ struct Group {
   int mesh_blocks = 0;   // blocks of 256 over the real mesh
   int fx_blocks = 0, fy_x = 0, fy_y = 0, fz_x = 0, fz_y = 0;
   size_t lds_spread = 0, lds_gather = 0, lds_fx = 0, lds_fy = 0, lds_fz = 0;
+  size_t lds_pair = 0;
+  int pair_npt = 0;         // most types of any replica
+  bool pair_cf_lds = true;  // every replica's coefficients fit in LDS
 };
 
 }  // namespace mfe
@@ -258,6 +298,11 @@ struct mfe_engine {
   float *t_coef_f = nullptr;
   int *t_off = nullptr, *t_nint = nullptr, *t_pairidx = nullptr;
   double *t_r2atat = nullptr, *t_r2umin = nullptr;
+  unsigned short *t_bin_k = nullptr;
+  mfe::PairRec *t_rec = nullptr;
+  int t_nbin = 0;
+  int t_lds_mode = 0;
+  size_t t_lds_bytes = 0;
   int t_nzl = 0;
   double rcut = 0.0;
 

@@ -268,21 +268,19 @@ __device__ __forceinline__ bool knot_weights(double r, double r0, double dinv, i
 }
 
 // ------------------------------------------------------------ pair forces
-// table intervals (all type pairs) the pair kernel keeps in LDS; larger
-// tables are searched in global memory
-constexpr int kPairLdsIntervals = 2048;
-// First interval (in storage order) with r2 >= zlow: zlow decreases along
-// the table, so this is a binary search for the same interval the
-// reference's linear walk stops at (Src/energy.F90:482-486).
-template <typename P>
-__device__ __forceinline__ int find_interval(P zl, int n, double r2) {
-  int lo = 0, hi = n - 1;
-  if (!(r2 >= zl[hi])) return -1;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (r2 >= zl[mid]) hi = mid; else lo = mid + 1;
-  }
-  return lo;
+// First interval (in storage order) with r2 >= zlow -- the interval the
+// reference's linear walk stops at (Src/energy.F90:482-486).  zlow
+// decreases along the table; the walk starts at the interval kb of r2's
+// bin and moves to lower r2 while r2 < zlow, then to higher r2 while the
+// next bound is still <= r2.  Both moves together reach the first k with
+// r2 >= zl(k) from any start, so the bin only sets the walk's length
+// (0-2 steps for bins about as wide as the intervals).
+template <typename Z>
+__device__ __forceinline__ int find_interval(const Z &zl, int n, double r2, int kb) {
+  int k = kb;
+  while (k < n - 1 && !(r2 >= zl(k))) ++k;
+  while (k > 0 && r2 >= zl(k - 1)) --k;
+  return k;
 }
 
 struct KahanF {
@@ -296,165 +294,286 @@ struct KahanF {
   __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// kPairLanes (8) lanes share one particle (cell order t) and split its
-// full list k = lane, lane+8, ...; their partial forces are combined by shuffles in
-// FP64 in a fixed order, so forces need no atomics and stay
-// deterministic.  Every pair is seen from both ends, so its energy and
-// virial carry weight 1/2.  semantics: 0 = vlist (r2 <= rcut2 kept),
-// 1 = cell list (r2 < rcut2 kept).
-template <bool MIXED>
-__global__ void __launch_bounds__(kBlock) k_pair(const SysDev *__restrict__ sys,
-                                                 int energy) {
+// Blocks are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
+// each XCD with its own L2.  This maps them to logical blocks so that each
+// XCD takes one contiguous range: in cell order that is a slab of the box,
+// whose positions (and its neighbours') stay in that XCD's L2.  A bijection
+// of [0, nblk).
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7, x = b & 7, k = b >> 3;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
+// list entries a lane loads together (indices, then positions)
+constexpr int kPairBatch = 4;
+
+// Pair forces over the Verlet list.  kPairLanes lanes share one particle
+// (cell order i) and split its list k = lane, lane + kPairLanes, ...; their
+// partial forces are combined by shuffles in FP64 in a fixed order, so
+// forces need no atomics and stay deterministic.  Every pair is seen from
+// both ends, so its energy and virial carry weight 1/2.  semantics: 0 =
+// vlist (r2 <= rcut2 kept), 1 = cell list (r2 < rcut2 kept).
+//
+// The grid is persistent: nblk_pair (<= kPairGridMax) blocks each stage
+// the tables in LDS once -- r2 bins, interval bounds and, with CF, the
+// polynomial coefficients -- and then walk one contiguous range of chunks
+// of kPairBlock / kPairLanes particles.  A lane loads the indices of
+// kPairBatch of its entries together, then their positions, then
+// evaluates them.  NT bounds the number of types (energy accumulators per
+// partner type).  Energies go per chunk into a per-wavefront LDS row
+// (fixed order), and per block into the partials.
+template <bool MIXED, int NT, bool CF>
+__global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
+k_pair(const SysDev *__restrict__ sys, int energy) {
+  using CT = typename std::conditional<MIXED, float, double>::type;
   const SysDev &V = sys[blockIdx.z];
-  if ((int)blockIdx.x >= V.nblk_pair) return;
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  const int nb = V.nblk_pair;
+  if (blk >= nb) return;
   const double *__restrict__ ro = V.ro_s;
   const int *__restrict__ type = V.type_s;
   const int np = V.np;
   const int *__restrict__ nn = V.nn;
   const int *__restrict__ nbr = V.nbr;
-  const int *__restrict__ sorted = V.c_sorted;
   const Box b = V.box;
   const TableDev t = V.tab;
   const int semantics = V.nlsem;
-  double *__restrict__ force = V.force;
-  double *__restrict__ part = V.part + V.off_pair;
   int *flags = V.flags;
-  __shared__ double sh[kBlock];
-  // the interval bounds of every table and the per-pair constants in LDS:
-  // the interval search then runs on LDS instead of chained global loads
-  __shared__ double zl_s[kPairLdsIntervals];
-  __shared__ double r2at_s[MFE_MAX_PAIRS], r2um_s[MFE_MAX_PAIRS];
-  __shared__ int off_s[MFE_MAX_PAIRS], nint_s[MFE_MAX_PAIRS];
+  extern __shared__ __align__(16) unsigned char pair_lds[];
+  __shared__ PairRec rec_s[MFE_MAX_PAIRS];
   __shared__ int pidx_s[MFE_MAX_TYPES * MFE_MAX_TYPES];
-  const bool lds_tab = t.nzl <= kPairLdsIntervals;
-  if (lds_tab)
-    for (int k = threadIdx.x; k < t.nzl; k += blockDim.x) zl_s[k] = t.zlow[k];
-  for (int k = threadIdx.x; k < t.nptpt; k += blockDim.x) {
-    r2at_s[k] = t.r2atat[k];
-    r2um_s[k] = t.r2umin[k];
-    off_s[k] = t.off[k];
-    nint_s[k] = t.nint[k];
+  __shared__ double wacc[kPairBlock / 64][MFE_MAX_PAIRS + 1];
+  const int P = t.nptpt, nbin = t.nbin, nzl = t.nzl, npt = t.npt;
+  const bool lds_zl = CF || t.lds_mode >= 1;
+  const size_t bb = pair_bin_bytes(P, nbin);
+  unsigned short *bin_s = reinterpret_cast<unsigned short *>(pair_lds);
+  double *zl_s = reinterpret_cast<double *>(pair_lds + bb);
+  CT *cf_s = reinterpret_cast<CT *>(pair_lds + bb + pair_zl_bytes(nzl));
+  for (int k = threadIdx.x; k < P * nbin; k += blockDim.x) bin_s[k] = t.bin_k[k];
+  if (lds_zl)
+    for (int k = threadIdx.x; k < nzl; k += blockDim.x) zl_s[k] = t.zlow[k];
+  if (CF) {
+    for (int k = threadIdx.x; k < nzl * kCoefStride; k += blockDim.x) {
+      if constexpr (MIXED) cf_s[k] = t.coef_f[k];
+      else cf_s[k] = t.coef[k];
+    }
   }
-  for (int k = threadIdx.x; k < t.npt * t.npt; k += blockDim.x) pidx_s[k] = t.pairidx[k];
+  for (int k = threadIdx.x; k < P; k += blockDim.x) rec_s[k] = t.rec[k];
+  for (int k = threadIdx.x; k < npt * npt; k += blockDim.x) pidx_s[k] = t.pairidx[k];
+  const int l64 = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = l64; k <= P; k += 64) wacc[wv][k] = 0.0;
   __syncthreads();
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = gid / kPairLanes, lane = gid % kPairLanes;
-  // energy per partner type (static indexing keeps it in registers)
-  double ej[MFE_MAX_TYPES];
+  // the 12 stored coefficients of interval iv (16-byte loads)
+  auto coefs = [&](int iv, CT *c) {
+    if constexpr (MIXED) {
+      const float4 *q = CF ? reinterpret_cast<const float4 *>(cf_s) + 3 * iv
+                           : reinterpret_cast<const float4 *>(t.coef_f) + 3 * iv;
 #pragma unroll
-  for (int k = 0; k < MFE_MAX_TYPES; ++k) ej[k] = 0.0;
+      for (int h = 0; h < 3; ++h) {
+        const float4 v = q[h];
+        c[4 * h] = v.x;
+        c[4 * h + 1] = v.y;
+        c[4 * h + 2] = v.z;
+        c[4 * h + 3] = v.w;
+      }
+    } else {
+      const double2 *q = CF ? reinterpret_cast<const double2 *>(cf_s) + 6 * iv
+                            : reinterpret_cast<const double2 *>(t.coef) + 6 * iv;
+#pragma unroll
+      for (int h = 0; h < 6; ++h) {
+        const double2 v = q[h];
+        c[2 * h] = v.x;
+        c[2 * h + 1] = v.y;
+      }
+    }
+  };
+  auto zlow = [&](int idx) -> double {
+    if constexpr (CF) return zl_s[idx];
+    else return lds_zl ? zl_s[idx] : t.zlow[idx];
+  };
+  const int lane = threadIdx.x % kPairLanes;
+  double ej[NT];
   double vir = 0.0;
+  double fx, fy, fz;
+  KahanF fxf, fyf, fzf;
   int ti = 0;
-  double fx = 0.0, fy = 0.0, fz = 0.0;
-  if (i < np) {
-    const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
-    ti = type[i];
-    KahanF fxf, fyf, fzf;
-    const int n = nn[i];
-    for (int k = lane; k < n; k += kPairLanes) {
-      const int j = nbr[(size_t)k * np + i];
-      double dx = xi - ro[3 * j], dy = yi - ro[3 * j + 1], dz = zi - ro[3 * j + 2];
-      const double r2 = pbcr2(dx, dy, dz, b);
-      if (semantics == 0 ? (r2 > t.rcut2) : !(r2 < t.rcut2)) continue;
-      const int tj = type[j];
-      const int p = pidx_s[ti * t.npt + tj];
-      double usum, fsum;
-      if (r2 < r2at_s[p]) {
-        usum = 1e10;
-        fsum = 1e10;
-      } else if (r2 < r2um_s[p]) {
-        atomicOr(&flags[0], ERR_OVERLAP);
-        continue;
-      } else {
-        const int o = off_s[p];
-        const int iv = lds_tab ? find_interval(zl_s + o, nint_s[p], r2)
-                               : find_interval(t.zlow + o, nint_s[p], r2);
-        if (iv < 0) {
-          atomicOr(&flags[0], ERR_TABLE);
-          continue;
-        }
-        const double d = r2 - (lds_tab ? zl_s[o + iv] : t.zlow[o + iv]);
-        if (MIXED) {
-          const float *cf = t.coef_f + (size_t)(o + iv) * 11;
-          const float df = (float)d;
-          const float fs = cf[6] + df * (cf[7] + df * (cf[8] + df * (cf[9] + df * cf[10])));
-          fsum = fs;
-          if (energy) {
-            const float us = cf[0] + df * (cf[1] + df * (cf[2] + df * (cf[3] + df * (cf[4] + df * cf[5]))));
-            usum = us;
-          } else {
-            usum = 0.0;
-          }
-        } else {
-          const double *cf = t.coef + (size_t)(o + iv) * 11;
-          fsum = cf[6] + d * (cf[7] + d * (cf[8] + d * (cf[9] + d * cf[10])));
-          usum = cf[0] + d * (cf[1] + d * (cf[2] + d * (cf[3] + d * (cf[4] + d * cf[5]))));
-        }
-        if (V.nknot) {
-          // additive spline term: u += sum c w, fsum -= u'/r
-          const double r = sqrt(r2);
-          int ki;
-          double w[4], dw[4];
-          if (knot_weights(r, V.knot_r0, V.knot_dinv, V.nknot, ki, w, dw)) {
-            const double *c = V.knot_c + (size_t)p * V.nknot + ki;
-            const double du = c[0] * w[0] + c[1] * w[1] + c[2] * w[2] + c[3] * w[3];
-            const double dup = (c[0] * dw[0] + c[1] * dw[1] + c[2] * dw[2] + c[3] * dw[3]) *
-                               V.knot_dinv;
-            usum += du;
-            fsum -= dup / r;
-          }
-        }
-      }
-      if (MIXED) {
-        const float fs = (float)fsum;
-        fxf.add(fs * (float)dx);
-        fyf.add(fs * (float)dy);
-        fzf.add(fs * (float)dz);
-      } else {
-        fx += fsum * dx;
-        fy += fsum * dy;
-        fz += fsum * dz;
-      }
+  // pair index of this particle with partner type u in bits 8u..8u+7
+  unsigned long long ptp = 0;
+  int err = 0;    // error bits, flagged once at the end
+  // one list entry: partner of type tj at (dx, dy, dz) = r_i - r_j.  The
+  // LDS reads come before any branch, so the entries of a batch overlap.
+  auto eval = [&](double dx, double dy, double dz, int tj) {
+    const double r2 = pbcr2(dx, dy, dz, b);
+    const bool in = semantics == 0 ? !(r2 > t.rcut2) : (r2 < t.rcut2);
+    const int p = (int)((ptp >> (8 * tj)) & 0xffu);
+    const PairRec R = rec_s[p];
+    // interval of r2 (of lo below the table: such an entry is an error
+    // or a hard core, and only its flags count)
+    const double r2c = fmax(r2, R.lo);
+    const float sf = __builtin_amdgcn_rcpf((float)r2c);
+    const float xb = fminf(fmaxf((sf - R.s_lo) * R.s_inv, 0.0f), (float)(nbin - 1));
+    const int bk = bin_s[p * nbin + (int)xb];
+    const int o = R.off, n = R.nint, kb = bk & (kSlowBin - 1);
+    const double z0 = zlow(o + kb), z1 = zlow(o + min(kb + 1, n - 1));
+    int iv;
+    double zv;
+    if (bk & kSlowBin) {
+      auto zl = [&](int k) -> double { return zlow(o + k); };
+      iv = find_interval(zl, n, r2c, kb);
+      zv = zlow(o + iv);
+    } else {
+      const bool at = r2c >= z0;
+      iv = at ? kb : min(kb + 1, n - 1);
+      zv = at ? z0 : z1;
+    }
+    CT c[kCoefStride];
+    coefs(o + iv, c);
+    const bool hc = r2 < R.r2at;
+    const bool ov = !hc && r2 < R.r2um;
+    const bool te = !hc && !ov && !(r2 >= R.lo);
+    if (in) err |= (ov ? ERR_OVERLAP : 0) | (te ? ERR_TABLE : 0);
+    if (!in || ov || te) return;
+    const double d = r2 - zv;
+    double usum, fsum;
+    if (MIXED) {
+      const float df = (float)d;
+      const float fs = c[6] + df * (c[7] + df * (c[8] + df * (c[9] + df * c[10])));
+      fsum = fs;
       if (energy) {
-#pragma unroll
-        for (int k2 = 0; k2 < MFE_MAX_TYPES; ++k2) ej[k2] += (k2 == tj) ? 0.5 * usum : 0.0;
-        vir -= 0.5 * fsum * r2;
+        const float us = c[0] + df * (c[1] + df * (c[2] + df * (c[3] + df * (c[4] + df * c[5]))));
+        usum = us;
+      } else {
+        usum = 0.0;
+      }
+    } else {
+      fsum = c[6] + d * (c[7] + d * (c[8] + d * (c[9] + d * c[10])));
+      usum = c[0] + d * (c[1] + d * (c[2] + d * (c[3] + d * (c[4] + d * c[5]))));
+    }
+    if (hc) {
+      usum = 1e10;
+      fsum = 1e10;
+    } else if (V.nknot) {
+      // additive spline term: u += sum c w, fsum -= u'/r
+      const double r = sqrt(r2);
+      int ki;
+      double w[4], dw[4];
+      if (knot_weights(r, V.knot_r0, V.knot_dinv, V.nknot, ki, w, dw)) {
+        const double *kc = V.knot_c + (size_t)p * V.nknot + ki;
+        const double du = kc[0] * w[0] + kc[1] * w[1] + kc[2] * w[2] + kc[3] * w[3];
+        const double dup = (kc[0] * dw[0] + kc[1] * dw[1] + kc[2] * dw[2] + kc[3] * dw[3]) *
+                           V.knot_dinv;
+        usum += du;
+        fsum -= dup / r;
       }
     }
     if (MIXED) {
-      fx = fxf.value();
-      fy = fyf.value();
-      fz = fzf.value();
+      const float fs = (float)fsum;
+      fxf.add(fs * (float)dx);
+      fyf.add(fs * (float)dy);
+      fzf.add(fs * (float)dz);
+    } else {
+      fx += fsum * dx;
+      fy += fsum * dy;
+      fz += fsum * dz;
     }
-  }
+    if (energy) {
 #pragma unroll
-  for (int w = 1; w < kPairLanes; w <<= 1) {
-    fx += __shfl_xor(fx, w, kPairLanes);
-    fy += __shfl_xor(fy, w, kPairLanes);
-    fz += __shfl_xor(fz, w, kPairLanes);
-  }
-  if (i < np && lane == 0) {
-    const int io = sorted[i];
-    force[3 * io] = fx;
-    force[3 * io + 1] = fy;
-    force[3 * io + 2] = fz;
-  }
-  if (energy) {
-    // component-major partials: component k of block b at k * nblk_pair + b
-    double *out = part + blockIdx.x;
-    const size_t cs = (size_t)V.nblk_pair;
-    for (int k = 0; k < t.nptpt; ++k) {
-      double v = 0.0;
-      if (i < np) {
+      for (int k2 = 0; k2 < NT; ++k2) ej[k2] += (k2 == tj) ? 0.5 * usum : 0.0;
+      vir -= 0.5 * fsum * r2;
+    }
+  };
+  // this block's chunks: an even share of the contiguous chunk range
+  constexpr int kChunk = kPairBlock / kPairLanes;
+  const int nch = (np + kChunk - 1) / kChunk;
+  const int ch0 = (int)((long long)blk * nch / nb), ch1 = (int)((long long)(blk + 1) * nch / nb);
+  for (int ch = ch0; ch < ch1; ++ch) {
+    const int i = ch * kChunk + threadIdx.x / kPairLanes;
 #pragma unroll
-        for (int u = 0; u < MFE_MAX_TYPES; ++u)
-          if (u < t.npt && pidx_s[ti * t.npt + u] == k) v += ej[u];
+    for (int k = 0; k < NT; ++k) ej[k] = 0.0;
+    fx = fy = fz = 0.0;
+    fxf = KahanF();
+    fyf = KahanF();
+    fzf = KahanF();
+    ti = 0;
+    if (i < np) {
+      const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
+      ti = type[i];
+      ptp = 0;
+      for (int u = 0; u < npt; ++u)
+        ptp |= (unsigned long long)pidx_s[ti * npt + u] << (8 * u);
+      const int n = nn[i];
+      // the loads are unconditional (rows clamped into the list; past its
+      // end the particle itself) so no branch makes the compiler wait for
+      // them one by one
+      const int last = n - 1;
+      for (int k0 = lane; k0 < n; k0 += kPairBatch * kPairLanes) {
+        int js[kPairBatch];
+#pragma unroll
+        for (int g = 0; g < kPairBatch; ++g) {
+          const int kk = k0 + g * kPairLanes;
+          const int jr = nbr[(size_t)max(0, min(kk, last)) * np + i];
+          js[g] = kk < n ? jr : i;
+        }
+        double xs[kPairBatch], ys[kPairBatch], zs[kPairBatch];
+        int ts[kPairBatch];
+#pragma unroll
+        for (int g = 0; g < kPairBatch; ++g) {
+          xs[g] = ro[3 * js[g]];
+          ys[g] = ro[3 * js[g] + 1];
+          zs[g] = ro[3 * js[g] + 2];
+          ts[g] = type[js[g]];
+        }
+#pragma unroll
+        for (int g = 0; g < kPairBatch; ++g)
+          if (k0 + g * kPairLanes < n) eval(xi - xs[g], yi - ys[g], zi - zs[g], ts[g]);
       }
-      double s2 = block_sum(v, sh);
-      if (threadIdx.x == 0) out[k * cs] = s2;
+      if (MIXED) {
+        fx = fxf.value();
+        fy = fyf.value();
+        fz = fzf.value();
+      }
     }
-    double s2 = block_sum(vir, sh);
-    if (threadIdx.x == 0) out[t.nptpt * cs] = s2;
+#pragma unroll
+    for (int w = 1; w < kPairLanes; w <<= 1) {
+      fx += __shfl_xor(fx, w, kPairLanes);
+      fy += __shfl_xor(fy, w, kPairLanes);
+      fz += __shfl_xor(fz, w, kPairLanes);
+    }
+    if (i < np && lane == 0) {
+      const int io = V.c_sorted[i];
+      V.force[3 * io] = fx;
+      V.force[3 * io + 1] = fy;
+      V.force[3 * io + 2] = fz;
+    }
+    if (energy) {
+      // pair-type energies of this chunk: a shuffle tree in each
+      // wavefront, added to the wavefront's own LDS row
+      for (int k = 0; k < P; ++k) {
+        double v = 0.0;
+        if (i < np) {
+#pragma unroll
+          for (int u = 0; u < NT; ++u)
+            if (u < npt && pidx_s[ti * npt + u] == k) v += ej[u];
+        }
+        for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
+        if (l64 == 0) wacc[wv][k] += v;
+      }
+    }
+  }
+  if (err) atomicOr(&flags[0], err);
+  if (energy) {
+    // the virial, then per-block partials, component-major (component k
+    // of block b at k * nblk_pair + b), the wavefronts summed in order
+    double v = vir;
+    for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
+    if (l64 == 0) wacc[wv][P] = v;
+    __syncthreads();
+    double *out = V.part + V.off_pair + blk;
+    for (int k = threadIdx.x; k <= P; k += blockDim.x) {
+      double s2 = wacc[0][k];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s2 += wacc[w][k];
+      out[(size_t)k * nb] = s2;
+    }
   }
 }
 
@@ -1476,12 +1595,19 @@ void launch_nlist_rebuild(const Group &g, bool gated) {
 void launch_pair(const Group &g, bool energy, bool sort_pos) {
   if (sort_pos)
     hipLaunchKernelGGL(k_sort_pos, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
-  if (g.prec == MFE_PREC_MIXED)
-    hipLaunchKernelGGL(k_pair<true>, grid(g.pair_blocks, g), dim3(kBlock), 0, g.stream, g.d,
-                       energy ? 1 : 0);
-  else
-    hipLaunchKernelGGL(k_pair<false>, grid(g.pair_blocks, g), dim3(kBlock), 0, g.stream, g.d,
-                       energy ? 1 : 0);
+  const dim3 gr = grid(g.pair_blocks, g);
+  const int e1 = energy ? 1 : 0;
+#define MFE_PAIR(M, NT, CF) \
+  hipLaunchKernelGGL((k_pair<M, NT, CF>), gr, dim3(kPairBlock), g.lds_pair, g.stream, g.d, e1)
+  const bool nt4 = g.pair_npt <= 4;
+  if (g.prec == MFE_PREC_MIXED) {
+    if (nt4) { if (g.pair_cf_lds) MFE_PAIR(true, 4, true); else MFE_PAIR(true, 4, false); }
+    else { if (g.pair_cf_lds) MFE_PAIR(true, 8, true); else MFE_PAIR(true, 8, false); }
+  } else {
+    if (nt4) { if (g.pair_cf_lds) MFE_PAIR(false, 4, true); else MFE_PAIR(false, 4, false); }
+    else { if (g.pair_cf_lds) MFE_PAIR(false, 8, true); else MFE_PAIR(false, 8, false); }
+  }
+#undef MFE_PAIR
 }
 
 template <typename R>
