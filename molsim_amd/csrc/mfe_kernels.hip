@@ -319,9 +319,10 @@ constexpr int kPairBatch = 4;
 // polynomial coefficients -- and then walk one contiguous range of chunks
 // of kPairBlock / kPairLanes particles.  A lane loads the indices of
 // kPairBatch of its entries together, then their positions, then
-// evaluates them.  NT bounds the number of types (energy accumulators per
-// partner type).  Energies go per chunk into a per-wavefront LDS row
-// (fixed order), and per block into the partials.
+// evaluates them.  NT bounds the number of types: up to 4 the energies
+// accumulate per pair type in registers over all chunks, else per partner
+// type, converted per chunk into a per-wavefront LDS row; either way in a
+// fixed order, and per block into the partials.
 template <bool MIXED, int NT, bool CF>
 __global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_pair(const SysDev *__restrict__ sys, int energy) {
@@ -392,7 +393,13 @@ k_pair(const SysDev *__restrict__ sys, int energy) {
     else return lds_zl ? zl_s[idx] : t.zlow[idx];
   };
   const int lane = threadIdx.x % kPairLanes;
-  double ej[NT];
+  // energy accumulators: per pair type for up to 4 types (kept over all
+  // chunks), else per partner type (converted to pair types per chunk)
+  constexpr bool kRegE = NT <= 4;
+  constexpr int NA = kRegE ? NT * (NT + 1) / 2 : NT;
+  double ej[NA];
+#pragma unroll
+  for (int k = 0; k < NA; ++k) ej[k] = 0.0;
   double vir = 0.0;
   double fx, fy, fz;
   KahanF fxf, fyf, fzf;
@@ -478,7 +485,7 @@ k_pair(const SysDev *__restrict__ sys, int energy) {
     }
     if (energy) {
 #pragma unroll
-      for (int k2 = 0; k2 < NT; ++k2) ej[k2] += (k2 == tj) ? 0.5 * usum : 0.0;
+      for (int k2 = 0; k2 < NA; ++k2) ej[k2] += (k2 == (kRegE ? p : tj)) ? 0.5 * usum : 0.0;
       vir -= 0.5 * fsum * r2;
     }
   };
@@ -488,8 +495,10 @@ k_pair(const SysDev *__restrict__ sys, int energy) {
   const int ch0 = (int)((long long)blk * nch / nb), ch1 = (int)((long long)(blk + 1) * nch / nb);
   for (int ch = ch0; ch < ch1; ++ch) {
     const int i = ch * kChunk + threadIdx.x / kPairLanes;
+    if (!kRegE) {
 #pragma unroll
-    for (int k = 0; k < NT; ++k) ej[k] = 0.0;
+      for (int k = 0; k < NA; ++k) ej[k] = 0.0;
+    }
     fx = fy = fz = 0.0;
     fxf = KahanF();
     fyf = KahanF();
@@ -545,14 +554,14 @@ k_pair(const SysDev *__restrict__ sys, int energy) {
       V.force[3 * io + 1] = fy;
       V.force[3 * io + 2] = fz;
     }
-    if (energy) {
+    if (energy && !kRegE) {
       // pair-type energies of this chunk: a shuffle tree in each
       // wavefront, added to the wavefront's own LDS row
       for (int k = 0; k < P; ++k) {
         double v = 0.0;
         if (i < np) {
 #pragma unroll
-          for (int u = 0; u < NT; ++u)
+          for (int u = 0; u < NA; ++u)
             if (u < npt && pidx_s[ti * npt + u] == k) v += ej[u];
         }
         for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
@@ -562,6 +571,16 @@ k_pair(const SysDev *__restrict__ sys, int energy) {
   }
   if (err) atomicOr(&flags[0], err);
   if (energy) {
+    if (kRegE) {
+      // the pair-type energies of all chunks: a shuffle tree in each
+      // wavefront
+#pragma unroll
+      for (int k = 0; k < NA; ++k) {
+        double v = ej[k];
+        for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
+        if (l64 == 0 && k < P) wacc[wv][k] = v;
+      }
+    }
     // the virial, then per-block partials, component-major (component k
     // of block b at k * nblk_pair + b), the wavefronts summed in order
     double v = vir;
@@ -1599,14 +1618,19 @@ void launch_pair(const Group &g, bool energy, bool sort_pos) {
   const int e1 = energy ? 1 : 0;
 #define MFE_PAIR(M, NT, CF) \
   hipLaunchKernelGGL((k_pair<M, NT, CF>), gr, dim3(kPairBlock), g.lds_pair, g.stream, g.d, e1)
-  const bool nt4 = g.pair_npt <= 4;
+#define MFE_PAIR_CF(M, NT) \
+  do { if (g.pair_cf_lds) MFE_PAIR(M, NT, true); else MFE_PAIR(M, NT, false); } while (0)
+  const int nt = g.pair_npt;
   if (g.prec == MFE_PREC_MIXED) {
-    if (nt4) { if (g.pair_cf_lds) MFE_PAIR(true, 4, true); else MFE_PAIR(true, 4, false); }
-    else { if (g.pair_cf_lds) MFE_PAIR(true, 8, true); else MFE_PAIR(true, 8, false); }
+    if (nt <= 3) MFE_PAIR_CF(true, 3);
+    else if (nt <= 4) MFE_PAIR_CF(true, 4);
+    else MFE_PAIR_CF(true, 8);
   } else {
-    if (nt4) { if (g.pair_cf_lds) MFE_PAIR(false, 4, true); else MFE_PAIR(false, 4, false); }
-    else { if (g.pair_cf_lds) MFE_PAIR(false, 8, true); else MFE_PAIR(false, 8, false); }
+    if (nt <= 3) MFE_PAIR_CF(false, 3);
+    else if (nt <= 4) MFE_PAIR_CF(false, 4);
+    else MFE_PAIR_CF(false, 8);
   }
+#undef MFE_PAIR_CF
 #undef MFE_PAIR
 }
 
