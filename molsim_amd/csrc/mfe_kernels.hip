@@ -239,10 +239,13 @@ __global__ void k_sort_pos(const SysDev *__restrict__ sys) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= V.np) return;
   const int i = V.c_sorted[t];
-  V.ro_s[3 * t] = V.ro[3 * i];
-  V.ro_s[3 * t + 1] = V.ro[3 * i + 1];
-  V.ro_s[3 * t + 2] = V.ro[3 * i + 2];
-  V.type_s[t] = V.type[i];
+  // every load before the first store
+  const double x = V.ro[3 * i], y = V.ro[3 * i + 1], z = V.ro[3 * i + 2];
+  const int ty = V.type[i];
+  V.ro_s[3 * t] = x;
+  V.ro_s[3 * t + 1] = y;
+  V.ro_s[3 * t + 2] = z;
+  V.type_s[t] = ty;
 }
 
 // ------------------------------------------------- spline pair term
@@ -1435,30 +1438,54 @@ __global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double r2 = 0.0;
   if (i < np) {
+    // every load before the first store (the arrays may alias as far as
+    // the compiler knows, which would serialise the loads behind stores)
     const double fac = 0.5 * (dt * dt);
-    double d[3];
-    double gn[3] = {0.0, 0.0, 0.0};
-    if (bd == 1) {
-      for (int m = 0; m < 3; ++m) gn[m] = V.noise[3 * i + m];
-    } else if (bd == 2) {
-      gauss3(V.seed, (unsigned)i, (unsigned)V.flags[3], gn);
-    }
+    double x[3], d[3], s[3];
+    const int cp = V.c_pos[i];
+#pragma unroll
     for (int m = 0; m < 3; ++m) {
-      if (bd) {
-        const int t = type[i];
-        d[m] = fac1[t] * forceo[3 * i + m] + fac2[t] * gn[m];
-      } else {
-        d[m] = dt * rod[3 * i + m] + fac * rodd[3 * i + m];
-      }
-      double x = ro[3 * i + m] + d[m];
-      if (fabs(x) > b.L2[m]) x = x - b.L[m] * round(x * b.Li[m]);
-      ro[3 * i + m] = x;
-      V.ro_s[3 * V.c_pos[i] + m] = x;
-      drosum[3 * i + m] = drosum[3 * i + m] + d[m];
+      x[m] = ro[3 * i + m];
+      s[m] = drosum[3 * i + m];
     }
-    r2 = drosum[3 * i] * drosum[3 * i];
-    r2 = r2 + drosum[3 * i + 1] * drosum[3 * i + 1];
-    r2 = r2 + drosum[3 * i + 2] * drosum[3 * i + 2];
+    if (bd) {
+      double gn[3] = {0.0, 0.0, 0.0};
+      if (bd == 1) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) gn[m] = V.noise[3 * i + m];
+      } else {
+        gauss3(V.seed, (unsigned)i, (unsigned)V.flags[3], gn);
+      }
+      const int t = type[i];
+      const double f1 = fac1[t], f2 = fac2[t];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) d[m] = f1 * forceo[3 * i + m] + f2 * gn[m];
+    } else {
+      double v[3], a[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        v[m] = rod[3 * i + m];
+        a[m] = rodd[3 * i + m];
+      }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) d[m] = dt * v[m] + fac * a[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double xm = x[m] + d[m];
+      if (fabs(xm) > b.L2[m]) xm = xm - b.L[m] * round(xm * b.Li[m]);
+      x[m] = xm;
+      s[m] = s[m] + d[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      ro[3 * i + m] = x[m];
+      V.ro_s[3 * cp + m] = x[m];
+      drosum[3 * i + m] = s[m];
+    }
+    r2 = s[0] * s[0];
+    r2 = r2 + s[1] * s[1];
+    r2 = r2 + s[2] * s[2];
     r2disp[i] = r2;
   }
   const double m = block_max(r2, sh);
@@ -1479,15 +1506,28 @@ __global__ void __launch_bounds__(kBlock) k_set_positions(const SysDev *__restri
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double r2 = 0.0;
   if (i < V.np) {
+    // every load before the first store
+    double xn[3], xo[3], s[3];
+    const int cp = V.c_pos[i];
+#pragma unroll
     for (int m = 0; m < 3; ++m) {
-      const double x = ro_new[3 * i + m];
-      double d = x - V.ro[3 * i + m];
-      if (fabs(d) > b.L2[m]) d = d - copysign(b.L[m], d);
-      V.ro[3 * i + m] = x;
-      V.ro_s[3 * V.c_pos[i] + m] = x;
-      V.drosum[3 * i + m] = V.drosum[3 * i + m] + d;
+      xn[m] = ro_new[3 * i + m];
+      xo[m] = V.ro[3 * i + m];
+      s[m] = V.drosum[3 * i + m];
     }
-    const double *ds = V.drosum + 3 * i;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double d = xn[m] - xo[m];
+      if (fabs(d) > b.L2[m]) d = d - copysign(b.L[m], d);
+      s[m] = s[m] + d;
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      V.ro[3 * i + m] = xn[m];
+      V.ro_s[3 * cp + m] = xn[m];
+      V.drosum[3 * i + m] = s[m];
+    }
+    const double *ds = s;
     r2 = ds[0] * ds[0];
     r2 = r2 + ds[1] * ds[1];
     r2 = r2 + ds[2] * ds[2];
@@ -1553,16 +1593,34 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ 
   double *__restrict__ rodd = V.rodd;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= V.np) return;
+  // every load before the first store
+  double f[3], aold[3], v[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) f[m] = force[3 * i + m];
+  if (mode == 1) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      aold[m] = rodd[3 * i + m];
+      v[m] = rod[3 * i + m];
+    }
+  }
+  const double qi = surf ? q[i] : 0.0;
+  const double mi = mode == 1 ? massip[i] : 0.0;
+  double es[3] = {0.0, 0.0, 0.0};
+  if (surf) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) es[m] = en[5 + m];
+  }
+#pragma unroll
   for (int m = 0; m < 3; ++m) {
-    double f = force[3 * i + m];
-    if (surf) f = f - surf_fac * q[i] * en[5 + m];
-    const double fo = f * ffac;
+    double fm = f[m];
+    if (surf) fm = fm - surf_fac * qi * es[m];
+    const double fo = fm * ffac;
     forceo[3 * i + m] = fo;
     if (mode == 1) {
-      const double aold = rodd[3 * i + m];
-      const double anew = massip[i] * fo;
+      const double anew = mi * fo;
       rodd[3 * i + m] = anew;
-      rod[3 * i + m] = rod[3 * i + m] + 0.5 * dt * (anew + aold);
+      rod[3 * i + m] = v[m] + 0.5 * dt * (anew + aold[m]);
     }
   }
 }
