@@ -282,6 +282,10 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
       v.sg.dkdr[m] = e->s[m] / e->box.L[m];
     }
     v.sg.order = e->order;
+    for (int m = 0; m < 3; ++m) {
+      v.sg.fB[m] = mfe::make_fastdiv(e->B[m]);
+      v.sg.fE[m] = mfe::make_fastdiv(e->B[m] + e->order - 1);
+    }
     v.hx = e->hx;
     v.nbricks = e->nbricks;
     MFE_SETP(v.mesh_r, e->mesh_r); MFE_SETP(v.mesh_c, e->mesh_c);
@@ -304,6 +308,13 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
     size_t l0, l1, l2;
     mfe::fft_geometry(e->prec, e->s, &v.fx_rpb, &v.fx_nblk, &v.fy_T, &v.fy_nbx, &v.fz_T,
                       &v.fz_nbx, &l0, &l1, &l2);
+    v.f_s1 = mfe::make_fastdiv(e->s[0]);
+    v.f_s2 = mfe::make_fastdiv(e->s[1]);
+    v.f_s3 = mfe::make_fastdiv(e->s[2]);
+    v.f_hx = mfe::make_fastdiv(e->s[0] / 2 + 1);
+    v.f_m = mfe::make_fastdiv(e->s[0] / 2);
+    v.f_fyT = mfe::make_fastdiv(v.fy_T);
+    v.f_fzT = mfe::make_fastdiv(v.fz_T);
   }
   MFE_SETP(v.k_vec, e->k_vec); MFE_SETP(v.k_sum, e->k_sum); v.nkvec = e->nkvec;
   v.nchain = e->nchain; v.bond_p = e->bond_p; MFE_SETP(v.ch_prev, e->ch_prev); MFE_SETP(v.ch_next, e->ch_next);

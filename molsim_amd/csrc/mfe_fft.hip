@@ -230,7 +230,7 @@ __device__ __forceinline__ R spread_point(const SysDev &V, const R *__restrict__
   const SpmeGeom &g = V.sg;
   const int h = g.order - 1;
   const int E0 = g.B[0] + h, E1 = g.B[1] + h, E2 = g.B[2] + h;
-  const int bc0 = x / g.B[0], bc1 = y / g.B[1], bc2 = z / g.B[2];
+  const int bc0 = fdiv(x, g.fB[0]), bc1 = fdiv(y, g.fB[1]), bc2 = fdiv(z, g.fB[2]);
   const int l0 = x - bc0 * g.B[0], l1 = y - bc1 * g.B[1], l2 = z - bc2 * g.B[2];
   R acc = R(0);
   for (int dz = 0; dz < 2; ++dz) {
@@ -280,20 +280,21 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
   batched_copy(
       nr * s1,
       [&](int e) {
-        const int r = e / s1, x = e - r * s1;
-        const int row = row0 + r, y = row % s2, z = row / s2;
+        const int r = fdiv(e, V.f_s1), x = e - r * s1;
+        const int row = row0 + r, z = fdiv(row, V.f_s2), y = row - z * s2;
         return spread_point<R>(V, tiles, x, y, z);
       },
       [&](int e, R v) {
-        const int r = e / s1, x = e - r * s1;
+        const int r = fdiv(e, V.f_s1), x = e - r * s1;
         ar[2 * (r * ld) + x] = v;
       });
   __syncthreads();
   cplx<R> *z = lds_fft(a, b, ph, nr, ld, tw_half, false);
   for (int e = threadIdx.x; e < nr * hx; e += blockDim.x) {
-    const int r = e / hx, k = e - r * hx;
-    cplx<R> zk = z[r * ld + (k % m)];
-    cplx<R> zm = conjc(z[r * ld + ((m - k) % m)]);
+    const int r = fdiv(e, V.f_hx), k = e - r * hx;
+    // k in [0, m]: k mod m and (m - k) mod m without divisions
+    cplx<R> zk = z[r * ld + (k == m ? 0 : k)];
+    cplx<R> zm = conjc(z[r * ld + (k == 0 ? 0 : m - k)]);
     cplx<R> ev = {(zk.x + zm.x) * R(0.5), (zk.y + zm.y) * R(0.5)};
     cplx<R> d = csub(zk, zm);
     cplx<R> od = {d.y * R(0.5), -d.x * R(0.5)};
@@ -324,13 +325,13 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
   batched_copy(
       nr * hx, [&](int e) { return cm[(size_t)row0 * hx + e]; },
       [&](int e, cplx<R> v) {
-        const int r = e / hx, k = e - r * hx;
+        const int r = fdiv(e, V.f_hx), k = e - r * hx;
         if (k == 0 || k == m) v.y = R(0);
         b[r * ld + k] = v;
       });
   __syncthreads();
   for (int e = threadIdx.x; e < nr * m; e += blockDim.x) {
-    const int r = e / m, k = e - r * m;
+    const int r = fdiv(e, V.f_m), k = e - r * m;
     cplx<R> xk = b[r * ld + k];
     cplx<R> xm = conjc(b[r * ld + (m - k)]);
     cplx<R> A = cadd(xk, xm), D = csub(xk, xm);
@@ -340,7 +341,7 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
   __syncthreads();
   cplx<R> *z = lds_fft(a, b, ph, nr, ld, tw_half, true);
   for (int e = threadIdx.x; e < nr * s1; e += blockDim.x) {
-    const int r = e / s1, x = e - r * s1;
+    const int r = fdiv(e, V.f_s1), x = e - r * s1;
     cplx<R> v = z[r * ld + (x >> 1)];
     real[(size_t)(row0 + r) * s1 + x] = (x & 1) ? v.y : v.x;
   }
@@ -363,21 +364,23 @@ __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, i
   const int kx0 = blockIdx.x * T;
   const int nt = min(T, hx - kx0);
   cplx<R> *base = cm + (size_t)z * s2 * hx + kx0;
+  // elements e = y T + c over the full tile width T; columns c >= nt (the
+  // last tile) are skipped
   batched_copy(
-      nt * s2,
+      T * s2,
       [&](int e) {
-        const int y = e / nt, c = e - y * nt;
-        return base[(size_t)y * hx + c];
+        const int y = fdiv(e, V.f_fyT), c = e - y * T;
+        return c < nt ? base[(size_t)y * hx + c] : cplx<R>{R(0), R(0)};
       },
       [&](int e, cplx<R> v) {
-        const int y = e / nt, c = e - y * nt;
-        a[c * ld + y] = v;
+        const int y = fdiv(e, V.f_fyT), c = e - y * T;
+        if (c < nt) a[c * ld + y] = v;
       });
   __syncthreads();
   cplx<R> *res = lds_fft(a, b, py, nt, ld, tw, inv != 0);
-  for (int e = threadIdx.x; e < nt * s2; e += blockDim.x) {
-    const int y = e / nt, c = e - y * nt;
-    base[(size_t)y * hx + c] = res[c * ld + y];
+  for (int e = threadIdx.x; e < T * s2; e += blockDim.x) {
+    const int y = fdiv(e, V.f_fyT), c = e - y * T;
+    if (c < nt) base[(size_t)y * hx + c] = res[c * ld + y];
   }
 }
 
@@ -408,14 +411,14 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   const size_t zstride = (size_t)s2 * hx;
   cplx<R> *base = cm + (size_t)y * hx + kx0;
   batched_copy(
-      nt * s3,
+      T * s3,
       [&](int e) {
-        const int z = e / nt, c = e - z * nt;
-        return base[z * zstride + c];
+        const int z = fdiv(e, V.f_fzT), c = e - z * T;
+        return c < nt ? base[z * zstride + c] : cplx<R>{R(0), R(0)};
       },
       [&](int e, cplx<R> v) {
-        const int z = e / nt, c = e - z * nt;
-        a[c * ld + z] = v;
+        const int z = fdiv(e, V.f_fzT), c = e - z * T;
+        if (c < nt) a[c * ld + z] = v;
       });
   __syncthreads();
   cplx<R> *f = lds_fft(a, b, pz, nt, ld, tw, false);
@@ -423,7 +426,7 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   const double gy = ax.g[1][ny], kky = ax.kk[1][ny];
   double esum = 0.0, vsum = 0.0;
   for (int e = threadIdx.x; e < nt * s3; e += blockDim.x) {
-    const int c = e / s3, z = e - c * s3;
+    const int c = fdiv(e, V.f_s3), z = e - c * s3;
     const int nx = kx0 + c, nz = min(z, s3 - z);
     const double m2 = ax.kk[0][nx] + kky + ax.kk[2][nz];
     double mf = 0.0;
@@ -441,9 +444,9 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   __syncthreads();
   cplx<R> *g = (f == a) ? b : a;
   cplx<R> *res = lds_fft(f, g, pz, nt, ld, tw, true);
-  for (int e = threadIdx.x; e < nt * s3; e += blockDim.x) {
-    const int z = e / nt, c = e - z * nt;
-    base[z * zstride + c] = res[c * ld + z];
+  for (int e = threadIdx.x; e < T * s3; e += blockDim.x) {
+    const int z = fdiv(e, V.f_fzT), c = e - z * T;
+    if (c < nt) base[z * zstride + c] = res[c * ld + z];
   }
   if (energy) {
     // shuffle trees in each wavefront, then the wavefronts in order

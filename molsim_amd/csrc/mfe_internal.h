@@ -149,9 +149,26 @@ struct PairParams {
 };
 
 // SPME mesh geometry: mesh s, spreading bricks B (nb per axis), dkdr = s/L.
+// Division by a divisor d fixed at set-up: q = (n * ceil(2^32 / d)) >> 32,
+// exact for 0 <= n with n * d < 2^32 (the error term n * e / 2^32, e < d,
+// stays below 1/d).  One 64-bit multiply instead of the ~25 instructions
+// of a run-time integer division.
+struct FastDiv {
+  unsigned long long m;
+  int d;
+};
+inline FastDiv make_fastdiv(int d) {
+  return FastDiv{((1ull << 32) + (unsigned long long)d - 1) / (unsigned long long)d, d};
+}
+__host__ __device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
+  return (int)(((unsigned long long)(unsigned)n * f.m) >> 32);
+}
+
 struct SpmeGeom {
   int s[3], B[3], nb[3], order;
   double dkdr[3];
+  FastDiv fB[3];   // the brick edges B
+  FastDiv fE[3];   // the extended tile edges B + order - 1
 };
 
 // Device-side view of one engine (one replica): every kernel reads the
@@ -210,6 +227,9 @@ struct SysDev {
   int fx_rpb, fx_nblk;   // x pass: rows per block, blocks
   int fy_T, fy_nbx;      // y pass: kx tile, tiles (grid y = s3)
   int fz_T, fz_nbx;      // conv pass: kx tile, tiles (grid y = s2)
+  // fast divisors of the FFT index arithmetic: s1, s2, s3, hx = s1/2+1,
+  // m = s1/2 and the y / conv tile widths
+  FastDiv f_s1, f_s2, f_s3, f_hx, f_m, f_fyT, f_fzT;
   // standard Ewald
   MFE_DPTR(const double) k_vec;
   MFE_DPTR(double) k_sum;

@@ -869,9 +869,9 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
   const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
   const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
   const int nstage = E0 * E1 * E2;
-  // staged in batches of 8 loads per lane, so the loads of a batch are in
-  // flight together
-  constexpr int kU = 8;
+  // staged in batches of kU loads per lane, so the loads of a batch are in
+  // flight together (one batch for bricks of 16 and order <= 8)
+  constexpr int kU = 24;
   for (int k0 = threadIdx.x; k0 < nstage; k0 += kU * blockDim.x) {
     R val[kU];
 #pragma unroll
@@ -879,8 +879,10 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       const int k = k0 + u * blockDim.x;
       val[u] = R(0);
       if (k < nstage) {
-        const int k01 = k / E0;
-        const int xe = k - k01 * E0, ye = k01 % E1, ze = k01 / E1;
+        // exact: the tile fits LDS (k < 40960) and its edges are short
+        const int k01 = fdiv(k, g.fE[0]);
+        const int ze = fdiv(k01, g.fE[1]);
+        const int xe = k - k01 * E0, ye = k01 - ze * E1;
         // region coordinates lie in [-(order-1), s + B): one wrap at most,
         // or (one-brick axis, B = s) possibly one more below zero
         int x = o0 - (ORDER - 1) + xe, y = o1 - (ORDER - 1) + ye, z = o2 - (ORDER - 1) + ze;
@@ -929,6 +931,16 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
     }
     __syncthreads();
     const int p = threadIdx.x >> 3, iz = threadIdx.x & 7;
+    // the force this lane updates, loaded now so the load overlaps the
+    // stencil sums
+    const bool owner = iz == 0 && p < nc;
+    const int idx = owner ? idxs[p] : 0;
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    if (owner) {
+      f0 = force[3 * idx];
+      f1 = force[3 * idx + 1];
+      f2 = force[3 * idx + 2];
+    }
     double ex = 0.0, ey = 0.0, ez = 0.0, ep = 0.0;
     if (p < nc && iz < ORDER) {
       const int lc = locs[p];
@@ -972,12 +984,11 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       ez += __shfl_xor(ez, w, 8);
       if (POT) ep += __shfl_xor(ep, w, 8);
     }
-    if (iz == 0 && p < nc) {
-      const int idx = idxs[p];
+    if (owner) {
       const double qd = (double)qsh[p];
-      force[3 * idx] -= qd * ex * g.dkdr[0] * efp;
-      force[3 * idx + 1] -= qd * ey * g.dkdr[1] * efp;
-      force[3 * idx + 2] -= qd * ez * g.dkdr[2] * efp;
+      force[3 * idx] = f0 - qd * ex * g.dkdr[0] * efp;
+      force[3 * idx + 1] = f1 - qd * ey * g.dkdr[1] * efp;
+      force[3 * idx + 2] = f2 - qd * ez * g.dkdr[2] * efp;
       // dE_rec/dq = efp * sum_stencil theta * (G * Q) (the convolved mesh)
       if (POT) V.phi_rec[idx] = efp * ep;
     }
