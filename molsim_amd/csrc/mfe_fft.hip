@@ -59,6 +59,13 @@ __device__ __forceinline__ void batched_copy(int n, Load load, Store store) {
   }
 }
 
+// a twiddle table of n entries copied to LDS (read by every stage)
+template <typename R>
+__device__ __forceinline__ void stage_twiddles(cplx<R> *dst, const cplx<R> *__restrict__ src,
+                                               int n) {
+  for (int k = threadIdx.x; k < n; k += blockDim.x) dst[k] = src[k];
+}
+
 // R-point DFT of v[0..RAD-1] in registers: V[q] = sum_j v[j] w^(j q), with
 // w = exp(-+2 pi i / RAD) taken from the length-n table tw (RAD divides n).
 template <typename R, int RAD>
@@ -277,6 +284,9 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
   R *ar = reinterpret_cast<R *>(a);
   const R *__restrict__ tiles = (const R *)V.spread_tiles;
   const int s2 = V.sg.s[1];
+  cplx<R> *twh_s = b + rows_per_blk * ld, *twf_s = twh_s + m;
+  stage_twiddles(twh_s, tw_half, m);
+  stage_twiddles(twf_s, tw_full, s1);
   batched_copy(
       nr * s1,
       [&](int e) {
@@ -289,7 +299,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
         ar[2 * (r * ld) + x] = v;
       });
   __syncthreads();
-  cplx<R> *z = lds_fft(a, b, ph, nr, ld, tw_half, false);
+  cplx<R> *z = lds_fft(a, b, ph, nr, ld, twh_s, false);
   for (int e = threadIdx.x; e < nr * hx; e += blockDim.x) {
     const int r = fdiv(e, V.f_hx), k = e - r * hx;
     // k in [0, m]: k mod m and (m - k) mod m without divisions
@@ -298,7 +308,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
     cplx<R> ev = {(zk.x + zm.x) * R(0.5), (zk.y + zm.y) * R(0.5)};
     cplx<R> d = csub(zk, zm);
     cplx<R> od = {d.y * R(0.5), -d.x * R(0.5)};
-    cplx<R> X = cadd(ev, cmul(tw_full[k], od));
+    cplx<R> X = cadd(ev, cmul(twf_s[k], od));
     cm[(size_t)(row0 + r) * hx + k] = X;
   }
 }
@@ -322,6 +332,9 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
   cplx<R> *b = a + rows_per_blk * ld;
   const int row0 = blockIdx.x * rows_per_blk;
   const int nr = min(rows_per_blk, nrows - row0);
+  cplx<R> *twh_s = b + rows_per_blk * ld, *twf_s = twh_s + m;
+  stage_twiddles(twh_s, tw_half, m);
+  stage_twiddles(twf_s, tw_full, s1);
   batched_copy(
       nr * hx, [&](int e) { return cm[(size_t)row0 * hx + e]; },
       [&](int e, cplx<R> v) {
@@ -335,15 +348,73 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
     cplx<R> xk = b[r * ld + k];
     cplx<R> xm = conjc(b[r * ld + (m - k)]);
     cplx<R> A = cadd(xk, xm), D = csub(xk, xm);
-    cplx<R> Dw = cmul(D, conjc(tw_full[k]));
+    cplx<R> Dw = cmul(D, conjc(twf_s[k]));
     a[r * ld + k] = {A.x - Dw.y, A.y + Dw.x};
   }
   __syncthreads();
-  cplx<R> *z = lds_fft(a, b, ph, nr, ld, tw_half, true);
+  cplx<R> *z = lds_fft(a, b, ph, nr, ld, twh_s, true);
   for (int e = threadIdx.x; e < nr * s1; e += blockDim.x) {
     const int r = fdiv(e, V.f_s1), x = e - r * s1;
     cplx<R> v = z[r * ld + (x >> 1)];
     real[(size_t)(row0 + r) * s1 + x] = (x & 1) ? v.y : v.x;
+  }
+}
+
+// ------------------------------------- inverse y and x of one z plane
+// Mixed precision, planes that fit LDS: one block transforms plane z back
+// along y and then x (c2r as in k_fft_x_inv), so the mesh is read once
+// and the real plane written once for both.  The plane is held transposed
+// (one sequence per kx) for the y transform.  LDS: two buffers of
+// max(s2 (m+1), hx (s2+1)) complex values, then the twiddles (m, s1, s2).
+// (The forward direction stays two passes: its x pass reads the spreading
+// tiles, which takes all CUs, not one per plane.)
+constexpr int kPlaneThreads = 1024;
+
+template <typename R>
+__global__ void __launch_bounds__(kPlaneThreads) k_fft_yx_inv(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  const int z = blockIdx.x;
+  if (z >= V.sg.s[2]) return;
+  const int s1 = V.sg.s[0], s2 = V.sg.s[1];
+  const int m = s1 / 2, hx = m + 1, ldx = m + 1, ldy = s2 + 1;
+  const int nbuf = max(s2 * ldx, hx * ldy);
+  extern __shared__ char smem[];
+  cplx<R> *A = reinterpret_cast<cplx<R> *>(smem), *B = A + nbuf;
+  cplx<R> *twh = B + nbuf, *twf = twh + m, *twy = twf + s1;
+  stage_twiddles(twh, (const cplx<R> *)V.tw_half, m);
+  stage_twiddles(twf, (const cplx<R> *)V.tw[0], s1);
+  stage_twiddles(twy, (const cplx<R> *)V.tw[1], s2);
+  const cplx<R> *__restrict__ cm = (const cplx<R> *)V.mesh_c + (size_t)z * s2 * hx;
+  batched_copy(
+      s2 * hx, [&](int e) { return cm[e]; },
+      [&](int e, cplx<R> v) {
+        const int y = fdiv(e, V.f_hx), kx = e - y * hx;
+        A[kx * ldy + y] = v;
+      });
+  __syncthreads();
+  cplx<R> *Y = lds_fft(A, B, V.plan[1], hx, ldy, twy, true);
+  cplx<R> *W = (Y == A) ? B : A;
+  // c2r of each row as in k_fft_x_inv: Re(X[0]), Re(X[m]) only
+  auto Xh = [&](int k, int y) {
+    cplx<R> v = Y[k * ldy + y];
+    if (k == 0 || k == m) v.y = R(0);
+    return v;
+  };
+  for (int e = threadIdx.x; e < s2 * m; e += blockDim.x) {
+    const int y = fdiv(e, V.f_m), k = e - y * m;
+    cplx<R> xk = Xh(k, y);
+    cplx<R> xm = conjc(Xh(m - k, y));
+    cplx<R> Av = cadd(xk, xm), D = csub(xk, xm);
+    cplx<R> Dw = cmul(D, conjc(twf[k]));
+    W[y * ldx + k] = {Av.x - Dw.y, Av.y + Dw.x};
+  }
+  __syncthreads();
+  cplx<R> *Zr = lds_fft(W, Y, V.plan_half, s2, ldx, twh, true);
+  R *__restrict__ real = (R *)V.mesh_r + (size_t)z * s2 * s1;
+  for (int e = threadIdx.x; e < s2 * s1; e += blockDim.x) {
+    const int y = fdiv(e, V.f_s1), x = e - y * s1;
+    const cplx<R> v = Zr[y * ldx + (x >> 1)];
+    real[e] = (x & 1) ? v.y : v.x;
   }
 }
 
@@ -364,6 +435,8 @@ __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, i
   const int kx0 = blockIdx.x * T;
   const int nt = min(T, hx - kx0);
   cplx<R> *base = cm + (size_t)z * s2 * hx + kx0;
+  cplx<R> *tw_s = b + T * ld;
+  stage_twiddles(tw_s, tw, s2);
   // elements e = y T + c over the full tile width T; columns c >= nt (the
   // last tile) are skipped
   batched_copy(
@@ -377,7 +450,7 @@ __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, i
         if (c < nt) a[c * ld + y] = v;
       });
   __syncthreads();
-  cplx<R> *res = lds_fft(a, b, py, nt, ld, tw, inv != 0);
+  cplx<R> *res = lds_fft(a, b, py, nt, ld, tw_s, inv != 0);
   for (int e = threadIdx.x; e < T * s2; e += blockDim.x) {
     const int y = fdiv(e, V.f_fyT), c = e - y * T;
     if (c < nt) base[(size_t)y * hx + c] = res[c * ld + y];
@@ -410,6 +483,8 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   const int nt = min(T, hx - kx0);
   const size_t zstride = (size_t)s2 * hx;
   cplx<R> *base = cm + (size_t)y * hx + kx0;
+  cplx<R> *tw_s = b + T * ld;
+  stage_twiddles(tw_s, tw, s3);
   batched_copy(
       T * s3,
       [&](int e) {
@@ -421,7 +496,7 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
         if (c < nt) a[c * ld + z] = v;
       });
   __syncthreads();
-  cplx<R> *f = lds_fft(a, b, pz, nt, ld, tw, false);
+  cplx<R> *f = lds_fft(a, b, pz, nt, ld, tw_s, false);
   const int ny = min(y, s2 - y);
   const double gy = ax.g[1][ny], kky = ax.kk[1][ny];
   double esum = 0.0, vsum = 0.0;
@@ -443,7 +518,7 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   }
   __syncthreads();
   cplx<R> *g = (f == a) ? b : a;
-  cplx<R> *res = lds_fft(f, g, pz, nt, ld, tw, true);
+  cplx<R> *res = lds_fft(f, g, pz, nt, ld, tw_s, true);
   for (int e = threadIdx.x; e < T * s3; e += blockDim.x) {
     const int z = fdiv(e, V.f_fzT), c = e - z * T;
     if (c < nt) base[z * zstride + c] = res[c * ld + z];
@@ -503,15 +578,25 @@ void fft_geometry(int prec, const int s[3], int *fx_rpb, int *fx_nblk, int *fy_T
   const int rpb = f ? rows_per_block<float>(s[0]) : rows_per_block<double>(s[0]);
   *fx_rpb = rpb;
   *fx_nblk = (s[1] * s[2] + rpb - 1) / rpb;
-  *lds_fx = 2 * (size_t)rpb * (s[0] / 2 + 1) * csz;
+  // two row buffers, then the half- and full-length twiddles
+  *lds_fx = (2 * (size_t)rpb * (s[0] / 2 + 1) + s[0] / 2 + s[0]) * csz;
   const int Ty = tile_width(hx, f ? tile_target<float>(s[1]) : tile_target<double>(s[1]));
   *fy_T = Ty;
   *fy_nbx = (hx + Ty - 1) / Ty;
-  *lds_fy = 2 * (size_t)Ty * (s[1] + 1) * csz;
+  *lds_fy = (2 * (size_t)Ty * (s[1] + 1) + s[1]) * csz;
   const int Tz = tile_width(hx, f ? tile_target<float>(s[2]) : tile_target<double>(s[2]));
   *fz_T = Tz;
   *fz_nbx = (hx + Tz - 1) / Tz;
-  *lds_fz = 2 * (size_t)Tz * (s[2] + 1) * csz;
+  *lds_fz = (2 * (size_t)Tz * (s[2] + 1) + s[2]) * csz;
+}
+
+size_t fft_plane_lds(int prec, const int s[3]) {
+  if (prec != MFE_PREC_MIXED) return 0;
+  const size_t m = s[0] / 2, hx = m + 1;
+  const size_t nbuf = std::max((size_t)s[1] * (m + 1), hx * (size_t)(s[1] + 1));
+  const size_t bytes = (2 * nbuf + m + s[0] + s[1]) * sizeof(cplx<float>);
+  // the gfx950 LDS of 160 KB per CU, less the kernels' static LDS
+  return bytes <= 150 * 1024 ? bytes : 0;
 }
 
 void fft_x_forward(const Group &g) {
@@ -522,6 +607,7 @@ void fft_x_forward(const Group &g) {
     hipLaunchKernelGGL(k_fft_x_fwd<double>, gr, dim3(256), g.lds_fx, g.stream, g.d);
 }
 void fft_x_inverse(const Group &g) {
+  if (g.fft_plane) return;  // done with the y inverse
   const dim3 gr(g.fx_blocks, 1, g.R);
   if (g.prec == MFE_PREC_MIXED)
     hipLaunchKernelGGL(k_fft_x_inv<float>, gr, dim3(256), g.lds_fx, g.stream, g.d);
@@ -529,6 +615,12 @@ void fft_x_inverse(const Group &g) {
     hipLaunchKernelGGL(k_fft_x_inv<double>, gr, dim3(256), g.lds_fx, g.stream, g.d);
 }
 void fft_y(const Group &g, int inverse) {
+  if (g.fft_plane && inverse) {
+    // y and x of each plane
+    hipLaunchKernelGGL(k_fft_yx_inv<float>, dim3(g.planes, 1, g.R), dim3(kPlaneThreads),
+                       g.lds_plane, g.stream, g.d);
+    return;
+  }
   const dim3 gr(g.fy_x, g.fy_y, g.R);
   if (g.prec == MFE_PREC_MIXED)
     hipLaunchKernelGGL(k_fft_y<float>, gr, dim3(256), g.lds_fy, g.stream, g.d, inverse);
@@ -545,3 +637,8 @@ void fft_conv(const Group &g, bool energy) {
 }
 
 }  // namespace mfe
+
+void mfe_allow_big_fft_lds(int bytes) {
+  (void)hipFuncSetAttribute((const void *)&mfe::k_fft_yx_inv<float>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
