@@ -883,19 +883,14 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
         const int k01 = fdiv(k, g.fE[0]);
         const int ze = fdiv(k01, g.fE[1]);
         const int xe = k - k01 * E0, ye = k01 - ze * E1;
-        // region coordinates lie in [-(order-1), s + B): one wrap at most,
-        // or (one-brick axis, B = s) possibly one more below zero
+        // region coordinates lie in [o - (order-1), o + B) with o + B <= s
+        // and s >= order: one wrap, below zero only
         int x = o0 - (ORDER - 1) + xe, y = o1 - (ORDER - 1) + ye, z = o2 - (ORDER - 1) + ze;
-        if (x < 0) x += g.s[0];
-        if (x < 0) x += g.s[0];
-        if (x >= g.s[0]) x -= g.s[0];
-        if (y < 0) y += g.s[1];
-        if (y < 0) y += g.s[1];
-        if (y >= g.s[1]) y -= g.s[1];
-        if (z < 0) z += g.s[2];
-        if (z < 0) z += g.s[2];
-        if (z >= g.s[2]) z -= g.s[2];
-        val[u] = mesh[((size_t)z * g.s[1] + y) * g.s[0] + x];
+        x += x < 0 ? g.s[0] : 0;
+        y += y < 0 ? g.s[1] : 0;
+        z += z < 0 ? g.s[2] : 0;
+        // meshes up to 1024 per axis: the offset fits 32 bits
+        val[u] = mesh[(unsigned)((z * g.s[1] + y) * g.s[0] + x)];
       }
     }
 #pragma unroll
