@@ -137,6 +137,7 @@ void free_all(mfe_engine *e) {
   dfree(e->t_bin_k); dfree(e->t_rec);
   dfree(e->c_count); dfree(e->c_start); dfree(e->c_of); dfree(e->c_slot);
   dfree(e->c_sorted); dfree(e->c_pos); dfree(e->nn); dfree(e->nbr); dfree(e->flags);
+  dfree(e->rebuild_bar); dfree(e->rebuild_sum);
   dfree(e->ro_s); dfree(e->type_s);
   dfree(e->mesh_r); dfree(e->mesh_c); dfree(e->b_count); dfree(e->b_start);
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
@@ -417,6 +418,9 @@ void sync_view(mfe_engine *e) {
   std::string err;
   mfe_engine *one[1] = {e};
   mfe_group_build(one, 1, e->d_sys, e->stream, e->grp, err);
+  e->grp.rebuild_bar = e->rebuild_bar;
+  e->grp.rebuild_sum = e->rebuild_sum;
+  e->grp.ncu = e->ncu;
   e->sys_dirty = false;
 }
 
@@ -435,6 +439,13 @@ int mfe_map_flag_error(mfe_engine *e, int err) {
   if (!err) return MFE_OK;
   mfe::launch_fill_int(e->flags, 1, 0, e->stream);
   MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+  if (err & mfe::ERR_SYNC) {
+    // the barrier state is undefined after a timed-out wait
+    mfe::launch_fill_int((int *)e->rebuild_bar, 2, 0, e->stream);
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->list_valid = false;
+    return fail(e, MFE_E_HIP, "list rebuild: a grid barrier timed out (CUs not all available)");
+  }
   if (err & mfe::ERR_CAPACITY) {
     e->list_valid = false;
     return fail(e, MFE_E_NEIGH_CAPACITY, "nneighpn > maxnneigh: raise maxnneigh");
@@ -514,6 +525,12 @@ int mfe_create(const mfe_config *cfg, mfe_engine **out) {
     e->max_np = cfg->max_np;
     MFE_HIP_TRY(hipSetDevice(e->device));
     MFE_HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    MFE_HIP_TRY(hipGetDeviceProperties(&prop, e->device));
+    e->ncu = prop.multiProcessorCount;
+    dalloc(e->rebuild_bar, 2);
+    dalloc(e->rebuild_sum, (size_t)e->ncu);
+    mfe::launch_fill_int((int *)e->rebuild_bar, 2, 0, e->stream);
     dalloc(e->flags, 4);
     dalloc(e->en, 8 + MFE_MAX_PAIRS);
     MFE_HIP_TRY(hipHostMalloc((void **)&e->h_en, (8 + MFE_MAX_PAIRS) * sizeof(double)));

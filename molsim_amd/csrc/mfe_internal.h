@@ -49,6 +49,7 @@ enum : int {
   ERR_OVERLAP = 1,    // r2 < r2umin
   ERR_TABLE = 2,      // no table interval for r2
   ERR_CAPACITY = 4,   // neighbour capacity exceeded
+  ERR_SYNC = 8,       // a grid barrier of the fused list rebuild timed out
 };
 
 template <typename R>
@@ -283,6 +284,10 @@ struct Group {
   size_t lds_plane = 0;
   int planes = 0;           // most z planes of any replica
   int pair_npt = 0;         // most types of any replica
+  // fused list rebuild (one engine): grid barrier state and CU count
+  unsigned *rebuild_bar = nullptr;
+  int *rebuild_sum = nullptr;
+  int ncu = 0;
   bool pair_cf_lds = true;  // every replica's coefficients fit in LDS
 };
 
@@ -306,6 +311,9 @@ struct mfe_engine {
 
   // particle state (device, xyz-interleaved FP64)
   double *ro = nullptr, *rod = nullptr, *rodd = nullptr;
+  unsigned *rebuild_bar = nullptr;  // grid barrier of k_rebuild [2]
+  int *rebuild_sum = nullptr;       // its per-block scan totals [ncu]
+  int ncu = 0;                      // compute units of the device
   double *force = nullptr;   // UTotal's force(3,na) before unit scaling
   double *forceo = nullptr;  // force * fac
   double *drosum = nullptr, *r2disp = nullptr;

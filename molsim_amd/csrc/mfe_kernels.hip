@@ -79,18 +79,20 @@ __global__ void k_fill_double(double *p, int n, double v) {
 // contiguous chunk; the chunk sums are scanned inside each wavefront with
 // shuffles and across the 16 wavefronts through LDS (one barrier).
 // which: 0 = cells of the neighbour list, 1 = SPME bricks.
-__global__ void __launch_bounds__(1024) k_scan(const SysDev *__restrict__ sys,
-                                               int which, int gated) {
-  const SysDev &V = sys[blockIdx.z];
-  if (gated && V.flags[1] == 0) return;
-  int *count = which ? V.b_count : V.c_count;
-  int *start = which ? V.b_start : V.c_start;
-  const int n = which ? V.nbricks : V.cg.ncells;
+__device__ __forceinline__ void ph_scan(int *count, int *start, int n) {
   __shared__ int wsum[16];
   const int per = (n + blockDim.x - 1) / blockDim.x;
   const int lo = threadIdx.x * per, hi = min(n, lo + per);
+  // the chunk in batches of 16 loads in flight
+  constexpr int kU = 16;
   int s = 0;
-  for (int i = lo; i < hi; ++i) s += count[i];
+  for (int i0 = lo; i0 < hi; i0 += kU) {
+    int v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) v[u] = i0 + u < hi ? count[i0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) s += v[u];
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   int x = s;
   for (int o = 1; o < 64; o <<= 1) {
@@ -102,12 +104,79 @@ __global__ void __launch_bounds__(1024) k_scan(const SysDev *__restrict__ sys,
   int wpre = 0;
   for (int w = 0; w < wave; ++w) wpre += wsum[w];
   int run = wpre + x - s;
-  for (int i = lo; i < hi; ++i) {
-    start[i] = run;
-    run += count[i];
-    count[i] = 0;
+  for (int i0 = lo; i0 < hi; i0 += kU) {
+    int v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) v[u] = i0 + u < hi ? count[i0 + u] : 0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (i0 + u < hi) {
+        start[i0 + u] = run;
+        count[i0 + u] = 0;
+      }
+      run += v[u];
+    }
   }
   if (threadIdx.x == blockDim.x - 1) start[n] = wpre + x;
+}
+
+// The same scan over all blocks of a grid (k_rebuild): each block scans its
+// contiguous range into start[] and leaves its total in bsum[block]; after
+// a grid barrier each block adds the totals of the blocks before it.
+__device__ __forceinline__ void ph_scan_local(const int *count, int *start, int n, int *bsum) {
+  __shared__ int wsum[16];
+  const int nb = gridDim.x, C = (n + nb - 1) / nb;
+  const int lo = min(n, blockIdx.x * C), hi = min(n, lo + C);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int carry = 0;
+  for (int base = lo; base < hi; base += blockDim.x) {
+    const int i = base + threadIdx.x;
+    const int v = i < hi ? count[i] : 0;
+    int x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    int wpre = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+      wpre += w < wave ? wsum[w] : 0;
+      tot += wsum[w];
+    }
+    if (i < hi) start[i] = carry + wpre + x - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = carry;
+}
+
+__device__ __forceinline__ void ph_scan_offset(int *count, int *start, int n, const int *bsum) {
+  __shared__ int wsum[16];
+  const int nb = gridDim.x, C = (n + nb - 1) / nb;
+  const int lo = min(n, blockIdx.x * C), hi = min(n, lo + C);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  // totals of the blocks before this one (nb <= blockDim), in a fixed order
+  int v = 0;
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += blockDim.x) v += bsum[k];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if (lane == 0) wsum[wave] = v;
+  __syncthreads();
+  int off = 0;
+  for (int w = 0; w < nw; ++w) off += wsum[w];
+  for (int i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+    start[i] += off;
+    count[i] = 0;
+  }
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) start[n] = off + bsum[nb - 1];
+}
+
+__global__ void __launch_bounds__(1024) k_scan(const SysDev *__restrict__ sys,
+                                               int which, int gated) {
+  const SysDev &V = sys[blockIdx.z];
+  if (gated && V.flags[1] == 0) return;
+  if (which) ph_scan(V.b_count, V.b_start, V.nbricks);
+  else ph_scan(V.c_count, V.c_start, V.cg.ncells);
 }
 
 // ------------------------------------------------------- neighbour lists
@@ -122,13 +191,13 @@ __device__ __forceinline__ int cell_of(const double *r, const CellGeom &g,
   return c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
 }
 
-__global__ void k_cell_count(const SysDev *__restrict__ sys, int gated) {
-  const SysDev &V = sys[blockIdx.z];
-  if (gated && V.flags[1] == 0) return;
+// The rebuild's phases, each over threads t0, t0 + nt, ...: run as five
+// kernels (batches, explicit builds) or as the phases of k_rebuild.
+__device__ __forceinline__ void ph_cell_count(const SysDev &V, int t0, int nt) {
   const double *__restrict__ ro = V.ro;
   const CellGeom g = V.cg;
   const Box b = V.box;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V.np; i += gridDim.x * blockDim.x) {
+  for (int i = t0; i < V.np; i += nt) {
     int c3[3];
     const int c = cell_of(ro + 3 * i, g, b, c3);
     V.c_of[i] = c;
@@ -136,30 +205,38 @@ __global__ void k_cell_count(const SysDev *__restrict__ sys, int gated) {
   }
 }
 
-__global__ void k_cell_scatter(const SysDev *__restrict__ sys, int gated) {
-  const SysDev &V = sys[blockIdx.z];
-  if (gated && V.flags[1] == 0) return;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < V.np; i += gridDim.x * blockDim.x)
-    V.c_sorted[V.c_start[V.c_of[i]] + V.c_slot[i]] = i;
+__device__ __forceinline__ void ph_cell_scatter(const SysDev &V, int t0, int nt) {
+  for (int i = t0; i < V.np; i += nt) V.c_sorted[V.c_start[V.c_of[i]] + V.c_slot[i]] = i;
 }
 
 // members of each cell in ascending particle index (deterministic lists)
-__global__ void k_cell_sort(const SysDev *__restrict__ sys, int gated) {
-  const SysDev &V = sys[blockIdx.z];
-  if (gated && V.flags[1] == 0) return;
-  const int ncells = V.cg.ncells;
+__device__ __forceinline__ void ph_cell_sort(const SysDev &V, int t0, int nt) {
   const int *start = V.c_start;
   int *sorted = V.c_sorted;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncells; c += gridDim.x * blockDim.x) {
-  const int lo = start[c], hi = start[c + 1];
-  for (int i = lo + 1; i < hi; ++i) {
-    int v = sorted[i], j = i - 1;
-    while (j >= lo && sorted[j] > v) {
-      sorted[j + 1] = sorted[j];
-      --j;
+  for (int c = t0; c < V.cg.ncells; c += nt) {
+    const int lo = start[c], hi = start[c + 1];
+    for (int i = lo + 1; i < hi; ++i) {
+      int v = sorted[i], j = i - 1;
+      while (j >= lo && sorted[j] > v) {
+        sorted[j + 1] = sorted[j];
+        --j;
+      }
+      sorted[j + 1] = v;
     }
-    sorted[j + 1] = v;
   }
+}
+
+// positions and types in cell order (ro_s, type_s)
+__device__ __forceinline__ void ph_sort_pos(const SysDev &V, int t0, int nt) {
+  for (int t = t0; t < V.np; t += nt) {
+    const int i = V.c_sorted[t];
+    // every load before the first store
+    const double x = V.ro[3 * i], y = V.ro[3 * i + 1], z = V.ro[3 * i + 2];
+    const int ty = V.type[i];
+    V.ro_s[3 * t] = x;
+    V.ro_s[3 * t + 1] = y;
+    V.ro_s[3 * t + 2] = z;
+    V.type_s[t] = ty;
   }
 }
 
@@ -168,12 +245,13 @@ __global__ void k_cell_sort(const SysDev *__restrict__ sys, int gated) {
 // (t -> i = sorted[t]) and store partners by their cell-order position, so
 // the pair kernel reads neighbouring positions from neighbouring memory;
 // the list is column-major, nbr[k*np + t].  The half list the reference
-// keeps is the checkerboard subset of this set.
-__global__ void __launch_bounds__(kBlock) k_nlist_build(const SysDev *__restrict__ sys,
-                                                        int gated) {
-  const SysDev &V = sys[blockIdx.z];
-  if (gated && V.flags[1] == 0) return;
+// keeps is the checkerboard subset of this set.  POS_S: the cell-ordered
+// positions are already in ro_s (read there, contiguously); else they are
+// read through sorted[] and ro_s / type_s are written here.
+template <bool POS_S>
+__device__ __forceinline__ void ph_nlist_build(const SysDev &V, int t0, int nt) {
   const double *__restrict__ ro = V.ro;
+  const double *__restrict__ ros = V.ro_s;
   const int np = V.np;
   const CellGeom g = V.cg;
   const Box b = V.box;
@@ -186,9 +264,11 @@ __global__ void __launch_bounds__(kBlock) k_nlist_build(const SysDev *__restrict
   int *__restrict__ nbr = V.nbr;
   int *flags = V.flags;
   double *drosum = V.drosum;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < np; t += gridDim.x * blockDim.x) {
+  for (int t = t0; t < np; t += nt) {
     const int i = sorted[t];
-    const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
+    const double xi = POS_S ? ros[3 * t] : ro[3 * i];
+    const double yi = POS_S ? ros[3 * t + 1] : ro[3 * i + 1];
+    const double zi = POS_S ? ros[3 * t + 2] : ro[3 * i + 2];
     const int c = cof[i];
     const int cx = c % g.n[0], cy = (c / g.n[0]) % g.n[1], cz = c / (g.n[0] * g.n[1]);
     const int rx = g.n[0] > 1 ? 1 : 0, ry = g.n[1] > 1 ? 1 : 0, rz = g.n[2] > 1 ? 1 : 0;
@@ -202,8 +282,18 @@ __global__ void __launch_bounds__(kBlock) k_nlist_build(const SysDev *__restrict
           const int cc = x + g.n[0] * (y + g.n[1] * z);
           for (int p = start[cc]; p < start[cc + 1]; ++p) {
             if (p == t) continue;
-            const int j = sorted[p];
-            double ddx = xi - ro[3 * j], ddy = yi - ro[3 * j + 1], ddz = zi - ro[3 * j + 2];
+            double xj, yj, zj;
+            if (POS_S) {
+              xj = ros[3 * p];
+              yj = ros[3 * p + 1];
+              zj = ros[3 * p + 2];
+            } else {
+              const int j = sorted[p];
+              xj = ro[3 * j];
+              yj = ro[3 * j + 1];
+              zj = ro[3 * j + 2];
+            }
+            double ddx = xi - xj, ddy = yi - yj, ddz = zi - zj;
             const double r2 = pbcr2(ddx, ddy, ddz, b);
             if (r2 < rs2) {
               if (n < maxnn) nbr[(size_t)n * np + t] = p;
@@ -220,32 +310,116 @@ __global__ void __launch_bounds__(kBlock) k_nlist_build(const SysDev *__restrict
     // cell-ordered positions and types for the pair kernel; k_move keeps
     // them current through c_pos until the next rebuild
     V.c_pos[i] = t;
-    V.ro_s[3 * t] = xi;
-    V.ro_s[3 * t + 1] = yi;
-    V.ro_s[3 * t + 2] = zi;
-    V.type_s[t] = V.type[i];
+    if (!POS_S) {
+      V.ro_s[3 * t] = xi;
+      V.ro_s[3 * t + 1] = yi;
+      V.ro_s[3 * t + 2] = zi;
+      V.type_s[t] = V.type[i];
+    }
     if (drosum) {
       drosum[3 * i] = 0.0;
       drosum[3 * i + 1] = 0.0;
       drosum[3 * i + 2] = 0.0;
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0 && gated) flags[2] += 1;
+}
+
+#define MFE_GATE                                \
+  const SysDev &V = sys[blockIdx.z];           \
+  if (gated && V.flags[1] == 0) return;        \
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+
+__global__ void k_cell_count(const SysDev *__restrict__ sys, int gated) {
+  MFE_GATE
+  ph_cell_count(V, t0, nt);
+}
+
+__global__ void k_cell_scatter(const SysDev *__restrict__ sys, int gated) {
+  MFE_GATE
+  ph_cell_scatter(V, t0, nt);
+}
+
+__global__ void k_cell_sort(const SysDev *__restrict__ sys, int gated) {
+  MFE_GATE
+  ph_cell_sort(V, t0, nt);
+}
+
+__global__ void __launch_bounds__(kBlock) k_nlist_build(const SysDev *__restrict__ sys,
+                                                        int gated) {
+  MFE_GATE
+  ph_nlist_build<false>(V, t0, nt);
+  if (t0 == 0 && gated) V.flags[2] += 1;
+}
+#undef MFE_GATE
+
+// Grid-wide barrier of k_rebuild, whose blocks are all resident (one per
+// CU): arrival counter bar[0] and generation bar[1].  Each wave waits for
+// its stores (in L2), the block meets at s_barrier; thread 0 then arrives with an
+// acquire-release increment (its release writes the XCD's L2 back, so
+// the other XCDs see the block's data), the last block to arrive resets
+// the counter and advances the generation (release), the others poll it
+// (relaxed) and take one acquire fence, which invalidates the CU's stale
+// lines for the whole block.  The wait is bounded: after kGridSpin polls a
+// block raises ERR_SYNC and goes on, so the kernel always ends (its list
+// is then void and the host reports the error and resets the barrier).
+constexpr int kGridSpin = 1 << 22;
+
+__device__ __forceinline__ void grid_sync(unsigned *bar, int *flags) {
+  // every wave waits for its own stores to be acknowledged (in L2) before
+  // the block barrier (a workgroup-scope barrier does not wait for them)
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived =
+        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    if (arrived == gridDim.x) {
+      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      int spins = 0;
+      while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+        if (++spins > kGridSpin) {
+          atomicOr(&flags[0], ERR_SYNC);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(8);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+  }
+  __syncthreads();
+}
+
+// The gated list rebuild of one engine as one kernel: it returns at once
+// when the skin test did not fire (one launch per step instead of five);
+// otherwise the phases run in order, separated by grid barriers.  Launched
+// with one 1024-thread block per CU, so all blocks are resident.
+__global__ void __launch_bounds__(1024) k_rebuild(const SysDev *__restrict__ sys, unsigned *bar,
+                                                  int *bsum) {
+  const SysDev &V = sys[0];
+  if (V.flags[1] == 0) return;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+  ph_cell_count(V, t0, nt);
+  grid_sync(bar, V.flags);
+  ph_scan_local(V.c_count, V.c_start, V.cg.ncells, bsum);
+  grid_sync(bar, V.flags);
+  ph_scan_offset(V.c_count, V.c_start, V.cg.ncells, bsum);
+  grid_sync(bar, V.flags);
+  ph_cell_scatter(V, t0, nt);
+  grid_sync(bar, V.flags);
+  ph_cell_sort(V, t0, nt);
+  grid_sync(bar, V.flags);
+  ph_sort_pos(V, t0, nt);
+  grid_sync(bar, V.flags);
+  ph_nlist_build<true>(V, t0, nt);
+  if (t0 == 0) V.flags[2] += 1;
 }
 
 // positions and types in cell order for the pair kernel (every UTotal)
 __global__ void k_sort_pos(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= V.np) return;
-  const int i = V.c_sorted[t];
-  // every load before the first store
-  const double x = V.ro[3 * i], y = V.ro[3 * i + 1], z = V.ro[3 * i + 2];
-  const int ty = V.type[i];
-  V.ro_s[3 * t] = x;
-  V.ro_s[3 * t + 1] = y;
-  V.ro_s[3 * t + 2] = z;
-  V.type_s[t] = ty;
+  ph_sort_pos(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
 }
 
 // ------------------------------------------------- spline pair term
@@ -1663,6 +1837,11 @@ void launch_fill_double(double *p, int n, double v, hipStream_t s) {
 }
 
 void launch_nlist_rebuild(const Group &g, bool gated) {
+  if (gated && g.R == 1 && g.rebuild_bar && g.ncu > 0) {
+    hipLaunchKernelGGL(k_rebuild, dim3(g.ncu), dim3(1024), 0, g.stream, g.d, g.rebuild_bar,
+                       g.rebuild_sum);
+    return;
+  }
   // gated kernels run on small grids (grid-stride loops): when the skin
   // test does not fire they cost one dispatch of a few hundred wavefronts
   const int gb = gated ? std::min(g.np_blocks, 256) : g.np_blocks;
