@@ -191,8 +191,8 @@ int mfe_get_pairs(mfe_engine *e, int32_t *nneighpn, int32_t *jpnlist,
 
 /* ---- measurement ------------------------------------------------------ */
 enum {
-  MFE_STAGE_PAIR = 0,   /* table pair forces                               */
-  MFE_STAGE_SPREAD = 1, /* binning + charge spreading                     */
+  MFE_STAGE_PAIR = 0,   /* table pair forces (the pair kernel)             */
+  MFE_STAGE_SPREAD = 1, /* charge spreading (the spreading kernel)        */
   MFE_STAGE_FFT_X = 2,  /* r2c along x (forward)                          */
   MFE_STAGE_FFT_Y = 3,  /* complex along y (forward)                      */
   MFE_STAGE_CONV = 4,   /* z forward + influence/energy/virial + z inverse */
@@ -200,10 +200,12 @@ enum {
   MFE_STAGE_STEP = 6,   /* one whole MD step                              */
   MFE_STAGE_NLIST = 7,  /* neighbour-list rebuild                         */
   MFE_STAGE_IFFT = 8,   /* y inverse + c2r along x                        */
-  MFE_NSTAGES = 9
+  MFE_STAGE_BIN = 9,    /* particles into mesh bricks (count, scan, records) */
+  MFE_NSTAGES = 10
 };
 /* Runs one stage nrep times on the engine's stream, bracketed by HIP
-   events on that stream, and returns the mean time per run in ms. */
+   events on that stream, and returns the mean time per run in ms.  One
+   full UTotal runs first, so each stage finds its inputs in place. */
 int mfe_time_stage(mfe_engine *e, int32_t stage, int32_t nrep, double *ms);
 
 /* ---- parameter adjoint ------------------------------------------------ */

@@ -1365,8 +1365,9 @@ int mfe_time_stage(mfe_engine *e, int32_t stage, int32_t nrep, double *ms) {
     MFE_HIP_TRY(hipEventRecord(a, e->stream));
     for (int r = 0; r < nrep; ++r) {
       switch (stage) {
-        case MFE_STAGE_PAIR: mfe::launch_pair(G, true); break;
-        case MFE_STAGE_SPREAD: mfe::launch_spme(G, true, SP_BIN, SP_SPREAD); break;
+        case MFE_STAGE_PAIR: mfe::launch_pair(G, true, false); break;
+        case MFE_STAGE_BIN: mfe::launch_spme(G, true, SP_BIN, SP_BIN); break;
+        case MFE_STAGE_SPREAD: mfe::launch_spme(G, true, SP_SPREAD, SP_SPREAD); break;
         case MFE_STAGE_FFT_X: mfe::launch_spme(G, true, SP_FFTX, SP_FFTX); break;
         case MFE_STAGE_FFT_Y: mfe::launch_spme(G, true, SP_FFTY, SP_FFTY); break;
         case MFE_STAGE_CONV: mfe::launch_spme(G, true, SP_CONV, SP_CONV); break;

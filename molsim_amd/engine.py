@@ -29,7 +29,7 @@ EWALD = {"none": 0, "std": 1, "spm": 2}
 NLIST = {"vlist": 0, "cell": 1}
 FORCES, ENERGY, VIRIAL = 1, 2, 4
 STAGES = {"pair": 0, "spread": 1, "fft_x": 2, "fft_y": 3, "conv": 4,
-          "gather": 5, "utotal": 6, "nlist": 7, "ifft": 8}
+          "gather": 5, "utotal": 6, "nlist": 7, "ifft": 8, "bin": 9}
 MAX_TYPES = 8
 MAX_PAIRS = MAX_TYPES * (MAX_TYPES + 1) // 2
 
