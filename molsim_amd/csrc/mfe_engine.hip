@@ -139,7 +139,7 @@ void free_all(mfe_engine *e) {
   dfree(e->c_sorted); dfree(e->c_pos); dfree(e->nn); dfree(e->nbr); dfree(e->flags);
   dfree(e->rebuild_bar); dfree(e->rebuild_sum);
   dfree(e->ro_s); dfree(e->type_s);
-  dfree(e->mesh_r); dfree(e->mesh_c); dfree(e->b_count); dfree(e->b_start);
+  dfree(e->mesh_r); dfree(e->mesh_c); dfree(e->b_count); dfree(e->b_start); dfree(e->b_done);
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
   dfree(e->spread_tiles);
   dfree(e->ax_g); dfree(e->ax_kk);
@@ -290,7 +290,7 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
     v.hx = e->hx;
     v.nbricks = e->nbricks;
     MFE_SETP(v.mesh_r, e->mesh_r); MFE_SETP(v.mesh_c, e->mesh_c);
-    MFE_SETP(v.b_count, e->b_count); MFE_SETP(v.b_start, e->b_start); MFE_SETP(v.b_id, e->b_id); MFE_SETP(v.b_slot, e->b_slot);
+    MFE_SETP(v.b_count, e->b_count); MFE_SETP(v.b_done, e->b_done); MFE_SETP(v.b_start, e->b_start); MFE_SETP(v.b_id, e->b_id); MFE_SETP(v.b_slot, e->b_slot);
     MFE_SETP(v.rec_frac, e->rec_frac); MFE_SETP(v.rec_meta, e->rec_meta);
     MFE_SETP(v.spread_tiles, e->spread_tiles);
     const int smax2 = std::max(e->s[0], std::max(e->s[1], e->s[2])) / 2 + 1;
@@ -468,8 +468,13 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
   else if (g.ewald == MFE_EWALD_STD) mfe::launch_std_ewald(g);
   if (g.chains) mfe::launch_bonds(g);
   if (g.surf) mfe::launch_surf(g);
-  mfe::launch_reduce(g, flags, skin, bd);
-  mfe::launch_finalize(g, mode, dt);
+  if (g.surf) {
+    // the surface field of the reduction enters the forces
+    mfe::launch_reduce(g, flags, skin, bd);
+    mfe::launch_finalize(g, mode, dt);
+  } else {
+    mfe::launch_reduce_finalize(g, flags, skin, bd, mode, dt);
+  }
 }
 
 // one VelVer step with the NList skin test and gated rebuild
@@ -838,6 +843,8 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
       dalloc(e->b_count, (size_t)e->nbricks);
       dalloc(e->b_start, (size_t)e->nbricks + 1);
       mfe::launch_fill_int(e->b_count, e->nbricks, 0, e->stream);
+      dalloc(e->b_done, 1);
+      mfe::launch_fill_int(e->b_done, 1, 0, e->stream);
       // per-axis influence tables (Essmann eqs. 3.9, 4.4; EwaldSetup
       // Src/energy.F90:6980-7018), b^2 with the reference's single-precision
       // exp(i 2 pi m k / s)

@@ -215,6 +215,7 @@ struct SysDev {
   MFE_DPTR(void) mesh_r;
   MFE_DPTR(void) mesh_c;
   MFE_DPTR(int) b_count;
+  MFE_DPTR(int) b_done;    // count blocks finished (the last one scans)
   MFE_DPTR(int) b_start;
   MFE_DPTR(int) b_id;
   MFE_DPTR(int) b_slot;
@@ -362,6 +363,7 @@ struct mfe_engine {
   void *mesh_r = nullptr;    // real mesh [s3][s2][s1]
   void *mesh_c = nullptr;    // complex mesh [s3][s2][hx]
   int *b_count = nullptr, *b_start = nullptr, *b_id = nullptr, *b_slot = nullptr;
+  int *b_done = nullptr;
   void *rec_frac = nullptr;  // per sorted particle: (fx, fy, fz, q)
   void *spread_tiles = nullptr;  // per brick: extended spreading tile
   size_t spread_tiles_bytes = 0;
@@ -440,6 +442,9 @@ void launch_bonds(const Group &g);
 void launch_surf(const Group &g);
 void launch_reduce(const Group &g, unsigned flags, bool skin, bool bd);
 void launch_finalize(const Group &g, int mode, double dt);
+// reduce and finalize in one launch (only without the surface term)
+void launch_reduce_finalize(const Group &g, unsigned flags, bool skin, bool bd, int mode,
+                            double dt);
 void launch_move(const Group &g, double dt, int bd);
 void launch_skin(const Group &g);
 void launch_set_positions(const Group &g, const double *ro_new);

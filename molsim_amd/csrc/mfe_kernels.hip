@@ -832,7 +832,8 @@ constexpr int kCountBins = 8192;
 __global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
   const int np = V.np;
-  if ((int)blockIdx.x * kCountThreads >= np) return;
+  const int nblk = (np + kCountThreads - 1) / kCountThreads;
+  if ((int)blockIdx.x >= nblk) return;
   const double *__restrict__ ro = V.ro;
   const int *__restrict__ sorted = V.c_sorted;
   const SpmeGeom g = V.sg;
@@ -846,29 +847,44 @@ __global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__re
     int mm[3];
     double fr[3];
     mesh_coords(ro + 3 * i, g, mm, fr);
-    id = mm[0] / g.B[0] + g.nb[0] * (mm[1] / g.B[1] + g.nb[1] * (mm[2] / g.B[2]));
+    id = fdiv(mm[0], g.fB[0]) + g.nb[0] * (fdiv(mm[1], g.fB[1]) + g.nb[1] * fdiv(mm[2], g.fB[2]));
   }
   if (V.nbricks > kCountBins) {
     if (act) {
       bid[i] = id;
       bslot[i] = atomicAdd(&bcount[id], 1);
     }
-    return;
+  } else {
+    for (int k = threadIdx.x; k < V.nbricks; k += kCountThreads) hist[k] = 0;
+    __syncthreads();
+    const int rank = act ? atomicAdd(&hist[id], 1) : 0;
+    __syncthreads();
+    // one global atomic per non-empty bin; its old value is the bin's base
+    for (int k = threadIdx.x; k < V.nbricks; k += kCountThreads) {
+      const int c = hist[k];
+      if (c) hist[k] = atomicAdd(&bcount[k], c);
+    }
+    __syncthreads();
+    if (act) {
+      bid[i] = id;
+      bslot[i] = hist[id] + rank;
+    }
   }
-  for (int k = threadIdx.x; k < V.nbricks; k += kCountThreads) hist[k] = 0;
+  // The last block to finish scans the counts into b_start and clears them
+  // (no separate scan launch).  Every count atomic of this block has
+  // returned -- its value was used -- before the block barrier, so the
+  // done counter is incremented after them; the last block takes an
+  // acquire fence before reading the counts.  Nothing waits.
+  __shared__ int last;
   __syncthreads();
-  const int rank = act ? atomicAdd(&hist[id], 1) : 0;
+  if (threadIdx.x == 0)
+    last = __hip_atomic_fetch_add(V.b_done, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+           nblk - 1;
   __syncthreads();
-  // one global atomic per non-empty bin; its old value is the bin's base
-  for (int k = threadIdx.x; k < V.nbricks; k += kCountThreads) {
-    const int c = hist[k];
-    if (c) hist[k] = atomicAdd(&bcount[k], c);
-  }
-  __syncthreads();
-  if (act) {
-    bid[i] = id;
-    bslot[i] = hist[id] + rank;
-  }
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  ph_scan(bcount, V.b_start, V.nbricks);
+  if (threadIdx.x == 0) __hip_atomic_store(V.b_done, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // particle records in brick order: fraction (x, y, z) and charge, plus the
@@ -1337,9 +1353,7 @@ constexpr int kRedComp = MFE_MAX_PAIRS + 12;
 // order keeps the result deterministic.
 constexpr int kReduceThreads = 1024;
 
-__global__ void __launch_bounds__(kReduceThreads) k_reduce(const SysDev *__restrict__ sys,
-                                                           int energy, int skin, int bd) {
-  const SysDev &V = sys[blockIdx.z];
+__device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin, int bd) {
   double *en = V.en;
   int *flags = V.flags;
   const int P = V.nptpt;
@@ -1431,6 +1445,11 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce(const SysDev *__restr
   en[2] = rec;
   en[3] = ubond;
   en[4] = vir;
+}
+
+__global__ void __launch_bounds__(kReduceThreads) k_reduce(const SysDev *__restrict__ sys,
+                                                           int energy, int skin, int bd) {
+  ph_reduce(sys[blockIdx.z], energy, skin, bd);
 }
 
 // --------------------------------------------------------------- adjoint
@@ -1758,21 +1777,16 @@ __global__ void __launch_bounds__(kBlock) k_skin_blocks(const SysDev *__restrict
 // UTotal's tail (Src/energy.F90:405-418) plus, for MD, GetLinAcc and the
 // velocity half of VelVer (Src/md.F90:348-358):
 //   mode 0: forceo = force*fac        mode 1: + VelVer kick
-__global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ sys,
-                                                     int mode, double dt) {
+// one particle of k_finalize; es = the reduced surface field en[5..7]
+__device__ __forceinline__ void ph_finalize(const SysDev &V, int i, int mode, double dt,
+                                            const double es[3]) {
 #pragma clang fp contract(off)
-  const SysDev &V = sys[blockIdx.z];
   const double *__restrict__ force = V.force;
   double *__restrict__ forceo = V.forceo;
-  const double *__restrict__ q = V.q;
-  const double *__restrict__ massip = V.massip;
   const double ffac = V.ffac, surf_fac = V.surf_fac;
   const int surf = V.lsurf;
-  const double *en = V.en;
   double *__restrict__ rod = V.rod;
   double *__restrict__ rodd = V.rodd;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= V.np) return;
   // every load before the first store
   double f[3], aold[3], v[3];
 #pragma unroll
@@ -1784,13 +1798,8 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ 
       v[m] = rod[3 * i + m];
     }
   }
-  const double qi = surf ? q[i] : 0.0;
-  const double mi = mode == 1 ? massip[i] : 0.0;
-  double es[3] = {0.0, 0.0, 0.0};
-  if (surf) {
-#pragma unroll
-    for (int m = 0; m < 3; ++m) es[m] = en[5 + m];
-  }
+  const double qi = surf ? V.q[i] : 0.0;
+  const double mi = mode == 1 ? V.massip[i] : 0.0;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     double fm = f[m];
@@ -1803,6 +1812,36 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ 
       rod[3 * i + m] = v[m] + 0.5 * dt * (anew + aold[m]);
     }
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ sys,
+                                                     int mode, double dt) {
+  const SysDev &V = sys[blockIdx.z];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V.np) return;
+  double es[3] = {0.0, 0.0, 0.0};
+  if (V.lsurf) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) es[m] = V.en[5 + m];
+  }
+  ph_finalize(V, i, mode, dt, es);
+}
+
+// Without the surface term the reduction and k_finalize are independent:
+// one launch in which blocks [0, F) finalize kReduceThreads particles each
+// and block F (F of the replica) reduces the partials.
+__global__ void __launch_bounds__(kReduceThreads) k_reduce_finalize(const SysDev *__restrict__ sys,
+                                                                    int energy, int skin, int bd,
+                                                                    int mode, double dt) {
+  const SysDev &V = sys[blockIdx.z];
+  const int F = (V.np + kReduceThreads - 1) / kReduceThreads;
+  if ((int)blockIdx.x < F) {
+    const int i = blockIdx.x * kReduceThreads + threadIdx.x;
+    const double es[3] = {0.0, 0.0, 0.0};
+    if (i < V.np) ph_finalize(V, i, mode, dt, es);
+    return;
+  }
+  if ((int)blockIdx.x == F) ph_reduce(V, energy, skin, bd);
 }
 
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
@@ -1882,10 +1921,10 @@ static void spme_stages(const Group &g, bool energy, int first, int last, bool p
   for (int st = first; st <= last; ++st) {
     switch (st) {
       case SP_BIN:
+        // counts, the last count block's scan, records
         hipLaunchKernelGGL(k_spme_count,
                            grid((g.np_blocks * kBlock + kCountThreads - 1) / kCountThreads, g),
                            dim3(kCountThreads), 0, g.stream, g.d);
-        hipLaunchKernelGGL(k_scan, grid(1, g), dim3(1024), 0, g.stream, g.d, 1, 0);
         hipLaunchKernelGGL(k_spme_scatter<R>, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream,
                            g.d);
         break;
@@ -1947,6 +1986,14 @@ void launch_reduce(const Group &g, unsigned flags, bool skin, bool bd) {
   const int energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
   hipLaunchKernelGGL(k_reduce, grid(1, g), dim3(kReduceThreads), 0, g.stream, g.d, energy,
                      skin ? 1 : 0, bd ? 1 : 0);
+}
+
+void launch_reduce_finalize(const Group &g, unsigned flags, bool skin, bool bd, int mode,
+                            double dt) {
+  const int energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
+  const int F = (g.np_blocks * kBlock + kReduceThreads - 1) / kReduceThreads;
+  hipLaunchKernelGGL(k_reduce_finalize, grid(F + 1, g), dim3(kReduceThreads), 0, g.stream, g.d,
+                     energy, skin ? 1 : 0, bd ? 1 : 0, mode, dt);
 }
 
 void launch_finalize(const Group &g, int mode, double dt) {
