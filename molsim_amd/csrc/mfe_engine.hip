@@ -829,7 +829,11 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
       // 256 (FP32) or 128 (FP64) x (3 order + 1) values and 256 ints
       const size_t chunk = e->prec == MFE_PREC_MIXED ? 256 : 128;
       const size_t lds_static = chunk * (3 * order + 1) * rsz + chunk * 4 + 256;
-      if ((size_t)lds_need + lds_static > 160 * 1024)
+      // k_gather: the staged tile plus a chunk of weights and derivatives,
+      // 256 particles (FP32) or 64 (FP64)
+      const size_t gchunk = e->prec == MFE_PREC_MIXED ? 256 : 64;
+      const size_t gather_static = gchunk * (6 * order + 1) * rsz + gchunk * 8 + 256;
+      if ((size_t)lds_need + lds_static > 160 * 1024 || gext + gather_static > 160 * 1024)
         return fail(e, MFE_E_BAD_PARAM, "mesh brick too large for LDS (mesh 2q, q prime > 16?)");
       const size_t M = (size_t)e->s[0] * e->s[1] * e->s[2];
       const size_t Mc = (size_t)e->hx * e->s[1] * e->s[2];

@@ -1034,9 +1034,9 @@ struct Acc3<double> {
 // CalcProp (Src/energy.F90:1259-1285): esum over the order^3 stencil of the
 // convolved mesh; force -= q * esum * dkdr * EpsiFourPi.  One block per
 // brick stages the mesh the brick's particles touch -- points
-// [o - (order-1), o + B) per axis -- in LDS; each particle is taken by 8
-// lanes, lane iz < order summing stencil plane iz (FP32 with compensated
-// row sums on the mixed path), and the planes are combined by shuffles in
+// [o - (order-1), o + B) per axis -- in LDS; each particle is taken by a
+// few lanes, each summing some of its stencil planes (FP32 with compensated
+// row sums on the mixed path), and the lanes are combined by shuffles in
 // FP64 in a fixed order.
 constexpr int kGatherLanes = 512;
 
@@ -1091,16 +1091,22 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
   }
   __syncthreads();
   const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
-  // particles in chunks of blockDim/8; the B-spline weights of a chunk are
-  // computed once (one thread per particle and axis) into LDS
-  constexpr int kGChunk = kGatherLanes / 8;
+  // particles in chunks of blockDim / GL; the B-spline weights of a chunk
+  // are computed once (one thread per particle and axis) into LDS.  GL
+  // lanes share a particle, lane l summing stencil planes l, l + GL, ...:
+  // two lanes on the FP32 path (no idle lanes at even orders, 256
+  // particles per chunk), eight on the FP64 path (its weights take twice
+  // the LDS)
+  constexpr int GL = sizeof(R) == 4 ? 2 : 8;
+  constexpr int kGChunk = kGatherLanes / GL;
+  constexpr int kPlanes = (ORDER + GL - 1) / GL;
   __shared__ R wv[kGChunk][3][ORDER], wd[kGChunk][3][ORDER];
   __shared__ R qsh[kGChunk];
   __shared__ int locs[kGChunk], idxs[kGChunk];
   for (int p0 = 0; p0 < n; p0 += kGChunk) {
     const int nc = min(kGChunk, n - p0);
-    if ((int)threadIdx.x < 3 * nc) {
-      const int c = threadIdx.x / 3, m = threadIdx.x - 3 * c;
+    for (int cm = threadIdx.x; cm < 3 * nc; cm += blockDim.x) {
+      const int c = cm / 3, m = cm - 3 * c;
       const int t = first + p0 + c;
       bspline<R, ORDER>(frac[4 * t + m], wv[c][m], wd[c][m]);
       if (m == 0) {
@@ -1115,10 +1121,10 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       }
     }
     __syncthreads();
-    const int p = threadIdx.x >> 3, iz = threadIdx.x & 7;
+    const int p = threadIdx.x / GL, l = threadIdx.x % GL;
     // the force this lane updates, loaded now so the load overlaps the
     // stencil sums
-    const bool owner = iz == 0 && p < nc;
+    const bool owner = l == 0 && p < nc;
     const int idx = owner ? idxs[p] : 0;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     if (owner) {
@@ -1127,7 +1133,7 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       f2 = force[3 * idx + 2];
     }
     double ex = 0.0, ey = 0.0, ez = 0.0, ep = 0.0;
-    if (p < nc && iz < ORDER) {
+    if (p < nc && l < ORDER) {
       const int lc = locs[p];
       const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = (lc >> 20) & 1023;
       R v[2][ORDER], d[2][ORDER];
@@ -1138,24 +1144,29 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
           v[m][k] = wv[p][m][k];
           d[m][k] = wd[p][m][k];
         }
-      const R vz = wv[p][2][iz], dz = wd[p][2][iz];
-      const R *plane = tile + (size_t)(lz - iz) * E1 * E0;
       Acc3<R> acc;
       double pacc = 0.0;
 #pragma unroll
-      for (int iy = 0; iy < ORDER; ++iy) {
-        const R *row = plane + (ly - iy) * E0;
-        R sx = R(0), sdx = R(0);
+      for (int kz = 0; kz < kPlanes; ++kz) {
+        const int iz = l + GL * kz;
+        if (iz >= ORDER) break;
+        const R vz = wv[p][2][iz], dz = wd[p][2][iz];
+        const R *plane = tile + (size_t)(lz - iz) * E1 * E0;
 #pragma unroll
-        for (int ix = 0; ix < ORDER; ++ix) {
-          const R qv = row[lx - ix];
-          sx += v[0][ix] * qv;
-          sdx += d[0][ix] * qv;
+        for (int iy = 0; iy < ORDER; ++iy) {
+          const R *row = plane + (ly - iy) * E0;
+          R sx = R(0), sdx = R(0);
+#pragma unroll
+          for (int ix = 0; ix < ORDER; ++ix) {
+            const R qv = row[lx - ix];
+            sx += v[0][ix] * qv;
+            sdx += d[0][ix] * qv;
+          }
+          acc.add(0, sdx * (v[1][iy] * vz));
+          acc.add(1, sx * (d[1][iy] * vz));
+          acc.add(2, sx * (v[1][iy] * dz));
+          if (POT) pacc += (double)sx * (double)(v[1][iy] * vz);
         }
-        acc.add(0, sdx * (v[1][iy] * vz));
-        acc.add(1, sx * (d[1][iy] * vz));
-        acc.add(2, sx * (v[1][iy] * dz));
-        if (POT) pacc += (double)sx * (double)(v[1][iy] * vz);
       }
       ex = acc.get(0);
       ey = acc.get(1);
@@ -1163,11 +1174,11 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       ep = pacc;
     }
 #pragma unroll
-    for (int w = 1; w < 8; w <<= 1) {
-      ex += __shfl_xor(ex, w, 8);
-      ey += __shfl_xor(ey, w, 8);
-      ez += __shfl_xor(ez, w, 8);
-      if (POT) ep += __shfl_xor(ep, w, 8);
+    for (int w = 1; w < GL; w <<= 1) {
+      ex += __shfl_xor(ex, w, GL);
+      ey += __shfl_xor(ey, w, GL);
+      ez += __shfl_xor(ez, w, GL);
+      if (POT) ep += __shfl_xor(ep, w, GL);
     }
     if (owner) {
       const double qd = (double)qsh[p];
