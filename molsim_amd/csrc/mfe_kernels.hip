@@ -65,6 +65,19 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
   return r;
 }
 
+// The partner value of an FP64 register in a lane quad, by DPP (no LDS):
+// kXor1 swaps lanes 0<->1 and 2<->3, kXor2 swaps 0<->2 and 1<->3 -- the
+// exchanges of __shfl_xor(x, 1 or 2) within a quad.  Every lane of the
+// wavefront must be active.
+constexpr int kXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+template <int CTRL>
+__device__ __forceinline__ double quad_swap(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
 __global__ void k_fill_int(int *p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -719,12 +732,14 @@ k_pair(const SysDev *__restrict__ sys, int energy) {
         fz = fzf.value();
       }
     }
-#pragma unroll
-    for (int w = 1; w < kPairLanes; w <<= 1) {
-      fx += __shfl_xor(fx, w, kPairLanes);
-      fy += __shfl_xor(fy, w, kPairLanes);
-      fz += __shfl_xor(fz, w, kPairLanes);
-    }
+    // the four lanes' partial forces: exchanges within the quad by DPP
+    static_assert(kPairLanes == 4, "quad exchanges assume four lanes per particle");
+    fx += quad_swap<kXor1>(fx);
+    fy += quad_swap<kXor1>(fy);
+    fz += quad_swap<kXor1>(fz);
+    fx += quad_swap<kXor2>(fx);
+    fy += quad_swap<kXor2>(fy);
+    fz += quad_swap<kXor2>(fz);
     if (i < np && lane == 0) {
       const int io = V.c_sorted[i];
       V.force[3 * io] = fx;
@@ -1173,12 +1188,20 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       ez = acc.get(2);
       ep = pacc;
     }
+    if constexpr (GL == 2) {
+      // the lane pair within a quad, by DPP
+      ex += quad_swap<kXor1>(ex);
+      ey += quad_swap<kXor1>(ey);
+      ez += quad_swap<kXor1>(ez);
+      if (POT) ep += quad_swap<kXor1>(ep);
+    } else {
 #pragma unroll
-    for (int w = 1; w < GL; w <<= 1) {
-      ex += __shfl_xor(ex, w, GL);
-      ey += __shfl_xor(ey, w, GL);
-      ez += __shfl_xor(ez, w, GL);
-      if (POT) ep += __shfl_xor(ep, w, GL);
+      for (int w = 1; w < GL; w <<= 1) {
+        ex += __shfl_xor(ex, w, GL);
+        ey += __shfl_xor(ey, w, GL);
+        ez += __shfl_xor(ez, w, GL);
+        if (POT) ep += __shfl_xor(ep, w, GL);
+      }
     }
     if (owner) {
       const double qd = (double)qsh[p];
