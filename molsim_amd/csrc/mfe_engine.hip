@@ -462,8 +462,9 @@ void mfe_sync_view(mfe_engine *e) { sync_view(e); }
 void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt, bool skin,
                        bool bd) {
   const bool energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) != 0;
-  // inside MD/BD steps (skin) k_move has already updated the cell-ordered copy
-  mfe::launch_pair(g, energy, !skin);
+  // inside MD/BD steps (skin) k_move has already updated the cell-ordered
+  // copy, and the pair launch takes the skin test's block candidates
+  mfe::launch_pair(g, energy, !skin, skin);
   if (g.ewald == MFE_EWALD_SPME) mfe::launch_spme(g, energy, SP_BIN, SP_GATHER);
   else if (g.ewald == MFE_EWALD_STD) mfe::launch_std_ewald(g);
   if (g.chains) mfe::launch_bonds(g);
@@ -480,7 +481,6 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
 // one VelVer step with the NList skin test and gated rebuild
 void mfe_launch_md_step(const mfe::Group &g, double dt, unsigned flags) {
   mfe::launch_move(g, dt, 0);
-  mfe::launch_skin(g);
   mfe_launch_utotal(g, flags, 1, dt, true, false);
   mfe::launch_nlist_rebuild(g, true);
 }
@@ -488,7 +488,6 @@ void mfe_launch_md_step(const mfe::Group &g, double dt, unsigned flags) {
 // one BDStep (noise: 1 = host-supplied buffer, 2 = counter-based stream)
 void mfe_launch_bd_step(const mfe::Group &g, int noise, unsigned flags) {
   mfe::launch_move(g, 0.0, noise);
-  mfe::launch_skin(g);
   mfe_launch_utotal(g, flags, 0, 0.0, true, true);
   mfe::launch_nlist_rebuild(g, true);
 }

@@ -432,7 +432,9 @@ namespace mfe {
 void launch_nlist_rebuild(const Group &g, bool gated);
 // sort_pos: refresh the cell-ordered positions first (not needed inside MD
 // and BD steps, whose k_move keeps them current)
-void launch_pair(const Group &g, bool energy, bool sort_pos = true);
+// skin: the pair launch also computes the skin test's block candidates
+// (MD/BD steps, after k_move), replacing launch_skin
+void launch_pair(const Group &g, bool energy, bool sort_pos = true, bool skin = false);
 void launch_spme(const Group &g, bool energy, int first_stage, int last_stage,
                  bool potential = false);
 // adjoint passes: real-space pair sums, per-particle dE/dq, column sums

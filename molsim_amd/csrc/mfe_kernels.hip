@@ -497,6 +497,50 @@ __device__ __forceinline__ int xcd_block(int b, int nblk) {
 // list entries a lane loads together (indices, then positions)
 constexpr int kPairBatch = 4;
 
+// k_skin_blocks' computation for skin blocks sb0 and sb0 + 1 (256
+// particles each, the k_move partition), one per half of a 512-thread
+// block: the same maxima in the same order, so the same candidates.
+__device__ __forceinline__ void skin_two_blocks(const SysDev &V, int sb0) {
+  __shared__ double wred[kPairBlock / 64];
+  const int sub = threadIdx.x >> 8, t = threadIdx.x & 255;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, w0 = 4 * sub;
+  const int sb = sb0 + sub;
+  const bool valid = sb < V.nblk_np;
+  const double *part_max = V.part + V.off_move;
+  // running maximum entering the block: the maxima of the earlier blocks
+  double pm = 0.0;
+  if (valid)
+    for (int b2 = t; b2 < sb; b2 += 256) pm = fmax(pm, part_max[b2]);
+  for (int w = 32; w > 0; w >>= 1) pm = fmax(pm, __shfl_down(pm, w, 64));
+  if (lane == 0) wred[wv] = pm;
+  __syncthreads();
+  const double P = fmax(fmax(wred[w0], wred[w0 + 1]), fmax(wred[w0 + 2], wred[w0 + 3]));
+  __syncthreads();
+  const int i = sb * 256 + t;
+  const double v = (valid && i < V.np) ? V.r2disp[i] : 0.0;
+  // inclusive max-scan inside the wavefront, then the earlier wavefronts
+  double inc = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const double y = __shfl_up(inc, off, 64);
+    if (lane >= off) inc = fmax(inc, y);
+  }
+  double excl = __shfl_up(inc, 1, 64);
+  if (lane == 0) excl = 0.0;
+  if (lane == 63) wred[wv] = inc;
+  __syncthreads();
+  double pw = 0.0;
+  for (int k = w0; k < wv; ++k) pw = fmax(pw, wred[k]);
+  const double before = fmax(P, fmax(pw, excl));
+  double cand = (valid && i < V.np && !(v > before)) ? v : 0.0;
+  __syncthreads();
+  for (int w = 32; w > 0; w >>= 1) cand = fmax(cand, __shfl_down(cand, w, 64));
+  if (lane == 0) wred[wv] = cand;
+  __syncthreads();
+  if (t == 0 && valid)
+    V.part[V.off_skin + sb] =
+        fmax(fmax(wred[w0], wred[w0 + 1]), fmax(wred[w0 + 2], wred[w0 + 3]));
+}
+
 // Pair forces over the Verlet list.  kPairLanes lanes share one particle
 // (cell order i) and split its list k = lane, lane + kPairLanes, ...; their
 // partial forces are combined by shuffles in FP64 in a fixed order, so
@@ -513,12 +557,18 @@ constexpr int kPairBatch = 4;
 // accumulate per pair type in registers over all chunks, else per partner
 // type, converted per chunk into a per-wavefront LDS row; either way in a
 // fixed order, and per block into the partials.
+// With skin (MD/BD steps) blocks from pgrid on take the skin test's block
+// candidates (two k_move blocks each) instead of a separate launch.
 template <bool MIXED, int NT, bool CF>
 __global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
-k_pair(const SysDev *__restrict__ sys, int energy) {
+k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
   using CT = typename std::conditional<MIXED, float, double>::type;
   const SysDev &V = sys[blockIdx.z];
-  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  if (skin && (int)blockIdx.x >= pgrid) {
+    skin_two_blocks(V, 2 * ((int)blockIdx.x - pgrid));
+    return;
+  }
+  const int blk = xcd_block(blockIdx.x, pgrid);
   const int nb = V.nblk_pair;
   if (blk >= nb) return;
   const double *__restrict__ ro = V.ro_s;
@@ -1927,13 +1977,15 @@ void launch_nlist_rebuild(const Group &g, bool gated) {
   hipLaunchKernelGGL(k_nlist_build, grid(gb, g), dim3(kBlock), 0, g.stream, g.d, gt);
 }
 
-void launch_pair(const Group &g, bool energy, bool sort_pos) {
+void launch_pair(const Group &g, bool energy, bool sort_pos, bool skin) {
   if (sort_pos)
     hipLaunchKernelGGL(k_sort_pos, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
-  const dim3 gr = grid(g.pair_blocks, g);
-  const int e1 = energy ? 1 : 0;
-#define MFE_PAIR(M, NT, CF) \
-  hipLaunchKernelGGL((k_pair<M, NT, CF>), gr, dim3(kPairBlock), g.lds_pair, g.stream, g.d, e1)
+  const int pgrid = std::max(g.pair_blocks, 1);
+  const dim3 gr = grid(pgrid + (skin ? (g.np_blocks + 1) / 2 : 0), g);
+  const int e1 = energy ? 1 : 0, s1 = skin ? 1 : 0;
+#define MFE_PAIR(M, NT, CF)                                                                  \
+  hipLaunchKernelGGL((k_pair<M, NT, CF>), gr, dim3(kPairBlock), g.lds_pair, g.stream, g.d, e1, \
+                     pgrid, s1)
 #define MFE_PAIR_CF(M, NT) \
   do { if (g.pair_cf_lds) MFE_PAIR(M, NT, true); else MFE_PAIR(M, NT, false); } while (0)
   const int nt = g.pair_npt;
