@@ -85,7 +85,6 @@ int refresh(mfe_batch *b) {
   int rc = mfe_group_build(b->eng.data(), R, b->d_sys, b->stream, b->grp, err);
   if (rc) return bfail(b, rc, err);
   const size_t lds = std::max(b->grp.lds_spread, b->grp.lds_gather);
-  // dynamic part; k_spread adds up to ~30 KB of static LDS
   if (lds > 32 * 1024) mfe_allow_big_spread_lds(b->grp.prec, (int)lds);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   b->graph = nullptr;

@@ -599,6 +599,8 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
       if (iptpn[i] < 1 || iptpn[i] > npt) return fail(e, MFE_E_BAD_PARAM, "iptpn out of range");
       t[i] = iptpn[i] - 1;
       mi[i] = massipt[t[i]];
+      // the mixed spreading's 2^-32 fixed point takes |q| < 2^19
+      if (!(std::fabs(az[i]) < 65536.0)) return fail(e, MFE_E_BAD_PARAM, "charge not finite or |q| >= 2^16");
     }
     if (np != e->np) {
       dalloc(e->ro, 3 * (size_t)np); dalloc(e->rod, 3 * (size_t)np);
@@ -824,15 +826,7 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
       e->spread_lds = (int)(ext * 8);
       const size_t gext = ext * rsz;
       const int lds_need = (int)std::max<size_t>((size_t)e->spread_lds, gext);
-      // k_spread also holds a chunk of candidate weights in static LDS:
-      // 256 (FP32) or 128 (FP64) x (3 order + 1) values and 256 ints
-      const size_t chunk = e->prec == MFE_PREC_MIXED ? 256 : 128;
-      const size_t lds_static = chunk * (3 * order + 1) * rsz + chunk * 4 + 256;
-      // k_gather: the staged tile plus a chunk of weights and derivatives,
-      // 256 particles (FP32) or 64 (FP64)
-      const size_t gchunk = e->prec == MFE_PREC_MIXED ? 256 : 64;
-      const size_t gather_static = gchunk * (6 * order + 1) * rsz + gchunk * 8 + 256;
-      if ((size_t)lds_need + lds_static > 160 * 1024 || gext + gather_static > 160 * 1024)
+      if ((size_t)lds_need > 160 * 1024)
         return fail(e, MFE_E_BAD_PARAM, "mesh brick too large for LDS (mesh 2q, q prime > 16?)");
       const size_t M = (size_t)e->s[0] * e->s[1] * e->s[2];
       const size_t Mc = (size_t)e->hx * e->s[1] * e->s[2];
@@ -884,7 +878,7 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
                                              : make_twiddles<double>(e->s[m], e->stream);
       e->tw_half = e->prec == MFE_PREC_MIXED ? make_twiddles<float>(e->s[0] / 2, e->stream)
                                              : make_twiddles<double>(e->s[0] / 2, e->stream);
-      if ((size_t)lds_need + lds_static > 64 * 1024) {
+      if ((size_t)lds_need > 64 * 1024) {
         // large bricks (meshes 2q with q prime) need the LDS above 64 KB
         mfe_allow_big_spread_lds(e->prec, lds_need);
       }

@@ -978,14 +978,21 @@ __global__ void k_spme_scatter(const SysDev *__restrict__ sys) {
 
 // LDS accumulator of the spreading: exact 64-bit fixed point (2^-32) for
 // the mixed path, so the sum does not depend on the order of the adds;
-// FP64 LDS atomics for the FP64 path.
+// FP64 LDS atomics for the FP64 path.  The fixed-point value is
+// round-to-nearest-even of v * 2^32: v * 2^32 + 1.5 * 2^52 in FP64 (the
+// product is exact, the sum rounds once to an integer because
+// |v| * 2^32 < 2^51), whose low mantissa bits minus those of 1.5 * 2^52 are
+// the two's-complement integer.
 template <typename R>
 struct SpreadAcc;
 template <>
 struct SpreadAcc<float> {
   typedef unsigned long long T;
   static __device__ __forceinline__ void add(T *p, float v) {
-    atomicAdd(p, (unsigned long long)(long long)__float2ll_rn(v * 4294967296.0f));
+    constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+    const double d = __fma_rn((double)v, 4294967296.0, kMagic);
+    atomicAdd(p, (unsigned long long)__double_as_longlong(d) -
+                     (unsigned long long)__double_as_longlong(kMagic));
   }
   static __device__ __forceinline__ float get(T v) {
     return (float)((double)(long long)v * (1.0 / 4294967296.0));
@@ -1000,6 +1007,21 @@ struct SpreadAcc<double> {
 
 constexpr int kMaxOrder = 8;
 
+// one particle record in brick order: fraction (x, y, z) and charge
+template <typename R>
+struct Rec4 {
+  R x, y, z, q;
+};
+__device__ __forceinline__ Rec4<float> load_rec(const float *__restrict__ p) {
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  return {v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ Rec4<double> load_rec(const double *__restrict__ p) {
+  const double2 a = reinterpret_cast<const double2 *>(p)[0];
+  const double2 b = reinterpret_cast<const double2 *>(p)[1];
+  return {a.x, a.y, b.x, b.y};
+}
+
 // Spreading, each particle once and no global atomics: one block per brick
 // takes the brick's own particles and adds their whole stencil -- points
 // m, m-1, ..., m-order+1 along each axis from meshmax m
@@ -1008,12 +1030,12 @@ constexpr int kMaxOrder = 8;
 // scratch array.  The forward x FFT reads each mesh point as its own
 // brick's tile value plus the low-side halos of the next bricks (at most 8
 // tiles) in a fixed order (spread_point in mfe_fft.hip).  Work is handed
-// out as (particle, stencil plane) items, order planes per particle.
-template <typename R> struct SpreadChunk { static constexpr int n = 256; };
-template <> struct SpreadChunk<double> { static constexpr int n = 128; };
-
-// 1024 threads share one brick's tile: the tile's LDS allows one block per
-// CU, so the block supplies the wavefronts that hide the load latencies
+// out as (particle, stencil plane) items, order planes per particle; each
+// item computes the particle's B-splines in registers (no staging, no
+// barrier between particles) and loads the record of its next item before
+// its own adds, so the load latency hides behind them.
+// 1024 threads share one brick's tile; without other LDS two blocks fit a
+// CU at 16-point bricks of order <= 7.
 constexpr int kSpreadThreads = 1024;
 
 template <typename R, int ORDER>
@@ -1025,59 +1047,63 @@ __global__ void __launch_bounds__(kSpreadThreads) k_spread(const SysDev *__restr
   const int *__restrict__ bstart = V.b_start;
   const SpmeGeom g = V.sg;
   typedef typename SpreadAcc<R>::T Acc;
-  constexpr int kSpreadChunk = SpreadChunk<R>::n;
   extern __shared__ char smem[];
   Acc *tile = reinterpret_cast<Acc *>(smem);
-  __shared__ R w[kSpreadChunk][3][ORDER];
-  __shared__ R qs[kSpreadChunk];
-  __shared__ int loc[kSpreadChunk];
   const int bx = blockIdx.x % g.nb[0];
   const int by = (blockIdx.x / g.nb[0]) % g.nb[1];
   const int bz = blockIdx.x / (g.nb[0] * g.nb[1]);
-  const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
+  // meshmax - brick origin + order-1: extended-tile coordinates
+  const int o0 = bx * g.B[0] - (ORDER - 1), o1 = by * g.B[1] - (ORDER - 1),
+            o2 = bz * g.B[2] - (ORDER - 1);
   const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
   const int ntile = E0 * E1 * E2;
-  for (int k = threadIdx.x; k < ntile; k += blockDim.x) tile[k] = Acc(0);
-  const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
-  for (int c0 = 0; c0 < n; c0 += kSpreadChunk) {
-    const int nc = min(kSpreadChunk, n - c0);
-    // one thread per (particle, axis) B-spline
-    for (int cm = threadIdx.x; cm < 3 * nc; cm += blockDim.x) {
-      const int c = cm / 3, m = cm - 3 * c;
-      const int rec = first + c0 + c;
-      bspline<R, ORDER>(frac[4 * rec + m], w[c][m], nullptr);
-      if (m == 0) {
-        const int2 md = meta[rec];
-        // meshmax in extended-tile coordinates: [order-1, B+order-2]
-        const int lx = (md.x & 1023) - o0 + ORDER - 1;
-        const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
-        const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
-        loc[c] = lx | (ly << 10) | (lz << 20);
-        qs[c] = frac[4 * rec + 3];
-      }
-    }
-    __syncthreads();
-    for (int r = threadIdx.x; r < nc * ORDER; r += blockDim.x) {
-      const int c = r / ORDER, iz = r - c * ORDER;
-      const int lc = loc[c];
-      const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = (lc >> 20) & 1023;
-      const R qz = qs[c] * w[c][2][iz];
-      R wx[ORDER];
-#pragma unroll
-      for (int j = 0; j < ORDER; ++j) wx[j] = w[c][0][j];
-      Acc *plane = tile + (size_t)(lz - iz) * E1 * E0;
-#pragma unroll
-      for (int jy = 0; jy < ORDER; ++jy) {
-        const R qyz = qz * w[c][1][jy];
-        Acc *row = plane + (ly - jy) * E0 + lx;
-#pragma unroll
-        for (int j = 0; j < ORDER; ++j) SpreadAcc<R>::add(&row[-j], qyz * wx[j]);
-      }
-    }
-    __syncthreads();
+  const int first = bstart[blockIdx.x], items = (bstart[blockIdx.x + 1] - first) * ORDER;
+  int r = threadIdx.x;
+  Rec4<R> f{};
+  int code = 0;
+  if (r < items) {
+    const int rec = first + r / ORDER;
+    f = load_rec(frac + 4 * rec);
+    code = meta[rec].x;
   }
+  for (int k = threadIdx.x; k < ntile; k += kSpreadThreads) tile[k] = Acc(0);
+  __syncthreads();
+  for (; r < items; r += kSpreadThreads) {
+    const int rn = r + kSpreadThreads;
+    Rec4<R> fn = f;
+    int cn = code;
+    if (rn < items) {
+      const int rec = first + rn / ORDER;
+      fn = load_rec(frac + 4 * rec);
+      cn = meta[rec].x;
+    }
+    const int iz = r % ORDER;
+    R wx[ORDER], wy[ORDER], wz[ORDER];
+    bspline<R, ORDER>(f.x, wx, nullptr);
+    bspline<R, ORDER>(f.y, wy, nullptr);
+    bspline<R, ORDER>(f.z, wz, nullptr);
+    R wzi = wz[0];
+#pragma unroll
+    for (int k = 1; k < ORDER; ++k)
+      if (iz == k) wzi = wz[k];
+    const int lx = (code & 1023) - o0;
+    const int ly = ((code >> 10) & 1023) - o1;
+    const int lz = ((code >> 20) & 1023) - o2;
+    const R qz = f.q * wzi;
+    Acc *plane = tile + (lz - iz) * E1 * E0;
+#pragma unroll
+    for (int jy = 0; jy < ORDER; ++jy) {
+      const R qyz = qz * wy[jy];
+      Acc *row = plane + (ly - jy) * E0 + lx;
+#pragma unroll
+      for (int j = 0; j < ORDER; ++j) SpreadAcc<R>::add(&row[-j], qyz * wx[j]);
+    }
+    f = fn;
+    code = cn;
+  }
+  __syncthreads();
   R *__restrict__ out = (R *)V.spread_tiles + (size_t)blockIdx.x * ntile;
-  for (int k = threadIdx.x; k < ntile; k += blockDim.x) out[k] = SpreadAcc<R>::get(tile[k]);
+  for (int k = threadIdx.x; k < ntile; k += kSpreadThreads) out[k] = SpreadAcc<R>::get(tile[k]);
 }
 
 // per-lane stencil sums: compensated FP32 on the mixed path, plain FP64
@@ -1100,9 +1126,11 @@ struct Acc3<double> {
 // convolved mesh; force -= q * esum * dkdr * EpsiFourPi.  One block per
 // brick stages the mesh the brick's particles touch -- points
 // [o - (order-1), o + B) per axis -- in LDS; each particle is taken by a
-// few lanes, each summing some of its stencil planes (FP32 with compensated
-// row sums on the mixed path), and the lanes are combined by shuffles in
-// FP64 in a fixed order.
+// few lanes, each computing the particle's B-splines and derivatives in
+// registers and summing some of its stencil planes (FP32 with compensated
+// row sums on the mixed path); the lanes are combined by lane exchanges in
+// FP64 in a fixed order.  No barrier between particles: each lane loads
+// its next particle's record while it sums the current one.
 constexpr int kGatherLanes = 512;
 
 template <typename R, int ORDER, bool POT>
@@ -1124,6 +1152,22 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
   const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
   const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
   const int nstage = E0 * E1 * E2;
+  // GL lanes share a particle, lane l summing stencil planes l, l + GL,
+  // ...: two lanes on the FP32 path (no idle lanes at even orders), eight
+  // on the FP64 path (one plane each)
+  constexpr int GL = sizeof(R) == 4 ? 2 : 8;
+  constexpr int kGStep = kGatherLanes / GL;
+  constexpr int kPlanes = (ORDER + GL - 1) / GL;
+  const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
+  const int l = threadIdx.x % GL;
+  int p = threadIdx.x / GL;
+  // the first particle's record is in flight during the staging
+  Rec4<R> f{};
+  int2 md = make_int2(0, 0);
+  if (p < n) {
+    f = load_rec(frac + 4 * (first + p));
+    md = meta[first + p];
+  }
   // staged in batches of kU loads per lane, so the loads of a batch are in
   // flight together (one batch for bricks of 16 and order <= 8)
   constexpr int kU = 24;
@@ -1155,67 +1199,48 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
     }
   }
   __syncthreads();
-  const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
-  // particles in chunks of blockDim / GL; the B-spline weights of a chunk
-  // are computed once (one thread per particle and axis) into LDS.  GL
-  // lanes share a particle, lane l summing stencil planes l, l + GL, ...:
-  // two lanes on the FP32 path (no idle lanes at even orders, 256
-  // particles per chunk), eight on the FP64 path (its weights take twice
-  // the LDS)
-  constexpr int GL = sizeof(R) == 4 ? 2 : 8;
-  constexpr int kGChunk = kGatherLanes / GL;
-  constexpr int kPlanes = (ORDER + GL - 1) / GL;
-  __shared__ R wv[kGChunk][3][ORDER], wd[kGChunk][3][ORDER];
-  __shared__ R qsh[kGChunk];
-  __shared__ int locs[kGChunk], idxs[kGChunk];
-  for (int p0 = 0; p0 < n; p0 += kGChunk) {
-    const int nc = min(kGChunk, n - p0);
-    for (int cm = threadIdx.x; cm < 3 * nc; cm += blockDim.x) {
-      const int c = cm / 3, m = cm - 3 * c;
-      const int t = first + p0 + c;
-      bspline<R, ORDER>(frac[4 * t + m], wv[c][m], wd[c][m]);
-      if (m == 0) {
-        const int2 md = meta[t];
-        // brick-local coordinates inside the staged region
-        const int lx = (md.x & 1023) - o0 + ORDER - 1;
-        const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
-        const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
-        locs[c] = lx | (ly << 10) | (lz << 20);
-        idxs[c] = md.y;
-        qsh[c] = frac[4 * t + 3];
-      }
-    }
-    __syncthreads();
-    const int p = threadIdx.x / GL, l = threadIdx.x % GL;
-    // the force this lane updates, loaded now so the load overlaps the
-    // stencil sums
-    const bool owner = l == 0 && p < nc;
-    const int idx = owner ? idxs[p] : 0;
+  for (; p < n; p += kGStep) {
+    // the force this particle's owner lane updates and the next particle's
+    // record, loaded now so the loads overlap the stencil sums
+    const bool owner = l == 0;
+    const int idx = md.y;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     if (owner) {
       f0 = force[3 * idx];
       f1 = force[3 * idx + 1];
       f2 = force[3 * idx + 2];
     }
+    const int pn = p + kGStep;
+    Rec4<R> fn = f;
+    int2 mn = md;
+    if (pn < n) {
+      fn = load_rec(frac + 4 * (first + pn));
+      mn = meta[first + pn];
+    }
     double ex = 0.0, ey = 0.0, ez = 0.0, ep = 0.0;
-    if (p < nc && l < ORDER) {
-      const int lc = locs[p];
-      const int lx = lc & 1023, ly = (lc >> 10) & 1023, lz = (lc >> 20) & 1023;
-      R v[2][ORDER], d[2][ORDER];
-#pragma unroll
-      for (int m = 0; m < 2; ++m)
-#pragma unroll
-        for (int k = 0; k < ORDER; ++k) {
-          v[m][k] = wv[p][m][k];
-          d[m][k] = wd[p][m][k];
-        }
+    if (l < ORDER) {
+      R v[3][ORDER], d[3][ORDER];
+      bspline<R, ORDER>(f.x, v[0], d[0]);
+      bspline<R, ORDER>(f.y, v[1], d[1]);
+      bspline<R, ORDER>(f.z, v[2], d[2]);
+      // brick-local coordinates inside the staged region
+      const int lx = (md.x & 1023) - o0 + ORDER - 1;
+      const int ly = ((md.x >> 10) & 1023) - o1 + ORDER - 1;
+      const int lz = ((md.x >> 20) & 1023) - o2 + ORDER - 1;
       Acc3<R> acc;
       double pacc = 0.0;
 #pragma unroll
       for (int kz = 0; kz < kPlanes; ++kz) {
         const int iz = l + GL * kz;
         if (iz >= ORDER) break;
-        const R vz = wv[p][2][iz], dz = wd[p][2][iz];
+        // weights of plane iz, selected among this step's GL planes
+        R vz = v[2][GL * kz], dz = d[2][GL * kz];
+#pragma unroll
+        for (int k = 1; k < GL; ++k)
+          if (GL * kz + k < ORDER && l == k) {
+            vz = v[2][GL * kz + k];
+            dz = d[2][GL * kz + k];
+          }
         const R *plane = tile + (size_t)(lz - iz) * E1 * E0;
 #pragma unroll
         for (int iy = 0; iy < ORDER; ++iy) {
@@ -1238,6 +1263,7 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       ez = acc.get(2);
       ep = pacc;
     }
+    // the GL lanes of a particle are all in or all out of the loop
     if constexpr (GL == 2) {
       // the lane pair within a quad, by DPP
       ex += quad_swap<kXor1>(ex);
@@ -1254,14 +1280,15 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       }
     }
     if (owner) {
-      const double qd = (double)qsh[p];
+      const double qd = (double)f.q;
       force[3 * idx] = f0 - qd * ex * g.dkdr[0] * efp;
       force[3 * idx + 1] = f1 - qd * ey * g.dkdr[1] * efp;
       force[3 * idx + 2] = f2 - qd * ez * g.dkdr[2] * efp;
       // dE_rec/dq = efp * sum_stencil theta * (G * Q) (the convolved mesh)
       if (POT) V.phi_rec[idx] = efp * ep;
     }
-    __syncthreads();
+    f = fn;
+    md = mn;
   }
 }
 
