@@ -268,8 +268,9 @@ __device__ __forceinline__ R spread_point(const SysDev &V, const R *__restrict__
 template <typename R>
 __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
-  if ((int)blockIdx.x >= V.fx_nblk) return;
-  const R *__restrict__ real = (const R *)V.mesh_r;
+  // neighbouring rows (which read the same spreading tiles) on one XCD
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blk >= V.fx_nblk) return;
   cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
   const int s1 = V.sg.s[0], nrows = V.sg.s[1] * V.sg.s[2], rows_per_blk = V.fx_rpb;
   const FftPlan &ph = V.plan_half;
@@ -279,7 +280,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
   const int m = s1 / 2, hx = m + 1, ld = m + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
   cplx<R> *b = a + rows_per_blk * ld;
-  const int row0 = blockIdx.x * rows_per_blk;
+  const int row0 = blk * rows_per_blk;
   const int nr = min(rows_per_blk, nrows - row0);
   R *ar = reinterpret_cast<R *>(a);
   const R *__restrict__ tiles = (const R *)V.spread_tiles;
@@ -422,7 +423,11 @@ __global__ void __launch_bounds__(kPlaneThreads) k_fft_yx_inv(const SysDev *__re
 template <typename R>
 __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, int inv) {
   const SysDev &V = sys[blockIdx.z];
-  if ((int)blockIdx.x >= V.fy_nbx || (int)blockIdx.y >= V.sg.s[2]) return;
+  // the tiles of one z plane share the cache lines of its rows: the blocks
+  // of a contiguous range of planes go to one XCD
+  const int L = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int bx = L % gridDim.x, z = L / gridDim.x;
+  if (bx >= V.fy_nbx || z >= V.sg.s[2]) return;
   cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
   const int s2 = V.sg.s[1], hx = V.hx, T = V.fy_T;
   const FftPlan &py = V.plan[1];
@@ -431,8 +436,7 @@ __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, i
   const int ld = s2 + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
   cplx<R> *b = a + T * ld;
-  const int z = blockIdx.y;
-  const int kx0 = blockIdx.x * T;
+  const int kx0 = bx * T;
   const int nt = min(T, hx - kx0);
   cplx<R> *base = cm + (size_t)z * s2 * hx + kx0;
   cplx<R> *tw_s = b + T * ld;
@@ -466,7 +470,11 @@ __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, i
 template <typename R>
 __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys, int energy) {
   const SysDev &V = sys[blockIdx.z];
-  if ((int)blockIdx.x >= V.fz_nbx || (int)blockIdx.y >= V.sg.s[1]) return;
+  // the tiles of one y line share the cache lines of its rows: the blocks
+  // of a contiguous range of lines go to one XCD
+  const int L = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int bx = L % gridDim.x, y = L / gridDim.x;
+  if (bx >= V.fz_nbx || y >= V.sg.s[1]) return;
   cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
   const int s1 = V.sg.s[0], s2 = V.sg.s[1], s3 = V.sg.s[2], hx = V.hx, T = V.fz_T;
   const FftPlan &pz = V.plan[2];
@@ -478,8 +486,7 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
   cplx<R> *b = a + T * ld;
   __shared__ double red[2][256];
-  const int y = blockIdx.y;
-  const int kx0 = blockIdx.x * T;
+  const int kx0 = bx * T;
   const int nt = min(T, hx - kx0);
   const size_t zstride = (size_t)s2 * hx;
   cplx<R> *base = cm + (size_t)y * hx + kx0;
@@ -541,7 +548,7 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
         e += red[0][k];
         v += red[1][k];
       }
-      const int blk = blockIdx.y * V.fz_nbx + blockIdx.x;
+      const int blk = y * V.fz_nbx + bx;
       part[blk] = e;
       part[V.nblk_conv + blk] = v;
     }

@@ -165,6 +165,16 @@ __host__ __device__ __forceinline__ int fdiv(int n, const FastDiv &f) {
   return (int)(((unsigned long long)(unsigned)n * f.m) >> 32);
 }
 
+// Blocks are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
+// each XCD with its own L2.  This maps them to logical blocks so that each
+// XCD takes one contiguous range, whose data (positions of a slab of the
+// box, rows of neighbouring mesh lines) then stays in that XCD's L2.  A
+// bijection of [0, nblk).
+__device__ __forceinline__ int xcd_block(int b, int nblk) {
+  const int q = nblk >> 3, r = nblk & 7, x = b & 7, k = b >> 3;
+  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 struct SpmeGeom {
   int s[3], B[3], nb[3], order;
   double dkdr[3];

@@ -484,16 +484,6 @@ struct KahanF {
   __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
-// Blocks are dispatched to the 8 XCDs round-robin (block b on XCD b % 8),
-// each XCD with its own L2.  This maps them to logical blocks so that each
-// XCD takes one contiguous range: in cell order that is a slab of the box,
-// whose positions (and its neighbours') stay in that XCD's L2.  A bijection
-// of [0, nblk).
-__device__ __forceinline__ int xcd_block(int b, int nblk) {
-  const int q = nblk >> 3, r = nblk & 7, x = b & 7, k = b >> 3;
-  return x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
-
 // list entries a lane loads together (indices, then positions)
 constexpr int kPairBatch = 4;
 
