@@ -465,6 +465,17 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
   // inside MD/BD steps (skin) k_move has already updated the cell-ordered
   // copy, and the pair launch takes the skin test's block candidates
   mfe::launch_pair(g, energy, !skin, skin);
+  // with SPME as the last force stage and no VelVer kick, the gather
+  // launch also finalizes each particle (forceo only) and reduces the
+  // partials.  With the kick the finalize phase reads and writes velocities
+  // and accelerations too, scattered in brick order: a separate launch in
+  // particle order is faster then.
+  const bool gather_fin = g.ewald == MFE_EWALD_SPME && !g.chains && !g.surf && mode == 0;
+  if (gather_fin) {
+    const mfe::FinArgs fa{energy ? 1 : 0, skin ? 1 : 0, bd ? 1 : 0, mode, dt};
+    mfe::launch_spme(g, energy, SP_BIN, SP_GATHER, false, &fa);
+    return;
+  }
   if (g.ewald == MFE_EWALD_SPME) mfe::launch_spme(g, energy, SP_BIN, SP_GATHER);
   else if (g.ewald == MFE_EWALD_STD) mfe::launch_std_ewald(g);
   if (g.chains) mfe::launch_bonds(g);

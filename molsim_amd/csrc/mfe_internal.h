@@ -445,8 +445,15 @@ void launch_nlist_rebuild(const Group &g, bool gated);
 // skin: the pair launch also computes the skin test's block candidates
 // (MD/BD steps, after k_move), replacing launch_skin
 void launch_pair(const Group &g, bool energy, bool sort_pos = true, bool skin = false);
+// What the gather launch also does when it is the last force stage (SPME
+// without bonds or surface term): each particle's finalize phase right
+// after its force, and the reductions in one extra block.
+struct FinArgs {
+  int energy, skin, bd, mode;
+  double dt;
+};
 void launch_spme(const Group &g, bool energy, int first_stage, int last_stage,
-                 bool potential = false);
+                 bool potential = false, const FinArgs *fin = nullptr);
 // adjoint passes: real-space pair sums, per-particle dE/dq, column sums
 void launch_adjoint(const Group &g, int ncol, double *out);
 void launch_std_ewald(const Group &g);
@@ -474,6 +481,8 @@ template <typename R, int O>
 const void *k_gather_ptr();
 template <typename R, int O>
 const void *k_gather_pot_ptr();
+template <typename R, int O>
+const void *k_gather_fin_ptr();
 
 // FFT passes (mfe_fft.hip)
 // LDS of the one-block-per-plane inverse y+x transform (mixed precision),

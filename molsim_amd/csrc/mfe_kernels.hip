@@ -874,9 +874,12 @@ __device__ __forceinline__ void mesh_coords(const double *r, const SpmeGeom &g,
   }
 }
 
-// Brick of every particle and its slot within the brick.  A block takes
-// 1024 consecutive particles of the cell order -- a compact piece of the
-// box, spanning a few dozen bricks -- counts them in an LDS histogram and
+// Brick of every particle and its slot within the brick, both indexed by
+// the particle's place t in the cell order, from the cell-ordered positions
+// (current whenever SPME runs: k_move or the pair launch refreshes them).
+// A block takes 1024 consecutive particles of the cell order -- a compact
+// piece of the box, spanning a few dozen bricks -- counts them in an LDS
+// histogram and
 // then adds each non-empty bin to the global count with one atomic, so
 // global atomics scale with the bricks a block touches, not with its
 // particles.  Meshes with more bricks than the histogram holds fall back to
@@ -889,25 +892,23 @@ __global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__re
   const int np = V.np;
   const int nblk = (np + kCountThreads - 1) / kCountThreads;
   if ((int)blockIdx.x >= nblk) return;
-  const double *__restrict__ ro = V.ro;
-  const int *__restrict__ sorted = V.c_sorted;
+  const double *__restrict__ ro_s = V.ro_s;
   const SpmeGeom g = V.sg;
   int *bcount = V.b_count, *bid = V.b_id, *bslot = V.b_slot;
   __shared__ int hist[kCountBins];
   const int t = blockIdx.x * kCountThreads + threadIdx.x;
   const bool act = t < np;
-  int i = 0, id = 0;
+  int id = 0;
   if (act) {
-    i = sorted[t];
     int mm[3];
     double fr[3];
-    mesh_coords(ro + 3 * i, g, mm, fr);
+    mesh_coords(ro_s + 3 * t, g, mm, fr);
     id = fdiv(mm[0], g.fB[0]) + g.nb[0] * (fdiv(mm[1], g.fB[1]) + g.nb[1] * fdiv(mm[2], g.fB[2]));
   }
   if (V.nbricks > kCountBins) {
     if (act) {
-      bid[i] = id;
-      bslot[i] = atomicAdd(&bcount[id], 1);
+      bid[t] = id;
+      bslot[t] = atomicAdd(&bcount[id], 1);
     }
   } else {
     for (int k = threadIdx.x; k < V.nbricks; k += kCountThreads) hist[k] = 0;
@@ -921,8 +922,8 @@ __global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__re
     }
     __syncthreads();
     if (act) {
-      bid[i] = id;
-      bslot[i] = hist[id] + rank;
+      bid[t] = id;
+      bslot[t] = hist[id] + rank;
     }
   }
   // The last block to finish scans the counts into b_start and clears them
@@ -943,22 +944,24 @@ __global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__re
 }
 
 // particle records in brick order: fraction (x, y, z) and charge, plus the
-// packed meshmax (10 bits per axis) and the particle index
+// packed meshmax (10 bits per axis) and the particle index; one thread per
+// place t of the cell order
 template <typename R>
 __global__ void k_spme_scatter(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
-  const double *__restrict__ ro = V.ro;
+  const double *__restrict__ ro_s = V.ro_s;
   const double *__restrict__ q = V.q;
   const SpmeGeom g = V.sg;
   const int *bid = V.b_id, *bslot = V.b_slot, *bstart = V.b_start;
   R *frac = (R *)V.rec_frac;
   int2 *meta = V.rec_meta;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= V.np) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= V.np) return;
+  const int i = V.c_sorted[t];
   int mm[3];
   double fr[3];
-  mesh_coords(ro + 3 * i, g, mm, fr);
-  const int pos = bstart[bid[i]] + bslot[i];
+  mesh_coords(ro_s + 3 * t, g, mm, fr);
+  const int pos = bstart[bid[t]] + bslot[t];
   frac[4 * pos] = (R)fr[0];
   frac[4 * pos + 1] = (R)fr[1];
   frac[4 * pos + 2] = (R)fr[2];
@@ -1123,9 +1126,28 @@ struct Acc3<double> {
 // its next particle's record while it sums the current one.
 constexpr int kGatherLanes = 512;
 
-template <typename R, int ORDER, bool POT>
-__global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restrict__ sys) {
+// the finalize phase split in its loads and its arithmetic and stores, so
+// a kernel can issue the loads early; and the reductions (defined below)
+struct FinState {
+  double aold[3], v[3], q, mi;
+};
+__device__ __forceinline__ void fin_load(const SysDev &V, int i, int mode, FinState &st);
+__device__ __forceinline__ void fin_apply(const SysDev &V, int i, const double f[3],
+                                         const FinState &st, int mode, double dt,
+                                         const double es[3]);
+__device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin, int bd);
+
+// FIN: also the finalize phase of each particle (its owner lane, after the
+// force) and, in block nbricks, the reductions, whose inputs come from
+// earlier launches.
+template <typename R, int ORDER, bool POT, bool FIN>
+__global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restrict__ sys,
+                                                         FinArgs fa) {
   const SysDev &V = sys[blockIdx.z];
+  if (FIN && (int)blockIdx.x == V.nbricks) {
+    ph_reduce(V, fa.energy, fa.skin, fa.bd);
+    return;
+  }
   if ((int)blockIdx.x >= V.nbricks) return;
   const R *__restrict__ frac = (const R *)V.rec_frac;
   const int2 *__restrict__ meta = V.rec_meta;
@@ -1195,10 +1217,12 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
     const bool owner = l == 0;
     const int idx = md.y;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+    FinState st;
     if (owner) {
       f0 = force[3 * idx];
       f1 = force[3 * idx + 1];
       f2 = force[3 * idx + 2];
+      if (FIN) fin_load(V, idx, fa.mode, st);
     }
     const int pn = p + kGStep;
     Rec4<R> fn = f;
@@ -1271,11 +1295,17 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
     }
     if (owner) {
       const double qd = (double)f.q;
-      force[3 * idx] = f0 - qd * ex * g.dkdr[0] * efp;
-      force[3 * idx + 1] = f1 - qd * ey * g.dkdr[1] * efp;
-      force[3 * idx + 2] = f2 - qd * ez * g.dkdr[2] * efp;
+      const double ft[3] = {f0 - qd * ex * g.dkdr[0] * efp, f1 - qd * ey * g.dkdr[1] * efp,
+                            f2 - qd * ez * g.dkdr[2] * efp};
+      force[3 * idx] = ft[0];
+      force[3 * idx + 1] = ft[1];
+      force[3 * idx + 2] = ft[2];
       // dE_rec/dq = efp * sum_stencil theta * (G * Q) (the convolved mesh)
       if (POT) V.phi_rec[idx] = efp * ep;
+      if (FIN) {
+        const double es[3] = {0.0, 0.0, 0.0};
+        fin_apply(V, idx, ft, st, fa.mode, fa.dt, es);
+      }
     }
     f = fn;
     md = mn;
@@ -1468,7 +1498,7 @@ __device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin,
   // component layout: [0,P) pair energies, P pair virial, P+1..P+2 rec
   // (u, vir), P+3..P+4 bond (u, vir), P+5..P+7 S, P+8 max r2, P+9 cand
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int c = wave; c < nc; c += kReduceThreads / 64) {
+  for (int c = wave; c < nc; c += (int)(blockDim.x >> 6)) {
     const double *src;
     int n;
     bool mx = false;
@@ -1879,40 +1909,50 @@ __global__ void __launch_bounds__(kBlock) k_skin_blocks(const SysDev *__restrict
 // velocity half of VelVer (Src/md.F90:348-358):
 //   mode 0: forceo = force*fac        mode 1: + VelVer kick
 // one particle of k_finalize; es = the reduced surface field en[5..7]
-__device__ __forceinline__ void ph_finalize(const SysDev &V, int i, int mode, double dt,
-                                            const double es[3]) {
+__device__ __forceinline__ void fin_load(const SysDev &V, int i, int mode, FinState &st) {
+  if (mode == 1) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      st.aold[m] = V.rodd[3 * i + m];
+      st.v[m] = V.rod[3 * i + m];
+    }
+  }
+  st.q = V.lsurf ? V.q[i] : 0.0;
+  st.mi = mode == 1 ? V.massip[i] : 0.0;
+}
+
+__device__ __forceinline__ void fin_apply(const SysDev &V, int i, const double f[3],
+                                         const FinState &st, int mode, double dt,
+                                         const double es[3]) {
 #pragma clang fp contract(off)
-  const double *__restrict__ force = V.force;
   double *__restrict__ forceo = V.forceo;
   const double ffac = V.ffac, surf_fac = V.surf_fac;
   const int surf = V.lsurf;
   double *__restrict__ rod = V.rod;
   double *__restrict__ rodd = V.rodd;
-  // every load before the first store
-  double f[3], aold[3], v[3];
-#pragma unroll
-  for (int m = 0; m < 3; ++m) f[m] = force[3 * i + m];
-  if (mode == 1) {
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      aold[m] = rodd[3 * i + m];
-      v[m] = rod[3 * i + m];
-    }
-  }
-  const double qi = surf ? V.q[i] : 0.0;
-  const double mi = mode == 1 ? V.massip[i] : 0.0;
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     double fm = f[m];
-    if (surf) fm = fm - surf_fac * qi * es[m];
+    if (surf) fm = fm - surf_fac * st.q * es[m];
     const double fo = fm * ffac;
     forceo[3 * i + m] = fo;
     if (mode == 1) {
-      const double anew = mi * fo;
+      const double anew = st.mi * fo;
       rodd[3 * i + m] = anew;
-      rod[3 * i + m] = v[m] + 0.5 * dt * (anew + aold[m]);
+      rod[3 * i + m] = st.v[m] + 0.5 * dt * (anew + st.aold[m]);
     }
   }
+}
+
+// every load before the first store
+__device__ __forceinline__ void ph_finalize(const SysDev &V, int i, int mode, double dt,
+                                            const double es[3]) {
+  double f[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) f[m] = V.force[3 * i + m];
+  FinState st;
+  fin_load(V, i, mode, st);
+  fin_apply(V, i, f, st, mode, dt, es);
 }
 
 __global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ sys,
@@ -1957,11 +1997,15 @@ const void *k_spread_ptr() {
 }
 template <typename R, int O>
 const void *k_gather_ptr() {
-  return (const void *)&k_gather<R, O, false>;
+  return (const void *)&k_gather<R, O, false, false>;
 }
 template <typename R, int O>
 const void *k_gather_pot_ptr() {
-  return (const void *)&k_gather<R, O, true>;
+  return (const void *)&k_gather<R, O, true, false>;
+}
+template <typename R, int O>
+const void *k_gather_fin_ptr() {
+  return (const void *)&k_gather<R, O, false, true>;
 }
 
 void philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
@@ -2020,7 +2064,8 @@ void launch_pair(const Group &g, bool energy, bool sort_pos, bool skin) {
 }
 
 template <typename R>
-static void spme_stages(const Group &g, bool energy, int first, int last, bool pot) {
+static void spme_stages(const Group &g, bool energy, int first, int last, bool pot,
+                        const FinArgs *fin) {
   for (int st = first; st <= last; ++st) {
     switch (st) {
       case SP_BIN:
@@ -2048,28 +2093,34 @@ static void spme_stages(const Group &g, bool energy, int first, int last, bool p
       case SP_CONV: fft_conv(g, energy); break;
       case SP_IFFTY: fft_y(g, 1); break;
       case SP_IFFTX: fft_x_inverse(g); break;
-      case SP_GATHER:
+      case SP_GATHER: {
+        const FinArgs fa = fin ? *fin : FinArgs{0, 0, 0, 0, 0.0};
         switch (g.order) {
-#define MFE_GATHER(O)                                                                     \
-  case O:                                                                                 \
-    if (pot)                                                                              \
-      hipLaunchKernelGGL((k_gather<R, O, true>), grid(g.bricks, g), dim3(kGatherLanes),   \
-                         g.lds_gather, g.stream, g.d);                                    \
-    else                                                                                  \
-      hipLaunchKernelGGL((k_gather<R, O, false>), grid(g.bricks, g), dim3(kGatherLanes),  \
-                         g.lds_gather, g.stream, g.d);                                    \
+#define MFE_GATHER(O)                                                                   \
+  case O:                                                                               \
+    if (pot)                                                                            \
+      hipLaunchKernelGGL((k_gather<R, O, true, false>), grid(g.bricks, g),              \
+                         dim3(kGatherLanes), g.lds_gather, g.stream, g.d, fa);          \
+    else if (fin)                                                                       \
+      hipLaunchKernelGGL((k_gather<R, O, false, true>), grid(g.bricks + 1, g),          \
+                         dim3(kGatherLanes), g.lds_gather, g.stream, g.d, fa);          \
+    else                                                                                \
+      hipLaunchKernelGGL((k_gather<R, O, false, false>), grid(g.bricks, g),             \
+                         dim3(kGatherLanes), g.lds_gather, g.stream, g.d, fa);          \
     break;
           MFE_GATHER(3) MFE_GATHER(4) MFE_GATHER(5) MFE_GATHER(6) MFE_GATHER(7) MFE_GATHER(8)
 #undef MFE_GATHER
         }
         break;
+      }
     }
   }
 }
 
-void launch_spme(const Group &g, bool energy, int first, int last, bool pot) {
-  if (g.prec == MFE_PREC_MIXED) spme_stages<float>(g, energy, first, last, pot);
-  else spme_stages<double>(g, energy, first, last, pot);
+void launch_spme(const Group &g, bool energy, int first, int last, bool pot,
+                 const FinArgs *fin) {
+  if (g.prec == MFE_PREC_MIXED) spme_stages<float>(g, energy, first, last, pot, fin);
+  else spme_stages<double>(g, energy, first, last, pot, fin);
 }
 
 void launch_std_ewald(const Group &g) {
@@ -2148,6 +2199,8 @@ void mfe_allow_big_spread_lds(int prec, int bytes) {
   (void)hipFuncSetAttribute((const void *)mfe::k_gather_ptr<R, O>(),          \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes); \
   (void)hipFuncSetAttribute((const void *)mfe::k_gather_pot_ptr<R, O>(),      \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes); \
+  (void)hipFuncSetAttribute((const void *)mfe::k_gather_fin_ptr<R, O>(),      \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (prec == MFE_PREC_MIXED) {
     MFE_ATTR(float, 3) MFE_ATTR(float, 4) MFE_ATTR(float, 5)
