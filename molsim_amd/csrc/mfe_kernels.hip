@@ -1969,20 +1969,23 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ 
 }
 
 // Without the surface term the reduction and k_finalize are independent:
-// one launch in which blocks [0, F) finalize kReduceThreads particles each
-// and block F (F of the replica) reduces the partials.
+// one launch in which block 0 reduces the partials (dispatched first, so
+// its chain of loads overlaps the rest) and blocks [1, F] finalize
+// kReduceThreads particles each (F of the replica).
 __global__ void __launch_bounds__(kReduceThreads) k_reduce_finalize(const SysDev *__restrict__ sys,
                                                                     int energy, int skin, int bd,
                                                                     int mode, double dt) {
   const SysDev &V = sys[blockIdx.z];
   const int F = (V.np + kReduceThreads - 1) / kReduceThreads;
-  if ((int)blockIdx.x < F) {
-    const int i = blockIdx.x * kReduceThreads + threadIdx.x;
-    const double es[3] = {0.0, 0.0, 0.0};
-    if (i < V.np) ph_finalize(V, i, mode, dt, es);
+  if (blockIdx.x == 0) {
+    ph_reduce(V, energy, skin, bd);
     return;
   }
-  if ((int)blockIdx.x == F) ph_reduce(V, energy, skin, bd);
+  if ((int)blockIdx.x <= F) {
+    const int i = ((int)blockIdx.x - 1) * kReduceThreads + threadIdx.x;
+    const double es[3] = {0.0, 0.0, 0.0};
+    if (i < V.np) ph_finalize(V, i, mode, dt, es);
+  }
 }
 
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }

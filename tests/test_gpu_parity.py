@@ -291,3 +291,19 @@ def test_odd_mesh_is_rejected():
         eng.set_ewald("spm", s.alpha, 4, (33, 32, 32))
     assert ei.value.code == 5
     eng.close()
+
+
+def test_charge_out_of_range_is_rejected():
+    # the mixed spreading accumulates in 2^-32 fixed point; charges are
+    # bounded (|q| < 2^16) and must be finite
+    s = S.c2()
+    eng = E.ForceEngine(s.np, "mixed")
+    eng.set_box(s.boxlen)
+    for bad in (70000.0, float("nan")):
+        az = np.array(s.az, dtype=np.float64)
+        az[3] = bad
+        with pytest.raises(E.MfeError) as ei:
+            eng.set_particles(s.iptpn, az, s.massipt)
+        assert E.ERRORS[ei.value.code] == "MFE_E_BAD_PARAM"
+    eng.set_particles(s.iptpn, s.az, s.massipt)
+    eng.close()
