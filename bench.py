@@ -177,6 +177,17 @@ def main():
         stages = {}
         npairs = 0
         host_rate = None
+        # Monte Carlo trial moves of one particle (DUTwoBodyEwaldRecSPM): host
+        # call to returned energy change, the mesh spread and transformed
+        # twice per trial
+        ro_h, _, _ = eng.download_state()
+        rng = np.random.default_rng(5)
+        eng.mc_delta_rec([1], ro_h[:1] + 1.0)
+        t1 = time.perf_counter()
+        for _ in range(20):
+            ip = int(rng.integers(sysm.np))
+            eng.mc_delta_rec([ip + 1], ro_h[ip:ip + 1] + rng.uniform(-1, 1, (1, 3)))
+        mc_us = 1e6 * (time.perf_counter() - t1) / 20
         if prec == "mixed":
             # UTotal with the host's buffers: positions in, forces out over
             # PCIe each call (mfe_upload_positions + mfe_compute)
@@ -196,7 +207,7 @@ def main():
             npairs = 2 * int(nn.sum())
         eng.close()
         return dict(sps=args.steps / dt, ms=1000 * dt / args.steps, f0=f0, e0=e0,
-                    en=en, stages=stages, npairs=npairs, host_rate=host_rate)
+                    en=en, stages=stages, npairs=npairs, host_rate=host_rate, mc_us=mc_us)
 
     mixed = run("mixed")
     fp64 = None if args.no_fp64 else run("fp64")
@@ -240,6 +251,8 @@ def main():
                      "traffic": measured_traffic(dom, args.units)},
         "stages": stage_report,
         "utotal_host_buffers_per_s": round(mixed["host_rate"], 1),
+        "mc_trial_move_us": {"mixed": round(mixed["mc_us"], 1),
+                             "fp64": round(fp64["mc_us"], 1) if fp64 else None},
     }
     if fp64 is not None:
         f_m, f_d = mixed["f0"], fp64["f0"]

@@ -158,6 +158,17 @@ int mfe_build_list(mfe_engine *e);
    current list: forceo(3,np) (may be NULL) and energies (may be NULL). */
 int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo,
                 mfe_energy *energy);
+/* Monte Carlo trial move, DUTwoBodyEwaldRecSPM (Src/denergy.F90:1413-1495,
+   reached from DUTwoBodyEwald, Src/denergy.F90:1258-1260): the change of
+   the SPME reciprocal energy when the natm distinct particles ianatm[]
+   (1-based) move from their current positions to rtm(3,natm), in the
+   units of mfe_energy.rec (EpsiFourPi included; the self term does not
+   change).  Nothing of the engine's state changes (the next force call
+   spreads the mesh again).  FP64 engines give the difference to ~1e-12 of
+   |E_rec|; mixed ones subtract two FP32-mesh energies and only reach ~1e-6
+   of |E_rec|.  SPME only (MFE_E_STATE otherwise). */
+int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm,
+                     const double *rtm, double *du_rec);
 /* nsteps of MDDriver(iSimulationStep) + NList(iSimulationStep): VelVer
    (Src/md.F90:252-410) with the skin-test rebuild (Src/nlist.F90:597-616),
    all on the device.  flags select which UTotal terms are reduced each

@@ -142,6 +142,7 @@ void free_all(mfe_engine *e) {
   dfree(e->mesh_r); dfree(e->mesh_c); dfree(e->b_count); dfree(e->b_start); dfree(e->b_done);
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
   dfree(e->spread_tiles);
+  dfree(e->mc_ia); dfree(e->mc_rtm); dfree(e->mc_out);
   dfree(e->ax_g); dfree(e->ax_kk);
   for (auto &t : e->tw) dfree(t);
   dfree(e->tw_half);
@@ -1090,6 +1091,66 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo, mfe_energy *energ
     int r = read_flags(e);
     if (r) return r;
     if (energy) energy_out(e, energy);
+    return MFE_OK;
+  });
+}
+
+// DUTwoBodyEwaldRecSPM (Src/denergy.F90:1413-1495) for a trial move of
+// natm particles: the current configuration is binned and spread, its
+// forward transforms and convolution give E_rec; the moved particles'
+// stencils are then exchanged in the spreading tiles and the same passes
+// give E_rec of the trial mesh.  du = EpsiFourPi (E_trial - E_current), the
+// units of mfe_energy.rec.  Positions, forces and the list are unchanged;
+// the next force call spreads the mesh again.
+int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const double *rtm,
+                     double *du_rec) {
+  if (!e || !ianatm || !rtm || !du_rec || natm < 1) return MFE_E_BAD_PARAM;
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (!e->have_ewald || e->ewald != MFE_EWALD_SPME)
+    return fail(e, MFE_E_STATE, "mfe_set_ewald with SPME first");
+  if (natm > e->np) return fail(e, MFE_E_BAD_PARAM, "natm > np");
+  std::vector<int> ia(natm);
+  {
+    std::vector<char> seen(e->np, 0);
+    for (int a = 0; a < natm; ++a) {
+      const int i = ianatm[a] - 1;
+      if (i < 0 || i >= e->np) return fail(e, MFE_E_BAD_PARAM, "ianatm out of range");
+      if (seen[i]) return fail(e, MFE_E_BAD_PARAM, "ianatm repeats a particle");
+      seen[i] = 1;
+      ia[a] = i;
+      for (int m = 0; m < 3; ++m)
+        if (!std::isfinite(rtm[3 * a + m])) return fail(e, MFE_E_BAD_PARAM, "rtm not finite");
+    }
+  }
+  if (!e->list_valid) {
+    rc = mfe_build_list(e);
+    if (rc) return rc;
+  }
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    if (e->mc_cap < natm) {
+      dalloc(e->mc_ia, (size_t)natm);
+      dalloc(e->mc_rtm, 3 * (size_t)natm);
+      e->mc_cap = natm;
+    }
+    if (!e->mc_out) dalloc(e->mc_out, 2);
+    upload(e->mc_ia, ia.data(), (size_t)natm, e->stream);
+    upload(e->mc_rtm, rtm, 3 * (size_t)natm, e->stream);
+    sync_view(e);
+    const mfe::Group &G = e->grp;
+    // current mesh: binning reads the cell-ordered positions
+    mfe::launch_sort_pos(G);
+    mfe::launch_spme(G, true, SP_BIN, SP_CONV);
+    mfe::launch_rec_sum(G, e->mc_out);
+    // trial mesh
+    mfe::launch_mc_stencils(G, natm, e->mc_ia, e->mc_rtm);
+    mfe::launch_spme(G, true, SP_FFTX, SP_CONV);
+    mfe::launch_rec_sum(G, e->mc_out + 1);
+    double h[2];
+    MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    *du_rec = e->efp * (h[1] - h[0]);
     return MFE_OK;
   });
 }

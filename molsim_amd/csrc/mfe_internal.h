@@ -373,6 +373,11 @@ struct mfe_engine {
   void *mesh_r = nullptr;    // real mesh [s3][s2][s1]
   void *mesh_c = nullptr;    // complex mesh [s3][s2][hx]
   int *b_count = nullptr, *b_start = nullptr, *b_id = nullptr, *b_slot = nullptr;
+  // Monte Carlo trial moves: moved particles, their trial positions, and the
+  // two reciprocal energies (current, trial)
+  int *mc_ia = nullptr;
+  double *mc_rtm = nullptr, *mc_out = nullptr;
+  int mc_cap = 0;
   int *b_done = nullptr;
   void *rec_frac = nullptr;  // per sorted particle: (fx, fy, fz, q)
   void *spread_tiles = nullptr;  // per brick: extended spreading tile
@@ -465,6 +470,13 @@ void launch_finalize(const Group &g, int mode, double dt);
 void launch_reduce_finalize(const Group &g, unsigned flags, bool skin, bool bd, int mode,
                             double dt);
 void launch_move(const Group &g, double dt, int bd);
+// the cell-ordered positions refreshed from the particle-ordered ones
+void launch_sort_pos(const Group &g);
+// Monte Carlo trial move: the natm stencils at the current positions of
+// particles ia[] taken out of the spreading tiles and those at rtm[] added
+void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm);
+// the reciprocal energy of the last convolution pass (its block partials)
+void launch_rec_sum(const Group &g, double *out);
 void launch_skin(const Group &g);
 void launch_set_positions(const Group &g, const double *ro_new);
 void launch_collect(const Group &g, int *flags, double *en, int nen);

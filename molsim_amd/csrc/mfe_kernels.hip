@@ -1312,6 +1312,66 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
   }
 }
 
+// ------------------------------------------------------- MC trial move
+// DUTwoBodyEwaldRecSPM (Src/denergy.F90:1413-1495): the generalized charge
+// mesh of a trial move is the current one less the moved particles' old
+// stencils plus their new ones (DUMakeGQMesh, :1432-1460).  The current
+// mesh is held as the spreading tiles, and each stencil point lies in the
+// extended tile of the brick of its particle's meshmax, the tile the
+// spreading adds it to.  One block: the stencils one after another (old,
+// then new, per moved particle), the points of one stencil over the
+// threads, so every add happens in a fixed order.  Values as the
+// spreading forms them: fraction and charge rounded to R, then
+// q w_z, times w_y, times w_x.
+template <typename R, int ORDER>
+__global__ void __launch_bounds__(256) k_mc_stencils(const SysDev *__restrict__ sys, int natm,
+                                                    const int *__restrict__ ia,
+                                                    const double *__restrict__ rtm) {
+  const SysDev &V = sys[blockIdx.z];
+  const SpmeGeom g = V.sg;
+  R *__restrict__ tiles = (R *)V.spread_tiles;
+  const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
+  const size_t ntile = (size_t)E0 * E1 * E2;
+  constexpr int NP = ORDER * ORDER * ORDER;
+  __shared__ R w[3][ORDER];
+  for (int a = 0; a < natm; ++a) {
+    const int i = ia[a];
+    const R q = (R)V.q[i];
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const double *r = sgn ? rtm + 3 * a : V.ro + 3 * i;
+      int mm[3];
+      double fr[3];
+      mesh_coords(r, g, mm, fr);
+      if (threadIdx.x < 3) bspline<R, ORDER>((R)fr[threadIdx.x], w[threadIdx.x], nullptr);
+      __syncthreads();
+      const int b0 = mm[0] / g.B[0], b1 = mm[1] / g.B[1], b2 = mm[2] / g.B[2];
+      const int lx = mm[0] - b0 * g.B[0] + ORDER - 1;
+      const int ly = mm[1] - b1 * g.B[1] + ORDER - 1;
+      const int lz = mm[2] - b2 * g.B[2] + ORDER - 1;
+      R *tile = tiles + (size_t)(b0 + g.nb[0] * (b1 + g.nb[1] * b2)) * ntile;
+      for (int k = threadIdx.x; k < NP; k += blockDim.x) {
+        const int j = k % ORDER, jy = (k / ORDER) % ORDER, iz = k / (ORDER * ORDER);
+        const R v = ((q * w[2][iz]) * w[1][jy]) * w[0][j];
+        R &t = tile[((size_t)(lz - iz) * E1 + (ly - jy)) * E0 + (lx - j)];
+        t = sgn ? t + v : t - v;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// the reciprocal energy sum of the last convolution pass: its block
+// partials in a fixed order (one wavefront, shuffle tree)
+__global__ void __launch_bounds__(64) k_rec_sum(const SysDev *__restrict__ sys,
+                                                double *__restrict__ out) {
+  const SysDev &V = sys[blockIdx.z];
+  const double *src = V.part + V.off_conv;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < V.nblk_conv; b += 64) acc += src[b];
+  for (int w = 32; w > 0; w >>= 1) acc += __shfl_down(acc, w, 64);
+  if (threadIdx.x == 0) out[blockIdx.z] = acc;
+}
+
 // --------------------------------------------------------- standard Ewald
 // UEwaldRecStd (Src/energy.F90:1050-1096) over the k-vectors of its kfac
 // table: one block per k-vector sums the four sign combinations of the
@@ -2039,6 +2099,30 @@ void launch_nlist_rebuild(const Group &g, bool gated) {
   hipLaunchKernelGGL(k_cell_scatter, grid(gb, g), dim3(kBlock), 0, g.stream, g.d, gt);
   hipLaunchKernelGGL(k_cell_sort, grid(gc, g), dim3(kBlock), 0, g.stream, g.d, gt);
   hipLaunchKernelGGL(k_nlist_build, grid(gb, g), dim3(kBlock), 0, g.stream, g.d, gt);
+}
+
+void launch_sort_pos(const Group &g) {
+  hipLaunchKernelGGL(k_sort_pos, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
+}
+
+void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm) {
+#define MFE_MC(R, O)                                                                       \
+  case O:                                                                                  \
+    hipLaunchKernelGGL((k_mc_stencils<R, O>), grid(1, g), dim3(256), 0, g.stream, g.d, natm, \
+                       ia, rtm);                                                           \
+    break;
+  if (g.prec == MFE_PREC_MIXED) {
+    switch (g.order) { MFE_MC(float, 3) MFE_MC(float, 4) MFE_MC(float, 5) MFE_MC(float, 6)
+                       MFE_MC(float, 7) MFE_MC(float, 8) }
+  } else {
+    switch (g.order) { MFE_MC(double, 3) MFE_MC(double, 4) MFE_MC(double, 5) MFE_MC(double, 6)
+                       MFE_MC(double, 7) MFE_MC(double, 8) }
+  }
+#undef MFE_MC
+}
+
+void launch_rec_sum(const Group &g, double *out) {
+  hipLaunchKernelGGL(k_rec_sum, grid(1, g), dim3(64), 0, g.stream, g.d, out);
 }
 
 void launch_pair(const Group &g, bool energy, bool sort_pos, bool skin) {

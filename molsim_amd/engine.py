@@ -76,7 +76,7 @@ SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_batch_create", "mfe_batch_destroy", "mfe_batch_last_error",
            "mfe_batch_failed_replica", "mfe_batch_build_list", "mfe_batch_compute",
            "mfe_batch_step_md", "mfe_batch_step_bd", "mfe_batch_get_energies",
-           "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint"]
+           "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint", "mfe_mc_delta_rec"]
 
 _lib = None
 
@@ -109,6 +109,7 @@ def lib():
         L.mfe_download_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.mfe_build_list.argtypes = [C.c_void_p]
         L.mfe_compute.argtypes = [C.c_void_p, C.c_uint32, _dp, C.POINTER(EnergyStruct)]
+        L.mfe_mc_delta_rec.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp]
         L.mfe_step_md.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_uint32]
         L.mfe_step_bd.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, C.c_uint32]
         L.mfe_get_energy.argtypes = [C.c_void_p, C.POINTER(EnergyStruct)]
@@ -330,6 +331,18 @@ class ForceEngine:
         e = EnergyStruct()
         self._check(self.L.mfe_compute(self.h, flags, _d(f), C.byref(e)), "mfe_compute")
         return f.reshape(-1, 3), self._energies(e)
+
+    def mc_delta_rec(self, ianatm, rtm):
+        """DUTwoBodyEwaldRecSPM: change of the SPME reciprocal energy (units of
+        Energies.rec) when particles ianatm (1-based) move to rtm[natm, 3]."""
+        ia = _i32(np.atleast_1d(ianatm))
+        r = _f64(np.asarray(rtm, dtype=np.float64).reshape(-1))
+        if r.size != 3 * ia.size:
+            raise ValueError("rtm must hold three coordinates per moved particle")
+        du = np.zeros(1)
+        self._check(self.L.mfe_mc_delta_rec(self.h, ia.size, _i(ia), _d(r), _d(du)),
+                    "mfe_mc_delta_rec")
+        return float(du[0])
 
     def step_md(self, nsteps, tstep, flags=FORCES | ENERGY | VIRIAL):
         self._check(self.L.mfe_step_md(self.h, int(nsteps), float(tstep), flags), "mfe_step_md")

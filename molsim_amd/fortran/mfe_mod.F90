@@ -245,6 +245,18 @@ module mfe_mod
          real(c_double), intent(out) :: energy
       end function
 
+      ! DUTwoBodyEwaldRecSPM for a trial move of natm particles ianatm
+      ! (1-based) to rtm(3,natm): du = change of u%rec
+      integer(c_int) function mfe_mc_delta_rec(eng, natm, ianatm, rtm, du) &
+            bind(c, name='mfe_mc_delta_rec')
+         import :: c_int, c_int32_t, c_ptr, c_double
+         type(c_ptr), value :: eng
+         integer(c_int32_t), value :: natm
+         integer(c_int32_t), intent(in) :: ianatm(*)
+         real(c_double), intent(in) :: rtm(*)
+         real(c_double), intent(out) :: du
+      end function
+
       ! batched replicas: engines(nrep) are the handles of set-up engines
       integer(c_int) function mfe_batch_create(engines, nrep, batch) bind(c, name='mfe_batch_create')
          import :: c_int, c_int32_t, c_ptr

@@ -1,0 +1,75 @@
+"""Monte Carlo trial moves: the SPME reciprocal-energy change of moving
+some particles (DUTwoBodyEwaldRecSPM, Src/denergy.F90:1413-1495) against
+the oracle's reciprocal energy (UEwaldRecSPM restated) of both
+configurations.
+
+Tolerances: FP64 |du - du_oracle| <= 1e-9 |E_rec| (each energy agrees to
+~1e-11 relative); mixed <= 1e-5 |E_rec| (two FP32-mesh energies are
+subtracted).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from molsim_amd import engine as E
+from molsim_amd import systems as S
+
+from helpers import efp, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_rec(s, ro):
+    setup = O.SpmeSetup(s.boxlen, s.alpha, s.order, (s.mesh,) * 3)
+    _, u, _ = O.spme_rec(setup, ro, s.az, efp())
+    return efp() * u
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-9), ("mixed", 1e-5)])
+def test_mc_delta_rec_vs_oracle(prec, tol):
+    s = S.c2()
+    print("case: MC trial moves c2", prec, flush=True)
+    eng = S.configure(s, prec)
+    f_before, en_before = eng.utotal()
+    e0 = oracle_rec(s, s.ro)
+    rng = np.random.default_rng(11)
+    for natm in (1, 3, 8):
+        ia = rng.choice(s.np, natm, replace=False)
+        rtm = s.ro[ia] + rng.uniform(-2.5, 2.5, (natm, 3))
+        du = eng.mc_delta_rec(ia + 1, rtm)
+        ro1 = s.ro.copy()
+        ro1[ia] = rtm
+        du0 = oracle_rec(s, ro1) - e0
+        print(f"  natm {natm}: du {du:.10e} oracle {du0:.10e}", flush=True)
+        assert abs(du - du0) <= tol * abs(e0)
+    # a particle moved across the box edge (the mesh is periodic)
+    rtm = s.ro[[9]].copy()
+    rtm[0, 0] += 0.5 * s.boxlen
+    du = eng.mc_delta_rec([10], rtm)
+    ro1 = s.ro.copy()
+    ro1[9] = rtm[0]
+    assert abs(du - (oracle_rec(s, ro1) - e0)) <= tol * abs(e0)
+    # no move, no change
+    assert abs(eng.mc_delta_rec([5], s.ro[4:5])) <= tol * abs(e0)
+    # the engine's own state is untouched (bitwise on the mixed path; the
+    # FP64 spreading adds in the order of its LDS atomics, which varies
+    # from run to run at the 1e-16 level)
+    f_after, en_after = eng.utotal()
+    if prec == "mixed":
+        assert np.array_equal(f_after, f_before)
+        assert en_after.tot == en_before.tot
+    else:
+        assert rel_rms(f_after, f_before) <= 1e-13
+        assert abs(en_after.tot - en_before.tot) <= 1e-13 * abs(en_before.tot)
+    eng.close()
+
+
+def test_mc_delta_rec_rejects_bad_moves():
+    s = S.c2()
+    eng = S.configure(s, "fp64")
+    for ia, rtm in (([3, 3], s.ro[[2, 2]]), ([0], s.ro[:1]), ([s.np + 1], s.ro[:1]),
+                    ([4], np.full((1, 3), np.nan))):
+        with pytest.raises(E.MfeError) as ei:
+            eng.mc_delta_rec(ia, rtm)
+        assert E.ERRORS[ei.value.code] == "MFE_E_BAD_PARAM"
+    eng.close()
