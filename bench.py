@@ -92,6 +92,28 @@ def cpu_baseline(sysm, nsteps):
     return nsteps / dt, dt
 
 
+def c5_adjoint(frames):
+    """Force-matching workload size: one 66000-particle frame of the
+    polyelectrolyte system scaled up (SPME 96^3, order 4, rc 18 A), the
+    model energy and its gradients with respect to charges, sigma, epsilon
+    and positions per adjoint call, mixed precision."""
+    s = S.c2(scale=11)
+    efp = E.epsi0_four_pi() / 78.4
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp,
+                         alpha=s.alpha)
+    eng = S.configure(s, "mixed", tables=tab)
+    eng.set_pair_params(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp, alpha=s.alpha)
+    eng.adjoint()
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        eng.adjoint()
+    dt = time.perf_counter() - t0
+    eng.close()
+    return {"n_particles": s.np, "mesh": s.mesh, "order": s.order, "frames": frames,
+            "adjoint_frames_per_s": round(frames / dt, 1),
+            "ms_per_frame": round(1000 * dt / frames, 3)}
+
+
 def c4_batched(nrep, steps, tol, seq_sample=16):
     """Batched BD of nrep replicas (N 500-4000, 0.1-1 M) with the
     autotuner's plan, against the same replicas stepped one engine at a
@@ -150,6 +172,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched-replica BD leg")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 66k-particle adjoint leg")
     ap.add_argument("--c4-replicas", type=int, default=512)
     ap.add_argument("--c4-steps", type=int, default=100)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -263,6 +286,8 @@ def main():
         out["energy_rel_mixed_vs_fp64"] = abs(mixed["e0"].tot - fp64["e0"].tot) / abs(fp64["e0"].tot)
     if not args.no_c4:
         out["c4_batched_bd"] = c4_batched(args.c4_replicas, args.c4_steps, 1e-3)
+    if not args.no_c5:
+        out["c5_adjoint"] = c5_adjoint(10)
     if not args.no_cpu_baseline:
         sps, secs = cpu_baseline(sysm, args.cpu_steps)
         out["cpu_baseline"] = {"value": round(sps, 4), "unit": "steps/s", "cores": 1,
