@@ -173,6 +173,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched-replica BD leg")
     ap.add_argument("--no-c5", action="store_true", help="skip the 66k-particle adjoint leg")
+    ap.add_argument("--drift-steps", type=int, default=5000,
+                    help="VelVer steps after the timed ones over which the total energy "
+                         "drift is reported (0: skip)")
     ap.add_argument("--c4-replicas", type=int, default=512)
     ap.add_argument("--c4-steps", type=int, default=100)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -197,6 +200,23 @@ def main():
         eng.step_md(args.steps, sysm.tstep, flags)
         en = eng.energy()
         dt = time.perf_counter() - t0
+        drift = None
+        if args.drift_steps:
+            # total energy before and after the configuration's run length of
+            # VelVer steps (untimed): potential from the engine, kinetic from
+            # the velocities (1 g/mol A^2/ps^2 = 10 J/mol)
+            mass = sysm.massat[sysm.iptpn - 1][:, None]
+
+            def etot():
+                _, rod, _ = eng.download_state()
+                ekin = 0.5 * float((mass * rod ** 2).sum()) * 1e-2
+                return eng.energy().tot + ekin, ekin
+
+            ea, ekin = etot()
+            eng.step_md(args.drift_steps, sysm.tstep, flags)
+            eb, _ = etot()
+            drift = {"steps": args.drift_steps, "e_tot_start_kj_mol": round(ea, 3),
+                     "e_tot_end_kj_mol": round(eb, 3), "drift_over_ekin": (eb - ea) / ekin}
         stages = {}
         npairs = 0
         host_rate = None
@@ -230,7 +250,8 @@ def main():
             npairs = 2 * int(nn.sum())
         eng.close()
         return dict(sps=args.steps / dt, ms=1000 * dt / args.steps, f0=f0, e0=e0,
-                    en=en, stages=stages, npairs=npairs, host_rate=host_rate, mc_us=mc_us)
+                    en=en, stages=stages, npairs=npairs, host_rate=host_rate, mc_us=mc_us,
+                    drift=drift)
 
     mixed = run("mixed")
     fp64 = None if args.no_fp64 else run("fp64")
@@ -274,6 +295,7 @@ def main():
                      "traffic": measured_traffic(dom, args.units)},
         "stages": stage_report,
         "utotal_host_buffers_per_s": round(mixed["host_rate"], 1),
+        "energy_drift": {"mixed": mixed["drift"], "fp64": fp64["drift"] if fp64 else None},
         "mc_trial_move_us": {"mixed": round(mixed["mc_us"], 1),
                              "fp64": round(fp64["mc_us"], 1) if fp64 else None},
     }

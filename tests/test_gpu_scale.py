@@ -91,7 +91,7 @@ def test_c3_md_steps_conserve_energy(c3):
     ro, rod, _ = em.download_state()
     assert np.isfinite(ro).all() and np.abs(ro).max() <= s.boxlen / 2
     massat = s.massat[s.iptpn - 1]
-    ekin = 0.5 * (massat[:, None] * rod ** 2).sum() * 1e-3 * 1e4  # g/mol A^2/ps^2 -> kJ/mol
+    ekin = 0.5 * (massat[:, None] * rod ** 2).sum() * 1e-2  # g/mol A^2/ps^2 = 10 J/mol
     assert np.isfinite(e1.tot) and ekin > 0
     em.close()
 
