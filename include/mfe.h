@@ -164,11 +164,17 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo,
    (1-based) move from their current positions to rtm(3,natm), in the
    units of mfe_energy.rec (EpsiFourPi included; the self term does not
    change).  Nothing of the engine's state changes (the next force call
-   spreads the mesh again).  FP64 engines give the difference to ~1e-12 of
-   |E_rec|; mixed ones subtract two FP32-mesh energies and only reach ~1e-6
-   of |E_rec|.  SPME only (MFE_E_STATE otherwise). */
+   spreads the mesh again).  The engine keeps the current configuration's
+   mesh and E_rec, as the reference keeps QMesh and urecold: reuse_current
+   = 1 starts from them and skips spreading and transforming the current
+   configuration (the caller states that nothing was accepted since); the
+   engine still spreads anew after any upload, step or setup call, and on
+   its first call.  FP64
+   engines give the difference to ~1e-12 of |E_rec|; mixed ones subtract
+   two FP32-mesh energies and only reach ~1e-6 of |E_rec|.  SPME only
+   (MFE_E_STATE otherwise). */
 int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm,
-                     const double *rtm, double *du_rec);
+                     const double *rtm, int32_t reuse_current, double *du_rec);
 /* nsteps of MDDriver(iSimulationStep) + NList(iSimulationStep): VelVer
    (Src/md.F90:252-410) with the skin-test rebuild (Src/nlist.F90:597-616),
    all on the device.  flags select which UTotal terms are reduced each

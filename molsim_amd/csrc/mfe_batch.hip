@@ -241,6 +241,7 @@ static int run_graph(mfe_batch *b, int kind, int32_t nsteps, double tstep, uint3
 
 int mfe_batch_step_md(mfe_batch *b, int32_t nsteps, double tstep, uint32_t flags) {
   if (!b || nsteps < 0) return MFE_E_BAD_PARAM;
+  for (mfe_engine *e : b->eng) e->mc_cached = false;
   return bguarded(b, [&]() -> int {
     int rc = prepare(b);
     if (rc) return rc;
@@ -250,6 +251,7 @@ int mfe_batch_step_md(mfe_batch *b, int32_t nsteps, double tstep, uint32_t flags
 
 int mfe_batch_step_bd(mfe_batch *b, int32_t nsteps, uint32_t flags) {
   if (!b || nsteps < 0) return MFE_E_BAD_PARAM;
+  for (mfe_engine *e : b->eng) e->mc_cached = false;
   return bguarded(b, [&]() -> int {
     for (size_t r = 0; r < b->eng.size(); ++r)
       if (!b->eng[r]->have_bd) return bfail(b, MFE_E_STATE, "BD factors not set (mfe_set_bd)", (int)r);

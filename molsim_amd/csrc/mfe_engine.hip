@@ -143,6 +143,7 @@ void free_all(mfe_engine *e) {
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
   dfree(e->spread_tiles);
   dfree(e->mc_ia); dfree(e->mc_rtm); dfree(e->mc_out);
+  if (e->mc_tiles) (void)hipFree(e->mc_tiles);
   dfree(e->ax_g); dfree(e->ax_kk);
   for (auto &t : e->tw) dfree(t);
   dfree(e->tw_half);
@@ -430,6 +431,7 @@ void sync_view(mfe_engine *e) {
 void touch(mfe_engine *e) {
   e->graph_dirty = true;
   e->sys_dirty = true;
+  e->mc_cached = false;
   ++e->version;
 }
 
@@ -992,6 +994,7 @@ int mfe_set_chains(mfe_engine *e, int32_t nchain, const int32_t *chain_first,
 
 int mfe_upload_positions(mfe_engine *e, const double *ro) {
   if (!e || !ro) return MFE_E_BAD_PARAM;
+  e->mc_cached = false;
   if (!e->have_particles || !e->have_box) return fail(e, MFE_E_STATE, "box and particles first");
   for (int i = 0; i < 3 * e->np; ++i) {
     const int m = i % 3;
@@ -1101,9 +1104,11 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo, mfe_energy *energ
 // stencils are then exchanged in the spreading tiles and the same passes
 // give E_rec of the trial mesh.  du = EpsiFourPi (E_trial - E_current), the
 // units of mfe_energy.rec.  Positions, forces and the list are unchanged;
-// the next force call spreads the mesh again.
+// the next force call spreads the mesh again.  The current tiles and
+// E_rec are kept (as the reference keeps QMesh and urecold); with
+// reuse_current a later call starts from them instead of spreading again.
 int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const double *rtm,
-                     double *du_rec) {
+                     int32_t reuse_current, double *du_rec) {
   if (!e || !ianatm || !rtm || !du_rec || natm < 1) return MFE_E_BAD_PARAM;
   int rc = check_ready(e);
   if (rc) return rc;
@@ -1135,14 +1140,32 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
       e->mc_cap = natm;
     }
     if (!e->mc_out) dalloc(e->mc_out, 2);
+    if (e->mc_tiles_bytes != e->spread_tiles_bytes) {
+      if (e->mc_tiles) MFE_HIP_TRY(hipFree(e->mc_tiles));
+      e->mc_tiles = nullptr;
+      e->mc_tiles_bytes = 0;
+      e->mc_cached = false;
+      MFE_HIP_TRY(hipMalloc(&e->mc_tiles, e->spread_tiles_bytes));
+      e->mc_tiles_bytes = e->spread_tiles_bytes;
+    }
     upload(e->mc_ia, ia.data(), (size_t)natm, e->stream);
     upload(e->mc_rtm, rtm, 3 * (size_t)natm, e->stream);
     sync_view(e);
     const mfe::Group &G = e->grp;
-    // current mesh: binning reads the cell-ordered positions
-    mfe::launch_sort_pos(G);
-    mfe::launch_spme(G, true, SP_BIN, SP_CONV);
-    mfe::launch_rec_sum(G, e->mc_out);
+    if (reuse_current && e->mc_cached) {
+      // the current configuration's tiles; its E_rec stays in mc_out[0]
+      MFE_HIP_TRY(hipMemcpyAsync(e->spread_tiles, e->mc_tiles, e->mc_tiles_bytes,
+                                 hipMemcpyDeviceToDevice, e->stream));
+    } else {
+      // current mesh: binning reads the cell-ordered positions
+      e->mc_cached = false;
+      mfe::launch_sort_pos(G);
+      mfe::launch_spme(G, true, SP_BIN, SP_SPREAD);
+      MFE_HIP_TRY(hipMemcpyAsync(e->mc_tiles, e->spread_tiles, e->mc_tiles_bytes,
+                                 hipMemcpyDeviceToDevice, e->stream));
+      mfe::launch_spme(G, true, SP_FFTX, SP_CONV);
+      mfe::launch_rec_sum(G, e->mc_out);
+    }
     // trial mesh
     mfe::launch_mc_stencils(G, natm, e->mc_ia, e->mc_rtm);
     mfe::launch_spme(G, true, SP_FFTX, SP_CONV);
@@ -1150,6 +1173,7 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
     double h[2];
     MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out, sizeof(h), hipMemcpyDeviceToHost, e->stream));
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->mc_cached = true;
     *du_rec = e->efp * (h[1] - h[0]);
     return MFE_OK;
   });
@@ -1157,6 +1181,7 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
 
 int mfe_step_md(mfe_engine *e, int32_t nsteps, double tstep, uint32_t flags) {
   if (!e || nsteps < 0) return MFE_E_BAD_PARAM;
+  e->mc_cached = false;
   int rc = check_ready(e);
   if (rc) return rc;
   if (!e->have_ewald) return fail(e, MFE_E_STATE, "mfe_set_ewald not called");
@@ -1211,6 +1236,7 @@ int mfe_set_bd(mfe_engine *e, const double *fac1, const double *fac2, uint64_t s
 int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1, const double *fac2,
                 const double *noise, uint32_t flags) {
   if (!e || nsteps < 0) return MFE_E_BAD_PARAM;
+  e->mc_cached = false;
   if ((fac1 == nullptr) != (fac2 == nullptr)) return fail(e, MFE_E_BAD_PARAM, "fac1 and fac2 go together");
   if (!fac1 && !e->have_bd) return fail(e, MFE_E_STATE, "BD factors not set (mfe_set_bd)");
   int rc = check_ready(e);

@@ -378,6 +378,11 @@ struct mfe_engine {
   int *mc_ia = nullptr;
   double *mc_rtm = nullptr, *mc_out = nullptr;
   int mc_cap = 0;
+  // the current configuration's spreading tiles and (in mc_out[0]) its
+  // E_rec, kept for callers whose positions have not changed since
+  void *mc_tiles = nullptr;
+  size_t mc_tiles_bytes = 0;
+  bool mc_cached = false;
   int *b_done = nullptr;
   void *rec_frac = nullptr;  // per sorted particle: (fx, fy, fz, q)
   void *spread_tiles = nullptr;  // per brick: extended spreading tile

@@ -109,7 +109,7 @@ def lib():
         L.mfe_download_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.mfe_build_list.argtypes = [C.c_void_p]
         L.mfe_compute.argtypes = [C.c_void_p, C.c_uint32, _dp, C.POINTER(EnergyStruct)]
-        L.mfe_mc_delta_rec.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp]
+        L.mfe_mc_delta_rec.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, C.c_int32, _dp]
         L.mfe_step_md.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_uint32]
         L.mfe_step_bd.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, C.c_uint32]
         L.mfe_get_energy.argtypes = [C.c_void_p, C.POINTER(EnergyStruct)]
@@ -332,15 +332,18 @@ class ForceEngine:
         self._check(self.L.mfe_compute(self.h, flags, _d(f), C.byref(e)), "mfe_compute")
         return f.reshape(-1, 3), self._energies(e)
 
-    def mc_delta_rec(self, ianatm, rtm):
+    def mc_delta_rec(self, ianatm, rtm, reuse_current=False):
         """DUTwoBodyEwaldRecSPM: change of the SPME reciprocal energy (units of
-        Energies.rec) when particles ianatm (1-based) move to rtm[natm, 3]."""
+        Energies.rec) when particles ianatm (1-based) move to rtm[natm, 3].
+        reuse_current: start from the mesh and energy of the previous call
+        (valid only if nothing changed the configuration since)."""
         ia = _i32(np.atleast_1d(ianatm))
         r = _f64(np.asarray(rtm, dtype=np.float64).reshape(-1))
         if r.size != 3 * ia.size:
             raise ValueError("rtm must hold three coordinates per moved particle")
         du = np.zeros(1)
-        self._check(self.L.mfe_mc_delta_rec(self.h, ia.size, _i(ia), _d(r), _d(du)),
+        self._check(self.L.mfe_mc_delta_rec(self.h, ia.size, _i(ia), _d(r),
+                                            1 if reuse_current else 0, _d(du)),
                     "mfe_mc_delta_rec")
         return float(du[0])
 

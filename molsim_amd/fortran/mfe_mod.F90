@@ -247,13 +247,14 @@ module mfe_mod
 
       ! DUTwoBodyEwaldRecSPM for a trial move of natm particles ianatm
       ! (1-based) to rtm(3,natm): du = change of u%rec
-      integer(c_int) function mfe_mc_delta_rec(eng, natm, ianatm, rtm, du) &
+      integer(c_int) function mfe_mc_delta_rec(eng, natm, ianatm, rtm, reuse, du) &
             bind(c, name='mfe_mc_delta_rec')
          import :: c_int, c_int32_t, c_ptr, c_double
          type(c_ptr), value :: eng
          integer(c_int32_t), value :: natm
          integer(c_int32_t), intent(in) :: ianatm(*)
          real(c_double), intent(in) :: rtm(*)
+         integer(c_int32_t), value :: reuse
          real(c_double), intent(out) :: du
       end function
 

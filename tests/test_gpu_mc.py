@@ -73,3 +73,37 @@ def test_mc_delta_rec_rejects_bad_moves():
             eng.mc_delta_rec(ia, rtm)
         assert E.ERRORS[ei.value.code] == "MFE_E_BAD_PARAM"
     eng.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("mixed", 0.0)])
+def test_mc_reuse_current_mesh(prec, tol):
+    """Trials that start from the kept mesh and energy (reuse_current) give
+    the energy changes of full evaluations, and the engine spreads anew
+    once its positions change (bitwise on the mixed path, whose spreading
+    does not depend on the order of its adds)."""
+    s = S.c2()
+    print("case: MC trials from the kept mesh c2", prec, flush=True)
+    eng = S.configure(s, prec)
+    e0 = abs(oracle_rec(s, s.ro))
+    rng = np.random.default_rng(3)
+    moves = [(rng.choice(s.np, n, replace=False), None) for n in (1, 2, 5)]
+    moves = [(ia, s.ro[ia] + rng.uniform(-2.0, 2.0, (ia.size, 3))) for ia, _ in moves]
+    full = [eng.mc_delta_rec(ia + 1, rtm) for ia, rtm in moves]
+    eng.mc_delta_rec(moves[0][0] + 1, moves[0][1])
+    kept = [eng.mc_delta_rec(ia + 1, rtm, reuse_current=True) for ia, rtm in moves]
+    for a, b in zip(full, kept):
+        assert abs(a - b) <= tol * e0, (a, b)
+    # an accepted move: new positions uploaded; a reuse request spreads anew
+    ia, rtm = moves[1]
+    ro1 = s.ro.copy()
+    ro1[ia] = rtm
+    ro1 -= s.boxlen * np.round(ro1 / s.boxlen)
+    eng.upload_positions(ro1)
+    ia2 = np.array([17])
+    rtm2 = ro1[ia2] + 1.0
+    du = eng.mc_delta_rec(ia2 + 1, rtm2, reuse_current=True)
+    ro2 = ro1.copy()
+    ro2[ia2] = rtm2
+    du0 = oracle_rec(s, ro2) - oracle_rec(s, ro1)
+    assert abs(du - du0) <= (1e-9 if prec == "fp64" else 1e-5) * e0
+    eng.close()
