@@ -37,6 +37,9 @@ program RefProbe
    external Pot_1_6_12
    real(8), allocatable :: args(:), vals(:), ders(:)
    real(8) :: Erf2
+   real(8) :: GauRandom, Random
+   integer(4) :: nrng, iseedg, idumr
+   real(8), allocatable :: rv(:)
    logical :: lexist
    character(200) :: proj
 
@@ -138,6 +141,27 @@ program RefProbe
             end do
          end do
       end do
+      close(ufile)
+   end if
+
+! ... random number streams: GauRandom and Random (Src/mollib.F90:2689-2766)
+!     from given seeds, 2 nrng and nrng values
+
+   inquire(file=trim(proj)//'.probe_rng', exist=lexist)
+   if (lexist) then
+      open(newunit=ufile, file=trim(proj)//'.probe_rng', access='stream', form='unformatted', status='old')
+      read(ufile) nrng, iseedg, idumr
+      close(ufile)
+      allocate(rv(2*nrng))
+      do i = 1, 2*nrng
+         rv(i) = GauRandom(iseedg)
+      end do
+      open(newunit=ufile, file=trim(proj)//'.probe_rngout', access='stream', form='unformatted', status='replace')
+      write(ufile) rv(1:2*nrng)
+      do i = 1, nrng
+         rv(i) = Random(idumr)
+      end do
+      write(ufile) rv(1:nrng)
       close(ufile)
    end if
 

@@ -152,9 +152,40 @@ def case(name, text, workdir, probe_pos=None, md=True):
     return d
 
 
+
+def c1_text():
+    ions = [("cat", 500, 23.0, 1.0, 4.0, KT298), ("an", 500, 35.4, -1.0, 4.0, KT298)]
+    body = (particle_nml(ions) + potential_nml(2, 15.0) +
+            " &nmlSetConfiguration txsetconf='random','random', radatset=1.6,1.6 /\n"
+            " &nmlMD integ='velver', tstep=0.002, nvvite=2, lsetvel=.true. /\n"
+            " &nmlIntList txintlist='vlist', inlist=0, drnlist=2.0 /\n"
+            " &nmlDump idump=1, ldpos=.true., ldliv=.true., ldfor=.true. /\n")
+    return system_nml("golden c1", [500, 500], 50.0, nstep2=5) + body
+
+
+def rng_streams():
+    """GauRandom and Random (Src/mollib.F90:2689-2766) from fixed seeds, run
+    inside the reference probe after the C1 setup: tests/golden/rng.npz."""
+    nrng, iseed, idum = 1000, 12345, -4711
+    work = tempfile.mkdtemp(prefix="mfe_golden_")
+    try:
+        d = os.path.join(work, "c1")
+        os.makedirs(d, exist_ok=True)
+        np.array([nrng, iseed, idum], "<i4").tofile(os.path.join(d, "c1.probe_rng"))
+        case("c1", c1_text(), work, md=False)
+        v = np.fromfile(os.path.join(d, "c1.probe_rngout"), "<f8")
+        np.savez_compressed(os.path.join(HERE, "rng.npz"), iseed=iseed, idum=idum,
+                            gaurandom=v[:2 * nrng], random=v[2 * nrng:])
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     if not os.path.exists(PROBE):
         sys.exit("build the reference first: make -f oracle/ref.mk")
+    if sys.argv[1:] == ["rng"]:
+        rng_streams()
+        return
     work = tempfile.mkdtemp(prefix="mfe_golden_")
     try:
         out = {}

@@ -5,6 +5,7 @@ from the amdflang build of the unmodified reference (tables, UTotal
 forces/energies/virial, Verlet lists, MD trajectory, Erf2,
 CardinalBSpline).  Every comparison here is bitwise.
 """
+import ctypes as C
 import hashlib
 import os
 
@@ -159,3 +160,30 @@ def test_velver_trajectory_bitwise(golden_dir):
         assert np.array_equal(rod.reshape(-1, 3), z["traj_vel"][k]), k
         if O.skin_test(drosum, drostep, 2.0):
             orc.set_list(ro)
+
+
+def test_gaurandom_and_random_streams_bitwise(golden_dir):
+    """GauRandom and Random (Src/mollib.F90:2689-2766) from fixed seeds, as
+    the reference probe drew them after the C1 setup.  GauRandom keeps one
+    deviate between calls (a SAVEd flag), so the setup may leave one
+    pending: the stream is compared from the first deviate of the seed,
+    which is at probe index 0 or 1."""
+    z = load(golden_dir, "rng.npz")
+    L = O.lib()
+    L.orc_random.restype = C.c_double
+
+    class RandomState(C.Structure):
+        _fields_ = [("ix", C.c_int32), ("iy", C.c_int32), ("am", C.c_double)]
+
+    L.orc_random.argtypes = [C.POINTER(C.c_int32), C.POINTER(RandomState)]
+    g_ref = z["gaurandom"]
+    seed = C.c_int32(int(z["iseed"]))
+    st = O.GauState(0, 0.0)
+    g = np.array([L.orc_gaurandom(C.byref(seed), C.byref(st)) for _ in range(g_ref.size)])
+    off = 0 if g[0] == g_ref[0] else 1
+    assert np.array_equal(g[:g_ref.size - off], g_ref[off:])
+    r_ref = z["random"]
+    idum = C.c_int32(int(z["idum"]))
+    rs = RandomState(-1, -1, 0.0)
+    r = np.array([L.orc_random(C.byref(idum), C.byref(rs)) for _ in range(r_ref.size)])
+    assert np.array_equal(r, r_ref)
