@@ -237,6 +237,11 @@ def main():
             return round(1e6 * (time.perf_counter() - t1) / n, 1)
 
         mc_us = {"after_accept": trials(10, False), "after_reject": trials(20, True)}
+        t1 = time.perf_counter()
+        for _ in range(20):
+            ip = int(rng.integers(sysm.np))
+            eng.mc_delta_pair([ip + 1], ro_h[ip:ip + 1] + rng.uniform(-0.5, 0.5, (1, 3)))
+        mc_us["pair_part"] = round(1e6 * (time.perf_counter() - t1) / 20, 1)
         if prec == "mixed":
             # UTotal with the host's buffers: positions in, forces out over
             # PCIe each call (mfe_upload_positions + mfe_compute)

@@ -175,6 +175,18 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo,
    (MFE_E_STATE otherwise). */
 int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm,
                      const double *rtm, int32_t reuse_current, double *du_rec);
+/* Monte Carlo trial move, pair part: UTwoBodyANew - UTwoBodyAOld
+   (Src/denergy.F90:289-495, from DUTwoBody :246-279).  The natm distinct
+   particles ianatm[] (1-based) against their partners in the engine's
+   current Verlet list that do not move, plus the moved pairs among
+   themselves; table walk and checks as the reference's, in FP64 on both
+   precision paths.  du_twob = change of the two-body energy summed over
+   pair types (the units of mfe_energy.twob0); overlap = 1 when a trial
+   pair is inside the hard core or below r2umin (the reference's
+   lhsoverlap: the move is rejected; du_twob is then 0).  The list is the
+   caller's to keep current (the reference's NList policy). */
+int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm,
+                      const double *rtm, double *du_twob, int32_t *overlap);
 /* nsteps of MDDriver(iSimulationStep) + NList(iSimulationStep): VelVer
    (Src/md.F90:252-410) with the skin-test rebuild (Src/nlist.F90:597-616),
    all on the device.  flags select which UTotal terms are reduced each

@@ -1098,6 +1098,69 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo, mfe_energy *energ
   });
 }
 
+namespace {
+
+// checks of a trial move (1-based distinct particles, finite positions)
+// and a valid list; ia gets the 0-based indices
+int mc_prepare(mfe_engine *e, int natm, const int32_t *ianatm, const double *rtm,
+               std::vector<int> &ia) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (natm > e->np) return fail(e, MFE_E_BAD_PARAM, "natm > np");
+  ia.assign(natm, 0);
+  std::vector<char> seen(e->np, 0);
+  for (int a = 0; a < natm; ++a) {
+    const int i = ianatm[a] - 1;
+    if (i < 0 || i >= e->np) return fail(e, MFE_E_BAD_PARAM, "ianatm out of range");
+    if (seen[i]) return fail(e, MFE_E_BAD_PARAM, "ianatm repeats a particle");
+    seen[i] = 1;
+    ia[a] = i;
+    for (int m = 0; m < 3; ++m)
+      if (!std::isfinite(rtm[3 * a + m])) return fail(e, MFE_E_BAD_PARAM, "rtm not finite");
+  }
+  if (!e->list_valid) return mfe_build_list(e);
+  return MFE_OK;
+}
+
+// device room for natm moved particles and the four results
+void mc_buffers(mfe_engine *e, int natm) {
+  if (e->mc_cap < natm) {
+    dalloc(e->mc_ia, (size_t)natm);
+    dalloc(e->mc_rtm, 3 * (size_t)natm);
+    e->mc_cap = natm;
+  }
+  if (!e->mc_out) dalloc(e->mc_out, 4);
+}
+
+}  // namespace
+
+// UTwoBodyANew - UTwoBodyAOld (Src/denergy.F90:289-495) for a trial move
+// of natm particles: partners from the engine's current list, as the
+// reference walks jpnlist.  du = change of the two-body energy (the sum
+// over pair types, kJ/mol); overlap = 1 when a trial pair lies inside the
+// hard core or below r2umin (the reference's lhsoverlap), du then 0.
+int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm, const double *rtm,
+                      double *du_twob, int32_t *overlap) {
+  if (!e || !ianatm || !rtm || !du_twob || !overlap || natm < 1) return MFE_E_BAD_PARAM;
+  std::vector<int> ia;
+  int rc = mc_prepare(e, natm, ianatm, rtm, ia);
+  if (rc) return rc;
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    mc_buffers(e, natm);
+    upload(e->mc_ia, ia.data(), (size_t)natm, e->stream);
+    upload(e->mc_rtm, rtm, 3 * (size_t)natm, e->stream);
+    sync_view(e);
+    mfe::launch_mc_pair(e->grp, natm, e->mc_ia, e->mc_rtm, e->mc_out + 2);
+    double h[2];
+    MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out + 2, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    *overlap = h[1] != 0.0 ? 1 : 0;
+    *du_twob = *overlap ? 0.0 : h[0];
+    return MFE_OK;
+  });
+}
+
 // DUTwoBodyEwaldRecSPM (Src/denergy.F90:1413-1495) for a trial move of
 // natm particles: the current configuration is binned and spread, its
 // forward transforms and convolution give E_rec; the moved particles'
@@ -1110,36 +1173,14 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo, mfe_energy *energ
 int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const double *rtm,
                      int32_t reuse_current, double *du_rec) {
   if (!e || !ianatm || !rtm || !du_rec || natm < 1) return MFE_E_BAD_PARAM;
-  int rc = check_ready(e);
+  std::vector<int> ia;
+  int rc = mc_prepare(e, natm, ianatm, rtm, ia);
   if (rc) return rc;
   if (!e->have_ewald || e->ewald != MFE_EWALD_SPME)
     return fail(e, MFE_E_STATE, "mfe_set_ewald with SPME first");
-  if (natm > e->np) return fail(e, MFE_E_BAD_PARAM, "natm > np");
-  std::vector<int> ia(natm);
-  {
-    std::vector<char> seen(e->np, 0);
-    for (int a = 0; a < natm; ++a) {
-      const int i = ianatm[a] - 1;
-      if (i < 0 || i >= e->np) return fail(e, MFE_E_BAD_PARAM, "ianatm out of range");
-      if (seen[i]) return fail(e, MFE_E_BAD_PARAM, "ianatm repeats a particle");
-      seen[i] = 1;
-      ia[a] = i;
-      for (int m = 0; m < 3; ++m)
-        if (!std::isfinite(rtm[3 * a + m])) return fail(e, MFE_E_BAD_PARAM, "rtm not finite");
-    }
-  }
-  if (!e->list_valid) {
-    rc = mfe_build_list(e);
-    if (rc) return rc;
-  }
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
-    if (e->mc_cap < natm) {
-      dalloc(e->mc_ia, (size_t)natm);
-      dalloc(e->mc_rtm, 3 * (size_t)natm);
-      e->mc_cap = natm;
-    }
-    if (!e->mc_out) dalloc(e->mc_out, 2);
+    mc_buffers(e, natm);
     if (e->mc_tiles_bytes != e->spread_tiles_bytes) {
       if (e->mc_tiles) MFE_HIP_TRY(hipFree(e->mc_tiles));
       e->mc_tiles = nullptr;

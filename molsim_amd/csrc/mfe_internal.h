@@ -482,6 +482,9 @@ void launch_sort_pos(const Group &g);
 void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm);
 // the reciprocal energy of the last convolution pass (its block partials)
 void launch_rec_sum(const Group &g, double *out);
+// Monte Carlo trial move, pair part: out[0] the two-body energy change,
+// out[1] 1 when a trial pair overlaps
+void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out);
 void launch_skin(const Group &g);
 void launch_set_positions(const Group &g, const double *ro_new);
 void launch_collect(const Group &g, int *flags, double *en, int nen);

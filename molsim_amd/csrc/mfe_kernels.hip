@@ -1360,6 +1360,80 @@ __global__ void __launch_bounds__(256) k_mc_stencils(const SysDev *__restrict__ 
   }
 }
 
+// UTwoBodyANew/UTwoBodyAOld (Src/denergy.F90:289-495) for a trial move of
+// natm particles: each moved particle against its list partners that stay
+// (u at the trial position minus u at the current one), plus the pairs of
+// moved particles among themselves, each once (both trial minus both
+// current, as its lptmdutwob branch).  The table is walked as the
+// reference walks ubuf, from the pair's first interval to the first with
+// r2 >= zlow, in FP64 on both precision paths.  A trial pair inside the
+// hard core or below r2umin is an overlap (its goto 400).  One block,
+// lanes over list entries, a fixed-order sum.
+__device__ __forceinline__ double mc_pair_u(const SysDev &V, int ti, int tj, double dx,
+                                            double dy, double dz, int &ovl) {
+  const TableDev &t = V.tab;
+  const double r2 = pbcr2(dx, dy, dz, V.box);
+  const bool in = V.nlsem == 0 ? !(r2 > t.rcut2) : (r2 < t.rcut2);
+  if (!in) return 0.0;
+  const int p = t.pairidx[ti * t.npt + tj];
+  if (r2 < t.r2atat[p] || r2 < t.r2umin[p]) {
+    ovl = 1;
+    return 0.0;
+  }
+  const int o = t.off[p], n = t.nint[p];
+  int k = 0;
+  while (k < n - 1 && r2 < t.zlow[o + k]) ++k;
+  const double *c = t.coef + (size_t)(o + k) * kCoefStride;
+  const double d = r2 - t.zlow[o + k];
+  return c[0] + d * (c[1] + d * (c[2] + d * (c[3] + d * (c[4] + d * c[5]))));
+}
+
+__global__ void __launch_bounds__(256) k_mc_pair(const SysDev *__restrict__ sys, int natm,
+                                                const int *__restrict__ ia,
+                                                const double *__restrict__ rtm,
+                                                double *__restrict__ out) {
+  const SysDev &V = sys[blockIdx.z];
+  const int np = V.np;
+  double acc = 0.0;
+  int ovl = 0, ovl_old = 0;
+  for (int a = 0; a < natm; ++a) {
+    const int i = ia[a], t = V.c_pos[i], ti = V.type[i];
+    const double xo = V.ro[3 * i], yo = V.ro[3 * i + 1], zo = V.ro[3 * i + 2];
+    const double xn = rtm[3 * a], yn = rtm[3 * a + 1], zn = rtm[3 * a + 2];
+    const int n = V.nn[t];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+      const int j = V.nbr[(size_t)k * np + t];
+      const int jp = V.c_sorted[j];
+      bool moved = false;
+      for (int b = 0; b < natm; ++b) moved |= ia[b] == jp;
+      if (moved) continue;
+      const int tj = V.type_s[j];
+      const double xj = V.ro_s[3 * j], yj = V.ro_s[3 * j + 1], zj = V.ro_s[3 * j + 2];
+      acc += mc_pair_u(V, ti, tj, xn - xj, yn - yj, zn - zj, ovl) -
+             mc_pair_u(V, ti, tj, xo - xj, yo - yj, zo - zj, ovl_old);
+    }
+  }
+  for (int q = threadIdx.x; q < natm * natm; q += blockDim.x) {
+    const int a = q / natm, b = q - a * natm;
+    if (b <= a) continue;
+    const int i = ia[a], j = ia[b], ti = V.type[i], tj = V.type[j];
+    acc += mc_pair_u(V, ti, tj, rtm[3 * a] - rtm[3 * b], rtm[3 * a + 1] - rtm[3 * b + 1],
+                     rtm[3 * a + 2] - rtm[3 * b + 2], ovl) -
+           mc_pair_u(V, ti, tj, V.ro[3 * i] - V.ro[3 * j], V.ro[3 * i + 1] - V.ro[3 * j + 1],
+                     V.ro[3 * i + 2] - V.ro[3 * j + 2], ovl_old);
+  }
+  __shared__ double red[256 / 64];
+  for (int w = 32; w > 0; w >>= 1) acc += __shfl_down(acc, w, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  const int any = __syncthreads_or(ovl);
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
+    out[2 * blockIdx.z] = s;
+    out[2 * blockIdx.z + 1] = any ? 1.0 : 0.0;
+  }
+}
+
 // the reciprocal energy sum of the last convolution pass: its block
 // partials in a fixed order (one wavefront, shuffle tree)
 __global__ void __launch_bounds__(64) k_rec_sum(const SysDev *__restrict__ sys,
@@ -2119,6 +2193,10 @@ void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *r
                        MFE_MC(double, 7) MFE_MC(double, 8) }
   }
 #undef MFE_MC
+}
+
+void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out) {
+  hipLaunchKernelGGL(k_mc_pair, grid(1, g), dim3(256), 0, g.stream, g.d, natm, ia, rtm, out);
 }
 
 void launch_rec_sum(const Group &g, double *out) {

@@ -258,6 +258,19 @@ module mfe_mod
          real(c_double), intent(out) :: du
       end function
 
+      ! UTwoBodyANew - UTwoBodyAOld for the same trial move: du = change of
+      ! the two-body energy, overlap = 1 for lhsoverlap
+      integer(c_int) function mfe_mc_delta_pair(eng, natm, ianatm, rtm, du, overlap) &
+            bind(c, name='mfe_mc_delta_pair')
+         import :: c_int, c_int32_t, c_ptr, c_double
+         type(c_ptr), value :: eng
+         integer(c_int32_t), value :: natm
+         integer(c_int32_t), intent(in) :: ianatm(*)
+         real(c_double), intent(in) :: rtm(*)
+         real(c_double), intent(out) :: du
+         integer(c_int32_t), intent(out) :: overlap
+      end function
+
       ! batched replicas: engines(nrep) are the handles of set-up engines
       integer(c_int) function mfe_batch_create(engines, nrep, batch) bind(c, name='mfe_batch_create')
          import :: c_int, c_int32_t, c_ptr

@@ -107,3 +107,38 @@ def test_mc_reuse_current_mesh(prec, tol):
     du0 = oracle_rec(s, ro2) - oracle_rec(s, ro1)
     assert abs(du - du0) <= (1e-9 if prec == "fp64" else 1e-5) * e0
     eng.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_mc_delta_pair_vs_oracle(prec):
+    """UTwoBodyANew - UTwoBodyAOld (Src/denergy.F90:289-495) against the
+    oracle's two-body energy of both configurations over the same list
+    (FP64 on both engine paths: <= 1e-10 of |u_twob|)."""
+    from helpers import oracle_for
+    s = S.c2()
+    print("case: MC pair deltas c2", prec, flush=True)
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    eng = S.configure(s, prec, tables=tab)
+    eng.build_list()
+    orc = oracle_for(s, tab)
+    orc.set_list(s.ro)
+    _, eo = orc.utotal(s.ro)
+    u0 = eo["twob0"]
+    rng = np.random.default_rng(5)
+    for natm in (1, 2, 6):
+        ia = rng.choice(s.np, natm, replace=False)
+        rtm = s.ro[ia] + rng.uniform(-0.8, 0.8, (natm, 3))
+        rtm -= s.boxlen * np.round(rtm / s.boxlen)
+        du, ov = eng.mc_delta_pair(ia + 1, rtm)
+        ro1 = s.ro.copy()
+        ro1[ia] = rtm
+        _, e1 = orc.utotal(ro1)
+        print(f"  natm {natm}: du {du:.10e} oracle {e1['twob0'] - u0:.10e} overlap {ov}",
+              flush=True)
+        assert not ov and du != 0.0
+        assert abs(du - (e1["twob0"] - u0)) <= 1e-10 * abs(u0)
+    # a particle moved onto another one overlaps
+    du, ov = eng.mc_delta_pair([1], s.ro[[1]] + 0.1)
+    assert ov and du == 0.0
+    eng.close()
