@@ -180,11 +180,28 @@ def rng_streams():
         shutil.rmtree(work, ignore_errors=True)
 
 
+def c1_cnf():
+    """The configuration file the reference writes at the end of the C1 run
+    (IOCnf('write'), Src/molsim.F90:945-955): tests/golden/c1_final.cnf, with
+    the run's last dumped positions and velocities for comparison."""
+    work = tempfile.mkdtemp(prefix="mfe_golden_")
+    try:
+        d = case("c1", c1_text(), work)
+        pos, liv, _ = dumps(d, "c1")
+        shutil.copyfile(os.path.join(d, "c1.cnf"), os.path.join(HERE, "c1_final.cnf"))
+        np.savez_compressed(os.path.join(HERE, "c1_final.npz"), ro=pos[-1], rod=liv[-1])
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     if not os.path.exists(PROBE):
         sys.exit("build the reference first: make -f oracle/ref.mk")
     if sys.argv[1:] == ["rng"]:
         rng_streams()
+        return
+    if sys.argv[1:] == ["cnf"]:
+        c1_cnf()
         return
     work = tempfile.mkdtemp(prefix="mfe_golden_")
     try:
