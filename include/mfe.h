@@ -91,8 +91,33 @@ typedef struct {
   double r2uminin, utoltab, ftoltab, umaxtab, fmaxtab, rumaxfac;
 } mfe_pot_1_6_12;
 
+/* EwaldSetup's parameter choice (iewaldopt; Src/energy.F90:6644-6690 for
+   the standard sum, 6863-6886 for SPME). Which fields are inputs and
+   which are outputs depends on iewaldopt, as in the reference's table:
+     0: ualphared, rcut (, ncut)      -> ualpha = ualphared/rcut
+     1: ualphared, rcut               -> ualpha, uewaldtol, ncut
+     2: uewaldtol, ualpha             -> rcut, ncut, ualphared
+     3: uewaldtol, rcut               -> ualpha, ualphared (, ncut: std)
+     4: uewaldtol, ncut               -> ualpha, rcut, ualphared
+   (SPME: 0 and 3 only.) The error estimates are Kolafa-Perram for charges
+   (lq2sum 0) and Wang-Holm for dipoles (lq2sum 1): AlphaToRcut,
+   AlphaToNcut, RcutToAlpha and NcutToAlpha (Src/energy.F90:7052-7124),
+   EwaldErrorUReal (Src/potential.F90:2995-3018). q2sum = sum(q^2)/np,
+   boxlenshort = minval(boxlen). Inputs out of range and iterations that do
+   not converge in 40 steps give MFE_E_BAD_PARAM, as they stop the
+   reference. */
+typedef struct {
+  int32_t iewaldopt;
+  int32_t spme;         /* 1: txewaldrec = 'spm' */
+  int32_t lq2sum;       /* 0: charges, 1: dipoles and no charges */
+  int32_t ncut;
+  double q2sum, boxlenshort, epsi_four_pi;
+  double uewaldtol, ualphared, ualpha, rcut;
+} mfe_ewald_opt;
+
 /* ---- library ---------------------------------------------------------- */
 int mfe_abi_version(void);
+int mfe_ewald_parameters(mfe_ewald_opt *p);
 /* Fills spec with IOPotTwoBody's defaults (Src/potential.F90:340-392,411). */
 int mfe_pot_defaults(mfe_pot_1_6_12 *spec);
 /* PotTwoBodyTab1/2/3 + SetUBuffer + CheckUBuffer for Pot_1_6_12 with

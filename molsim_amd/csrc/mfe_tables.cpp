@@ -222,3 +222,132 @@ extern "C" int mfe_build_tables_1_6_12(const mfe_pot_1_6_12 *s, double *ubuf,
 }
 
 extern "C" int mfe_abi_version(void) { return MFE_ABI_VERSION; }
+
+namespace {
+
+constexpr double kPiE = 3.1415926535897932;  // MolModule's Pi
+
+// The fixed-point iterations of EwaldSetup (Src/energy.F90:7052-7124),
+// each expression in the reference's order of operations; 40 iterations
+// at most (the reference stops otherwise).
+bool alpha_to_rcut(const mfe_ewald_opt &p, double &rcut) {
+  const double L = p.boxlenshort, a = p.ualpha, t = p.uewaldtol / (p.q2sum * p.epsi_four_pi);
+  const double y = p.lq2sum == 0 ? t * std::sqrt(2.0 * (L * L * L)) * (a * a)
+                                 : (std::sqrt(15.0) / 4.0) * t * std::sqrt(L * L * L) / (a * a);
+  double xold = 1.0;
+  for (int i = 1; i <= 40; ++i) {
+    const double xnew = p.lq2sum == 0 ? (1.0 / a) * std::sqrt(-std::log(y * std::pow(xold, 1.5)))
+                                      : (1.0 / a) * std::sqrt(-std::log(y / std::pow(xold, 0.5)));
+    if (std::fabs(xold - xnew) / xnew < 1.0e-6) {
+      rcut = xnew;
+      return true;
+    }
+    xold = xnew;
+  }
+  return false;
+}
+
+bool alpha_to_ncut(const mfe_ewald_opt &p, int &ncut) {
+  const double L = p.boxlenshort, a = p.ualpha, t = p.uewaldtol / (p.q2sum * p.epsi_four_pi);
+  const double y = p.lq2sum == 0 ? t * (kPiE * kPiE / ((a * a) * L))
+                                 : (std::sqrt(15.0) / 4.0) * t * L / (a * a);
+  double xold = 1.0;
+  for (int i = 1; i <= 40; ++i) {
+    const double g = 1.0 + 1.0 / (a * L * std::sqrt(xold));
+    const double xnew = p.lq2sum == 0 ? (a * L / kPiE) * std::sqrt(-std::log(y * xold / g))
+                                      : (a * L / kPiE) * std::sqrt(-std::log(y / xold / g));
+    if (std::fabs(xold - xnew) / xnew < 1.0e-6) {
+      ncut = (int)(xnew + 1.0);
+      return true;
+    }
+    xold = xnew;
+  }
+  return false;
+}
+
+bool rcut_to_alpha(const mfe_ewald_opt &p, double &alpha) {
+  const double L = p.boxlenshort, r = p.rcut, t = p.uewaldtol / (p.q2sum * p.epsi_four_pi);
+  const double lr = L * r;
+  const double y = p.lq2sum == 0 ? t * std::sqrt(2.0 * (lr * lr * lr))
+                                 : (3.0 / 4.0) * t * std::sqrt((L * L * L) / r);
+  double xold = 1.0e-1;
+  for (int i = 1; i <= 40; ++i) {
+    const double xnew = p.lq2sum == 0 ? (1.0 / r) * std::sqrt(-std::log(y * (xold * xold)))
+                                      : (1.0 / r) * std::sqrt(-std::log(y / (xold * xold)));
+    if (std::fabs(xold - xnew) / xnew < 1.0e-6) {
+      alpha = xnew;
+      return true;
+    }
+    xold = xnew;
+  }
+  return false;
+}
+
+bool ncut_to_alpha(const mfe_ewald_opt &p, double &alpha) {
+  const double L = p.boxlenshort, t = p.uewaldtol / (p.q2sum * p.epsi_four_pi);
+  const int n = p.ncut;
+  const double y = p.lq2sum == 0 ? t * (kPiE * kPiE * n) / L : (3.0 / 4.0) * t * L / n;
+  // sqrt(real(ncut)): default (single) precision in the reference
+  const double sn = (double)std::sqrt((float)n);
+  double xold = 1.0;
+  for (int i = 1; i <= 40; ++i) {
+    const double g = 1.0 + 1.0 / (xold * L * sn);
+    const double xnew = (kPiE * n / L) / std::sqrt(-std::log(y / (xold * xold) / g));
+    if (std::fabs(xold - xnew) / xnew < 1.0e-6) {
+      alpha = xnew;
+      return true;
+    }
+    xold = xnew;
+  }
+  return false;
+}
+
+// EwaldErrorUReal (Src/potential.F90:2995-3018), leading order for dipoles
+double ewald_error_ureal(const mfe_ewald_opt &p) {
+  const double L = p.boxlenshort, r = p.rcut, fac = (p.ualpha * r) * (p.ualpha * r);
+  if (p.lq2sum == 0)
+    return p.epsi_four_pi * p.q2sum * std::sqrt(r / (2.0 * (L * L * L))) / fac * std::exp(-fac);
+  const double rl = r * L;
+  return p.epsi_four_pi * p.q2sum * 4.0 / std::sqrt(15.0 * (rl * rl * rl)) * 1.0 * fac *
+         std::exp(-fac);
+}
+
+}  // namespace
+
+extern "C" int mfe_ewald_parameters(mfe_ewald_opt *p) {
+  if (!p || (p->lq2sum != 0 && p->lq2sum != 1) || !(p->q2sum > 0.0) ||
+      !(p->boxlenshort > 0.0) || !(p->epsi_four_pi > 0.0))
+    return MFE_E_BAD_PARAM;
+  const int opt = p->iewaldopt;
+  if (p->spme ? (opt != 0 && opt != 3) : (opt < 0 || opt > 4)) return MFE_E_BAD_PARAM;
+  bool ok = true;
+  switch (opt) {
+    case 0:
+      if (!(p->ualphared > 0.0) || !(p->rcut > 0.0) || (!p->spme && p->ncut <= 0))
+        return MFE_E_BAD_PARAM;
+      p->ualpha = p->ualphared / p->rcut;
+      break;
+    case 1:
+      if (!(p->ualphared > 0.0) || !(p->rcut > 0.0)) return MFE_E_BAD_PARAM;
+      p->ualpha = p->ualphared / p->rcut;
+      p->uewaldtol = ewald_error_ureal(*p);
+      ok = alpha_to_ncut(*p, p->ncut);
+      break;
+    case 2:
+      if (!(p->uewaldtol > 0.0) || !(p->ualpha > 0.0)) return MFE_E_BAD_PARAM;
+      ok = alpha_to_rcut(*p, p->rcut) && alpha_to_ncut(*p, p->ncut);
+      p->ualphared = p->ualpha * p->rcut;
+      break;
+    case 3:
+      if (!(p->uewaldtol > 0.0) || !(p->rcut > 0.0)) return MFE_E_BAD_PARAM;
+      ok = rcut_to_alpha(*p, p->ualpha) && (p->spme || alpha_to_ncut(*p, p->ncut));
+      p->ualphared = p->ualpha * p->rcut;
+      break;
+    case 4:
+      if (!(p->uewaldtol > 0.0) || p->ncut <= 0) return MFE_E_BAD_PARAM;
+      ok = ncut_to_alpha(*p, p->ualpha) && alpha_to_rcut(*p, p->rcut);
+      p->ualphared = p->ualpha * p->rcut;
+      break;
+  }
+  return ok ? MFE_OK : MFE_E_BAD_PARAM;
+}

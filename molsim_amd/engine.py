@@ -65,6 +65,13 @@ class PotSpec(C.Structure):
                 ("fmaxtab", C.c_double), ("rumaxfac", C.c_double)]
 
 
+class EwaldOpt(C.Structure):
+    _fields_ = [("iewaldopt", C.c_int32), ("spme", C.c_int32), ("lq2sum", C.c_int32),
+                ("ncut", C.c_int32), ("q2sum", C.c_double), ("boxlenshort", C.c_double),
+                ("epsi_four_pi", C.c_double), ("uewaldtol", C.c_double),
+                ("ualphared", C.c_double), ("ualpha", C.c_double), ("rcut", C.c_double)]
+
+
 # every symbol include/mfe.h declares
 SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_create", "mfe_destroy", "mfe_last_error", "mfe_set_box",
@@ -77,7 +84,7 @@ SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_batch_failed_replica", "mfe_batch_build_list", "mfe_batch_compute",
            "mfe_batch_step_md", "mfe_batch_step_bd", "mfe_batch_get_energies",
            "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint", "mfe_mc_delta_rec",
-           "mfe_mc_delta_pair"]
+           "mfe_mc_delta_pair", "mfe_ewald_parameters"]
 
 _lib = None
 
@@ -162,6 +169,21 @@ class Tables:
     iubuflow: np.ndarray
     nugrid: np.ndarray
     r2umin: np.ndarray
+
+
+def ewald_parameters(iewaldopt, *, q2sum, boxlenshort, epsi_four_pi, lq2sum=0, spme=False,
+                     uewaldtol=0.0, ualphared=0.0, ualpha=0.0, rcut=0.0, ncut=0) -> dict:
+    """EwaldSetup's parameter choice for iewaldopt 0-4 (mfe_ewald_parameters):
+    the inputs the option uses, the rest determined; returns every field."""
+    p = EwaldOpt(int(iewaldopt), int(bool(spme)), int(lq2sum), int(ncut), float(q2sum),
+                 float(boxlenshort), float(epsi_four_pi), float(uewaldtol), float(ualphared),
+                 float(ualpha), float(rcut))
+    L = lib()
+    L.mfe_ewald_parameters.argtypes = [C.POINTER(EwaldOpt)]
+    rc = L.mfe_ewald_parameters(C.byref(p))
+    if rc:
+        raise MfeError(rc, "mfe_ewald_parameters: input out of range or no convergence")
+    return {k: getattr(p, k) for k, _ in EwaldOpt._fields_}
 
 
 def pot_defaults() -> PotSpec:

@@ -18,6 +18,12 @@ module mfe_mod
       integer(c_int32_t) :: precision, device, max_np, reserved
    end type mfe_config
 
+   type, bind(c) :: mfe_ewald_opt   ! EwaldSetup's iewaldopt inputs and outputs
+      integer(c_int32_t) :: iewaldopt, spme, lq2sum, ncut
+      real(c_double) :: q2sum, boxlenshort, epsi_four_pi
+      real(c_double) :: uewaldtol, ualphared, ualpha, rcut
+   end type mfe_ewald_opt
+
    type, bind(c) :: mfe_energies   ! struct mfe_energy
       real(c_double) :: tot, twob0, rec, bond, virial
       integer(c_int32_t) :: nptpt, rebuilds
@@ -256,6 +262,12 @@ module mfe_mod
          real(c_double), intent(in) :: rtm(*)
          integer(c_int32_t), value :: reuse
          real(c_double), intent(out) :: du
+      end function
+
+      ! EwaldSetup's iewaldopt parameter choice
+      integer(c_int) function mfe_ewald_parameters(p) bind(c, name='mfe_ewald_parameters')
+         import :: c_int, mfe_ewald_opt
+         type(mfe_ewald_opt), intent(inout) :: p
       end function
 
       ! UTwoBodyANew - UTwoBodyAOld for the same trial move: du = change of

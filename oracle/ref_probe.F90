@@ -78,6 +78,13 @@ program RefProbe
    write(ufile) rcut, ualpha, EpsiFourPi
    close(ufile)
 
+! ... Ewald parameters as EwaldSetup chose them (iewaldopt)
+
+   open(newunit=ufile, file=trim(proj)//'.probe_ewald', access='stream', form='unformatted', status='replace')
+   write(ufile) iewaldopt, lq2sum, ncut
+   write(ufile) uewaldtol, ualphared, ualpha, rcut, q2sum, boxlenshort, EpsiFourPi
+   close(ufile)
+
 ! ... forces at given positions
 
    inquire(file=trim(proj)//'.probe_pos', exist=lexist)

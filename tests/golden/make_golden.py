@@ -194,6 +194,34 @@ def c1_cnf():
         shutil.rmtree(work, ignore_errors=True)
 
 
+def ewald_options():
+    """EwaldSetup's parameter choice for iewaldopt 0-4 (std,
+    Src/energy.F90:6644-6690, 7052-7124) on the C1 system: what the
+    reference settles on for each set of inputs (tests/golden/ewald_opt.npz)."""
+    ions = [("cat", 500, 23.0, 1.0, 4.0, KT298), ("an", 500, 35.4, -1.0, 4.0, KT298)]
+    cases = [(0, ""), (1, ""), (2, ", uewaldtol=1.0d-3, ualpha=0.21"),
+             (3, ", uewaldtol=1.0d-3"), (3, ", uewaldtol=1.0d-5"), (4, ", uewaldtol=1.0d-3"),
+             (4, ", uewaldtol=1.0d-4, ncut=12")]
+    rows_i, rows_d = [], []
+    work = tempfile.mkdtemp(prefix="mfe_golden_")
+    try:
+        for k, (opt, extra) in enumerate(cases):
+            body = (particle_nml(ions) +
+                    potential_nml(2, 15.0, extra=f", iewaldopt={opt}{extra}") +
+                    " &nmlSetConfiguration txsetconf='random','random', radatset=1.6,1.6 /\n"
+                    " &nmlMD integ='velver', tstep=0.002, nvvite=2, lsetvel=.true. /\n"
+                    " &nmlIntList txintlist='vlist', inlist=0, drnlist=2.0 /\n")
+            text = system_nml("golden ewald", [500, 500], 50.0, nstep2=1, dump=False) + body
+            d = case(f"e{k}", text, work, md=False)
+            b = open(os.path.join(d, f"e{k}.probe_ewald"), "rb").read()
+            rows_i.append(struct.unpack("<3i", b[:12]))
+            rows_d.append(struct.unpack("<7d", b[12:68]))
+        np.savez_compressed(os.path.join(HERE, "ewald_opt.npz"), ints=np.array(rows_i),
+                            reals=np.array(rows_d))
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     if not os.path.exists(PROBE):
         sys.exit("build the reference first: make -f oracle/ref.mk")
@@ -202,6 +230,9 @@ def main():
         return
     if sys.argv[1:] == ["cnf"]:
         c1_cnf()
+        return
+    if sys.argv[1:] == ["ewald"]:
+        ewald_options()
         return
     work = tempfile.mkdtemp(prefix="mfe_golden_")
     try:

@@ -107,3 +107,33 @@ def test_autotune_mesh_rule():
     sysm = S.c4_replica(0)
     t = A.with_params(sysm, 0.2, 5, 30)
     assert (t.alpha, t.order, t.mesh) == (0.2, 5, 30) and abs(t.rcut - 15.0) < 1e-12
+
+
+def test_ewald_parameters_match_reference(golden_dir):
+    """iewaldopt 0-4 (Src/energy.F90:6644-6690, 7052-7124): the parameters
+    the reference settles on for the C1 system (1000 ions, L = 50 A,
+    rc = 15 A), bit for bit (tests/golden/ewald_opt.npz)."""
+    z = np.load(os.path.join(golden_dir, "ewald_opt.npz"))
+    for (opt, lq, ncut), (tol, ared, alpha, rcut, q2, lshort, efp) in zip(z["ints"], z["reals"]):
+        inp = dict(q2sum=q2, boxlenshort=lshort, epsi_four_pi=efp, lq2sum=int(lq))
+        if opt in (0, 1):
+            inp.update(ualphared=3.0, rcut=15.0, ncut=8)
+        elif opt == 2:
+            inp.update(uewaldtol=tol, ualpha=0.21)
+        elif opt == 3:
+            inp.update(uewaldtol=tol, rcut=15.0)
+        else:
+            inp.update(uewaldtol=tol, ncut=int(ncut))
+        r = E.ewald_parameters(int(opt), **inp)
+        assert r["ualpha"] == alpha, (opt, r["ualpha"], alpha)
+        assert r["rcut"] == rcut and r["ncut"] == ncut, (opt, r)
+        assert r["uewaldtol"] == tol and (opt == 0 or r["ualphared"] == ared), (opt, r)
+    # SPME takes options 0 and 3 only, and 3 gives the same alpha as std
+    s = E.ewald_parameters(3, q2sum=1.0, boxlenshort=50.0, epsi_four_pi=z["reals"][0][6],
+                           spme=True, uewaldtol=1e-3, rcut=15.0)
+    assert s["ualpha"] == z["reals"][3][2]
+    with pytest.raises(E.MfeError):
+        E.ewald_parameters(2, q2sum=1.0, boxlenshort=50.0, epsi_four_pi=17.7, spme=True,
+                           uewaldtol=1e-3, ualpha=0.2)
+    with pytest.raises(E.MfeError):
+        E.ewald_parameters(3, q2sum=1.0, boxlenshort=50.0, epsi_four_pi=17.7, rcut=15.0)
