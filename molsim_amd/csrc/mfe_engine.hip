@@ -373,11 +373,6 @@ int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStre
     g.np_blocks = std::max(g.np_blocks, nblocks256(e->np));
     g.pair_blocks = std::max(g.pair_blocks, e->nblk_pair);
     g.lds_pair = std::max(g.lds_pair, e->t_lds_bytes);
-    if (e->ewald == MFE_EWALD_SPME) {
-      if (!e->plane_lds) g.fft_plane = false;
-      g.lds_plane = std::max(g.lds_plane, e->plane_lds);
-      g.planes = std::max(g.planes, e->s[2]);
-    }
     g.pair_npt = std::max(g.pair_npt, e->npt);
     if (e->t_lds_mode < 2) g.pair_cf_lds = false;
     g.cell_blocks = std::max(g.cell_blocks, nblocks256(e->cg.ncells));
@@ -896,8 +891,6 @@ int mfe_set_ewald(mfe_engine *e, int32_t kind, double alpha, int32_t order,
         // large bricks (meshes 2q with q prime) need the LDS above 64 KB
         mfe_allow_big_spread_lds(e->prec, lds_need);
       }
-      e->plane_lds = mfe::fft_plane_lds(e->prec, e->s);
-      if (e->plane_lds) mfe_allow_big_fft_lds((int)e->plane_lds);
     } else if (kind == MFE_EWALD_STD) {
       if (ncut < 1) return fail(e, MFE_E_BAD_PARAM, "ncut < 1");
       e->ncut = ncut;

@@ -320,7 +320,9 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
 template <typename R>
 __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
-  if ((int)blockIdx.x >= V.fx_nblk) return;
+  // neighbouring rows (which the y inverse wrote together) on one XCD
+  const int blk = xcd_block(blockIdx.x, gridDim.x);
+  if (blk >= V.fx_nblk) return;
   const cplx<R> *__restrict__ cm = (const cplx<R> *)V.mesh_c;
   R *__restrict__ real = (R *)V.mesh_r;
   const int s1 = V.sg.s[0], nrows = V.sg.s[1] * V.sg.s[2], rows_per_blk = V.fx_rpb;
@@ -331,7 +333,7 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
   const int m = s1 / 2, hx = m + 1, ld = m + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
   cplx<R> *b = a + rows_per_blk * ld;
-  const int row0 = blockIdx.x * rows_per_blk;
+  const int row0 = blk * rows_per_blk;
   const int nr = min(rows_per_blk, nrows - row0);
   cplx<R> *twh_s = b + rows_per_blk * ld, *twf_s = twh_s + m;
   stage_twiddles(twh_s, tw_half, m);
@@ -358,64 +360,6 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
     const int r = fdiv(e, V.f_s1), x = e - r * s1;
     cplx<R> v = z[r * ld + (x >> 1)];
     real[(size_t)(row0 + r) * s1 + x] = (x & 1) ? v.y : v.x;
-  }
-}
-
-// ------------------------------------- inverse y and x of one z plane
-// Mixed precision, planes that fit LDS: one block transforms plane z back
-// along y and then x (c2r as in k_fft_x_inv), so the mesh is read once
-// and the real plane written once for both.  The plane is held transposed
-// (one sequence per kx) for the y transform.  LDS: two buffers of
-// max(s2 (m+1), hx (s2+1)) complex values, then the twiddles (m, s1, s2).
-// (The forward direction stays two passes: its x pass reads the spreading
-// tiles, which takes all CUs, not one per plane.)
-constexpr int kPlaneThreads = 1024;
-
-template <typename R>
-__global__ void __launch_bounds__(kPlaneThreads) k_fft_yx_inv(const SysDev *__restrict__ sys) {
-  const SysDev &V = sys[blockIdx.z];
-  const int z = blockIdx.x;
-  if (z >= V.sg.s[2]) return;
-  const int s1 = V.sg.s[0], s2 = V.sg.s[1];
-  const int m = s1 / 2, hx = m + 1, ldx = m + 1, ldy = s2 + 1;
-  const int nbuf = max(s2 * ldx, hx * ldy);
-  extern __shared__ char smem[];
-  cplx<R> *A = reinterpret_cast<cplx<R> *>(smem), *B = A + nbuf;
-  cplx<R> *twh = B + nbuf, *twf = twh + m, *twy = twf + s1;
-  stage_twiddles(twh, (const cplx<R> *)V.tw_half, m);
-  stage_twiddles(twf, (const cplx<R> *)V.tw[0], s1);
-  stage_twiddles(twy, (const cplx<R> *)V.tw[1], s2);
-  const cplx<R> *__restrict__ cm = (const cplx<R> *)V.mesh_c + (size_t)z * s2 * hx;
-  batched_copy(
-      s2 * hx, [&](int e) { return cm[e]; },
-      [&](int e, cplx<R> v) {
-        const int y = fdiv(e, V.f_hx), kx = e - y * hx;
-        A[kx * ldy + y] = v;
-      });
-  __syncthreads();
-  cplx<R> *Y = lds_fft(A, B, V.plan[1], hx, ldy, twy, true);
-  cplx<R> *W = (Y == A) ? B : A;
-  // c2r of each row as in k_fft_x_inv: Re(X[0]), Re(X[m]) only
-  auto Xh = [&](int k, int y) {
-    cplx<R> v = Y[k * ldy + y];
-    if (k == 0 || k == m) v.y = R(0);
-    return v;
-  };
-  for (int e = threadIdx.x; e < s2 * m; e += blockDim.x) {
-    const int y = fdiv(e, V.f_m), k = e - y * m;
-    cplx<R> xk = Xh(k, y);
-    cplx<R> xm = conjc(Xh(m - k, y));
-    cplx<R> Av = cadd(xk, xm), D = csub(xk, xm);
-    cplx<R> Dw = cmul(D, conjc(twf[k]));
-    W[y * ldx + k] = {Av.x - Dw.y, Av.y + Dw.x};
-  }
-  __syncthreads();
-  cplx<R> *Zr = lds_fft(W, Y, V.plan_half, s2, ldx, twh, true);
-  R *__restrict__ real = (R *)V.mesh_r + (size_t)z * s2 * s1;
-  for (int e = threadIdx.x; e < s2 * s1; e += blockDim.x) {
-    const int y = fdiv(e, V.f_s1), x = e - y * s1;
-    const cplx<R> v = Zr[y * ldx + (x >> 1)];
-    real[e] = (x & 1) ? v.y : v.x;
   }
 }
 
@@ -597,15 +541,6 @@ void fft_geometry(int prec, const int s[3], int *fx_rpb, int *fx_nblk, int *fy_T
   *lds_fz = (2 * (size_t)Tz * (s[2] + 1) + s[2]) * csz;
 }
 
-size_t fft_plane_lds(int prec, const int s[3]) {
-  if (prec != MFE_PREC_MIXED) return 0;
-  const size_t m = s[0] / 2, hx = m + 1;
-  const size_t nbuf = std::max((size_t)s[1] * (m + 1), hx * (size_t)(s[1] + 1));
-  const size_t bytes = (2 * nbuf + m + s[0] + s[1]) * sizeof(cplx<float>);
-  // the gfx950 LDS of 160 KB per CU, less the kernels' static LDS
-  return bytes <= 150 * 1024 ? bytes : 0;
-}
-
 void fft_x_forward(const Group &g) {
   const dim3 gr(g.fx_blocks, 1, g.R);
   if (g.prec == MFE_PREC_MIXED)
@@ -614,7 +549,6 @@ void fft_x_forward(const Group &g) {
     hipLaunchKernelGGL(k_fft_x_fwd<double>, gr, dim3(256), g.lds_fx, g.stream, g.d);
 }
 void fft_x_inverse(const Group &g) {
-  if (g.fft_plane) return;  // done with the y inverse
   const dim3 gr(g.fx_blocks, 1, g.R);
   if (g.prec == MFE_PREC_MIXED)
     hipLaunchKernelGGL(k_fft_x_inv<float>, gr, dim3(256), g.lds_fx, g.stream, g.d);
@@ -622,12 +556,6 @@ void fft_x_inverse(const Group &g) {
     hipLaunchKernelGGL(k_fft_x_inv<double>, gr, dim3(256), g.lds_fx, g.stream, g.d);
 }
 void fft_y(const Group &g, int inverse) {
-  if (g.fft_plane && inverse) {
-    // y and x of each plane
-    hipLaunchKernelGGL(k_fft_yx_inv<float>, dim3(g.planes, 1, g.R), dim3(kPlaneThreads),
-                       g.lds_plane, g.stream, g.d);
-    return;
-  }
   const dim3 gr(g.fy_x, g.fy_y, g.R);
   if (g.prec == MFE_PREC_MIXED)
     hipLaunchKernelGGL(k_fft_y<float>, gr, dim3(256), g.lds_fy, g.stream, g.d, inverse);
@@ -645,7 +573,3 @@ void fft_conv(const Group &g, bool energy) {
 
 }  // namespace mfe
 
-void mfe_allow_big_fft_lds(int bytes) {
-  (void)hipFuncSetAttribute((const void *)&mfe::k_fft_yx_inv<float>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
-}

@@ -291,9 +291,6 @@ struct Group {
   int fx_blocks = 0, fy_x = 0, fy_y = 0, fz_x = 0, fz_y = 0;
   size_t lds_spread = 0, lds_gather = 0, lds_fx = 0, lds_fy = 0, lds_fz = 0;
   size_t lds_pair = 0;
-  bool fft_plane = true;    // every replica takes the inverse y+x plane transform
-  size_t lds_plane = 0;
-  int planes = 0;           // most z planes of any replica
   int pair_npt = 0;         // most types of any replica
   // fused list rebuild (one engine): grid barrier state and CU count
   unsigned *rebuild_bar = nullptr;
@@ -344,7 +341,6 @@ struct mfe_engine {
   mfe::PairRec *t_rec = nullptr;
   int t_nbin = 0;
   int t_lds_mode = 0;
-  size_t plane_lds = 0;     // fft_plane_lds of the mesh (0: separate passes)
   size_t t_lds_bytes = 0;
   int t_nzl = 0;
   double rcut = 0.0;
@@ -505,9 +501,6 @@ template <typename R, int O>
 const void *k_gather_fin_ptr();
 
 // FFT passes (mfe_fft.hip)
-// LDS of the one-block-per-plane inverse y+x transform (mixed precision),
-// 0 when a plane does not fit
-size_t fft_plane_lds(int prec, const int s[3]);
 void fft_x_forward(const Group &g);
 void fft_y(const Group &g, int inverse);
 void fft_conv(const Group &g, bool energy);
@@ -525,7 +518,6 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v);
 void mfe_sync_view(mfe_engine *e);   // throws HipError
 int mfe_map_flag_error(mfe_engine *e, int bits);
 void mfe_allow_big_spread_lds(int prec, int bytes);
-void mfe_allow_big_fft_lds(int bytes);
 void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt, bool skin,
                        bool bd);
 void mfe_launch_md_step(const mfe::Group &g, double dt, unsigned flags);
