@@ -165,6 +165,16 @@ int mfe_set_neighbor(mfe_engine *e, int32_t kind, double drnlist,
    Src/potential.F90:3561), p the integer power, eq the bond length. */
 int mfe_set_chains(mfe_engine *e, int32_t nchain, const int32_t *chain_first,
                    const int32_t *chain_len, double k, int32_t p, double eq);
+/* Bond angles of the chains for UAngle (Src/energy.F90:5409-5481, called
+   after UBond from UTotal, Src/energy.F90:387): angle(ict)%k, %p and %eq
+   of nmlPotentialChain, k per degree^2 and eq in degrees, scaled as
+   IOPotChain does (Src/potential.F90:3558-3562): k / (pi/180)^2 / p
+   whatever p is, eq * pi/180.  k = 0 (the reference's
+   default) leaves the term out.  u%angle enters u%tot and carries no
+   virial; read it with mfe_get_angle_energy. */
+int mfe_set_angles(mfe_engine *e, double k, int32_t p, double eq);
+/* u%angle of the last call that reduced energies (kJ/mol, like u%bond). */
+int mfe_get_angle_energy(mfe_engine *e, double *u_angle);
 
 /* ---- state ------------------------------------------------------------ */
 /* ro(3,np), positions inside the box (Src/mol.F90:1087). */

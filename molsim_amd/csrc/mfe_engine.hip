@@ -178,7 +178,7 @@ void layout_partials(mfe_engine *e) {
   int off = 0;
   e->off_pair = off; off += e->nblk_pair * (e->nptpt + 1);
   e->off_conv = off; off += 2 * conv;
-  e->off_bond = off; off += 2 * nb;
+  e->off_bond = off; off += 3 * nb;
   e->off_surf = off; off += 3 * nb;
   e->off_std = off; off += 2 * e->nkvec;
   e->off_move = off; off += nb;
@@ -322,6 +322,7 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   MFE_SETP(v.k_vec, e->k_vec); MFE_SETP(v.k_sum, e->k_sum); v.nkvec = e->nkvec;
   v.nchain = e->nchain; v.bond_p = e->bond_p; MFE_SETP(v.ch_prev, e->ch_prev); MFE_SETP(v.ch_next, e->ch_next);
   v.bond_k = e->bond_k; v.bond_eq = e->bond_eq;
+  v.ang_k = e->ang_k; v.ang_eq = e->ang_eq; v.ang_p = e->ang_p;
   MFE_SETP(v.bd_fac1, e->bd_fac1); MFE_SETP(v.bd_fac2, e->bd_fac2); MFE_SETP(v.noise, e->bd_noise); v.seed = e->bd_seed;
   v.nknot = e->nknot;
   v.knot_r0 = e->knot_r0;
@@ -516,7 +517,7 @@ void energy_out(mfe_engine *e, mfe_energy *en) {
 }
 
 void fetch_energy(mfe_engine *e) {
-  MFE_HIP_TRY(hipMemcpyAsync(e->h_en, e->en, (8 + MFE_MAX_PAIRS) * sizeof(double),
+  MFE_HIP_TRY(hipMemcpyAsync(e->h_en, e->en, mfe::kEnLen * sizeof(double),
                              hipMemcpyDeviceToHost, e->stream));
 }
 
@@ -545,13 +546,13 @@ int mfe_create(const mfe_config *cfg, mfe_engine **out) {
     dalloc(e->rebuild_sum, (size_t)e->ncu);
     mfe::launch_fill_int((int *)e->rebuild_bar, 2, 0, e->stream);
     dalloc(e->flags, 4);
-    dalloc(e->en, 8 + MFE_MAX_PAIRS);
-    MFE_HIP_TRY(hipHostMalloc((void **)&e->h_en, (8 + MFE_MAX_PAIRS) * sizeof(double)));
+    dalloc(e->en, mfe::kEnLen);
+    MFE_HIP_TRY(hipHostMalloc((void **)&e->h_en, mfe::kEnLen * sizeof(double)));
     MFE_HIP_TRY(hipHostMalloc((void **)&e->h_flags, 4 * sizeof(int)));
     dalloc(e->bd_fac1, MFE_MAX_TYPES);
     dalloc(e->bd_fac2, MFE_MAX_TYPES);
     mfe::launch_fill_int(e->flags, 4, 0, e->stream);
-    mfe::launch_fill_double(e->en, 8 + MFE_MAX_PAIRS, 0.0, e->stream);
+    mfe::launch_fill_double(e->en, mfe::kEnLen, 0.0, e->stream);
     mfe::launch_fill_double(e->bd_fac1, MFE_MAX_TYPES, 0.0, e->stream);
     mfe::launch_fill_double(e->bd_fac2, MFE_MAX_TYPES, 0.0, e->stream);
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
@@ -983,6 +984,30 @@ int mfe_set_chains(mfe_engine *e, int32_t nchain, const int32_t *chain_first,
     touch(e);
     return MFE_OK;
   });
+}
+
+// UAngle's parameters (Src/energy.F90:5409-5481) for the chains of
+// mfe_set_chains: the namelist force constant k (per degree^2) and
+// equilibrium angle eq (degrees) scaled as IOPotChain does
+// (Src/potential.F90:3558-3562): k / (pi/180)^2 / p and eq * pi/180.
+int mfe_set_angles(mfe_engine *e, double k, int32_t p, double eq) {
+  if (!e) return MFE_E_BAD_PARAM;
+  if (!e->have_particles) return fail(e, MFE_E_STATE, "mfe_set_particles first");
+  if (p < 1 || !(k >= 0.0) || !std::isfinite(k) || !std::isfinite(eq))
+    return fail(e, MFE_E_BAD_PARAM, "angle: k < 0, p < 1 or not finite");
+  const double sclang = 3.14159265358979323846 / 180.0;  // DegToRad
+  e->ang_k = k / (sclang * sclang) / p;
+  e->ang_eq = eq * sclang;
+  e->ang_p = p;
+  touch(e);
+  return MFE_OK;
+}
+
+// u%angle of the last force call that reduced energies
+int mfe_get_angle_energy(mfe_engine *e, double *u_angle) {
+  if (!e || !u_angle) return MFE_E_BAD_PARAM;
+  *u_angle = e->h_en ? e->h_en[mfe::kEnAngle] : 0.0;
+  return MFE_OK;
 }
 
 int mfe_upload_positions(mfe_engine *e, const double *ro) {

@@ -32,6 +32,11 @@
 
 namespace mfe {
 
+// an engine's energy vector: [tot, twob0, rec, bond, virial, S(3),
+// twob(1..MFE_MAX_PAIRS), angle]
+constexpr int kEnAngle = 8 + MFE_MAX_PAIRS;
+constexpr int kEnLen = kEnAngle + 1;
+
 // lanes of the pair kernel that share one particle's neighbour list
 constexpr int kPairLanes = 4;
 
@@ -251,6 +256,10 @@ struct SysDev {
   MFE_DPTR(const int) ch_prev;
   MFE_DPTR(const int) ch_next;
   double bond_k, bond_eq;
+  // UAngle: k already divided by (pi/180)^2 and p, eq in rad; ang_k == 0
+  // leaves the angle term out, as the reference does
+  double ang_k, ang_eq;
+  int ang_p;
   // Brownian dynamics: fac1/fac2 per type, noise (host-supplied) or the
   // counter-based generator keyed by seed
   MFE_DPTR(const double) bd_fac1;
@@ -399,6 +408,8 @@ struct mfe_engine {
   int *ch_prev = nullptr, *ch_next = nullptr;
   double bond_k = 0.0, bond_eq = 0.0;
   int bond_p = 2;
+  double ang_k = 0.0, ang_eq = 0.0;
+  int ang_p = 2;
 
   // reductions
   double *part = nullptr;

@@ -1529,9 +1529,49 @@ __global__ void __launch_bounds__(kBlock) k_ewald_force(const SysDev *__restrict
 }
 
 // ------------------------------------------------------------------ bonds
+// One bond angle of UAngle (Src/energy.F90:5409-5481), centred on bead c
+// with neighbours jm (earlier) and jp (later): unit vectors from jm and jp
+// to c, theta = acos(clamped cosine), u = k (theta - eq)^p with k already
+// divided by p.  fm and fp are the forces on jm and jp; c receives
+// -(fm + fp).  sin(theta) = 0 divides by zero, as in the reference.
+__device__ __forceinline__ double angle_at(const double *__restrict__ ro, int c, int jm, int jp,
+                                           const Box &b, double k, double eq, int p,
+                                           double fm[3], double fp[3]) {
+  double dxm = ro[3 * c] - ro[3 * jm], dym = ro[3 * c + 1] - ro[3 * jm + 1],
+         dzm = ro[3 * c + 2] - ro[3 * jm + 2];
+  const double dmi = 1.0 / sqrt(pbcr2(dxm, dym, dzm, b));
+  dxm *= dmi;
+  dym *= dmi;
+  dzm *= dmi;
+  double dxp = ro[3 * c] - ro[3 * jp], dyp = ro[3 * c + 1] - ro[3 * jp + 1],
+         dzp = ro[3 * c + 2] - ro[3 * jp + 2];
+  const double dpi = 1.0 / sqrt(pbcr2(dxp, dyp, dzp, b));
+  dxp *= dpi;
+  dyp *= dpi;
+  dzp *= dpi;
+  const double cosine = dxm * dxp + dym * dyp + dzm * dzp;
+  const double theta = acos(fmax(-1.0, fmin(cosine, 1.0)));
+  const double term = theta - eq;
+  double pw = 1.0;
+  for (int e = 1; e < p; ++e) pw *= term;
+  double fac = k * pw;
+  const double u = fac * term;
+  fac = -p * fac / sin(theta);
+  fm[0] = (fac * dmi) * (dxp - cosine * dxm);
+  fm[1] = (fac * dmi) * (dyp - cosine * dym);
+  fm[2] = (fac * dmi) * (dzp - cosine * dzm);
+  fp[0] = (fac * dpi) * (dxm - cosine * dxp);
+  fp[1] = (fac * dpi) * (dym - cosine * dyp);
+  fp[2] = (fac * dpi) * (dzm - cosine * dzp);
+  return u;
+}
+
 // UBond (Src/energy.F90:5335-5372): u = k (r - eq)^p with k already
 // divided by p; each particle gathers the forces of its (at most two)
-// bonds, and the bond to the next bead carries the energy.
+// bonds, and the bond to the next bead carries the energy.  With an angle
+// force constant, also UAngle (Src/energy.F90:5409-5481): each particle
+// gathers the forces of the (at most three) angles it takes part in, each
+// angle evaluated by angle_at, and the centre bead carries the energy.
 __global__ void __launch_bounds__(kBlock) k_bonds(const SysDev *__restrict__ sys,
                                                   int energy) {
   const SysDev &V = sys[blockIdx.z];
@@ -1547,7 +1587,7 @@ __global__ void __launch_bounds__(kBlock) k_bonds(const SysDev *__restrict__ sys
   double *part = V.part + V.off_bond;
   __shared__ double sh[kBlock];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double ub = 0.0, vir = 0.0;
+  double ub = 0.0, vir = 0.0, ua = 0.0;
   if (i < np) {
     double f[3] = {0.0, 0.0, 0.0};
     const int nb2[2] = {next[i], prev[i]};
@@ -1574,6 +1614,33 @@ __global__ void __launch_bounds__(kBlock) k_bonds(const SysDev *__restrict__ sys
       f[2] += sg * fac * dz;
       if (s == 0) vir -= fac * r2;
     }
+    if (V.ang_k != 0.0) {
+      const double ka = V.ang_k, eqa = V.ang_eq;
+      const int pa = V.ang_p;
+      double fm[3], fp[3];
+      const int jm = prev[i], jp = next[i];
+      // i as the later bead of the angle centred on jm
+      if (jm >= 0 && prev[jm] >= 0) {
+        (void)angle_at(ro, jm, prev[jm], i, b, ka, eqa, pa, fm, fp);
+        f[0] += fp[0];
+        f[1] += fp[1];
+        f[2] += fp[2];
+      }
+      // i as the centre
+      if (jm >= 0 && jp >= 0) {
+        ua += angle_at(ro, i, jm, jp, b, ka, eqa, pa, fm, fp);
+        f[0] -= fm[0] + fp[0];
+        f[1] -= fm[1] + fp[1];
+        f[2] -= fm[2] + fp[2];
+      }
+      // i as the earlier bead of the angle centred on jp
+      if (jp >= 0 && next[jp] >= 0) {
+        (void)angle_at(ro, jp, i, next[jp], b, ka, eqa, pa, fm, fp);
+        f[0] += fm[0];
+        f[1] += fm[1];
+        f[2] += fm[2];
+      }
+    }
     force[3 * i] += f[0];
     force[3 * i + 1] += f[1];
     force[3 * i + 2] += f[2];
@@ -1581,9 +1648,11 @@ __global__ void __launch_bounds__(kBlock) k_bonds(const SysDev *__restrict__ sys
   if (energy) {
     double s1 = block_sum(ub, sh);
     double s2 = block_sum(vir, sh);
+    double s3 = block_sum(ua, sh);
     if (threadIdx.x == 0) {
       part[blockIdx.x] = s1;
       part[V.nblk_np + blockIdx.x] = s2;
+      part[2 * V.nblk_np + blockIdx.x] = s3;
     }
   }
 }
@@ -1622,7 +1691,7 @@ __device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin,
   double *en = V.en;
   int *flags = V.flags;
   const int P = V.nptpt;
-  const int nc = P + 10;
+  const int nc = P + 11;
   const int nb = V.nblk_np;
   const int nrec = V.ewald == MFE_EWALD_SPME ? V.nblk_conv
                                              : (V.ewald == MFE_EWALD_STD ? V.nkvec : 0);
@@ -1630,7 +1699,8 @@ __device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin,
   __shared__ double red[kRedComp];
   if (bd && threadIdx.x == 0) flags[3] += 1;
   // component layout: [0,P) pair energies, P pair virial, P+1..P+2 rec
-  // (u, vir), P+3..P+4 bond (u, vir), P+5..P+7 S, P+8 max r2, P+9 cand
+  // (u, vir), P+3..P+4 bond (u, vir), P+5..P+7 S, P+8 max r2, P+9 cand,
+  // P+10 angle u
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int c = wave; c < nc; c += (int)(blockDim.x >> 6)) {
     const double *src;
@@ -1648,6 +1718,9 @@ __device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin,
     } else if (c <= P + 7) {
       src = V.part + V.off_surf + (size_t)(c - P - 5) * nb;
       n = V.lsurf ? nb : 0;
+    } else if (c == P + 10) {
+      src = V.part + V.off_bond + (size_t)2 * nb;
+      n = (energy && V.nchain > 0 && V.ang_k != 0.0) ? nb : 0;
     } else {
       src = V.part + (c == P + 8 ? V.off_move : V.off_skin);
       n = skin ? nb : 0;
@@ -1705,11 +1778,13 @@ __device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin,
     rec = V.efp * urec;
     vir += V.efp * virrec;
   }
-  en[0] = twob0 + rec + ubond;
+  const double uangle = v[P + 10];
+  en[0] = twob0 + rec + ubond + uangle;
   en[1] = twob0;
   en[2] = rec;
   en[3] = ubond;
   en[4] = vir;
+  en[kEnAngle] = uangle;
 }
 
 __global__ void __launch_bounds__(kReduceThreads) k_reduce(const SysDev *__restrict__ sys,

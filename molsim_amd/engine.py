@@ -76,7 +76,8 @@ class EwaldOpt(C.Structure):
 SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_create", "mfe_destroy", "mfe_last_error", "mfe_set_box",
            "mfe_set_particles", "mfe_set_tables", "mfe_set_ewald",
-           "mfe_set_neighbor", "mfe_set_chains", "mfe_upload_positions",
+           "mfe_set_neighbor", "mfe_set_chains", "mfe_set_angles",
+           "mfe_get_angle_energy", "mfe_upload_positions",
            "mfe_upload_velocities", "mfe_download_state", "mfe_build_list",
            "mfe_compute", "mfe_step_md", "mfe_step_bd", "mfe_get_energy",
            "mfe_get_pairs", "mfe_time_stage", "mfe_set_bd", "mfe_download_forces",
@@ -110,6 +111,8 @@ def lib():
         L.mfe_set_ewald.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32, _ip,
                                     C.c_int32, C.c_int32, C.c_int32, C.c_double]
         L.mfe_set_neighbor.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32]
+        L.mfe_set_angles.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.c_double]
+        L.mfe_get_angle_energy.argtypes = [C.c_void_p, _dp]
         L.mfe_set_chains.argtypes = [C.c_void_p, C.c_int32, _ip, _ip, C.c_double,
                                      C.c_int32, C.c_double]
         L.mfe_upload_positions.argtypes = [C.c_void_p, _dp]
@@ -324,6 +327,17 @@ class ForceEngine:
         ln = _i32([c[1] for c in chains])
         self._check(self.L.mfe_set_chains(self.h, first.size, _i(first), _i(ln), float(k),
                                           int(p), float(eq)), "mfe_set_chains")
+
+    def set_angles(self, k, p, eq):
+        """UAngle of the chains: namelist k (per degree^2), p, eq (degrees)."""
+        self._check(self.L.mfe_set_angles(self.h, float(k), int(p), float(eq)),
+                    "mfe_set_angles")
+
+    def angle_energy(self):
+        """u%angle of the last call that reduced energies."""
+        u = np.zeros(1)
+        self._check(self.L.mfe_get_angle_energy(self.h, _d(u)), "mfe_get_angle_energy")
+        return float(u[0])
 
     # --- state
     def upload_positions(self, ro):

@@ -135,6 +135,21 @@ module mfe_mod
          real(c_double), value :: eq
       end function
 
+      integer(c_int) function mfe_set_angles(eng, k, p, eq) bind(c, name='mfe_set_angles')
+         import :: c_int, c_int32_t, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), value :: k
+         integer(c_int32_t), value :: p
+         real(c_double), value :: eq
+      end function
+
+      integer(c_int) function mfe_get_angle_energy(eng, u_angle) &
+            bind(c, name='mfe_get_angle_energy')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: u_angle
+      end function
+
       integer(c_int) function mfe_upload_positions(eng, ro) bind(c, name='mfe_upload_positions')
          import :: c_int, c_ptr, c_double
          type(c_ptr), value :: eng

@@ -92,13 +92,14 @@ class System(C.Structure):
                 ("spme", C.POINTER(Spme)), ("nchain", C.c_int),
                 ("chain_first", _ip), ("chain_len", _ip),
                 ("bond_k", C.c_double), ("bond_p", C.c_double),
-                ("bond_eq", C.c_double)]
+                ("bond_eq", C.c_double), ("angle_k", C.c_double),
+                ("angle_p", C.c_double), ("angle_eq", C.c_double)]
 
 
 class Energy(C.Structure):
     _fields_ = [("tot", C.c_double), ("twob0", C.c_double), ("rec", C.c_double),
                 ("bond", C.c_double), ("virial", C.c_double),
-                ("twob", C.c_double * 64)]
+                ("twob", C.c_double * 64), ("angle", C.c_double)]
 
 
 class GauState(C.Structure):
@@ -141,6 +142,8 @@ def _declare(L):
                                  _dp]
     L.orc_ubond.argtypes = [C.POINTER(Box), _dp, C.c_int, _ip, _ip, C.c_double,
                             C.c_double, C.c_double, _dp, _dp, _dp]
+    L.orc_uangle.argtypes = [C.POINTER(Box), _dp, C.c_int, _ip, _ip, C.c_double,
+                             C.c_double, C.c_double, _dp, _dp]
     L.orc_utotal.argtypes = [C.POINTER(System), _dp, _dp, C.POINTER(Energy)]
     L.orc_velver_move.argtypes = [C.POINTER(Box), C.c_int, C.c_double, _dp,
                                   _dp, _dp, _dp]
@@ -427,7 +430,7 @@ class Oracle:
                  ewald="spm", alpha=0.0, epsi_four_pi=None, ncut=8,
                  ncut_cube=False, lsurf=False, order=4, mesh=(32, 32, 32),
                  nlist="vlist", drnlist=2.0, chains=None, bond=(0.0, 2, 0.0),
-                 maxnneigh=None):
+                 maxnneigh=None, angle=(0.0, 2, 180.0)):
         self.boxlen = f64(np.broadcast_to(np.asarray(boxlen, float), (3,)))
         self.iptpn = i32(iptpn)
         self.az = f64(az)
@@ -454,6 +457,7 @@ class Oracle:
         else:
             self.chain_first = self.chain_len = None
         self.bond = bond
+        self.angle = angle          # namelist (k per degree^2, p, eq in degrees)
         self.nn = self.jl = None
         self.cells = None
 
@@ -498,6 +502,10 @@ class Oracle:
             # namelist k is divided by p at setup (Src/potential.F90:3561)
             kin, pw, eq = float(self.bond[0]), int(self.bond[1]), float(self.bond[2])
             s.bond_k, s.bond_p, s.bond_eq = kin / pw, float(pw), eq
+            # angle k per rad^p divided by p, eq in rad (Src/potential.F90:3558-3562)
+            sclang = 3.1415926535897932 / 180.0
+            ka, pa, eqa = float(self.angle[0]), int(self.angle[1]), float(self.angle[2])
+            s.angle_k, s.angle_p, s.angle_eq = ka / sclang ** 2 / pa, float(pa), eqa * sclang
         forceo = np.zeros(3 * self.np)
         e = Energy()
         rc = lib().orc_utotal(C.byref(s), _d(ro), _d(forceo), C.byref(e))
@@ -505,8 +513,25 @@ class Oracle:
             raise RuntimeError(f"UTotal error {rc}")
         nptpt = self.npt * (self.npt + 1) // 2
         en = dict(tot=e.tot, twob0=e.twob0, twob=np.array(e.twob[:nptpt]),
-                  rec=e.rec, bond=e.bond, virial=e.virial)
+                  rec=e.rec, bond=e.bond, angle=e.angle, virial=e.virial)
         return forceo.reshape(self.np, 3), en
+
+
+def uangle(boxlen, ro, chains, k, p, eq):
+    """UAngle (Src/energy.F90:5409-5481) of chains [(first, len), ...]
+    (1-based) with the namelist k (per degree^2 whatever p is), p and eq (degrees).
+    Returns (force[np,3] in kJ/mol/A, u_angle)."""
+    ro = f64(ro).reshape(-1)
+    b = make_box(boxlen)
+    first = i32([c[0] for c in chains])
+    ln = i32([c[1] for c in chains])
+    sclang = 3.1415926535897932 / 180.0
+    force = np.zeros_like(ro)
+    u = C.c_double(0.0)
+    lib().orc_uangle(C.byref(b), _d(ro), first.size, _i(first), _i(ln),
+                     float(k) / sclang ** 2 / int(p), float(int(p)), float(eq) * sclang,
+                     _d(force), C.byref(u))
+    return force.reshape(-1, 3), u.value
 
 
 def velver_move(boxlen, tstep, ro, rod, rodd):

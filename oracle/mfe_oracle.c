@@ -1204,6 +1204,69 @@ void orc_ubond(const orc_box *b, const double *ro, int nchain,
   *virial += vir;
 }
 
+/* UAngle (Src/energy.F90:5409-5481): k already divided by (pi/180)^2 and p,
+ * eq in rad.  The loop carries the previous bond's unit vector, reversed,
+ * as the reference does; no virial. */
+void orc_uangle(const orc_box *b, const double *ro, int nchain,
+                const int *chain_first, const int *chain_len, double k,
+                double p, double eq, double *force, double *u_angle) {
+  double ua = 0.0;
+  int ip_pow = (int)p;
+  if (k == 0.0) {
+    *u_angle = 0.0;
+    return;
+  }
+  for (int ic = 0; ic < nchain; ++ic) {
+    if (chain_len[ic] < 3) continue;
+    int jm = chain_first[ic] - 1, ip = chain_first[ic];
+    double dxm = ro[3 * ip] - ro[3 * jm], dym = ro[3 * ip + 1] - ro[3 * jm + 1];
+    double dzm = ro[3 * ip + 2] - ro[3 * jm + 2], r2;
+    orc_pbcr2(b, &dxm, &dym, &dzm, &r2);
+    double dmi = 1.0 / sqrt(r2);
+    dxm = dxm * dmi;
+    dym = dym * dmi;
+    dzm = dzm * dmi;
+    for (int iseg = 2; iseg <= chain_len[ic] - 1; ++iseg) {
+      int jp = chain_first[ic] - 1 + iseg;
+      double dxp = ro[3 * ip] - ro[3 * jp], dyp = ro[3 * ip + 1] - ro[3 * jp + 1];
+      double dzp = ro[3 * ip + 2] - ro[3 * jp + 2];
+      orc_pbcr2(b, &dxp, &dyp, &dzp, &r2);
+      double dpi = 1.0 / sqrt(r2);
+      dxp = dxp * dpi;
+      dyp = dyp * dpi;
+      dzp = dzp * dpi;
+      double cosine = dxm * dxp + dym * dyp + dzm * dzp;
+      double theta = acos(fmax(-1.0, fmin(cosine, 1.0)));
+      double term = theta - eq;
+      double fac = k * ipow(term, ip_pow - 1);
+      ua = ua + fac * term;
+      fac = -ip_pow * fac / sin(theta);
+      double fxm = (fac * dmi) * (dxp - cosine * dxm);
+      double fym = (fac * dmi) * (dyp - cosine * dym);
+      double fzm = (fac * dmi) * (dzp - cosine * dzm);
+      double fxp = (fac * dpi) * (dxm - cosine * dxp);
+      double fyp = (fac * dpi) * (dym - cosine * dyp);
+      double fzp = (fac * dpi) * (dzm - cosine * dzp);
+      force[3 * jm] += fxm;
+      force[3 * jm + 1] += fym;
+      force[3 * jm + 2] += fzm;
+      force[3 * jp] += fxp;
+      force[3 * jp + 1] += fyp;
+      force[3 * jp + 2] += fzp;
+      force[3 * ip] -= (fxm + fxp);
+      force[3 * ip + 1] -= (fym + fyp);
+      force[3 * ip + 2] -= (fzm + fzp);
+      jm = ip;
+      ip = jp;
+      dmi = dpi;
+      dxm = -dxp;
+      dym = -dyp;
+      dzm = -dzp;
+    }
+  }
+  *u_angle = ua;
+}
+
 /* ---------------------------------------------------------------------- */
 /* UTotal: Src/energy.F90:280-434 (monoatomic, charged, serial)           */
 /* ---------------------------------------------------------------------- */
@@ -1248,6 +1311,9 @@ int orc_utotal(const orc_system *s, const double *ro, double *forceo,
     orc_ubond(&s->box, ro, s->nchain, s->chain_first, s->chain_len, s->bond_k,
               s->bond_p, s->bond_eq, force, &e->bond, &virial);
     e->tot = e->tot + e->bond;
+    orc_uangle(&s->box, ro, s->nchain, s->chain_first, s->chain_len,
+               s->angle_k, s->angle_p, s->angle_eq, force, &e->angle);
+    e->tot = e->tot + e->angle;
   }
   double fac = orc_force_unit_fac();
   for (int i = 0; i < 3 * np; ++i) forceo[i] = force[i] * fac;

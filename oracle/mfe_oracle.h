@@ -165,6 +165,13 @@ void orc_ewald_surf(const double boxlen[3], int na, const double *r,
                     const double *az, double epsi_four_pi, double *force,
                     double *u_rec, double *virrec);
 
+/* ---- angles (Src/energy.F90:5409-5481) ---- */
+/* k and eq in the stored units of IOPotChain (Src/potential.F90:3558-3562):
+ * k = namelist k / (pi/180)^2 / p, eq = namelist eq * pi/180. */
+void orc_uangle(const orc_box *b, const double *ro, int nchain,
+                const int *chain_first, const int *chain_len, double k,
+                double p, double eq, double *force, double *u_angle);
+
 /* ---- bonds (Src/energy.F90:5335-5400) ---- */
 /* Chains are runs of consecutive particles: chain c spans particles
  * chain_first[c] .. chain_first[c]+chain_len[c]-1 (1-based).  k is the
@@ -196,10 +203,12 @@ typedef struct {
   int nchain;                /* bonds */
   const int *chain_first, *chain_len;
   double bond_k, bond_p, bond_eq;
+  double angle_k, angle_p, angle_eq; /* angles (k = 0: none), stored units */
 } orc_system;
 typedef struct {
   double tot, twob0, rec, bond, virial;
   double twob[64];           /* u%twob(1:nptpt) */
+  double angle;              /* u%angle */
 } orc_energy;
 /* Evaluates UTotal.  forceo receives force*fac (kJ/mol/A -> internal
  * units of Src/energy.F90:408). Returns 0 or an error code. */

@@ -154,6 +154,45 @@ def test_spme_c2_vs_oracle(prec, ftol, etol):
     eng.close()
 
 
+@pytest.mark.parametrize("prec,ftol,etol", [("fp64", 1e-10, 1e-11), ("mixed", 1e-5, 1e-6)])
+@pytest.mark.parametrize("angle", [(0.02, 2, 109.5), (0.0005, 3, 150.0)])
+def test_utotal_with_angles_vs_oracle(prec, ftol, etol, angle):
+    """UAngle (Src/energy.F90:5409-5481) on the chains of the polyelectrolyte
+    system: u%angle within 1e-12 of the oracle on both precision paths (the
+    angle term is FP64 on both), u%tot and forces within the path's bound,
+    the virial unchanged by the angles, and the angle term's own forces
+    (engine with minus engine without) against the oracle's UAngle."""
+    s = S.c2()
+    say("case: UTotal with angles, c2 chains", prec, angle)
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    eng = S.configure(s, prec, tables=tab)
+    f0, en0 = eng.utotal()
+    eng.set_angles(*angle)
+    f, en = eng.utotal()
+    ua = eng.angle_energy()
+    orc = oracle_for(s, tab)
+    orc.angle = angle
+    orc.set_list(s.ro)
+    fo, eo = orc.utotal(s.ro)
+    say(f"  u%angle {ua:.15e} vs {eo['angle']:.15e}  force rel-RMS {rel_rms(f, fo):.3e}")
+    assert eo["angle"] != 0.0   # negative where an odd power meets theta < eq
+    assert abs(ua - eo["angle"]) <= 1e-12 * abs(eo["angle"])
+    assert abs(en.tot - eo["tot"]) <= etol * abs(eo["tot"])
+    assert abs((en.tot - en0.tot) - ua) <= 1e-9 * abs(en.tot)
+    # FP64 spreading's atomics reorder sums between calls (~1e-16)
+    assert abs(en.virial - en0.virial) <= 1e-12 * abs(en0.virial)
+    assert rel_rms(f, fo) <= ftol
+    fa, _ = O.uangle(s.boxlen, s.ro, s.chains, *angle)
+    assert rel_rms(f - f0, fa * E.force_unit_fac()) <= 1e-8
+    # switching the angles off restores the plain result
+    eng.set_angles(0.0, 2, 180.0)
+    f2, en2 = eng.utotal()
+    assert eng.angle_energy() == 0.0
+    assert abs(en2.tot - en0.tot) <= 1e-12 * abs(en0.tot)
+    eng.close()
+
+
 @pytest.mark.parametrize("mesh,order", [(32, 4), (48, 5), (40, 6), (24, 8), (18, 3), (36, 7)])
 def test_spme_meshes_and_orders(mesh, order):
     """Mixed-radix meshes (2, 3, 5) and every supported spline order."""
