@@ -222,6 +222,36 @@ def ewald_options():
         shutil.rmtree(work, ignore_errors=True)
 
 
+def chain_angles():
+    """UTotal of the chain case's probe positions (chains.npz) with bond
+    angles on (UAngle, Src/energy.F90:5409-5481): for each (k, p, eq) of the
+    namelist, the reference's force, forceo and energies -> chain_angles.npz."""
+    z = np.load(os.path.join(HERE, "chains.npz"))
+    ch = [("pe", 40, 30.0, -1.0, 4.0, KT298), ("cat", 40, 23.0, 1.0, 4.0, KT298)]
+    k_bond = float(z["bond"][0])
+    angles = [(0.02, 2, 109.5), (0.0005, 3, 150.0)]
+    work = tempfile.mkdtemp(prefix="mfe_golden_")
+    try:
+        res = {}
+        for a, (ka, pa, eqa) in enumerate(angles):
+            text = (system_nml("golden chain angles", None, 60.0, nstep2=1) +
+                    particle_nml(ch, chain=(1, 4, 10)) + potential_nml(2, 20.0, ncut=5) +
+                    f" &nmlPotentialChain bond = {k_bond!r}, 2, 5.0,"
+                    f" angle = {ka!r}, {pa}, {eqa!r} /\n"
+                    " &nmlSetConfiguration txsetconf='chainrandom','random', radatset=1.6,1.6 /\n"
+                    " &nmlMD integ='velver', tstep=0.002, nvvite=2, lsetvel=.true. /\n"
+                    " &nmlIntList txintlist='vlist', inlist=0, drnlist=2.0 /\n")
+            d = case(f"ang{a}", text, work, probe_pos=z["probe_pos"], md=False)
+            f = read_force(os.path.join(d, f"ang{a}.probe_f"), 3)
+            for k_ in ("force", "forceo", "u_tot", "u_twob", "u_rec", "u_bond", "virial"):
+                res[f"{k_}{a}"] = f[k_]
+            print("angles", (ka, pa, eqa), "u_tot", f["u_tot"], "u_bond", f["u_bond"])
+        np.savez_compressed(os.path.join(HERE, "chain_angles.npz"),
+                            angles=np.array(angles, float), **res)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     if not os.path.exists(PROBE):
         sys.exit("build the reference first: make -f oracle/ref.mk")
@@ -233,6 +263,9 @@ def main():
         return
     if sys.argv[1:] == ["ewald"]:
         ewald_options()
+        return
+    if sys.argv[1:] == ["angles"]:
+        chain_angles()
         return
     work = tempfile.mkdtemp(prefix="mfe_golden_")
     try:

@@ -31,7 +31,7 @@ def say(*a):
 def golden():
     here = os.path.join(os.path.dirname(__file__), "golden")
     return {k: np.load(os.path.join(here, k + ".npz"))
-            for k in ("c1_std", "c1_cell_surf", "chains")}
+            for k in ("c1_std", "c1_cell_surf", "chains", "chain_angles")}
 
 
 def c1_from_golden(z):
@@ -128,6 +128,32 @@ def test_utotal_bonds_vs_reference(golden):
     assert abs(en.bond - float(z["ref_u_bond"])) <= 1e-12 * max(abs(float(z["ref_u_bond"])), 1e-3)
     assert rel_rms(f, z["ref_forceo"]) <= 2e-6
     assert abs(en.tot - float(z["ref_u_tot"])) <= 1e-7 * abs(float(z["ref_u_tot"]))
+    eng.close()
+
+
+@pytest.mark.parametrize("a", [0, 1])
+def test_utotal_angles_vs_reference(golden, a):
+    """UAngle on the golden chains against the reference run with bond
+    angles on (p = 2 and p = 3): u%angle (the reference's u%tot less its
+    u%tot without angles) and u%tot, forces within the FP64 path's bound."""
+    z, za = golden["chains"], golden["chain_angles"]
+    ka, pa, eqa = za["angles"][a]
+    s = S.System("chains", 60.0, z["probe_pos"].copy(), np.zeros_like(z["probe_pos"]),
+                 np.repeat([1, 2], z["nppt"]).astype(np.int32), z["zat"], z["sigat"],
+                 z["epsat"], np.array([30.0, 23.0]), rcut=20.0, alpha=float(z["alpha"]),
+                 ewald="std", ncut=5, chains=[(1 + c * 10, 10) for c in range(4)],
+                 bond=(float(z["bond"][0]), 2, float(z["bond"][2])))
+    say("case: UTotal with angles, golden chains", (ka, pa, eqa))
+    eng = S.configure(s, "fp64")
+    eng.set_angles(float(ka), int(pa), float(eqa))
+    f, en = eng.utotal()
+    ref_tot = float(za[f"u_tot{a}"])
+    ref_angle = ref_tot - float(z["ref_u_tot"])
+    ua = eng.angle_energy()
+    say(f"  u%angle {ua:.15e} vs reference {ref_angle:.15e}")
+    assert abs(ua - ref_angle) <= 1e-12 * abs(ref_tot)
+    assert abs(en.tot - ref_tot) <= 1e-7 * abs(ref_tot)
+    assert rel_rms(f, za[f"forceo{a}"]) <= 2e-6
     eng.close()
 
 

@@ -136,6 +136,37 @@ def test_utotal_chains_bonds_bitwise(golden_dir):
     assert e["virial"] == z["ref_virial"]
 
 
+@pytest.mark.parametrize("a", [0, 1])
+def test_utotal_chains_angles_bitwise(golden_dir, a):
+    """The chain case with bond angles on (UAngle, Src/energy.F90:5409-5481),
+    p = 2 and p = 3, against the reference run at the same positions
+    (tests/golden/chain_angles.npz, drawn by the reference probe)."""
+    z = load(golden_dir, "chains.npz")
+    za = load(golden_dir, "chain_angles.npz")
+    ka, pa, eqa = za["angles"][a]
+    efp = float(z["epsi_four_pi"])
+    t = O.build_tables(z["zat"], z["sigat"], z["epsat"], rcut=20.0,
+                       epsi_four_pi=efp, alpha=float(z["alpha"]))
+    iptpn = np.repeat([1, 2], z["nppt"])
+    az = np.repeat(z["zat"], z["nppt"])
+    nper = int(z["nper"])
+    chains = [(1 + c * nper, nper) for c in range(int(z["nchain"]))]
+    orc = O.Oracle(60.0, iptpn, az, t, 20.0, ewald="std", alpha=float(z["alpha"]),
+                   epsi_four_pi=efp, ncut=5, chains=chains, bond=tuple(z["bond"]),
+                   angle=(float(ka), int(pa), float(eqa)))
+    pos = z["probe_pos"]
+    orc.set_list(pos)
+    f, e = orc.utotal(pos)
+    # the reference's u%angle is its u%tot less the terms without angles
+    ref_angle = za[f"u_tot{a}"] - z["ref_u_tot"]
+    assert e["angle"] != 0.0
+    assert abs(e["angle"] - ref_angle) <= 1e-12 * abs(za[f"u_tot{a}"])
+    assert e["tot"] == za[f"u_tot{a}"]
+    assert e["bond"] == za[f"u_bond{a}"]
+    assert e["virial"] == za[f"virial{a}"]
+    assert np.array_equal(f, za[f"forceo{a}"])
+
+
 def test_velver_trajectory_bitwise(golden_dir):
     """Five VelVer steps (Src/md.F90:252-410) with the skin-test rebuilds.
 
