@@ -202,7 +202,8 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo,
    spreads the mesh again).  The engine keeps the current configuration's
    mesh and E_rec, as the reference keeps QMesh and urecold: reuse_current
    = 1 starts from them and skips spreading and transforming the current
-   configuration (the caller states that nothing was accepted since); the
+   configuration (the caller states that nothing was accepted since, other
+   than through mfe_mc_accept); the
    engine still spreads anew after any upload, step or setup call, and on
    its first call.  FP64
    engines give the difference to ~1e-12 of |E_rec|; mixed ones subtract
@@ -222,6 +223,19 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm,
    caller's to keep current (the reference's NList policy). */
 int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm,
                       const double *rtm, double *du_twob, int32_t *overlap);
+/* Monte Carlo accepted move: EwaldUpdateArray for 'spm'
+   (Src/denergy.F90:1776-1819: QMesh = QMeshTM, urecold = urecnew) with the
+   MC driver's position update ro(:,ianatm) = rtm and NList's skin
+   bookkeeping (the list is rebuilt on the device only when the
+   reference's skin test fires, as after mfe_upload_positions).  rtm must
+   lie inside the box.  On an FP64 engine whose last mfe_mc_delta_rec was
+   this same move, its trial mesh and E_rec become the current ones, so the
+   next trial spreads nothing; otherwise (any other move, or a mixed
+   engine, whose FP32 tiles would keep the exchange's rounding) the next
+   trial spreads anew.  reuse_current = 1 is valid after this call in both
+   cases. */
+int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm,
+                  const double *rtm);
 /* nsteps of MDDriver(iSimulationStep) + NList(iSimulationStep): VelVer
    (Src/md.F90:252-410) with the skin-test rebuild (Src/nlist.F90:597-616),
    all on the device.  flags select which UTotal terms are reduced each

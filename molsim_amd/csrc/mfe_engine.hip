@@ -1233,7 +1233,71 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
     MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out, sizeof(h), hipMemcpyDeviceToHost, e->stream));
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     e->mc_cached = true;
+    e->mc_last_ia = ia;
+    e->mc_last_rtm.assign(rtm, rtm + 3 * (size_t)natm);
     *du_rec = e->efp * (h[1] - h[0]);
+    return MFE_OK;
+  });
+}
+
+// EwaldUpdateArray (Src/denergy.F90:1776-1819, 'spm': QMesh = QMeshTM,
+// urecold = urecnew) with the MC driver's position update of an accepted
+// move (ro(:,ianatm) = rtm) and NList's skin bookkeeping.  On an FP64
+// engine whose last mfe_mc_delta_rec was this move, its trial mesh (the
+// kept tiles with the moved stencils exchanged again, before the positions
+// change) and its E_rec become the current ones, so the next trial with
+// reuse_current spreads nothing; otherwise the next trial spreads anew.
+int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const double *rtm) {
+  if (!e || !ianatm || !rtm || natm < 1) return MFE_E_BAD_PARAM;
+  std::vector<int> ia;
+  int rc = mc_prepare(e, natm, ianatm, rtm, ia);
+  if (rc) return rc;
+  for (int a = 0; a < natm; ++a)
+    for (int m = 0; m < 3; ++m)
+      if (!(std::fabs(rtm[3 * a + m]) <= e->box.L2[m]))
+        return fail(e, MFE_E_BAD_PARAM, "rtm outside the box [-L/2, L/2]");
+  if (!e->list_valid || !e->have_positions || !e->have_neighbor)
+    return fail(e, MFE_E_STATE, "positions and neighbour list first");
+  // FP64 only: mixed tiles are FP32, and exchanging stencils in them
+  // would leave FP32 rounding that a fresh spread does not have
+  const bool keep = e->prec == MFE_PREC_FP64 && e->mc_cached && e->have_ewald &&
+                    e->ewald == MFE_EWALD_SPME &&
+                    e->mc_tiles_bytes == e->spread_tiles_bytes && e->mc_last_ia == ia &&
+                    e->mc_last_rtm.size() == 3 * (size_t)natm &&
+                    std::equal(e->mc_last_rtm.begin(), e->mc_last_rtm.end(), rtm);
+  e->mc_cached = false;
+  e->mc_last_ia.clear();
+  e->mc_last_rtm.clear();
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    mc_buffers(e, natm);
+    upload(e->mc_ia, ia.data(), (size_t)natm, e->stream);
+    upload(e->mc_rtm, rtm, 3 * (size_t)natm, e->stream);
+    sync_view(e);
+    const mfe::Group &G = e->grp;
+    if (keep) {
+      // trial tiles from the current ones (old stencils read from ro,
+      // which still holds the old positions), then kept as current
+      MFE_HIP_TRY(hipMemcpyAsync(e->spread_tiles, e->mc_tiles, e->mc_tiles_bytes,
+                                 hipMemcpyDeviceToDevice, e->stream));
+      mfe::launch_mc_stencils(G, natm, e->mc_ia, e->mc_rtm);
+      MFE_HIP_TRY(hipMemcpyAsync(e->mc_tiles, e->spread_tiles, e->mc_tiles_bytes,
+                                 hipMemcpyDeviceToDevice, e->stream));
+      MFE_HIP_TRY(hipMemcpyAsync(e->mc_out, e->mc_out + 1, sizeof(double),
+                                 hipMemcpyDeviceToDevice, e->stream));
+    }
+    // the accepted positions, then NList(iSimulationStep) as after an upload
+    if (!e->ro_stage) dalloc(e->ro_stage, 3 * (size_t)e->max_np);
+    MFE_HIP_TRY(hipMemcpyAsync(e->ro_stage, e->ro, 3 * (size_t)e->np * sizeof(double),
+                               hipMemcpyDeviceToDevice, e->stream));
+    mfe::launch_mc_place(G, natm, e->mc_ia, e->mc_rtm, e->ro_stage);
+    mfe::launch_set_positions(G, e->ro_stage);
+    mfe::launch_skin(G);
+    mfe::launch_reduce(G, 0u, true, false);
+    mfe::launch_nlist_rebuild(G, true);
+    int r = read_flags(e);
+    if (r) return r;
+    e->mc_cached = keep;
     return MFE_OK;
   });
 }

@@ -388,6 +388,10 @@ struct mfe_engine {
   void *mc_tiles = nullptr;
   size_t mc_tiles_bytes = 0;
   bool mc_cached = false;
+  // the move of the last mfe_mc_delta_rec (0-based indices, positions):
+  // mfe_mc_accept of the same move keeps its trial mesh and E_rec
+  std::vector<int> mc_last_ia;
+  std::vector<double> mc_last_rtm;
   int *b_done = nullptr;
   void *rec_frac = nullptr;  // per sorted particle: (fx, fy, fz, q)
   void *spread_tiles = nullptr;  // per brick: extended spreading tile
@@ -486,6 +490,10 @@ void launch_move(const Group &g, double dt, int bd);
 void launch_sort_pos(const Group &g);
 // Monte Carlo trial move: the natm stencils at the current positions of
 // particles ia[] taken out of the spreading tiles and those at rtm[] added
+// ro_stage[ia[a]] = rtm[a] for a < natm (a trial move written into a copy
+// of the positions)
+void launch_mc_place(const Group &g, int natm, const int *ia, const double *rtm,
+                     double *ro_stage);
 void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm);
 // the reciprocal energy of the last convolution pass (its block partials)
 void launch_rec_sum(const Group &g, double *out);

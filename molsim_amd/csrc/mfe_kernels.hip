@@ -1312,6 +1312,16 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
   }
 }
 
+// an accepted trial move written into a copy of the positions
+__global__ void __launch_bounds__(256) k_mc_place(int natm, const int *__restrict__ ia,
+                                                  const double *__restrict__ rtm,
+                                                  double *__restrict__ ro_stage) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= natm) return;
+  const int i = ia[a];
+  for (int m = 0; m < 3; ++m) ro_stage[3 * i + m] = rtm[3 * a + m];
+}
+
 // ------------------------------------------------------- MC trial move
 // DUTwoBodyEwaldRecSPM (Src/denergy.F90:1413-1495): the generalized charge
 // mesh of a trial move is the current one less the moved particles' old
@@ -2252,6 +2262,12 @@ void launch_nlist_rebuild(const Group &g, bool gated) {
 
 void launch_sort_pos(const Group &g) {
   hipLaunchKernelGGL(k_sort_pos, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d);
+}
+
+void launch_mc_place(const Group &g, int natm, const int *ia, const double *rtm,
+                     double *ro_stage) {
+  hipLaunchKernelGGL(k_mc_place, dim3((natm + 255) / 256), dim3(256), 0, g.stream, natm, ia, rtm,
+                     ro_stage);
 }
 
 void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm) {

@@ -85,7 +85,7 @@ SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_batch_failed_replica", "mfe_batch_build_list", "mfe_batch_compute",
            "mfe_batch_step_md", "mfe_batch_step_bd", "mfe_batch_get_energies",
            "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint", "mfe_mc_delta_rec",
-           "mfe_mc_delta_pair", "mfe_ewald_parameters"]
+           "mfe_mc_delta_pair", "mfe_mc_accept", "mfe_ewald_parameters"]
 
 _lib = None
 
@@ -122,6 +122,7 @@ def lib():
         L.mfe_compute.argtypes = [C.c_void_p, C.c_uint32, _dp, C.POINTER(EnergyStruct)]
         L.mfe_mc_delta_rec.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, C.c_int32, _dp]
         L.mfe_mc_delta_pair.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp, _ip]
+        L.mfe_mc_accept.argtypes = [C.c_void_p, C.c_int32, _ip, _dp]
         L.mfe_step_md.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_uint32]
         L.mfe_step_bd.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp, C.c_uint32]
         L.mfe_get_energy.argtypes = [C.c_void_p, C.POINTER(EnergyStruct)]
@@ -397,6 +398,16 @@ class ForceEngine:
         self._check(self.L.mfe_mc_delta_pair(self.h, ia.size, _i(ia), _d(r), _d(du), _i(ov)),
                     "mfe_mc_delta_pair")
         return float(du[0]), bool(ov[0])
+
+    def mc_accept(self, ianatm, rtm):
+        """Accepted move (EwaldUpdateArray + ro(:, ianatm) = rtm + NList):
+        after mc_delta_rec of the same move, the next mc_delta_rec may pass
+        reuse_current=True."""
+        ia = _i32(np.atleast_1d(ianatm))
+        r = _f64(np.asarray(rtm, dtype=np.float64).reshape(-1))
+        if r.size != 3 * ia.size:
+            raise ValueError("rtm must hold three coordinates per moved particle")
+        self._check(self.L.mfe_mc_accept(self.h, ia.size, _i(ia), _d(r)), "mfe_mc_accept")
 
     def step_md(self, nsteps, tstep, flags=FORCES | ENERGY | VIRIAL):
         self._check(self.L.mfe_step_md(self.h, int(nsteps), float(tstep), flags), "mfe_step_md")

@@ -298,6 +298,15 @@ module mfe_mod
          integer(c_int32_t), intent(out) :: overlap
       end function
 
+      ! accepted move: EwaldUpdateArray ('spm') + ro(:,ianatm) = rtm + NList
+      integer(c_int) function mfe_mc_accept(eng, natm, ianatm, rtm) bind(c, name='mfe_mc_accept')
+         import :: c_int, c_int32_t, c_ptr, c_double
+         type(c_ptr), value :: eng
+         integer(c_int32_t), value :: natm
+         integer(c_int32_t), intent(in) :: ianatm(*)
+         real(c_double), intent(in) :: rtm(*)
+      end function
+
       ! batched replicas: engines(nrep) are the handles of set-up engines
       integer(c_int) function mfe_batch_create(engines, nrep, batch) bind(c, name='mfe_batch_create')
          import :: c_int, c_int32_t, c_ptr

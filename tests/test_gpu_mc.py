@@ -142,3 +142,84 @@ def test_mc_delta_pair_vs_oracle(prec):
     du, ov = eng.mc_delta_pair([1], s.ro[[1]] + 0.1)
     assert ov and du == 0.0
     eng.close()
+
+
+def _wrap(r, L):
+    return r - L * np.round(r / L)
+
+
+def _free_move(eng, s, ro, rng, natm, step):
+    """A trial move that the reference's MC would not reject for overlap
+    (lhsoverlap of UTwoBodyANew), drawn again until it does not overlap."""
+    while True:
+        ia = rng.choice(s.np, natm, replace=False)
+        rtm = _wrap(ro[ia] + rng.uniform(-step, step, (natm, 3)), s.boxlen)
+        if not eng.mc_delta_pair(ia + 1, rtm)[1]:
+            return ia, rtm
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_mc_accept_keeps_the_trial_mesh(prec):
+    """EwaldUpdateArray (Src/denergy.F90:1776-1819) with the accepted
+    positions: after mc_accept of the move just tried, the next trial with
+    reuse_current equals a fresh spread of the accepted configuration
+    uploaded whole (FP64, from the kept trial mesh: <= 1e-12 |E_rec|; mixed,
+    which spreads anew after an accept: bitwise) and the oracle's energy
+    change; positions, forces and energies after the accepts equal those of
+    the uploaded configuration."""
+    s = S.c2()
+    print("case: MC accept c2", prec, flush=True)
+    eng = S.configure(s, prec)
+    ref = S.configure(s, prec)
+    eng.utotal()
+    ref.utotal()
+    rng = np.random.default_rng(23)
+    ro = s.ro.copy()
+    tol = 1e-9 if prec == "fp64" else 1e-5
+    for it, natm in enumerate((1, 3, 2)):
+        ia, rtm = _free_move(eng, s, ro, rng, natm, 2.0)
+        eng.mc_delta_rec(ia + 1, rtm, reuse_current=it > 0)
+        eng.mc_accept(ia + 1, rtm)
+        ro[ia] = rtm
+        ref.upload_positions(ro)
+        ib = rng.choice(s.np, 2, replace=False)
+        rtb = _wrap(ro[ib] + rng.uniform(-2.0, 2.0, (2, 3)), s.boxlen)
+        d_keep = eng.mc_delta_rec(ib + 1, rtb, reuse_current=True)
+        d_ref = ref.mc_delta_rec(ib + 1, rtb)
+        e0 = oracle_rec(s, ro)
+        ro_b = ro.copy()
+        ro_b[ib] = rtb
+        d_orc = oracle_rec(s, ro_b) - e0
+        print(f"  accept {it} natm {natm}: next trial kept {d_keep:.12e} "
+              f"fresh {d_ref:.12e} oracle {d_orc:.12e}", flush=True)
+        if prec == "mixed":
+            assert d_keep == d_ref
+        else:
+            assert abs(d_keep - d_ref) <= 1e-12 * abs(e0)
+        assert abs(d_keep - d_orc) <= tol * abs(e0)
+    # an accept of a move that was not the last one tried: the next trial
+    # spreads anew and still agrees
+    ia, rtm = _free_move(eng, s, ro, rng, 2, 1.5)
+    eng.mc_accept(ia + 1, rtm)
+    ro[ia] = rtm
+    ref.upload_positions(ro)
+    r0 = _wrap(ro[[0]] + 0.7, s.boxlen)
+    d_keep = eng.mc_delta_rec([1], r0, reuse_current=True)
+    d_ref = ref.mc_delta_rec([1], r0)
+    assert abs(d_keep - d_ref) <= 1e-12 * abs(oracle_rec(s, ro))
+    # the engine's state is that of the uploaded configuration
+    r1 = eng.download_state()[0].reshape(-1, 3)
+    assert np.array_equal(r1, ro)
+    f1, en1 = eng.utotal()
+    f2, en2 = ref.utotal()
+    if prec == "mixed":
+        assert np.array_equal(f1, f2)
+        assert en1.tot == en2.tot
+    else:
+        assert rel_rms(f1, f2) <= 1e-12
+        assert abs(en1.tot - en2.tot) <= 1e-12 * abs(en2.tot)
+    # a move outside the box is refused
+    with pytest.raises(RuntimeError):
+        eng.mc_accept([1], np.array([[0.6 * s.boxlen, 0.0, 0.0]]))
+    eng.close()
+    ref.close()
