@@ -1276,13 +1276,18 @@ int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const doub
     sync_view(e);
     const mfe::Group &G = e->grp;
     if (keep) {
-      // trial tiles from the current ones (old stencils read from ro,
-      // which still holds the old positions), then kept as current
-      MFE_HIP_TRY(hipMemcpyAsync(e->spread_tiles, e->mc_tiles, e->mc_tiles_bytes,
-                                 hipMemcpyDeviceToDevice, e->stream));
+      // the moved stencils exchanged in the kept tiles themselves (old
+      // stencils read from ro, which still holds the old positions): for
+      // this one launch the view's spreading tiles are the kept ones.
+      // sync_view waits for the stream, and the second view upload is
+      // ordered after the launch on the same stream.
+      std::swap(e->spread_tiles, e->mc_tiles);
+      e->sys_dirty = true;
+      sync_view(e);
       mfe::launch_mc_stencils(G, natm, e->mc_ia, e->mc_rtm);
-      MFE_HIP_TRY(hipMemcpyAsync(e->mc_tiles, e->spread_tiles, e->mc_tiles_bytes,
-                                 hipMemcpyDeviceToDevice, e->stream));
+      std::swap(e->spread_tiles, e->mc_tiles);
+      e->sys_dirty = true;
+      sync_view(e);
       MFE_HIP_TRY(hipMemcpyAsync(e->mc_out, e->mc_out + 1, sizeof(double),
                                  hipMemcpyDeviceToDevice, e->stream));
     }
