@@ -220,7 +220,11 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm,
    pair types (the units of mfe_energy.twob0); overlap = 1 when a trial
    pair is inside the hard core or below r2umin (the reference's
    lhsoverlap: the move is rejected; du_twob is then 0).  The list is the
-   caller's to keep current (the reference's NList policy). */
+   caller's to keep current (the reference's NList policy).  On an engine
+   set up with MFE_NL_CELL the partners are, as for UTwoBodyANewCellList /
+   UTwoBodyAOldCellList (Src/denergy.F90:728-928, chosen for lCellList at
+   :145-147), every particle with r2 < rcut2 of the new or old position,
+   however far the move. */
 int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm,
                       const double *rtm, double *du_twob, int32_t *overlap);
 /* Monte Carlo accepted move: EwaldUpdateArray for 'spm'

@@ -1404,21 +1404,38 @@ __global__ void __launch_bounds__(256) k_mc_pair(const SysDev *__restrict__ sys,
                                                 double *__restrict__ out) {
   const SysDev &V = sys[blockIdx.z];
   const int np = V.np;
+  // cell-list semantics (UTwoBodyANew/OldCellList, Src/denergy.F90:728-928):
+  // every particle within rcut of the old or new position, as the cells
+  // around each position hold them; Verlet list (UTwoBodyANew/Old): the
+  // moved particle's list partners
+  const bool all = V.nlsem == 1;
   double acc = 0.0;
   int ovl = 0, ovl_old = 0;
   for (int a = 0; a < natm; ++a) {
     const int i = ia[a], t = V.c_pos[i], ti = V.type[i];
     const double xo = V.ro[3 * i], yo = V.ro[3 * i + 1], zo = V.ro[3 * i + 2];
     const double xn = rtm[3 * a], yn = rtm[3 * a + 1], zn = rtm[3 * a + 2];
-    const int n = V.nn[t];
+    const int n = all ? np : V.nn[t];
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
-      const int j = V.nbr[(size_t)k * np + t];
-      const int jp = V.c_sorted[j];
+      int jp, tj;
+      double xj, yj, zj;
+      if (all) {
+        jp = k;
+        tj = V.type[k];
+        xj = V.ro[3 * k];
+        yj = V.ro[3 * k + 1];
+        zj = V.ro[3 * k + 2];
+      } else {
+        const int j = V.nbr[(size_t)k * np + t];  // cell-ordered slot
+        jp = V.c_sorted[j];
+        tj = V.type_s[j];
+        xj = V.ro_s[3 * j];
+        yj = V.ro_s[3 * j + 1];
+        zj = V.ro_s[3 * j + 2];
+      }
       bool moved = false;
       for (int b = 0; b < natm; ++b) moved |= ia[b] == jp;
       if (moved) continue;
-      const int tj = V.type_s[j];
-      const double xj = V.ro_s[3 * j], yj = V.ro_s[3 * j + 1], zj = V.ro_s[3 * j + 2];
       acc += mc_pair_u(V, ti, tj, xn - xj, yn - yj, zn - zj, ovl) -
              mc_pair_u(V, ti, tj, xo - xj, yo - yj, zo - zj, ovl_old);
     }

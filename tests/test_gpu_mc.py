@@ -144,6 +144,48 @@ def test_mc_delta_pair_vs_oracle(prec):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_mc_delta_pair_cell_list_vs_oracle(prec):
+    """UTwoBodyANewCellList - UTwoBodyAOldCellList (Src/denergy.F90:728-928,
+    chosen for lCellList at :145-147): every particle within rcut (r2 <
+    rcut2) of the old and the new position, including moves far beyond the
+    Verlet skin, against the oracle's UTwoBodyACellList energy of both
+    configurations (<= 1e-10 of |u_twob|)."""
+    from helpers import oracle_for
+    s = S.c2()
+    print("case: MC pair deltas, cell list, c2", prec, flush=True)
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    eng = S.configure(s, prec, tables=tab, nlist="cell")
+    eng.build_list()
+    orc = oracle_for(s, tab, nlist="cell")
+    orc.set_list(s.ro)
+    _, eo = orc.utotal(s.ro)
+    u0 = eo["twob0"]
+    rng = np.random.default_rng(9)
+    done = 0
+    for natm, step in ((1, 0.8), (3, 0.8), (1, 12.0), (4, 25.0)):
+        while True:
+            ia = rng.choice(s.np, natm, replace=False)
+            rtm = _wrap(s.ro[ia] + rng.uniform(-step, step, (natm, 3)), s.boxlen)
+            du, ov = eng.mc_delta_pair(ia + 1, rtm)
+            if not ov:
+                break
+        ro1 = s.ro.copy()
+        ro1[ia] = rtm
+        orc.set_list(ro1)
+        _, e1 = orc.utotal(ro1)
+        print(f"  natm {natm} step {step}: du {du:.10e} oracle {e1['twob0'] - u0:.10e}",
+              flush=True)
+        assert du != 0.0
+        assert abs(du - (e1["twob0"] - u0)) <= 1e-10 * abs(u0)
+        done += 1
+    assert done == 4
+    du, ov = eng.mc_delta_pair([1], s.ro[[1]] + 0.1)
+    assert ov and du == 0.0
+    eng.close()
+
+
 def _wrap(r, L):
     return r - L * np.round(r / L)
 
