@@ -1147,7 +1147,8 @@ void mc_buffers(mfe_engine *e, int natm) {
     dalloc(e->mc_rtm, 3 * (size_t)natm);
     e->mc_cap = natm;
   }
-  if (!e->mc_out) dalloc(e->mc_out, 4);
+  // [E_rec current, E_rec trial, du_twob, overlap, pair-delta block partials]
+  if (!e->mc_out) dalloc(e->mc_out, 4 + 2 * (size_t)mfe::kMcPairBlocks);
 }
 
 }  // namespace
@@ -1169,7 +1170,8 @@ int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm, const 
     upload(e->mc_ia, ia.data(), (size_t)natm, e->stream);
     upload(e->mc_rtm, rtm, 3 * (size_t)natm, e->stream);
     sync_view(e);
-    mfe::launch_mc_pair(e->grp, natm, e->mc_ia, e->mc_rtm, e->mc_out + 2);
+    mfe::launch_mc_pair(e->grp, natm, e->mc_ia, e->mc_rtm, e->mc_out + 2, e->mc_out + 4,
+                        e->nlkind == MFE_NL_CELL);
     double h[2];
     MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out + 2, sizeof(h), hipMemcpyDeviceToHost, e->stream));
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));

@@ -499,7 +499,12 @@ void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *r
 void launch_rec_sum(const Group &g, double *out);
 // Monte Carlo trial move, pair part: out[0] the two-body energy change,
 // out[1] 1 when a trial pair overlaps
-void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out);
+// all: every particle a candidate partner (cell-list semantics), over
+// kMcPairBlocks blocks whose partials part[2 * kMcPairBlocks] are summed
+// in a fixed order; else the Verlet-list partners in one block
+constexpr int kMcPairBlocks = 512;
+void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out,
+                    double *part, bool all);
 void launch_skin(const Group &g);
 void launch_set_positions(const Group &g, const double *ro_new);
 void launch_collect(const Group &g, int *flags, double *en, int nen);

@@ -1401,9 +1401,11 @@ __device__ __forceinline__ double mc_pair_u(const SysDev &V, int ti, int tj, dou
 __global__ void __launch_bounds__(256) k_mc_pair(const SysDev *__restrict__ sys, int natm,
                                                 const int *__restrict__ ia,
                                                 const double *__restrict__ rtm,
-                                                double *__restrict__ out) {
+                                                double *__restrict__ out,
+                                                double *__restrict__ part) {
   const SysDev &V = sys[blockIdx.z];
   const int np = V.np;
+  const int stride = gridDim.x * blockDim.x;
   // cell-list semantics (UTwoBodyANew/OldCellList, Src/denergy.F90:728-928):
   // every particle within rcut of the old or new position, as the cells
   // around each position hold them; Verlet list (UTwoBodyANew/Old): the
@@ -1416,7 +1418,7 @@ __global__ void __launch_bounds__(256) k_mc_pair(const SysDev *__restrict__ sys,
     const double xo = V.ro[3 * i], yo = V.ro[3 * i + 1], zo = V.ro[3 * i + 2];
     const double xn = rtm[3 * a], yn = rtm[3 * a + 1], zn = rtm[3 * a + 2];
     const int n = all ? np : V.nn[t];
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
       int jp, tj;
       double xj, yj, zj;
       if (all) {
@@ -1440,7 +1442,8 @@ __global__ void __launch_bounds__(256) k_mc_pair(const SysDev *__restrict__ sys,
              mc_pair_u(V, ti, tj, xo - xj, yo - yj, zo - zj, ovl_old);
     }
   }
-  for (int q = threadIdx.x; q < natm * natm; q += blockDim.x) {
+  for (int q = blockIdx.x == 0 ? (int)threadIdx.x : natm * natm; q < natm * natm;
+       q += blockDim.x) {
     const int a = q / natm, b = q - a * natm;
     if (b <= a) continue;
     const int i = ia[a], j = ia[b], ti = V.type[i], tj = V.type[j];
@@ -1456,8 +1459,31 @@ __global__ void __launch_bounds__(256) k_mc_pair(const SysDev *__restrict__ sys,
   if (threadIdx.x == 0) {
     double s = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) s += red[w];
-    out[2 * blockIdx.z] = s;
-    out[2 * blockIdx.z + 1] = any ? 1.0 : 0.0;
+    if (gridDim.x == 1) {
+      out[2 * blockIdx.z] = s;
+      out[2 * blockIdx.z + 1] = any ? 1.0 : 0.0;
+    } else {
+      part[2 * blockIdx.x] = s;
+      part[2 * blockIdx.x + 1] = any ? 1.0 : 0.0;
+    }
+  }
+}
+
+// the block partials of a many-block k_mc_pair in a fixed order
+__global__ void __launch_bounds__(64) k_mc_pair_sum(const double *__restrict__ part, int nblk,
+                                                    double *__restrict__ out) {
+  double s = 0.0, o = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += 64) {
+    s += part[2 * b];
+    o = fmax(o, part[2 * b + 1]);
+  }
+  for (int w = 32; w > 0; w >>= 1) {
+    s += __shfl_down(s, w, 64);
+    o = fmax(o, __shfl_down(o, w, 64));
+  }
+  if (threadIdx.x == 0) {
+    out[0] = s;
+    out[1] = o;
   }
 }
 
@@ -2303,8 +2329,16 @@ void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *r
 #undef MFE_MC
 }
 
-void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out) {
-  hipLaunchKernelGGL(k_mc_pair, grid(1, g), dim3(256), 0, g.stream, g.d, natm, ia, rtm, out);
+void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out,
+                    double *part, bool all) {
+  if (!all) {
+    hipLaunchKernelGGL(k_mc_pair, grid(1, g), dim3(256), 0, g.stream, g.d, natm, ia, rtm, out,
+                       part);
+    return;
+  }
+  hipLaunchKernelGGL(k_mc_pair, dim3(kMcPairBlocks), dim3(256), 0, g.stream, g.d, natm, ia, rtm,
+                     out, part);
+  hipLaunchKernelGGL(k_mc_pair_sum, dim3(1), dim3(64), 0, g.stream, part, kMcPairBlocks, out);
 }
 
 void launch_rec_sum(const Group &g, double *out) {
