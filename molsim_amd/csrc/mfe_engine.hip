@@ -1279,17 +1279,8 @@ int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const doub
     const mfe::Group &G = e->grp;
     if (keep) {
       // the moved stencils exchanged in the kept tiles themselves (old
-      // stencils read from ro, which still holds the old positions): for
-      // this one launch the view's spreading tiles are the kept ones.
-      // sync_view waits for the stream, and the second view upload is
-      // ordered after the launch on the same stream.
-      std::swap(e->spread_tiles, e->mc_tiles);
-      e->sys_dirty = true;
-      sync_view(e);
-      mfe::launch_mc_stencils(G, natm, e->mc_ia, e->mc_rtm);
-      std::swap(e->spread_tiles, e->mc_tiles);
-      e->sys_dirty = true;
-      sync_view(e);
+      // stencils read from ro, which still holds the old positions)
+      mfe::launch_mc_stencils(G, natm, e->mc_ia, e->mc_rtm, e->mc_tiles);
       MFE_HIP_TRY(hipMemcpyAsync(e->mc_out, e->mc_out + 1, sizeof(double),
                                  hipMemcpyDeviceToDevice, e->stream));
     }

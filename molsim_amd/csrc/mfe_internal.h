@@ -494,7 +494,9 @@ void launch_sort_pos(const Group &g);
 // of the positions)
 void launch_mc_place(const Group &g, int natm, const int *ia, const double *rtm,
                      double *ro_stage);
-void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm);
+// tiles: nullptr = the spreading tiles, else a buffer of their layout
+void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm,
+                        void *tiles = nullptr);
 // the reciprocal energy of the last convolution pass (its block partials)
 void launch_rec_sum(const Group &g, double *out);
 // Monte Carlo trial move, pair part: out[0] the two-body energy change,

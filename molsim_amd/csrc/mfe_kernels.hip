@@ -1336,10 +1336,12 @@ __global__ void __launch_bounds__(256) k_mc_place(int natm, const int *__restric
 template <typename R, int ORDER>
 __global__ void __launch_bounds__(256) k_mc_stencils(const SysDev *__restrict__ sys, int natm,
                                                     const int *__restrict__ ia,
-                                                    const double *__restrict__ rtm) {
+                                                    const double *__restrict__ rtm,
+                                                    void *tiles_in) {
   const SysDev &V = sys[blockIdx.z];
   const SpmeGeom g = V.sg;
-  R *__restrict__ tiles = (R *)V.spread_tiles;
+  // the spreading tiles, or the given ones of the same layout
+  R *__restrict__ tiles = (R *)(tiles_in ? tiles_in : (void *)V.spread_tiles);
   const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
   const size_t ntile = (size_t)E0 * E1 * E2;
   constexpr int NP = ORDER * ORDER * ORDER;
@@ -2313,11 +2315,12 @@ void launch_mc_place(const Group &g, int natm, const int *ia, const double *rtm,
                      ro_stage);
 }
 
-void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm) {
+void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm,
+                        void *tiles) {
 #define MFE_MC(R, O)                                                                       \
   case O:                                                                                  \
     hipLaunchKernelGGL((k_mc_stencils<R, O>), grid(1, g), dim3(256), 0, g.stream, g.d, natm, \
-                       ia, rtm);                                                           \
+                       ia, rtm, tiles);                                                    \
     break;
   if (g.prec == MFE_PREC_MIXED) {
     switch (g.order) { MFE_MC(float, 3) MFE_MC(float, 4) MFE_MC(float, 5) MFE_MC(float, 6)
