@@ -131,6 +131,39 @@ def test_utotal_bonds_vs_reference(golden):
     eng.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_reference_ubuf_drives_the_engine(prec):
+    """MOLSIM's own ubuf/iubuflow/r2umin (the reference run's tables for the
+    polyelectrolyte potential, tests/golden/tables_c2.npz) handed to
+    mfe_set_tables as a MOLSIM host would hand its arrays: the engine gives
+    the same forces and energies as with the tables it builds itself
+    (bitwise), and matches the oracle on the reference's tables."""
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "tables_c2.npz"))
+    s = S.c2()
+    assert float(z["rcut"]) == s.rcut and float(z["alpha"]) == s.alpha
+    say("case: reference ubuf through mfe_set_tables", prec)
+    ref_tab = E.Tables(z["ubuf"].copy(), z["iubuflow"].astype(np.int32), z["nugrid"].copy(),
+                       z["r2umin"].copy())
+    own_tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                             alpha=s.alpha)
+    eng = S.configure(s, prec, tables=ref_tab)
+    f, en = eng.utotal()
+    eng.close()
+    eng = S.configure(s, prec, tables=own_tab)
+    f2, en2 = eng.utotal()
+    eng.close()
+    if prec == "mixed":
+        assert np.array_equal(f, f2) and en.tot == en2.tot
+    else:
+        assert rel_rms(f, f2) <= 1e-13 and abs(en.tot - en2.tot) <= 1e-13 * abs(en2.tot)
+    orc = oracle_for(s, ref_tab)
+    orc.set_list(s.ro)
+    fo, eo = orc.utotal(s.ro)
+    assert rel_rms(f, fo) <= (1e-10 if prec == "fp64" else 1e-5)
+    etol = 1e-11 if prec == "fp64" else 1e-6
+    assert abs(en.twob0 - eo["twob0"]) <= etol * abs(eo["twob0"])
+
+
 @pytest.mark.parametrize("a", [0, 1])
 def test_utotal_angles_vs_reference(golden, a):
     """UAngle on the golden chains against the reference run with bond
