@@ -150,6 +150,7 @@ def _declare(L):
     L.orc_velver_kick.argtypes = [C.c_int, C.c_double, _dp, _dp, _dp, _dp, _dp]
     L.orc_bd_step.argtypes = [C.POINTER(Box), C.c_int, _ip, _dp, _dp, _dp, _dp,
                               _dp, _dp]
+    L.orc_bd_factors.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
     L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                     C.POINTER(C.c_uint32)]
     L.orc_bd_noise.argtypes = [C.c_uint64, C.c_int, C.c_uint32, _dp]
@@ -587,6 +588,14 @@ def model_pair_energy(boxlen, iptpn, ro, zat, sigat, epsat, efp, alpha, rcut, *,
                                     float(ljushift), float(rcut), int(nk), float(r0),
                                     float(r1), _d(c.reshape(-1)), _d(fk))
     return e, fk.reshape(-1, 3)
+
+
+def bd_factors(tstep, temp, dcoeff):
+    """BDStep's fac1, fac2 per type (Src/bd.F90:136-139)."""
+    d = f64(dcoeff)
+    f1, f2 = np.zeros(d.size), np.zeros(d.size)
+    lib().orc_bd_factors(d.size, float(tstep), float(temp), _d(d), _d(f1), _d(f2))
+    return f1, f2
 
 
 def bd_step(boxlen, iptpn, fac1, fac2, forceo, noise, ro):

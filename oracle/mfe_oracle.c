@@ -1358,6 +1358,20 @@ void orc_bd_step(const orc_box *b, int np, const int *iptpn,
   }
 }
 
+void orc_bd_factors(int npt, double tstep, double temp, const double *dcoeff,
+                    double *fac1, double *fac2) {
+  /* IOScale defaults and derived scales, in its order of operations */
+  const double scllen = 1.0e-10, sclmas = 1.0e-03, scltem = 1.0, scltim = 1.0e-12;
+  const double scldif = 1.0e-09, gas = 8.314472;
+  const double sclvel = scllen / scltim, sclacc = sclvel / scltim;
+  const double sclfor = sclacc * sclmas;
+  /* left to right, as Fortran evaluates a*b*c/d/e */
+  for (int t = 0; t < npt; ++t) {
+    fac1[t] = tstep * scltim * dcoeff[t] * scldif * sclfor / (gas * temp * scltem) / scllen;
+    fac2[t] = sqrt(2.0 * dcoeff[t] * scldif * tstep * scltim) / scllen;
+  }
+}
+
 /* GauRandom (Src/mollib.F90:2739-2766): polar Box-Muller over the inline
    Lehmer generator iseed = ibclr(iseed*16087, 31) (32-bit wrap-around) */
 double orc_gaurandom(int32_t *iseed, orc_gau_state *st) {

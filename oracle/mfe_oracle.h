@@ -228,6 +228,11 @@ void orc_velver_kick(int np, double tstep, const double *massip,
 void orc_bd_step(const orc_box *b, int np, const int *iptpn,
                  const double *fac1, const double *fac2, const double *forceo,
                  const double *noise, double *ro, double *drostep);
+/* BDStep's per-type factors (Src/bd.F90:136-139) with IOScale's default
+ * units (Src/molsim.F90:786-801): fac1 = dt D/(R T) in A per unit forceo,
+ * fac2 = sqrt(2 D dt) in A; tstep in ps, dcoeff in 1e-9 m^2/s, temp in K. */
+void orc_bd_factors(int npt, double tstep, double temp, const double *dcoeff,
+                    double *fac1, double *fac2);
 
 /* ---- random numbers (Src/mollib.F90:2689-2766) ---- */
 typedef struct { int flag; double save; } orc_gau_state;

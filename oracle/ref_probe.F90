@@ -15,6 +15,10 @@
 !                           np, maxnneigh, force(3,np), forceo(3,np),
 !                           u%tot, u%twob(0:nptpt), u%rec, u%bond, virial,
 !                           nneighpn(1:np), jpnlist(1:maxnneigh,1:np)
+!   <project>.probe_bdout : (only if <project>.probe_pos and .probe_bd exist,
+!                           the latter one integer nbd, and the input is a
+!                           BD run) np, nbd, iseed, then for each of nbd
+!                           BDSteps forceo, ro before and drostep, ro after
 !   <project>.probe_math  : (only if <project>.probe_args exists: n reals)
 !                           Erf2(x) for each x, then for orders 3..8
 !                           CardinalBSpline(order,x) values and derivs.
@@ -38,7 +42,7 @@ program RefProbe
    real(8), allocatable :: args(:), vals(:), ders(:)
    real(8) :: Erf2
    real(8) :: GauRandom, Random
-   integer(4) :: nrng, iseedg, idumr
+   integer(4) :: nrng, iseedg, idumr, nbd, ibd
    real(8), allocatable :: rv(:)
    logical :: lexist
    character(200) :: proj
@@ -103,6 +107,29 @@ program RefProbe
       write(ufile) nneighpn(1:np)
       write(ufile) jpnlist(1:maxnneigh,1:np)
       close(ufile)
+
+! ... Brownian dynamics steps from there (only if <project>.probe_bd exists:
+!     one integer, the number of steps): iseed before the first step, then
+!     per step forceo(3,np) and ro(3,np) before BDStep, drostep(3,np) and
+!     ro(3,np) after it, as BDDriver runs BDStep and UTotal
+!     (Src/bd.F90:54-58); the list is rebuilt before each UTotal
+
+      inquire(file=trim(proj)//'.probe_bd', exist=lexist)
+      if (lexist .and. lbd) then
+         open(newunit=ufile, file=trim(proj)//'.probe_bd', access='stream', form='unformatted', status='old')
+         read(ufile) nbd
+         close(ufile)
+         open(newunit=ufile, file=trim(proj)//'.probe_bdout', access='stream', form='unformatted', status='replace')
+         write(ufile) np, nbd, iseed
+         do ibd = 1, nbd
+            write(ufile) forceo(1:3,1:np), ro(1:3,1:np)
+            call BDStep
+            write(ufile) drostep(1:3,1:np), ro(1:3,1:np)
+            if (lvlist) call SetVList
+            call UTotal(iSimulationStep)
+         end do
+         close(ufile)
+      end if
    end if
 
 ! ... math routines

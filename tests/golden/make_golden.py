@@ -47,7 +47,8 @@ def records(path):
 
 
 def system_nml(title, np_list, boxlen, nstep2=1, dump=True, method="md"):
-    return (f" &nmlSystem txtitle='{title}', txmethod='{method}', txensemb='nve',"
+    ens = "nvt" if method == "bd" else "nve"
+    return (f" &nmlSystem txtitle='{title}', txmethod='{method}', txensemb='{ens}',"
             f" txbc='xyz', txstart='setconf', nstep1=1, nstep2={nstep2},"
             f" boxlen=3*{boxlen}, temp=298.0, prsr=0.1013, iseed=1,"
             f" ldump={'.true.' if dump else '.false.'}, itest=0, ipart=0,"
@@ -252,11 +253,56 @@ def chain_angles():
         shutil.rmtree(work, ignore_errors=True)
 
 
+def bd_steps():
+    """BDStep (Src/bd.F90:120-155) as BDDriver runs it: a 2:1 electrolyte of
+    99 ions (297 deviates per step, an odd count, so GauRandom's spare
+    deviate carries across steps) with per-type diffusion coefficients; the
+    reference's forceo, positions and displacements over four steps from a
+    recorded iseed -> tests/golden/bd.npz."""
+    ions = [("c2", 33, 40.0, 2.0, 4.0, KT298), ("an", 66, 35.4, -1.0, 4.0, KT298)]
+    dcoeff, tstep, nbd = (1.33, 2.03), 0.005, 4
+    body = (particle_nml(ions) + potential_nml(2, 12.0, ncut=5) +
+            " &nmlSetConfiguration txsetconf='random','random', radatset=1.6,1.6 /\n"
+            f" &nmlBD tstep={tstep!r}, dcoeff={dcoeff[0]!r},{dcoeff[1]!r} /\n"
+            " &nmlIntList txintlist='vlist', inlist=0, drnlist=2.0 /\n")
+    text = system_nml("golden bd", None, 40.0, nstep2=1, dump=False, method="bd") + body
+    work = tempfile.mkdtemp(prefix="mfe_golden_")
+    try:
+        # positions: the setup's own random configuration, read back from
+        # a first probe run with the force dump
+        d = os.path.join(work, "bd")
+        os.makedirs(d, exist_ok=True)
+        rng = np.random.default_rng(11)
+        pos = []
+        while len(pos) < 99:
+            p = rng.uniform(-20.0, 20.0, 3)
+            if all(np.sum((np.asarray(q) - p - 40.0 * np.round((np.asarray(q) - p) / 40.0)) ** 2)
+                   > 3.2 ** 2 for q in pos):
+                pos.append(p)
+        pos = np.array(pos)
+        np.array([nbd], "<i4").tofile(os.path.join(d, "bd.probe_bd"))
+        case("bd", text, work, probe_pos=pos, md=False)
+        b = open(os.path.join(d, "bd.probe_bdout"), "rb").read()
+        np_, nb, iseed = struct.unpack("<3i", b[:12])
+        v = np.frombuffer(b[12:], "<f8").reshape(nb, 4, np_, 3)
+        np.savez_compressed(os.path.join(HERE, "bd.npz"), boxlen=40.0, tstep=tstep,
+                            temp=298.0, dcoeff=np.array(dcoeff),
+                            iptpn=np.array([1] * 33 + [2] * 66, np.int32),
+                            iseed=iseed, forceo=v[:, 0], ro_before=v[:, 1],
+                            drostep=v[:, 2], ro_after=v[:, 3])
+        print("bd: iseed", iseed, "steps", nb, "max |drostep|", np.abs(v[:, 2]).max())
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def main():
     if not os.path.exists(PROBE):
         sys.exit("build the reference first: make -f oracle/ref.mk")
     if sys.argv[1:] == ["rng"]:
         rng_streams()
+        return
+    if sys.argv[1:] == ["bd"]:
+        bd_steps()
         return
     if sys.argv[1:] == ["cnf"]:
         c1_cnf()

@@ -100,6 +100,9 @@ def test_fortran_host_matches_python_host():
     exe = os.path.join(ROOT, "molsim_amd", "mfe_host")
     if not os.path.exists(exe):
         pytest.skip("Fortran host not built")
+    srcs = [os.path.join(ROOT, "molsim_amd", "fortran", f) for f in ("mfe_mod.F90", "mfe_host.F90")]
+    if any(os.path.getmtime(f) > os.path.getmtime(exe) for f in srcs):
+        pytest.skip("Fortran host older than its sources: build again")
     s = S.c2()
     s.chains = []
     say("case: Fortran host c2 (no bonds), mixed")

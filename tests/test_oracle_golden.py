@@ -218,3 +218,44 @@ def test_gaurandom_and_random_streams_bitwise(golden_dir):
     rs = RandomState(-1, -1, 0.0)
     r = np.array([L.orc_random(C.byref(idum), C.byref(rs)) for _ in range(r_ref.size)])
     assert np.array_equal(r, r_ref)
+
+
+def test_bd_step_bitwise(golden_dir):
+    """BDStep (Src/bd.F90:120-155) against the reference run of four steps
+    (bd.npz, drawn by the reference probe): fac1/fac2 from the unit formula
+    (:136-139), GauRandom's deviates drawn from the recorded iseed in the
+    reference's order (x, y, z per particle, 297 per step, so the spare
+    deviate carries across steps), the displacement and PBC -- every
+    displacement and position bitwise, given the reference's forceo."""
+    z = load(golden_dir, "bd.npz")
+    fac1, fac2 = O.bd_factors(float(z["tstep"]), float(z["temp"]), z["dcoeff"])
+    L = O.lib()
+    seed = C.c_int32(int(z["iseed"]))
+    st = O.GauState(0, 0.0)
+    iptpn = z["iptpn"]
+    nstep, np_ = z["forceo"].shape[:2]
+    ro = z["ro_before"][0].copy()
+    for k in range(nstep):
+        assert np.array_equal(ro, z["ro_before"][k]), k
+        noise = np.array([L.orc_gaurandom(C.byref(seed), C.byref(st))
+                          for _ in range(3 * np_)])
+        dro = O.bd_step(float(z["boxlen"]), iptpn, fac1, fac2,
+                        z["forceo"][k].reshape(-1), noise, ro.reshape(-1))
+        assert np.array_equal(dro.reshape(-1, 3), z["drostep"][k]), k
+        assert np.array_equal(ro, z["ro_after"][k]), k
+        ro = z["ro_before"][k + 1].copy() if k + 1 < nstep else ro
+    # the displacement is not dominated by either term alone
+    f_part = np.abs(fac1[iptpn - 1][:, None] * z["forceo"][0]).mean()
+    n_part = np.abs(z["drostep"][0]).mean()
+    assert 1e-4 < f_part / n_part < 1e2
+
+
+def test_bd_factors_closed_form():
+    """fac1 = dt D / (R T) and fac2 = sqrt(2 D dt), in A and kJ/(mol A):
+    D = 1e-9 m^2/s = 0.1 A^2/ps; R T in kJ/mol."""
+    f1, f2 = O.bd_factors(0.005, 298.0, [1.0, 2.0])
+    rt = 8.314472e-3 * 298.0          # kJ/mol
+    d = np.array([1.0, 2.0]) * 0.1    # A^2/ps
+    # forceo is in g A/(mol ps^2) = 1e-2 kJ/(mol A)
+    assert np.allclose(f1, 0.005 * d / rt * 1e-2, rtol=1e-13)
+    assert np.allclose(f2, np.sqrt(2 * d * 0.005), rtol=1e-13)
