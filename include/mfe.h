@@ -133,6 +133,13 @@ int mfe_build_tables_1_6_12(const mfe_pot_1_6_12 *spec, double *ubuf,
 int mfe_create(const mfe_config *cfg, mfe_engine **out);
 int mfe_destroy(mfe_engine *e);
 const char *mfe_last_error(const mfe_engine *e);
+/* No reference counterpart.  Bound on the polls of each grid-barrier wait
+   of the single-engine list rebuild (polls < 0: the default, 2^22).  A
+   wait that runs out ends the rebuild in every block; the call then
+   returns MFE_E_HIP, the positions stay where they were, and the next
+   call builds the list from scratch.  Small bounds exist to exercise
+   that path from the host. */
+int mfe_set_sync_limit(mfe_engine *e, int32_t polls);
 
 /* ---- setup (replaces IOSystem box setup, Particle, IOPotTwoBody,
         EwaldSetup, IONList) ------------------------------------------- */
@@ -315,6 +322,31 @@ int mfe_set_knots(mfe_engine *e, int32_t nknot, double r_lo, double r_hi,
 int mfe_adjoint(mfe_engine *e, double *dE_dq, double *dE_dzat,
                 double *dE_dsigat, double *dE_depsat, double *dE_dknot,
                 double *dE_dro, double *energy);
+
+/* ---- force matching --------------------------------------------------- */
+/* No reference counterpart.  The forces of the adjoint's model (the pair
+   model of mfe_set_pair_params and mfe_set_knots evaluated exactly, in
+   place of the tables, plus SPME, bonds, angles and the surface term) at
+   the current positions: force(3,np) = -dE/dro in kJ/(mol A), UTotal's
+   force before the unit scaling.  The arithmetic is the engine's: FP32
+   model terms with FP64 sums on the mixed path, FP64 on the FP64 path. */
+int mfe_model_forces(mfe_engine *e, double *force);
+/* nframes configurations ro[nframes][np][3] (inside the box) and their
+   target forces f_target[nframes][np][3] (kJ/(mol A)), kept in device
+   memory for mfe_fm_loss_grad. */
+int mfe_fm_set_frames(mfe_engine *e, int32_t nframes, const double *ro,
+                      const double *f_target);
+/* Loss L = sum over frames and particles of |F_model - F_target|^2 and its
+   gradient with respect to the per-type charges zat, sigat and epsat
+   [npt] and the spline coefficients c[nptpt][nknot] (any output may be
+   NULL).  Per frame, on the device: list build, model forces with the
+   reciprocal field kept, residual, the pair part of the gradient, the
+   reciprocal part through a second spreading (of the residual-weighted
+   B-spline derivatives), convolution and potential gather, and the
+   surface term.  The engine's positions are those of the last frame
+   afterwards.  SPME or no Ewald sum. */
+int mfe_fm_loss_grad(mfe_engine *e, double *loss, double *dL_dzat,
+                     double *dL_dsigat, double *dL_depsat, double *dL_dknot);
 
 /* ---- batched replicas ------------------------------------------------- */
 /* No reference counterpart: MOLSIM runs one system per process.  A batch

@@ -23,7 +23,8 @@ INCLUDE = os.path.join(ROOT, "include")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libmfe.so")
 
-HIP_SOURCES = ["mfe_engine.hip", "mfe_kernels.hip", "mfe_fft.hip", "mfe_batch.hip"]
+HIP_SOURCES = ["mfe_engine.hip", "mfe_kernels.hip", "mfe_fft.hip", "mfe_batch.hip",
+               "mfe_fm.hip"]
 CXX_SOURCES = ["mfe_tables.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

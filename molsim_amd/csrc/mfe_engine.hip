@@ -153,6 +153,8 @@ void free_all(mfe_engine *e) {
   dfree(e->ro_stage);
   dfree(e->knot_c); dfree(e->phi_rec); dfree(e->adj_phi); dfree(e->adj_dq);
   dfree(e->adj_part); dfree(e->adj_out);
+  dfree(e->fm_ro); dfree(e->fm_tgt); dfree(e->fm_fld); dfree(e->fm_res); dfree(e->fm_res_s);
+  dfree(e->fm_vec); dfree(e->fm_part); dfree(e->fm_acc); dfree(e->fm_qr);
   if (e->h_en) (void)hipHostFree(e->h_en);
   if (e->h_flags) (void)hipHostFree(e->h_flags);
   e->h_en = nullptr;
@@ -334,6 +336,11 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   MFE_SETP(v.adj_dq, e->adj_dq);
   MFE_SETP(v.adj_part, e->adj_part);
   v.adj_ncol = 1 + 2 * e->npt + e->nptpt * e->nknot;
+  MFE_SETP(v.fm_fld, e->fm_fld);
+  MFE_SETP(v.fm_res, e->fm_res);
+  MFE_SETP(v.fm_res_s, e->fm_res_s);
+  MFE_SETP(v.fm_vec, e->fm_vec);
+  v.fm_ncol = 1 + 3 * e->npt + e->nptpt * e->nknot;
   MFE_SETP(v.part, e->part); MFE_SETP(v.en, e->en);
   v.off_pair = e->off_pair; v.off_conv = e->off_conv; v.off_bond = e->off_bond;
   v.off_surf = e->off_surf; v.off_std = e->off_std; v.off_move = e->off_move;
@@ -419,6 +426,7 @@ void sync_view(mfe_engine *e) {
   e->grp.rebuild_bar = e->rebuild_bar;
   e->grp.rebuild_sum = e->rebuild_sum;
   e->grp.ncu = e->ncu;
+  e->grp.sync_spin = e->sync_spin;
   e->sys_dirty = false;
 }
 
@@ -439,8 +447,11 @@ int mfe_map_flag_error(mfe_engine *e, int err) {
   mfe::launch_fill_int(e->flags, 1, 0, e->stream);
   MFE_HIP_TRY(hipStreamSynchronize(e->stream));
   if (err & mfe::ERR_SYNC) {
-    // the barrier state is undefined after a timed-out wait
+    // the barrier state is undefined after a timed-out wait, and the cell
+    // counts may hold the failed rebuild's increments: both are cleared
+    // before the list is built again from scratch
     mfe::launch_fill_int((int *)e->rebuild_bar, 2, 0, e->stream);
+    if (e->c_count) mfe::launch_fill_int(e->c_count, e->ncells_alloc, 0, e->stream);
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     e->list_valid = false;
     return fail(e, MFE_E_HIP, "list rebuild: a grid barrier timed out (CUs not all available)");
@@ -472,7 +483,7 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
   const bool gather_fin = g.ewald == MFE_EWALD_SPME && !g.chains && !g.surf && mode == 0;
   if (gather_fin) {
     const mfe::FinArgs fa{energy ? 1 : 0, skin ? 1 : 0, bd ? 1 : 0, mode, dt};
-    mfe::launch_spme(g, energy, SP_BIN, SP_GATHER, false, &fa);
+    mfe::launch_spme(g, energy, SP_BIN, SP_GATHER, mfe::GM_FORCE, &fa);
     return;
   }
   if (g.ewald == MFE_EWALD_SPME) mfe::launch_spme(g, energy, SP_BIN, SP_GATHER);
@@ -579,6 +590,15 @@ int mfe_destroy(mfe_engine *e) {
 }
 
 const char *mfe_last_error(const mfe_engine *e) { return e ? e->err.c_str() : "null engine"; }
+
+int mfe_set_sync_limit(mfe_engine *e, int32_t polls) {
+  if (!e) return MFE_E_BAD_PARAM;
+  e->sync_spin = polls < 0 ? -1 : polls;
+  e->sys_dirty = true;
+  e->graph_dirty = true;
+  ++e->version;
+  return MFE_OK;
+}
 
 int mfe_set_box(mfe_engine *e, const double boxlen[3]) {
   if (!e || !boxlen) return MFE_E_BAD_PARAM;
@@ -696,7 +716,12 @@ int mfe_set_tables(mfe_engine *e, const double *ubuf, int32_t nbuf, const int32_
         for (int c = 11; c < mfe::kCoefStride; ++c) cf.push_back(0.0);
       }
     }
+    // FP32 copy for the mixed path; the free twelfth slot of each record
+    // carries the rounding of the interval constant w0, w0 - (float)w0, so
+    // that energy passes add w0 to the FP32 remainder polynomial in FP64
     std::vector<float> cff(cf.begin(), cf.end());
+    for (size_t r = 0; r + mfe::kCoefStride <= cf.size(); r += mfe::kCoefStride)
+      cff[r + 11] = (float)(cf[r] - (double)cff[r]);
     // bins uniform in s = 1/r2, where the tables' intervals are roughly
     // evenly spaced.  The kernel computes the bin in FP32 from r2 clamped
     // to [lo, inf) and clamps it into [0, nbin); each bin is built over
@@ -1488,7 +1513,7 @@ int mfe_adjoint(mfe_engine *e, double *dE_dq, double *dE_dzat, double *dE_dsigat
     const mfe::Group &G = e->grp;
     const unsigned f = MFE_FORCES | MFE_ENERGY | MFE_VIRIAL;
     mfe::launch_pair(G, true);
-    if (G.ewald == MFE_EWALD_SPME) mfe::launch_spme(G, true, SP_BIN, SP_GATHER, true);
+    if (G.ewald == MFE_EWALD_SPME) mfe::launch_spme(G, true, SP_BIN, SP_GATHER, mfe::GM_POT);
     if (G.chains) mfe::launch_bonds(G);
     if (G.surf) mfe::launch_surf(G);
     mfe::launch_reduce(G, f, false, false);
@@ -1516,6 +1541,166 @@ int mfe_adjoint(mfe_engine *e, double *dE_dq, double *dE_dzat, double *dE_dsigat
     }
     // the model energy: exact pair sum + reciprocal (with self/surface) + bonds
     if (energy) *energy = out[0] + e->h_en[2] + e->h_en[3];
+    return MFE_OK;
+  });
+}
+
+namespace {
+
+// device room of the force-matching passes for the engine's np
+void fm_buffers(mfe_engine *e) {
+  const int np = e->np;
+  const int ncol = 1 + 3 * e->npt + e->nptpt * e->nknot;
+  const int ncolp = 2 * e->npt + e->nptpt * e->nknot;
+  const int nb = (np + 255) / 256, nb4 = mfe::fm_pair_blocks(np);
+  const size_t part = (size_t)nb * 4 + (size_t)nb * e->npt + (size_t)nb4 * std::max(ncolp, 1);
+  bool changed = false;
+  if (e->fm_np != np || !e->fm_fld) {
+    dalloc(e->fm_fld, 3 * (size_t)np);
+    dalloc(e->fm_res, 3 * (size_t)np);
+    dalloc(e->fm_res_s, 3 * (size_t)np);
+    dfree(e->fm_vec);
+    MFE_HIP_TRY(hipMalloc(&e->fm_vec, 4 * (size_t)np * sizeof(double)));
+    dalloc(e->phi_rec, (size_t)e->max_np);
+    dalloc(e->adj_phi, (size_t)e->max_np);
+    dalloc(e->adj_dq, (size_t)e->max_np);
+    dalloc(e->fm_qr, 3);
+    e->fm_np = np;
+    changed = true;
+  }
+  if (part > e->fm_part_cap) {
+    dalloc(e->fm_part, part);
+    e->fm_part_cap = part;
+  }
+  if (ncol > e->fm_acc_cap) {
+    dalloc(e->fm_acc, (size_t)ncol);
+    e->fm_acc_cap = ncol;
+  }
+  if (changed) touch(e);
+}
+
+// the model's forces at the engine's positions (list current): pair model,
+// SPME with the reciprocal field kept per particle, bonds and angles, the
+// surface term; forceo = force * ffac as UTotal leaves it
+void fm_forward(mfe_engine *e) {
+  const mfe::Group &G = e->grp;
+  mfe::launch_fm_pair(G, false, nullptr, 0, e->np);
+  if (G.ewald == MFE_EWALD_SPME) mfe::launch_spme(G, false, SP_BIN, SP_GATHER, mfe::GM_FIELD);
+  if (G.chains) mfe::launch_bonds(G);
+  if (G.surf) mfe::launch_surf(G);
+  mfe::launch_reduce(G, 0u, false, false);
+  mfe::launch_finalize(G, 0, 0.0);
+}
+
+int fm_check(mfe_engine *e) {
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (!e->have_ewald) return fail(e, MFE_E_STATE, "mfe_set_ewald not called");
+  if (!e->have_pp) return fail(e, MFE_E_STATE, "mfe_set_pair_params not called");
+  if (e->ewald == MFE_EWALD_STD)
+    return fail(e, MFE_E_BAD_PARAM, "force matching covers SPME and no-Ewald systems");
+  return MFE_OK;
+}
+
+}  // namespace
+
+int mfe_model_forces(mfe_engine *e, double *force) {
+  if (!e || !force) return MFE_E_BAD_PARAM;
+  int rc = fm_check(e);
+  if (rc) return rc;
+  if (!e->list_valid) {
+    rc = mfe_build_list(e);
+    if (rc) return rc;
+  }
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    e->mc_cached = false;
+    fm_buffers(e);
+    sync_view(e);
+    fm_forward(e);
+    std::vector<double> fo(3 * (size_t)e->np);
+    MFE_HIP_TRY(hipMemcpyAsync(fo.data(), e->forceo, fo.size() * sizeof(double),
+                               hipMemcpyDeviceToHost, e->stream));
+    int r = read_flags(e);
+    if (r) return r;
+    const double inv = 1.0 / e->h_sys.ffac;
+    for (size_t k = 0; k < fo.size(); ++k) force[k] = fo[k] * inv;
+    return MFE_OK;
+  });
+}
+
+int mfe_fm_set_frames(mfe_engine *e, int32_t nframes, const double *ro, const double *f_target) {
+  if (!e || nframes < 1 || !ro || !f_target) return MFE_E_BAD_PARAM;
+  if (!e->have_particles || !e->have_box) return fail(e, MFE_E_STATE, "box and particles first");
+  const size_t n3 = 3 * (size_t)e->np;
+  for (size_t k = 0; k < n3 * nframes; ++k) {
+    const int m = (int)(k % 3);
+    if (!(std::fabs(ro[k]) <= e->box.L2[m]))
+      return fail(e, MFE_E_BAD_PARAM, "frame position outside the box [-L/2, L/2]");
+    if (!std::isfinite(f_target[k])) return fail(e, MFE_E_BAD_PARAM, "target not finite");
+  }
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    if (nframes != e->fm_nframes || e->fm_frame_np != e->np) {
+      dalloc(e->fm_ro, n3 * nframes);
+      dalloc(e->fm_tgt, n3 * nframes);
+    }
+    upload(e->fm_ro, ro, n3 * nframes, e->stream);
+    upload(e->fm_tgt, f_target, n3 * nframes, e->stream);
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    e->fm_nframes = nframes;
+    e->fm_frame_np = e->np;
+    return MFE_OK;
+  });
+}
+
+int mfe_fm_loss_grad(mfe_engine *e, double *loss, double *dL_dzat, double *dL_dsigat,
+                     double *dL_depsat, double *dL_dknot) {
+  if (!e) return MFE_E_BAD_PARAM;
+  int rc = fm_check(e);
+  if (rc) return rc;
+  if (e->fm_nframes < 1 || e->fm_frame_np != e->np)
+    return fail(e, MFE_E_STATE, "mfe_fm_set_frames first (for this np)");
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    e->mc_cached = false;
+    fm_buffers(e);
+    sync_view(e);
+    const mfe::Group &G = e->grp;
+    const int np = e->np, npt = e->npt;
+    const int ncol = 1 + 3 * npt + e->nptpt * e->nknot;
+    const int ncolp = 2 * npt + e->nptpt * e->nknot;
+    const int nb = (np + 255) / 256, nb4 = mfe::fm_pair_blocks(np);
+    double *part_r = e->fm_part;                       // [nb][4]
+    double *part_q = part_r + (size_t)nb * 4;          // [nb][npt]
+    double *part_p = part_q + (size_t)nb * npt;        // [nb4][ncolp]
+    const size_t n3 = 3 * (size_t)np;
+    mfe::launch_fill_double(e->fm_acc, ncol, 0.0, e->stream);
+    for (int f = 0; f < e->fm_nframes; ++f) {
+      MFE_HIP_TRY(hipMemcpyAsync(e->ro, e->fm_ro + n3 * f, n3 * sizeof(double),
+                                 hipMemcpyDeviceToDevice, e->stream));
+      mfe::launch_nlist_rebuild(G, false);
+      fm_forward(e);
+      mfe::launch_fm_resid(G, e->fm_tgt + n3 * f, part_r);
+      mfe::launch_fm_acc(G, part_r, nb, 4, 1, e->fm_acc);
+      mfe::launch_fm_pair(G, true, part_p, ncolp, np);
+      mfe::launch_fm_acc(G, part_p, nb4, ncolp, ncolp, e->fm_acc + 1 + npt);
+      if (G.ewald == MFE_EWALD_SPME) mfe::launch_fm_rec_backward(G);
+      mfe::launch_fm_q(G, part_r, e->fm_qr, part_q);
+      mfe::launch_fm_acc(G, part_q, nb, npt, npt, e->fm_acc + 1);
+    }
+    std::vector<double> out(ncol);
+    MFE_HIP_TRY(hipMemcpyAsync(out.data(), e->fm_acc, ncol * sizeof(double),
+                               hipMemcpyDeviceToHost, e->stream));
+    int r = read_flags(e);
+    e->list_valid = (r == MFE_OK);
+    e->have_positions = true;
+    if (r) return r;
+    if (loss) *loss = out[0];
+    if (dL_dzat) std::copy(out.begin() + 1, out.begin() + 1 + npt, dL_dzat);
+    if (dL_dsigat) std::copy(out.begin() + 1 + npt, out.begin() + 1 + 2 * npt, dL_dsigat);
+    if (dL_depsat) std::copy(out.begin() + 1 + 2 * npt, out.begin() + 1 + 3 * npt, dL_depsat);
+    if (dL_dknot) std::copy(out.begin() + 1 + 3 * npt, out.end(), dL_dknot);
     return MFE_OK;
   });
 }

@@ -278,6 +278,15 @@ struct SysDev {
   MFE_DPTR(double) adj_dq;     // dE/dq per particle
   MFE_DPTR(double) adj_part;   // per-block partials [blocks][adj_ncol]
   int adj_ncol;
+  // force matching: reciprocal field per particle (forward gather), the
+  // residual force - target (particle and cell order), the target of the
+  // frame in hand, and the residual spreading weights in brick order
+  MFE_DPTR(double) fm_fld;
+  MFE_DPTR(double) fm_res;
+  MFE_DPTR(double) fm_res_s;
+  MFE_DPTR(const double) fm_tgt;
+  MFE_DPTR(void) fm_vec;
+  int fm_ncol;
   // reductions and output scaling
   MFE_DPTR(double) part;
   MFE_DPTR(double) en;
@@ -305,6 +314,7 @@ struct Group {
   unsigned *rebuild_bar = nullptr;
   int *rebuild_sum = nullptr;
   int ncu = 0;
+  int sync_spin = -1;       // polls before a barrier wait gives up (-1: default)
   bool pair_cf_lds = true;  // every replica's coefficients fit in LDS
 };
 
@@ -331,6 +341,7 @@ struct mfe_engine {
   unsigned *rebuild_bar = nullptr;  // grid barrier of k_rebuild [2]
   int *rebuild_sum = nullptr;       // its per-block scan totals [ncu]
   int ncu = 0;                      // compute units of the device
+  int sync_spin = -1;               // k_rebuild's barrier poll bound (-1: default)
   double *force = nullptr;   // UTotal's force(3,na) before unit scaling
   double *forceo = nullptr;  // force * fac
   double *drosum = nullptr, *r2disp = nullptr;
@@ -434,6 +445,16 @@ struct mfe_engine {
   double *phi_rec = nullptr, *adj_phi = nullptr, *adj_dq = nullptr;
   double *adj_part = nullptr, *adj_out = nullptr;
   int adj_cap = 0;
+  // force matching: device-resident frames (positions, target forces), the
+  // model forces' reciprocal field, residuals, residual spreading weights,
+  // block partials and the accumulated loss and gradient
+  double *fm_ro = nullptr, *fm_tgt = nullptr;
+  int fm_nframes = 0, fm_frame_np = 0;
+  double *fm_fld = nullptr, *fm_res = nullptr, *fm_res_s = nullptr;
+  void *fm_vec = nullptr;
+  double *fm_part = nullptr, *fm_acc = nullptr, *fm_qr = nullptr;
+  size_t fm_part_cap = 0;
+  int fm_acc_cap = 0, fm_np = 0;
 
   // Brownian dynamics
   double *bd_fac1 = nullptr, *bd_fac2 = nullptr;  // [MFE_MAX_TYPES]
@@ -473,8 +494,10 @@ struct FinArgs {
   int energy, skin, bd, mode;
   double dt;
 };
+// what the gather does beside the forces (k_gather's MODE)
+enum { GM_FORCE = 0, GM_POT = 1, GM_FIELD = 2, GM_POTONLY = 3 };
 void launch_spme(const Group &g, bool energy, int first_stage, int last_stage,
-                 bool potential = false, const FinArgs *fin = nullptr);
+                 int gather_mode = GM_FORCE, const FinArgs *fin = nullptr);
 // adjoint passes: real-space pair sums, per-particle dE/dq, column sums
 void launch_adjoint(const Group &g, int ncol, double *out);
 void launch_std_ewald(const Group &g);
@@ -513,6 +536,22 @@ void launch_collect(const Group &g, int *flags, double *en, int nen);
 void launch_fill_int(int *p, int n, int v, hipStream_t s);
 void launch_fill_double(double *p, int n, double v, hipStream_t s);
 
+// force matching (mfe_fm.hip): model pair forces (backward false: written
+// over force[]) or the pair part of the loss gradient (backward true:
+// per-particle dL/dq into adj_dq, block partials [dsig, deps, dknot] of
+// ncol columns); the residual and its partials [|r|^2, q r]; the
+// reciprocal part (residual mesh spread, convolved, potential gathered);
+// dL/dq per particle with per-type partials; column sums into acc
+void launch_fm_pair(const Group &g, bool backward, double *part, int ncol, int np);
+int fm_pair_blocks(int np);
+void launch_fm_resid(const Group &g, const double *tgt, double *part);
+void launch_fm_rec_backward(const Group &g);
+void launch_fm_q(const Group &g, const double *rpart, double *qr, double *part);
+void launch_fm_acc(const Group &g, const double *part, int nblk, int ncol, int cols,
+                   double *acc);
+template <typename R, int O>
+const void *k_fm_spread_ptr();
+
 // per-replica launch geometry of the FFT passes (mfe_fft.hip)
 void fft_geometry(int prec, const int s[3], int *fx_rpb, int *fx_nblk, int *fy_T,
                   int *fy_nbx, int *fz_T, int *fz_nbx, size_t *lds_fx, size_t *lds_fy,
@@ -520,11 +559,7 @@ void fft_geometry(int prec, const int s[3], int *fx_rpb, int *fx_nblk, int *fy_T
 template <typename R, int O>
 const void *k_spread_ptr();
 template <typename R, int O>
-const void *k_gather_ptr();
-template <typename R, int O>
-const void *k_gather_pot_ptr();
-template <typename R, int O>
-const void *k_gather_fin_ptr();
+const void *k_gather_ptr(int mode, bool fin);
 
 // FFT passes (mfe_fft.hip)
 void fft_x_forward(const Group &g);

@@ -16,67 +16,11 @@
 #include <cmath>
 #include <type_traits>
 
+#include "mfe_device.h"
 #include "mfe_internal.h"
 
 namespace mfe {
 namespace {
-
-constexpr int kBlock = 256;
-
-// ---------------------------------------------------------------- helpers
-// PBCr2 for the box (Src/molaux.F90:159-182): one shift per axis, then
-// r2 = dx**2 + dy**2 + dz**2 summed left to right without fusion, so that
-// list membership is decided on the reference's FP64 value.
-__device__ __forceinline__ double pbcr2(double &dx, double &dy, double &dz,
-                                        const Box &b) {
-#pragma clang fp contract(off)
-  if (fabs(dx) > b.L2[0]) dx = dx - copysign(b.L[0], dx);
-  if (fabs(dy) > b.L2[1]) dy = dy - copysign(b.L[1], dy);
-  if (fabs(dz) > b.L2[2]) dz = dz - copysign(b.L[2], dz);
-  double r2 = dx * dx;
-  r2 = r2 + dy * dy;
-  r2 = r2 + dz * dz;
-  return r2;
-}
-
-// Block-wide sum (max) in a fixed order: a shuffle tree inside each
-// wavefront, then the wavefront results in wavefront order.  Every thread
-// gets the result; sh needs one slot per wavefront.
-template <typename T>
-__device__ __forceinline__ T block_sum(T v, T *sh) {
-  for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[wave] = v;
-  __syncthreads();
-  T r = sh[0];
-  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r += sh[k];
-  return r;
-}
-
-__device__ __forceinline__ double block_max(double v, double *sh) {
-  for (int w = 32; w > 0; w >>= 1) v = fmax(v, __shfl_down(v, w, 64));
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) sh[wave] = v;
-  __syncthreads();
-  double r = sh[0];
-  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = fmax(r, sh[k]);
-  return r;
-}
-
-// The partner value of an FP64 register in a lane quad, by DPP (no LDS):
-// kXor1 swaps lanes 0<->1 and 2<->3, kXor2 swaps 0<->2 and 1<->3 -- the
-// exchanges of __shfl_xor(x, 1 or 2) within a quad.  Every lane of the
-// wavefront must be active.
-constexpr int kXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
-constexpr int kXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
-template <int CTRL>
-__device__ __forceinline__ double quad_swap(double x) {
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
 
 __global__ void k_fill_int(int *p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -372,17 +316,22 @@ __global__ void __launch_bounds__(kBlock) k_nlist_build(const SysDev *__restrict
 // the other XCDs see the block's data), the last block to arrive resets
 // the counter and advances the generation (release), the others poll it
 // (relaxed) and take one acquire fence, which invalidates the CU's stale
-// lines for the whole block.  The wait is bounded: after kGridSpin polls a
-// block raises ERR_SYNC and goes on, so the kernel always ends (its list
-// is then void and the host reports the error and resets the barrier).
+// lines for the whole block.  The wait is bounded: after `spin` polls a
+// block raises ERR_SYNC and leaves.  A block goes on to the next phase
+// only when every block has arrived (so the data of the finished phase is
+// complete) and no block has given up; otherwise it returns false, the
+// kernel ends in every block, and the host reports the error, clears the
+// barrier and the cell counts, and rebuilds the list from scratch.
 constexpr int kGridSpin = 1 << 22;
 
-__device__ __forceinline__ void grid_sync(unsigned *bar, int *flags) {
+__device__ __forceinline__ bool grid_sync(unsigned *bar, int *flags, int spin) {
+  __shared__ int ok_s;
   // every wave waits for its own stores to be acknowledged (in L2) before
   // the block barrier (a workgroup-scope barrier does not wait for them)
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   if (threadIdx.x == 0) {
+    int ok = 1;
     const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const unsigned arrived =
         __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
@@ -392,39 +341,47 @@ __device__ __forceinline__ void grid_sync(unsigned *bar, int *flags) {
     } else {
       int spins = 0;
       while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-        if (++spins > kGridSpin) {
+        if (++spins > spin) {
           atomicOr(&flags[0], ERR_SYNC);
+          ok = 0;
           break;
         }
         __builtin_amdgcn_s_sleep(8);
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
+    if (ok && (__hip_atomic_load(&flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) &
+               ERR_SYNC))
+      ok = 0;
+    ok_s = ok;
   }
   __syncthreads();
+  return ok_s != 0;
 }
 
 // The gated list rebuild of one engine as one kernel: it returns at once
-// when the skin test did not fire (one launch per step instead of five);
-// otherwise the phases run in order, separated by grid barriers.  Launched
-// with one 1024-thread block per CU, so all blocks are resident.
+// when the skin test did not fire (one launch per step instead of five),
+// or after a failed rebuild earlier in the same call (the list is void
+// until the host rebuilds it); otherwise the phases run in order,
+// separated by grid barriers.  Launched with one 1024-thread block per
+// CU, so all blocks are resident.
 __global__ void __launch_bounds__(1024) k_rebuild(const SysDev *__restrict__ sys, unsigned *bar,
-                                                  int *bsum) {
+                                                  int *bsum, int spin) {
   const SysDev &V = sys[0];
-  if (V.flags[1] == 0) return;
+  if (V.flags[1] == 0 || (V.flags[0] & ERR_SYNC)) return;
   const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   ph_cell_count(V, t0, nt);
-  grid_sync(bar, V.flags);
+  if (!grid_sync(bar, V.flags, spin)) return;
   ph_scan_local(V.c_count, V.c_start, V.cg.ncells, bsum);
-  grid_sync(bar, V.flags);
+  if (!grid_sync(bar, V.flags, spin)) return;
   ph_scan_offset(V.c_count, V.c_start, V.cg.ncells, bsum);
-  grid_sync(bar, V.flags);
+  if (!grid_sync(bar, V.flags, spin)) return;
   ph_cell_scatter(V, t0, nt);
-  grid_sync(bar, V.flags);
+  if (!grid_sync(bar, V.flags, spin)) return;
   ph_cell_sort(V, t0, nt);
-  grid_sync(bar, V.flags);
+  if (!grid_sync(bar, V.flags, spin)) return;
   ph_sort_pos(V, t0, nt);
-  grid_sync(bar, V.flags);
+  if (!grid_sync(bar, V.flags, spin)) return;
   ph_nlist_build<true>(V, t0, nt);
   if (t0 == 0) V.flags[2] += 1;
 }
@@ -433,28 +390,6 @@ __global__ void __launch_bounds__(1024) k_rebuild(const SysDev *__restrict__ sys
 __global__ void k_sort_pos(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
   ph_sort_pos(V, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
-}
-
-// ------------------------------------------------- spline pair term
-// Uniform cubic B-spline on knots r0 + k/dinv: on interval i (fraction f)
-// the weights of coefficients i..i+3 are (1-f)^3/6, (3f^3-6f^2+4)/6,
-// (-3f^3+3f^2+3f+1)/6, f^3/6 (de Boor, A Practical Guide to Splines, ch. IX),
-// and their derivatives in f.
-__device__ __forceinline__ bool knot_weights(double r, double r0, double dinv, int nknot,
-                                             int &i, double w[4], double dw[4]) {
-  const double x = (r - r0) * dinv;
-  if (!(x >= 0.0) || !(x < (double)(nknot - 3))) return false;
-  i = (int)x;
-  const double f = x - i, f2 = f * f, f3 = f2 * f, g = 1.0 - f;
-  w[0] = g * g * g / 6.0;
-  w[1] = (3.0 * f3 - 6.0 * f2 + 4.0) / 6.0;
-  w[2] = (-3.0 * f3 + 3.0 * f2 + 3.0 * f + 1.0) / 6.0;
-  w[3] = f3 / 6.0;
-  dw[0] = -0.5 * g * g;
-  dw[1] = 1.5 * f2 - 2.0 * f;
-  dw[2] = -1.5 * f2 + f + 0.5;
-  dw[3] = 0.5 * f2;
-  return true;
 }
 
 // ------------------------------------------------------------ pair forces
@@ -472,17 +407,6 @@ __device__ __forceinline__ int find_interval(const Z &zl, int n, double r2, int 
   while (k > 0 && r2 >= zl(k - 1)) --k;
   return k;
 }
-
-struct KahanF {
-  float s = 0.f, c = 0.f;
-  __device__ __forceinline__ void add(float v) {
-    float y = v - c;
-    float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ double value() const { return (double)s - (double)c; }
-};
 
 // list entries a lane loads together (indices, then positions)
 constexpr int kPairBatch = 4;
@@ -677,8 +601,12 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
       const float fs = c[6] + df * (c[7] + df * (c[8] + df * (c[9] + df * c[10])));
       fsum = fs;
       if (energy) {
-        const float us = c[0] + df * (c[1] + df * (c[2] + df * (c[3] + df * (c[4] + df * c[5]))));
-        usum = us;
+        // the interval constant w0 in FP64 (its FP32 value plus the stored
+        // rounding in slot 11), the remainder polynomial in FP32: the
+        // constant's rounding is the same for every pair of the interval
+        // and would not average out over the sum
+        const float up = df * (c[1] + df * (c[2] + df * (c[3] + df * (c[4] + df * c[5]))));
+        usum = ((double)c[0] + (double)c[11]) + (double)up;
       } else {
         usum = 0.0;
       }
@@ -829,51 +757,6 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
 }
 
 // ------------------------------------------------------------------ SPME
-// CardinalBSpline (Src/mollib.F90:1629-1683) at fraction x in [0,1):
-// v[i] is the weight of grid point kmax - i, d[i] its derivative.
-template <typename R, int ORDER>
-__device__ __forceinline__ void bspline(R x, R *v, R *d) {
-  R last[ORDER];
-  v[0] = x;
-  v[1] = R(1) - x;
-#pragma unroll
-  for (int i = 2; i < ORDER; ++i) v[i] = R(0);
-#pragma unroll
-  for (int i = 0; i < ORDER; ++i) last[i] = R(0);
-#pragma unroll
-  for (int n = 3; n <= ORDER; ++n) {
-#pragma unroll
-    for (int i = 0; i < n - 1; ++i) last[i] = v[i];
-    // 1/(n-1) folded at compile time (the reference divides; the results
-    // differ by at most an ulp)
-    const R inv = R(1.0 / (n - 1));
-    v[0] = x * last[0] * inv;
-#pragma unroll
-    for (int i = 1; i <= n - 2; ++i) {
-      const R a = R(i) + x;
-      v[i] = (a * last[i] + (R(n) - a) * last[i - 1]) * inv;
-    }
-    v[n - 1] = (R(1) - x) * last[n - 2] * inv;
-  }
-  if (d) {
-    d[0] = last[0];
-#pragma unroll
-    for (int i = 1; i <= ORDER - 2; ++i) d[i] = last[i] - last[i - 1];
-    d[ORDER - 1] = -last[ORDER - 2];
-  }
-}
-
-__device__ __forceinline__ void mesh_coords(const double *r, const SpmeGeom &g,
-                                            int mm[3], double fr[3]) {
-  for (int m = 0; m < 3; ++m) {
-    const double u = g.dkdr[m] * r[m];
-    const double f = floor(u);
-    fr[m] = u - f;
-    int k = (int)f % g.s[m];
-    mm[m] = k < 0 ? k + g.s[m] : k;
-  }
-}
-
 // Brick of every particle and its slot within the brick, both indexed by
 // the particle's place t in the cell order, from the cell-ordered positions
 // (current whenever SPME runs: k_move or the pair launch refreshes them).
@@ -967,52 +850,6 @@ __global__ void k_spme_scatter(const SysDev *__restrict__ sys) {
   frac[4 * pos + 2] = (R)fr[2];
   frac[4 * pos + 3] = (R)q[i];
   meta[pos] = make_int2(mm[0] | (mm[1] << 10) | (mm[2] << 20), i);
-}
-
-// LDS accumulator of the spreading: exact 64-bit fixed point (2^-32) for
-// the mixed path, so the sum does not depend on the order of the adds;
-// FP64 LDS atomics for the FP64 path.  The fixed-point value is
-// round-to-nearest-even of v * 2^32: v * 2^32 + 1.5 * 2^52 in FP64 (the
-// product is exact, the sum rounds once to an integer because
-// |v| * 2^32 < 2^51), whose low mantissa bits minus those of 1.5 * 2^52 are
-// the two's-complement integer.
-template <typename R>
-struct SpreadAcc;
-template <>
-struct SpreadAcc<float> {
-  typedef unsigned long long T;
-  static __device__ __forceinline__ void add(T *p, float v) {
-    constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
-    const double d = __fma_rn((double)v, 4294967296.0, kMagic);
-    atomicAdd(p, (unsigned long long)__double_as_longlong(d) -
-                     (unsigned long long)__double_as_longlong(kMagic));
-  }
-  static __device__ __forceinline__ float get(T v) {
-    return (float)((double)(long long)v * (1.0 / 4294967296.0));
-  }
-};
-template <>
-struct SpreadAcc<double> {
-  typedef double T;
-  static __device__ __forceinline__ void add(T *p, double v) { atomicAdd(p, v); }
-  static __device__ __forceinline__ double get(T v) { return v; }
-};
-
-constexpr int kMaxOrder = 8;
-
-// one particle record in brick order: fraction (x, y, z) and charge
-template <typename R>
-struct Rec4 {
-  R x, y, z, q;
-};
-__device__ __forceinline__ Rec4<float> load_rec(const float *__restrict__ p) {
-  const float4 v = *reinterpret_cast<const float4 *>(p);
-  return {v.x, v.y, v.z, v.w};
-}
-__device__ __forceinline__ Rec4<double> load_rec(const double *__restrict__ p) {
-  const double2 a = reinterpret_cast<const double2 *>(p)[0];
-  const double2 b = reinterpret_cast<const double2 *>(p)[1];
-  return {a.x, a.y, b.x, b.y};
 }
 
 // Spreading, each particle once and no global atomics: one block per brick
@@ -1137,10 +974,15 @@ __device__ __forceinline__ void fin_apply(const SysDev &V, int i, const double f
                                          const double es[3]);
 __device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin, int bd);
 
+// MODE: GM_FORCE adds the force; GM_POT also writes efp * phi_i (the
+// reciprocal potential, dE_rec/dq_i); GM_FIELD also writes the reciprocal
+// field -efp grad phi_i (the force per unit charge); GM_POTONLY writes
+// efp * phi_i of the mesh and leaves the forces alone (the backward pass
+// of the force-matching loss interpolates its own convolved mesh).
 // FIN: also the finalize phase of each particle (its owner lane, after the
 // force) and, in block nbricks, the reductions, whose inputs come from
 // earlier launches.
-template <typename R, int ORDER, bool POT, bool FIN>
+template <typename R, int ORDER, int MODE, bool FIN>
 __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restrict__ sys,
                                                          FinArgs fa) {
   const SysDev &V = sys[blockIdx.z];
@@ -1218,7 +1060,7 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
     const int idx = md.y;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     FinState st;
-    if (owner) {
+    if (owner && MODE != GM_POTONLY) {
       f0 = force[3 * idx];
       f1 = force[3 * idx + 1];
       f2 = force[3 * idx + 2];
@@ -1266,10 +1108,12 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
             sx += v[0][ix] * qv;
             sdx += d[0][ix] * qv;
           }
-          acc.add(0, sdx * (v[1][iy] * vz));
-          acc.add(1, sx * (d[1][iy] * vz));
-          acc.add(2, sx * (v[1][iy] * dz));
-          if (POT) pacc += (double)sx * (double)(v[1][iy] * vz);
+          if (MODE != GM_POTONLY) {
+            acc.add(0, sdx * (v[1][iy] * vz));
+            acc.add(1, sx * (d[1][iy] * vz));
+            acc.add(2, sx * (v[1][iy] * dz));
+          }
+          if (MODE == GM_POT || MODE == GM_POTONLY) pacc += (double)sx * (double)(v[1][iy] * vz);
         }
       }
       ex = acc.get(0);
@@ -1283,25 +1127,33 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       ex += quad_swap<kXor1>(ex);
       ey += quad_swap<kXor1>(ey);
       ez += quad_swap<kXor1>(ez);
-      if (POT) ep += quad_swap<kXor1>(ep);
+      if (MODE == GM_POT || MODE == GM_POTONLY) ep += quad_swap<kXor1>(ep);
     } else {
 #pragma unroll
       for (int w = 1; w < GL; w <<= 1) {
         ex += __shfl_xor(ex, w, GL);
         ey += __shfl_xor(ey, w, GL);
         ez += __shfl_xor(ez, w, GL);
-        if (POT) ep += __shfl_xor(ep, w, GL);
+        if (MODE == GM_POT || MODE == GM_POTONLY) ep += __shfl_xor(ep, w, GL);
       }
     }
     if (owner) {
       const double qd = (double)f.q;
-      const double ft[3] = {f0 - qd * ex * g.dkdr[0] * efp, f1 - qd * ey * g.dkdr[1] * efp,
-                            f2 - qd * ez * g.dkdr[2] * efp};
-      force[3 * idx] = ft[0];
-      force[3 * idx + 1] = ft[1];
-      force[3 * idx + 2] = ft[2];
+      const double fld[3] = {-ex * g.dkdr[0] * efp, -ey * g.dkdr[1] * efp,
+                             -ez * g.dkdr[2] * efp};
+      const double ft[3] = {f0 + qd * fld[0], f1 + qd * fld[1], f2 + qd * fld[2]};
+      if (MODE != GM_POTONLY) {
+        force[3 * idx] = ft[0];
+        force[3 * idx + 1] = ft[1];
+        force[3 * idx + 2] = ft[2];
+      }
       // dE_rec/dq = efp * sum_stencil theta * (G * Q) (the convolved mesh)
-      if (POT) V.phi_rec[idx] = efp * ep;
+      if (MODE == GM_POT || MODE == GM_POTONLY) V.phi_rec[idx] = efp * ep;
+      if (MODE == GM_FIELD) {
+        V.fm_fld[3 * idx] = fld[0];
+        V.fm_fld[3 * idx + 1] = fld[1];
+        V.fm_fld[3 * idx + 2] = fld[2];
+      }
       if (FIN) {
         const double es[3] = {0.0, 0.0, 0.0};
         fin_apply(V, idx, ft, st, fa.mode, fa.dt, es);
@@ -2015,7 +1867,9 @@ __global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
                                                  double dt, int bd) {
 #pragma clang fp contract(off)
   const SysDev &V = sys[blockIdx.z];
-  if ((int)blockIdx.x >= V.nblk_np) return;
+  // after a failed list rebuild (a grid barrier gave up) the state stays
+  // where it was for the rest of the call, which then reports the error
+  if ((int)blockIdx.x >= V.nblk_np || (V.flags[0] & ERR_SYNC)) return;
   double *__restrict__ ro = V.ro;
   const double *__restrict__ rod = V.rod;
   const double *__restrict__ rodd = V.rodd;
@@ -2263,16 +2117,14 @@ const void *k_spread_ptr() {
   return (const void *)&k_spread<R, O>;
 }
 template <typename R, int O>
-const void *k_gather_ptr() {
-  return (const void *)&k_gather<R, O, false, false>;
-}
-template <typename R, int O>
-const void *k_gather_pot_ptr() {
-  return (const void *)&k_gather<R, O, true, false>;
-}
-template <typename R, int O>
-const void *k_gather_fin_ptr() {
-  return (const void *)&k_gather<R, O, false, true>;
+const void *k_gather_ptr(int mode, bool fin) {
+  if (fin) return (const void *)&k_gather<R, O, GM_FORCE, true>;
+  switch (mode) {
+    case GM_POT: return (const void *)&k_gather<R, O, GM_POT, false>;
+    case GM_FIELD: return (const void *)&k_gather<R, O, GM_FIELD, false>;
+    case GM_POTONLY: return (const void *)&k_gather<R, O, GM_POTONLY, false>;
+    default: return (const void *)&k_gather<R, O, GM_FORCE, false>;
+  }
 }
 
 void philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
@@ -2290,7 +2142,7 @@ void launch_fill_double(double *p, int n, double v, hipStream_t s) {
 void launch_nlist_rebuild(const Group &g, bool gated) {
   if (gated && g.R == 1 && g.rebuild_bar && g.ncu > 0) {
     hipLaunchKernelGGL(k_rebuild, dim3(g.ncu), dim3(1024), 0, g.stream, g.d, g.rebuild_bar,
-                       g.rebuild_sum);
+                       g.rebuild_sum, g.sync_spin >= 0 ? g.sync_spin : kGridSpin);
     return;
   }
   // gated kernels run on small grids (grid-stride loops): when the skin
@@ -2374,7 +2226,7 @@ void launch_pair(const Group &g, bool energy, bool sort_pos, bool skin) {
 }
 
 template <typename R>
-static void spme_stages(const Group &g, bool energy, int first, int last, bool pot,
+static void spme_stages(const Group &g, bool energy, int first, int last, int gmode,
                         const FinArgs *fin) {
   for (int st = first; st <= last; ++st) {
     switch (st) {
@@ -2408,14 +2260,20 @@ static void spme_stages(const Group &g, bool energy, int first, int last, bool p
         switch (g.order) {
 #define MFE_GATHER(O)                                                                   \
   case O:                                                                               \
-    if (pot)                                                                            \
-      hipLaunchKernelGGL((k_gather<R, O, true, false>), grid(g.bricks, g),              \
+    if (fin)                                                                            \
+      hipLaunchKernelGGL((k_gather<R, O, GM_FORCE, true>), grid(g.bricks + 1, g),       \
                          dim3(kGatherLanes), g.lds_gather, g.stream, g.d, fa);          \
-    else if (fin)                                                                       \
-      hipLaunchKernelGGL((k_gather<R, O, false, true>), grid(g.bricks + 1, g),          \
+    else if (gmode == GM_POT)                                                           \
+      hipLaunchKernelGGL((k_gather<R, O, GM_POT, false>), grid(g.bricks, g),            \
+                         dim3(kGatherLanes), g.lds_gather, g.stream, g.d, fa);          \
+    else if (gmode == GM_FIELD)                                                         \
+      hipLaunchKernelGGL((k_gather<R, O, GM_FIELD, false>), grid(g.bricks, g),          \
+                         dim3(kGatherLanes), g.lds_gather, g.stream, g.d, fa);          \
+    else if (gmode == GM_POTONLY)                                                       \
+      hipLaunchKernelGGL((k_gather<R, O, GM_POTONLY, false>), grid(g.bricks, g),        \
                          dim3(kGatherLanes), g.lds_gather, g.stream, g.d, fa);          \
     else                                                                                \
-      hipLaunchKernelGGL((k_gather<R, O, false, false>), grid(g.bricks, g),             \
+      hipLaunchKernelGGL((k_gather<R, O, GM_FORCE, false>), grid(g.bricks, g),          \
                          dim3(kGatherLanes), g.lds_gather, g.stream, g.d, fa);          \
     break;
           MFE_GATHER(3) MFE_GATHER(4) MFE_GATHER(5) MFE_GATHER(6) MFE_GATHER(7) MFE_GATHER(8)
@@ -2427,10 +2285,10 @@ static void spme_stages(const Group &g, bool energy, int first, int last, bool p
   }
 }
 
-void launch_spme(const Group &g, bool energy, int first, int last, bool pot,
+void launch_spme(const Group &g, bool energy, int first, int last, int gmode,
                  const FinArgs *fin) {
-  if (g.prec == MFE_PREC_MIXED) spme_stages<float>(g, energy, first, last, pot, fin);
-  else spme_stages<double>(g, energy, first, last, pot, fin);
+  if (g.prec == MFE_PREC_MIXED) spme_stages<float>(g, energy, first, last, gmode, fin);
+  else spme_stages<double>(g, energy, first, last, gmode, fin);
 }
 
 void launch_std_ewald(const Group &g) {
@@ -2503,14 +2361,15 @@ void launch_skin(const Group &g) {
 // Bricks above 64 KB of LDS (meshes whose only divisors >= order-1 are
 // large, e.g. 2*23) need the per-kernel dynamic LDS limit raised.
 void mfe_allow_big_spread_lds(int prec, int bytes) {
-#define MFE_ATTR(R, O)                                                       \
-  (void)hipFuncSetAttribute((const void *)mfe::k_spread_ptr<R, O>(),          \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes); \
-  (void)hipFuncSetAttribute((const void *)mfe::k_gather_ptr<R, O>(),          \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes); \
-  (void)hipFuncSetAttribute((const void *)mfe::k_gather_pot_ptr<R, O>(),      \
-                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes); \
-  (void)hipFuncSetAttribute((const void *)mfe::k_gather_fin_ptr<R, O>(),      \
+#define MFE_ATTR(R, O)                                                              \
+  (void)hipFuncSetAttribute((const void *)mfe::k_spread_ptr<R, O>(),                 \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);        \
+  for (int gm = 0; gm < 4; ++gm)                                                     \
+    (void)hipFuncSetAttribute(mfe::k_gather_ptr<R, O>(gm, false),                     \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);      \
+  (void)hipFuncSetAttribute(mfe::k_gather_ptr<R, O>(mfe::GM_FORCE, true),             \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);        \
+  (void)hipFuncSetAttribute(mfe::k_fm_spread_ptr<R, O>(),                             \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (prec == MFE_PREC_MIXED) {
     MFE_ATTR(float, 3) MFE_ATTR(float, 4) MFE_ATTR(float, 5)

@@ -85,7 +85,8 @@ SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_batch_failed_replica", "mfe_batch_build_list", "mfe_batch_compute",
            "mfe_batch_step_md", "mfe_batch_step_bd", "mfe_batch_get_energies",
            "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint", "mfe_mc_delta_rec",
-           "mfe_mc_delta_pair", "mfe_mc_accept", "mfe_ewald_parameters"]
+           "mfe_mc_delta_pair", "mfe_mc_accept", "mfe_ewald_parameters",
+           "mfe_set_sync_limit", "mfe_model_forces", "mfe_fm_set_frames", "mfe_fm_loss_grad"]
 
 _lib = None
 
@@ -105,6 +106,10 @@ def lib():
         L.mfe_destroy.argtypes = [C.c_void_p]
         L.mfe_last_error.argtypes = [C.c_void_p]
         L.mfe_last_error.restype = C.c_char_p
+        L.mfe_set_sync_limit.argtypes = [C.c_void_p, C.c_int32]
+        L.mfe_model_forces.argtypes = [C.c_void_p, _dp]
+        L.mfe_fm_set_frames.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
+        L.mfe_fm_loss_grad.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.mfe_set_box.argtypes = [C.c_void_p, _dp]
         L.mfe_set_particles.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _ip, _dp, _dp]
         L.mfe_set_tables.argtypes = [C.c_void_p, _dp, C.c_int32, _ip, _dp, _dp, C.c_double]
@@ -412,6 +417,10 @@ class ForceEngine:
     def step_md(self, nsteps, tstep, flags=FORCES | ENERGY | VIRIAL):
         self._check(self.L.mfe_step_md(self.h, int(nsteps), float(tstep), flags), "mfe_step_md")
 
+    def set_sync_limit(self, polls):
+        """Poll bound of the list rebuild's grid-barrier waits (< 0: default)."""
+        self._check(self.L.mfe_set_sync_limit(self.h, int(polls)), "mfe_set_sync_limit")
+
     def set_bd(self, fac1, fac2, seed=0):
         """BDStep factors per type and the seed of the device noise stream."""
         self._check(self.L.mfe_set_bd(self.h, _d(_f64(fac1)), _d(_f64(fac2)), int(seed)),
@@ -469,6 +478,39 @@ class ForceEngine:
         out["dknot"] = out["dknot"][:nptpt * nk].reshape(nptpt, nk)
         out["dro"] = out["dro"].reshape(-1, 3)
         out["E"] = e.value
+        return out
+
+    def model_forces(self):
+        """Forces of the adjoint's model at the current positions:
+        -dE/dro [np, 3] in kJ/(mol A)."""
+        f = np.zeros(3 * self.np)
+        self._check(self.L.mfe_model_forces(self.h, _d(f)), "mfe_model_forces")
+        return f.reshape(-1, 3)
+
+    def fm_set_frames(self, ro, f_target):
+        """Frames ro[nf, np, 3] and their target forces (kJ/(mol A)), kept
+        on the device for fm_loss_grad."""
+        ro = _f64(ro).reshape(-1)
+        ft = _f64(f_target).reshape(-1)
+        nf = ro.size // (3 * self.np)
+        if nf < 1 or ro.size != 3 * self.np * nf or ft.size != ro.size:
+            raise ValueError("frames and targets must be [nf, np, 3]")
+        self._check(self.L.mfe_fm_set_frames(self.h, nf, _d(ro), _d(ft)), "mfe_fm_set_frames")
+        self.fm_nframes = nf
+
+    def fm_loss_grad(self):
+        """Force-matching loss over the stored frames and its gradient:
+        dict(loss, dzat[npt], dsigat[npt], depsat[npt], dknot[nptpt, nknot])."""
+        nk = getattr(self, "nknot", 0)
+        nptpt = self.npt * (self.npt + 1) // 2
+        out = dict(dzat=np.zeros(self.npt), dsigat=np.zeros(self.npt),
+                   depsat=np.zeros(self.npt), dknot=np.zeros(max(nptpt * nk, 1)))
+        loss = C.c_double()
+        self._check(self.L.mfe_fm_loss_grad(self.h, C.byref(loss), _d(out["dzat"]),
+                                            _d(out["dsigat"]), _d(out["depsat"]),
+                                            _d(out["dknot"])), "mfe_fm_loss_grad")
+        out["dknot"] = out["dknot"][:nptpt * nk].reshape(nptpt, nk)
+        out["loss"] = loss.value
         return out
 
     def download_forces(self):

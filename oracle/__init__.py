@@ -159,6 +159,11 @@ def _declare(L):
                                         _dp, C.c_double, C.c_double, C.c_int, C.c_int,
                                         C.c_double, C.c_double, C.c_double, C.c_int,
                                         C.c_double, C.c_double, _dp, _dp]
+    L.orc_model_pair_forces.restype = C.c_double
+    L.orc_model_pair_forces.argtypes = [C.POINTER(Box), C.c_int, _ip, _dp, C.c_int, _dp, _dp,
+                                        _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.c_double, C.c_double, C.c_double, C.c_int,
+                                        C.c_double, C.c_double, _dp, _dp]
     L.orc_gaurandom.restype = C.c_double
     L.orc_gaurandom.argtypes = [C.POINTER(C.c_int32), C.POINTER(GauState)]
 
@@ -535,6 +540,42 @@ def uangle(boxlen, ro, chains, k, p, eq):
     return force.reshape(-1, 3), u.value
 
 
+def ubond(boxlen, ro, chains, k, p, eq):
+    """UBond (Src/energy.F90:5335-5400) of chains [(first, len), ...]
+    (1-based), namelist k (divided by p here, Src/potential.F90:3561), p and
+    eq: (force[np, 3] in kJ/(mol A), u_bond)."""
+    ro = f64(ro).reshape(-1)
+    b = make_box(boxlen)
+    first = i32([c[0] for c in chains])
+    ln = i32([c[1] for c in chains])
+    force = np.zeros_like(ro)
+    u = np.zeros(1)
+    vir = np.zeros(1)
+    lib().orc_ubond(C.byref(b), _d(ro), first.size, _i(first), _i(ln), float(k) / int(p),
+                    float(int(p)), float(eq), _d(force), _d(u), _d(vir))
+    return force.reshape(-1, 3), float(u[0])
+
+
+def model_forces(boxlen, iptpn, ro, zat, sigat, epsat, efp, alpha, rcut, *, spme=None,
+                 lsurf=False, chains=None, bond=(0.0, 2, 0.0), angle=(0.0, 2, 180.0),
+                 knots=None, lewald=True, lljcut=True):
+    """Forces of the model the engine's force matching uses: the pair model
+    (model_pair_forces), SPME (spme_rec with the SpmeSetup `spme`), the
+    surface term, bonds and angles -- -dE/dro [np, 3] in kJ/(mol A)."""
+    _, f = model_pair_forces(boxlen, iptpn, ro, zat, sigat, epsat, efp, alpha, rcut,
+                             lewald=lewald, lljcut=lljcut, knots=knots)
+    az = f64(zat)[i32(iptpn) - 1]
+    if spme is not None:
+        f = f + spme_rec(spme, ro, az, efp)[0]
+    if lsurf:
+        f = f + ewald_surf(boxlen, ro, az, efp)[0]
+    if chains:
+        f = f + ubond(boxlen, ro, chains, *bond)[0]
+        if angle[0] != 0.0:
+            f = f + uangle(boxlen, ro, chains, *angle)[0]
+    return f
+
+
 def velver_move(boxlen, tstep, ro, rod, rodd):
     b = make_box(boxlen)
     drostep = np.zeros_like(ro)
@@ -588,6 +629,30 @@ def model_pair_energy(boxlen, iptpn, ro, zat, sigat, epsat, efp, alpha, rcut, *,
                                     float(ljushift), float(rcut), int(nk), float(r0),
                                     float(r1), _d(c.reshape(-1)), _d(fk))
     return e, fk.reshape(-1, 3)
+
+
+def model_pair_forces(boxlen, iptpn, ro, zat, sigat, epsat, efp, alpha, rcut, *,
+                      lewald=True, lljcut=True, ljrcut=None, ljushift=1.0, knots=None):
+    """The same pair model through a cell grid: (energy, force[np, 3]) with
+    force = -dE/dro in kJ/(mol A).  knots = (r0, r1, c[nptpt, nknot])."""
+    b = make_box(boxlen)
+    ro = f64(ro).reshape(-1)
+    np_ = ro.size // 3
+    if ljrcut is None:
+        ljrcut = 2.0 ** float(np.float32(1.0) / np.float32(6.0))
+    f = np.zeros(3 * np_)
+    if knots is None:
+        nk, r0, r1, c = 0, 0.0, 1.0, np.zeros(1)
+    else:
+        r0, r1, c = knots
+        c = f64(c)
+        nk = c.shape[-1]
+    e = lib().orc_model_pair_forces(C.byref(b), np_, _i(i32(iptpn)), _d(ro), len(zat),
+                                    _d(f64(zat)), _d(f64(sigat)), _d(f64(epsat)), float(efp),
+                                    float(alpha), int(lewald), int(lljcut), float(ljrcut),
+                                    float(ljushift), float(rcut), int(nk), float(r0),
+                                    float(r1), _d(c.reshape(-1)), _d(f))
+    return e, f.reshape(-1, 3)
 
 
 def bd_factors(tstep, temp, dcoeff):

@@ -1523,3 +1523,109 @@ double orc_model_pair_energy(const orc_box *b, int np, const int *iptpn, const d
   }
   return e;
 }
+
+/* one pair of the model: energy and fr = -(du/dr)/r */
+static double model_pair(double r2, int ti, int tj, int npt, const double *zat,
+                         const double *sigat, const double *epsat, double efp, double alpha,
+                         int lewald, int lljcut, double ljrcut, double ljushift, int nknot,
+                         double r0, double dinv, const double *c, double *fr) {
+  const double r = sqrt(r2);
+  const double qq = zat[ti] * zat[tj] * efp;
+  double u, dudr;
+  if (lewald) {
+    const double ar = alpha * r;
+    const double ec = 1.0 - orc_erf2(ar);
+    u = qq * ec / r;
+    dudr = -qq * (ec / r2 + 1.1283791670955126 * alpha * exp(-ar * ar) / r);
+  } else {
+    u = qq / r;
+    dudr = -qq / r2;
+  }
+  const double sig = 0.5 * (sigat[ti] + sigat[tj]);
+  const double eps = sqrt(epsat[ti] * epsat[tj]);
+  if (!lljcut || r < ljrcut * sig) {
+    const double x = sig / r, s6 = x * x * x * x * x * x;
+    u += 4.0 * eps * (s6 * s6 - s6) + (lljcut ? ljushift * eps : 0.0);
+    dudr -= 24.0 * eps * (2.0 * s6 * s6 - s6) / r;
+  }
+  if (nknot > 3) {
+    int ki;
+    double w[4], dw[4];
+    if (orc_knot_weights(r, r0, dinv, nknot, &ki, w, dw)) {
+      const int a = ti < tj ? ti : tj, bb = ti < tj ? tj : ti;
+      const int p = a * npt - a * (a - 1) / 2 + (bb - a);
+      const double *cp = c + (size_t)p * nknot + ki;
+      u += cp[0] * w[0] + cp[1] * w[1] + cp[2] * w[2] + cp[3] * w[3];
+      dudr += (cp[0] * dw[0] + cp[1] * dw[1] + cp[2] * dw[2] + cp[3] * dw[3]) * dinv;
+    }
+  }
+  *fr = -dudr / r;
+  return u;
+}
+
+double orc_model_pair_forces(const orc_box *b, int np, const int *iptpn, const double *ro,
+                             int npt, const double *zat, const double *sigat,
+                             const double *epsat, double efp, double alpha, int lewald,
+                             int lljcut, double ljrcut, double ljushift, double rcut,
+                             int nknot, double r0, double r1, const double *c,
+                             double *force) {
+  const double dinv = nknot > 3 ? (nknot - 3) / (r1 - r0) : 0.0;
+  const double rc2 = rcut * rcut;
+  double e = 0.0;
+  for (int k = 0; k < 3 * np; ++k) force[k] = 0.0;
+  int n[3];
+  int cells = 1;
+  for (int m = 0; m < 3; ++m) {
+    n[m] = (int)floor(b->boxlen[m] / rcut);
+    if (n[m] < 3) cells = 0;
+  }
+#define MODEL_PAIR(i, j)                                                                   \
+  do {                                                                                     \
+    double dx = ro[3 * (i)] - ro[3 * (j)], dy = ro[3 * (i) + 1] - ro[3 * (j) + 1],         \
+           dz = ro[3 * (i) + 2] - ro[3 * (j) + 2], r2, fr;                                 \
+    orc_pbcr2(b, &dx, &dy, &dz, &r2);                                                      \
+    if (r2 <= rc2) {                                                                       \
+      e += model_pair(r2, iptpn[i] - 1, iptpn[j] - 1, npt, zat, sigat, epsat, efp, alpha,  \
+                      lewald, lljcut, ljrcut, ljushift, nknot, r0, dinv, c, &fr);          \
+      force[3 * (i)] += fr * dx; force[3 * (i) + 1] += fr * dy; force[3 * (i) + 2] += fr * dz; \
+      force[3 * (j)] -= fr * dx; force[3 * (j) + 1] -= fr * dy; force[3 * (j) + 2] -= fr * dz; \
+    }                                                                                      \
+  } while (0)
+  if (!cells) {
+    for (int i = 0; i < np; ++i)
+      for (int j = i + 1; j < np; ++j) MODEL_PAIR(i, j);
+    return e;
+  }
+  const int nc = n[0] * n[1] * n[2];
+  int *head = (int *)malloc(sizeof(int) * (size_t)nc);
+  int *next = (int *)malloc(sizeof(int) * (size_t)np);
+  int *cof = (int *)malloc(sizeof(int) * 3 * (size_t)np);
+  for (int k = 0; k < nc; ++k) head[k] = -1;
+  for (int i = np - 1; i >= 0; --i) {
+    int cc[3];
+    for (int m = 0; m < 3; ++m) {
+      int v = (int)floor((ro[3 * i + m] + 0.5 * b->boxlen[m]) * n[m] / b->boxlen[m]);
+      cc[m] = v < 0 ? 0 : (v >= n[m] ? n[m] - 1 : v);
+      cof[3 * i + m] = cc[m];
+    }
+    const int id = cc[0] + n[0] * (cc[1] + n[1] * cc[2]);
+    next[i] = head[id];
+    head[id] = i;
+  }
+  for (int i = 0; i < np; ++i) {
+    for (int dz = -1; dz <= 1; ++dz)
+      for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int x = (cof[3 * i] + dx + n[0]) % n[0];
+          const int y = (cof[3 * i + 1] + dy + n[1]) % n[1];
+          const int z = (cof[3 * i + 2] + dz + n[2]) % n[2];
+          for (int j = head[x + n[0] * (y + n[1] * z)]; j >= 0; j = next[j])
+            if (j > i) MODEL_PAIR(i, j);
+        }
+  }
+#undef MODEL_PAIR
+  free(head);
+  free(next);
+  free(cof);
+  return e;
+}

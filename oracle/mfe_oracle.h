@@ -262,6 +262,17 @@ double orc_model_pair_energy(const orc_box *b, int np, const int *iptpn, const d
                              int lljcut, double ljrcut, double ljushift, double rcut,
                              int nknot, double r0, double r1, const double *c,
                              double *fknot);
+/* The same pair model, its energy (return value) and forces force[3*np]
+   (kJ/(mol A), -dE/dro), over the pairs with r2 <= rcut^2 found through a
+   cell grid of edge >= rcut (all pairs when an axis has fewer than three
+   cells); the force-matching loss of the engine is built on these model
+   forces. */
+double orc_model_pair_forces(const orc_box *b, int np, const int *iptpn, const double *ro,
+                             int npt, const double *zat, const double *sigat,
+                             const double *epsat, double efp, double alpha, int lewald,
+                             int lljcut, double ljrcut, double ljushift, double rcut,
+                             int nknot, double r0, double r1, const double *c,
+                             double *force);
 typedef struct { int32_t ix, iy; double am; } orc_random_state;
 double orc_random(int32_t *idum, orc_random_state *st);
 

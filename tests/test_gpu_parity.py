@@ -405,3 +405,69 @@ def test_charge_out_of_range_is_rejected():
         assert E.ERRORS[ei.value.code] == "MFE_E_BAD_PARAM"
     eng.set_particles(s.iptpn, s.az, s.massipt)
     eng.close()
+
+
+def test_rebuild_barrier_give_up_recovers():
+    """The single-engine list rebuild waits at grid barriers with a bounded
+    number of polls.  With the bound set to 0 from the host, a rebuild that
+    fires gives up at its first barrier: every block leaves, the call
+    reports MFE_E_HIP, and the next call builds the list from scratch (cell
+    counts and barrier cleared) -- forces then equal a fresh engine's at the
+    same positions, and a later gated rebuild with the default bound works."""
+    s = S.c2()
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    say("case: c2 rebuild barrier gives up (poll bound 0), fp64")
+    eng = S.configure(s, "fp64", tables=tab)
+    eng.utotal()
+    shift = 1.1 * np.array([0.6, 0.64, 0.48])
+
+    def moved(ro):
+        ro = ro + shift
+        return ro - s.boxlen * np.round(ro / s.boxlen)
+
+    def check_fresh(ro, f, en):
+        fresh = S.configure(s, "fp64", tables=tab)
+        fresh.upload_positions(ro)
+        f2, en2 = fresh.utotal()
+        fresh.close()
+        assert rel_rms(f, f2) <= 1e-13 and abs(en.tot - en2.tot) <= 1e-13 * abs(en2.tot)
+
+    # host upload: the skin test fires, the rebuild gives up
+    eng.set_sync_limit(0)
+    ro1 = moved(s.ro)
+    with pytest.raises(E.MfeError) as ei:
+        eng.upload_positions(ro1)
+    assert ei.value.code == 6
+    eng.set_sync_limit(-1)
+    f, en = eng.utotal()
+    check_fresh(ro1, f, en)
+    # a gated rebuild with the default bound after the recovery
+    ro2 = moved(ro1)
+    r0 = eng.energy().rebuilds
+    eng.upload_positions(ro2)
+    f, en = eng.utotal()
+    assert en.rebuilds == r0 + 1
+    check_fresh(ro2, f, en)
+    # MD steps: a shift of 0.99 A keeps the list (1.98 A < skin 2 A); the
+    # thermal motion of the first steps then fires the skin test inside the
+    # step graph, whose rebuild gives up; the rest of the call leaves the
+    # positions where they are and the call reports the error
+    ro3 = ro2 + 0.99 * shift / 1.1
+    ro3 -= s.boxlen * np.round(ro3 / s.boxlen)
+    eng.upload_positions(ro3)
+    assert eng.energy().rebuilds == r0 + 1
+    eng.set_sync_limit(0)
+    with pytest.raises(E.MfeError) as ei:
+        eng.step_md(10, s.tstep)
+    assert ei.value.code == 6
+    ro4, _, _ = eng.download_state()
+    d = ro4 - ro3
+    d -= s.boxlen * np.round(d / s.boxlen)
+    assert np.isfinite(ro4).all() and np.abs(d).max() < 0.5
+    eng.set_sync_limit(-1)
+    eng.upload_positions(ro4)
+    f, en = eng.utotal()
+    check_fresh(ro4, f, en)
+    eng.step_md(5, s.tstep)
+    eng.close()
