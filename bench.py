@@ -15,6 +15,7 @@ force relative RMS of mixed vs FP64 is reported beside it.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -63,11 +64,31 @@ def algorithmic_bytes(sysm, stage, npairs=0):
     }[stage]
 
 
-def cpu_baseline(sysm, nsteps):
-    """The FP64 oracle (serial C restatement of the reference path) on the
-    same system: MD steps with UTotal, timed on one host core."""
+def cpu_info():
+    """CPU model, logical CPUs of the machine and those this process may use."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def cpu_baseline(sysm, nsteps, all_cores=False):
+    """The FP64 oracle (C restatement of the reference path) on the same
+    system: MD steps with UTotal, timed on the host -- serially on one core,
+    or with all_cores its OpenMP twin (the same source with its particle
+    loops, FFT rows and spreading split over threads)."""
     import oracle as O
     from molsim_amd import engine as EE
+    O.use_all_cores(all_cores)
     efp = EE.epsi0_four_pi() / 78.4
     tab = EE.build_tables(sysm.zat, sysm.sigat, sysm.epsat, rcut=sysm.rcut,
                           epsi_four_pi=efp, alpha=sysm.alpha)
@@ -89,7 +110,63 @@ def cpu_baseline(sysm, nsteps):
         if O.skin_test(drosum, dro, sysm.drnlist):
             orc.set_list(ro)
     dt = time.perf_counter() - t0
+    O.use_all_cores(False)
     return nsteps / dt, dt
+
+
+def c5_force_matching(nframes, bd_steps=10):
+    """Force matching at the size of the force-matching configuration: the
+    polyelectrolyte system scaled to 66,000 particles (SPME 96^3, order 4,
+    rc 18 A) with a 64-knot spline term per type pair.  Frames are BD
+    snapshots of the mixed engine (bd_steps steps apart); targets are the
+    model forces of an FP64 engine with perturbed parameters.  Timed: the
+    loss and its gradient over all frames (list build, model forces,
+    residual, backward pass per frame), frames resident on the device."""
+    s = S.c2(scale=11)
+    efp = E.epsi0_four_pi() / 78.4
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp,
+                         alpha=s.alpha)
+    rng = np.random.default_rng(9)
+    c = 0.05 * rng.normal(size=(6, 64))
+    knots = (3.5, s.rcut, c)
+    eng = S.configure(s, "mixed", tables=tab)
+    eng.set_pair_params(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp, alpha=s.alpha)
+    eng.set_knots(*knots)
+    eng.utotal()
+    f1, f2 = E.bd_factors(0.005, 298.0, [1.0, 1.0, 1.0])
+    eng.set_bd(f1, f2, 4242)
+    frames = np.zeros((nframes, s.np, 3))
+    for k in range(nframes):
+        eng.step_bd(nsteps=bd_steps)
+        frames[k], _, _ = eng.download_state()
+    # targets: FP64 model forces with perturbed parameters
+    st = dataclasses.replace(s, zat=s.zat * np.array([1.03, 0.98, 1.02]),
+                             sigat=s.sigat * np.array([1.02, 0.99, 1.01]),
+                             epsat=s.epsat * np.array([0.95, 1.06, 1.03]))
+    tab_t = E.build_tables(st.zat, st.sigat, st.epsat, rcut=st.rcut, epsi_four_pi=efp,
+                           alpha=st.alpha)
+    ref = S.configure(st, "fp64", tables=tab_t)
+    ref.set_pair_params(st.zat, st.sigat, st.epsat, rcut=st.rcut, epsi_four_pi=efp,
+                        alpha=st.alpha)
+    ref.set_knots(knots[0], knots[1], c + 0.02 * rng.normal(size=c.shape))
+    targets = np.zeros_like(frames)
+    for k in range(nframes):
+        ref.upload_positions(frames[k])
+        targets[k] = ref.model_forces()
+    ref.close()
+    eng.fm_set_frames(frames, targets)
+    eng.fm_loss_grad()
+    t0 = time.perf_counter()
+    g = eng.fm_loss_grad()
+    dt = time.perf_counter() - t0
+    eng.close()
+    return {"n_particles": s.np, "mesh": s.mesh, "order": s.order, "frames": nframes,
+            "knots_per_pair": 64, "precision": "mixed (fp32 model terms, fp64 sums)",
+            "loss": g["loss"], "frames_per_s": round(nframes / dt, 1),
+            "ms_per_frame": round(1000 * dt / nframes, 4),
+            "targets": "fp64 model forces with charges x(1.03, 0.98, 1.02), "
+                       "sigma x(1.02, 0.99, 1.01), epsilon x(0.95, 1.06, 1.03), "
+                       "knots + N(0, 0.02)"}
 
 
 def c5_adjoint(frames):
@@ -125,7 +202,8 @@ def c4_batched(nrep, steps, tol, seq_sample=16):
     t_tune = time.perf_counter() - t0
     tuned = plan.apply(reps)
     cache = {}
-    fac1, fac2 = 2.0e-7, 0.0316
+    # BDStep's factors for D = 1e-9 m^2/s at 298 K and dt = 5 fs
+    fac1, fac2 = (float(v[0]) for v in E.bd_factors(0.005, 298.0, [1.0]))
     engs = [S.configure(s, "mixed", tables=A._tables(s, cache)) for s in tuned]
     for r, e in enumerate(engs):
         e.set_bd(np.full(2, fac1), np.full(2, fac2), 1000 + r)
@@ -172,10 +250,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched-replica BD leg")
-    ap.add_argument("--no-c5", action="store_true", help="skip the 66k-particle adjoint leg")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the 66k-particle force-matching and adjoint legs")
     ap.add_argument("--drift-steps", type=int, default=5000,
                     help="VelVer steps after the timed ones over which the total energy "
                          "drift is reported (0: skip)")
+    ap.add_argument("--c5-frames", type=int, default=256)
     ap.add_argument("--c4-replicas", type=int, default=512)
     ap.add_argument("--c4-steps", type=int, default=100)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -195,11 +275,12 @@ def main():
         eng.build_list()
         f0, e0 = eng.utotal()
         eng.step_md(args.warmup, sysm.tstep, flags)
-        eng.energy()  # synchronises
+        reb0 = eng.energy().rebuilds  # synchronises
         t0 = time.perf_counter()
         eng.step_md(args.steps, sysm.tstep, flags)
         en = eng.energy()
         dt = time.perf_counter() - t0
+        rebuilds_timed = en.rebuilds - reb0
         drift = None
         if args.drift_steps:
             # total energy before and after the configuration's run length of
@@ -283,7 +364,7 @@ def main():
         eng.close()
         return dict(sps=args.steps / dt, ms=1000 * dt / args.steps, f0=f0, e0=e0,
                     en=en, stages=stages, npairs=npairs, host_rate=host_rate, mc_us=mc_us,
-                    drift=drift)
+                    drift=drift, rebuilds_timed=rebuilds_timed)
 
     mixed = run("mixed")
     fp64 = None if args.no_fp64 else run("fp64")
@@ -293,7 +374,8 @@ def main():
     npairs = mixed["npairs"]
     # one kernel per stage ("bin" and "ifft" are two kernels each, "utotal"
     # the whole evaluation: reported, not roofline candidates)
-    kern = {k: v for k, v in mixed["stages"].items() if k not in ("utotal", "bin", "ifft")}
+    kern = {k: v for k, v in mixed["stages"].items()
+            if k not in ("utotal", "bin", "ifft") and algorithmic_bytes(sysm, k, npairs) > 0}
     dom = max(kern, key=kern.get)
     nbytes = algorithmic_bytes(sysm, dom, npairs)
     achieved = nbytes / (kern[dom] * 1e-3) / 1e9
@@ -326,6 +408,7 @@ def main():
                      "algorithmic_bytes": nbytes,
                      "traffic": measured_traffic(dom, args.units)},
         "stages": stage_report,
+        "rebuilds_in_timed_steps": mixed["rebuilds_timed"],
         "utotal_host_buffers_per_s": round(mixed["host_rate"], 1),
         "energy_drift": {"mixed": mixed["drift"], "fp64": fp64["drift"] if fp64 else None},
         "mc_trial_move_us": {"mixed": mixed["mc_us"], "fp64": fp64["mc_us"] if fp64 else None},
@@ -337,16 +420,35 @@ def main():
         out["mixed_over_fp64"] = round(mixed["sps"] / fp64["sps"], 3)
         out["force_rel_rms_mixed_vs_fp64"] = rel
         out["energy_rel_mixed_vs_fp64"] = abs(mixed["e0"].tot - fp64["e0"].tot) / abs(fp64["e0"].tot)
+        # each term's error relative to the total energy (the bound's scale):
+        # pair sum, reciprocal part (self term exact on both), total
+        em, ed = mixed["e0"], fp64["e0"]
+        out["energy_terms_rel_to_total"] = {
+            "twob0": (em.twob0 - ed.twob0) / abs(ed.tot),
+            "rec": (em.rec - ed.rec) / abs(ed.tot),
+            "tot": (em.tot - ed.tot) / abs(ed.tot)}
+        out["fp64_rebuilds_in_timed_steps"] = fp64["rebuilds_timed"]
     if not args.no_c4:
         out["c4_batched_bd"] = c4_batched(args.c4_replicas, args.c4_steps, 1e-3)
     if not args.no_c5:
-        out["c5_adjoint"] = c5_adjoint(10)
+        out["c5_force_matching"] = c5_force_matching(args.c5_frames)
+        out["c5_energy_adjoint"] = c5_adjoint(10)
     if not args.no_cpu_baseline:
+        model, nproc, usable = cpu_info()
         sps, secs = cpu_baseline(sysm, args.cpu_steps)
         out["cpu_baseline"] = {"value": round(sps, 4), "unit": "steps/s", "cores": 1,
                                "kind": "port",
                                "sample": f"{args.cpu_steps} MD steps of the same 200k-ion "
-                                         f"system with the FP64 oracle ({secs:.1f} s)"}
+                                         f"system with the FP64 oracle, serial, one core "
+                                         f"({secs:.1f} s)",
+                               "cpu_model": model, "nproc": nproc, "cpus_usable": usable}
+        nthr = int(os.environ.get("OMP_NUM_THREADS", usable))
+        sps2, secs2 = cpu_baseline(sysm, args.cpu_steps, all_cores=True)
+        out["cpu_baseline_all_cores"] = {
+            "value": round(sps2, 4), "unit": "steps/s", "cores": nthr, "kind": "port",
+            "sample": f"{args.cpu_steps} MD steps, the oracle's OpenMP twin "
+                      f"({nthr} threads; not the reference's summation order) ({secs2:.1f} s)",
+            "cpu_model": model, "nproc": nproc}
     if args.copy_probe_mb:
         # three device-to-device copies of a known size through the HIP
         # runtime (its copy kernel is the counters' calibration reference)

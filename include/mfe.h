@@ -133,6 +133,14 @@ int mfe_build_tables_1_6_12(const mfe_pot_1_6_12 *spec, double *ubuf,
 int mfe_create(const mfe_config *cfg, mfe_engine **out);
 int mfe_destroy(mfe_engine *e);
 const char *mfe_last_error(const mfe_engine *e);
+/* No reference counterpart (MOLSIM only warns about a net charge,
+   Src/potential.F90:621-625).  on = 1 adds the neutralising-background
+   term of a system with net charge Q = sum az to the Ewald energy (both
+   kinds): u%rec += EpsiFourPi * (-pi Q^2 / (2 V ualpha^2)), virial
+   -3 times that term (it scales as 1/V), no force; mfe_adjoint's dE/dq
+   gains EpsiFourPi * (-pi Q / (V ualpha^2)).  Off by default, as the
+   reference computes it.  Zero for neutral systems. */
+int mfe_set_net_charge(mfe_engine *e, int32_t on);
 /* No reference counterpart.  Bound on the polls of each grid-barrier wait
    of the single-engine list rebuild (polls < 0: the default, 2^22).  A
    wait that runs out ends the rebuild in every block; the call then

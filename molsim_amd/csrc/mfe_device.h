@@ -101,6 +101,47 @@ struct KahanF {
   __device__ __forceinline__ double value() const { return (double)s - (double)c; }
 };
 
+// a / D for a small integer D, as the reference divides.  Powers of two
+// multiply exactly.  Otherwise the product with the rounded reciprocal is
+// refined by one FMA step (q + (a - q D) / D), which gives the correctly
+// rounded quotient: multiplying by RN(1/3) or RN(1/5) alone rounds every
+// quotient the same way, and on the FP32 path that bias (5e-8 per axis at
+// order 6, so 1.4e-7 on the spread charge and 3e-7 on the reciprocal
+// energy) does not average out over the mesh.
+template <typename R, int D>
+__device__ __forceinline__ R div_small(R a) {
+  constexpr R inv = R(1.0 / D);
+  if constexpr ((D & (D - 1)) == 0) {
+    return a * inv;
+  } else if constexpr (sizeof(R) == 4) {
+    const float q = a * inv;
+    const float r = __builtin_fmaf(-q, float(D), a);
+    return __builtin_fmaf(r, inv, q);
+  } else {
+    const double q = a * inv;
+    const double r = __builtin_fma(-q, double(D), a);
+    return __builtin_fma(r, inv, q);
+  }
+}
+
+// stages n = N..ORDER of the Schoenberg recursion (compile-time n, so the
+// divisor is a constant of each stage)
+template <typename R, int ORDER, int N>
+__device__ __forceinline__ void bspline_stage(R x, R *v, R *last) {
+  if constexpr (N <= ORDER) {
+#pragma unroll
+    for (int i = 0; i < N - 1; ++i) last[i] = v[i];
+    v[0] = div_small<R, N - 1>(x * last[0]);
+#pragma unroll
+    for (int i = 1; i <= N - 2; ++i) {
+      const R a = R(i) + x;
+      v[i] = div_small<R, N - 1>(a * last[i] + (R(N) - a) * last[i - 1]);
+    }
+    v[N - 1] = div_small<R, N - 1>((R(1) - x) * last[N - 2]);
+    bspline_stage<R, ORDER, N + 1>(x, v, last);
+  }
+}
+
 // CardinalBSpline (Src/mollib.F90:1629-1683) at fraction x in [0,1):
 // v[i] is the weight of grid point kmax - i, d[i] its derivative.
 template <typename R, int ORDER>
@@ -112,21 +153,7 @@ __device__ __forceinline__ void bspline(R x, R *v, R *d) {
   for (int i = 2; i < ORDER; ++i) v[i] = R(0);
 #pragma unroll
   for (int i = 0; i < ORDER; ++i) last[i] = R(0);
-#pragma unroll
-  for (int n = 3; n <= ORDER; ++n) {
-#pragma unroll
-    for (int i = 0; i < n - 1; ++i) last[i] = v[i];
-    // 1/(n-1) folded at compile time (the reference divides; the results
-    // differ by at most an ulp)
-    const R inv = R(1.0 / (n - 1));
-    v[0] = x * last[0] * inv;
-#pragma unroll
-    for (int i = 1; i <= n - 2; ++i) {
-      const R a = R(i) + x;
-      v[i] = (a * last[i] + (R(n) - a) * last[i - 1]) * inv;
-    }
-    v[n - 1] = (R(1) - x) * last[n - 2] * inv;
-  }
+  bspline_stage<R, ORDER, 3>(x, v, last);
   if (d) {
     d[0] = last[0];
 #pragma unroll

@@ -91,13 +91,21 @@ bool plan_ok(const mfe::FftPlan &p) {
 
 template <typename R>
 void* make_twiddles(int n, hipStream_t s) {
-  std::vector<mfe::cplx<R>> h(n);
+  std::vector<mfe::Twiddle<R>> h(n);
   for (int e = 0; e < n; ++e) {
-    const double a = -2.0 * kPi * (double)e / (double)n;
-    h[e].x = (R)std::cos(a);
-    h[e].y = (R)std::sin(a);
+    // the angle reduced to the first octant's accuracy: cos/sin of
+    // 2 pi e / n in long double
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)e /
+                          (long double)n;
+    const long double c = std::cos(a), sn = std::sin(a);
+    h[e].x = (R)c;
+    h[e].y = (R)sn;
+    if constexpr (sizeof(R) == 4) {
+      h[e].lx = (float)(c - (long double)h[e].x);
+      h[e].ly = (float)(sn - (long double)h[e].y);
+    }
   }
-  mfe::cplx<R> *d = nullptr;
+  mfe::Twiddle<R> *d = nullptr;
   dalloc(d, (size_t)n);
   upload(d, h.data(), (size_t)n, s);
   MFE_HIP_TRY(hipStreamSynchronize(s));
@@ -138,7 +146,7 @@ void free_all(mfe_engine *e) {
   dfree(e->c_count); dfree(e->c_start); dfree(e->c_of); dfree(e->c_slot);
   dfree(e->c_sorted); dfree(e->c_pos); dfree(e->nn); dfree(e->nbr); dfree(e->flags);
   dfree(e->rebuild_bar); dfree(e->rebuild_sum);
-  dfree(e->ro_s); dfree(e->type_s);
+  dfree(e->ro_s); dfree(e->type_s); dfree(e->pos_f); dfree(e->t_rec_f); dfree(e->t_zlow_f);
   dfree(e->mesh_r); dfree(e->mesh_c); dfree(e->b_count); dfree(e->b_start); dfree(e->b_done);
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
   dfree(e->spread_tiles);
@@ -204,6 +212,20 @@ void set_cell_geom(mfe_engine *e) {
     g.cinv[m] = n / e->box.L[m];
   }
   g.ncells = g.n[0] * g.n[1] * g.n[2];
+  for (int m = 0; m < 3; ++m) {
+    const int n = g.n[m];
+    const double L = e->box.L[m];
+    g.hf[m] = (double)(float)(L / n);
+    // {-W, -hf, 0, hf, W}: W the offset across the periodic boundary
+    const double W = n > 1 ? L - (n - 1) * g.hf[m] : L;
+    g.off[m][0] = (float)(-W);
+    g.off[m][1] = (float)(-g.hf[m]);
+    g.off[m][2] = 0.0f;
+    g.off[m][3] = (float)g.hf[m];
+    g.off[m][4] = (float)W;
+    g.L[m] = (float)L;
+    g.L2[m] = (float)(0.5 * L);
+  }
   if (g.ncells > e->ncells_alloc) {
     dalloc(e->c_count, (size_t)g.ncells);
     dalloc(e->c_start, (size_t)g.ncells + 1);
@@ -213,11 +235,28 @@ void set_cell_geom(mfe_engine *e) {
   e->cg = g;
 }
 
-int auto_maxnn(const mfe_engine *e) {
+// expected Verlet-list length (partners within rcut + drnlist)
+double mean_partners(const mfe_engine *e) {
   const double rs = e->rcut + e->drnlist;
   const double vol = e->box.L[0] * e->box.L[1] * e->box.L[2];
-  const double mean = 4.0 / 3.0 * kPi * rs * rs * rs * e->np / vol;
-  int m = (int)(2.0 * mean + 64.0);
+  return 4.0 / 3.0 * kPi * rs * rs * rs * e->np / vol;
+}
+
+// lanes per particle of the mixed pair kernel: a lane evaluates batches of
+// four entries, so lists of up to ~12 keep one lane busy, up to ~48 two.
+// Boxes with an axis of fewer than three list cells take four (their
+// kernel variant forms separations in FP64).
+bool pair_pos64_for(const mfe_engine *e) {
+  return e->cg.n[0] < 3 || e->cg.n[1] < 3 || e->cg.n[2] < 3;
+}
+int pair_lanes_for(const mfe_engine *e) {
+  if (pair_pos64_for(e)) return 4;
+  const double m = mean_partners(e);
+  return m <= 12.0 ? 1 : (m <= 48.0 ? 2 : 4);
+}
+
+int auto_maxnn(const mfe_engine *e) {
+  int m = (int)(2.0 * mean_partners(e) + 64.0);
   return std::min(m, std::max(e->np - 1, 1));
 }
 
@@ -262,6 +301,7 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   MFE_SETP(t.zlow, e->t_zlow); MFE_SETP(t.coef, e->t_coef); MFE_SETP(t.coef_f, e->t_coef_f); MFE_SETP(t.off, e->t_off);
   MFE_SETP(t.nint, e->t_nint); MFE_SETP(t.r2atat, e->t_r2atat); MFE_SETP(t.r2umin, e->t_r2umin); MFE_SETP(t.pairidx, e->t_pairidx);
   MFE_SETP(t.bin_k, e->t_bin_k); MFE_SETP(t.rec, e->t_rec);
+  MFE_SETP(t.rec_f, e->t_rec_f); MFE_SETP(t.zlow_f, e->t_zlow_f);
   t.nbin = e->t_nbin;
   t.lds_mode = e->t_lds_mode;
   t.npt = e->npt; t.nptpt = e->nptpt; t.rcut2 = e->rcut * e->rcut; t.nzl = e->t_nzl;
@@ -272,6 +312,7 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   v.cg = e->cg;
   MFE_SETP(v.c_count, e->c_count); MFE_SETP(v.c_start, e->c_start); MFE_SETP(v.c_of, e->c_of); MFE_SETP(v.c_slot, e->c_slot);
   MFE_SETP(v.c_sorted, e->c_sorted); MFE_SETP(v.c_pos, e->c_pos); MFE_SETP(v.ro_s, e->ro_s); MFE_SETP(v.type_s, e->type_s); MFE_SETP(v.nn, e->nn);
+  MFE_SETP(v.pos_f, e->pos_f);
   MFE_SETP(v.nbr, e->nbr); MFE_SETP(v.flags, e->flags);
   v.ewald = e->ewald;
   v.lsurf = (e->lsurf && e->ewald != MFE_EWALD_NONE) ? 1 : 0;
@@ -279,6 +320,14 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   v.efp = e->efp;
   v.self_energy = e->self_energy;
   v.vol = e->box.L[0] * e->box.L[1] * e->box.L[2];
+  v.net_energy = v.net_dq = 0.0;
+  if (e->net_charge && e->ewald != MFE_EWALD_NONE && e->alpha > 0.0) {
+    double qsum = 0.0;
+    for (double qv : e->h_q) qsum += qv;
+    const double a2 = e->alpha * e->alpha;
+    v.net_energy = -kPi * qsum * qsum / (2.0 * v.vol * a2);
+    v.net_dq = -kPi * qsum / (v.vol * a2);
+  }
   if (e->ewald == MFE_EWALD_SPME) {
     for (int m = 0; m < 3; ++m) {
       v.sg.s[m] = e->s[m];
@@ -347,6 +396,8 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   v.off_skin = e->off_skin;
   v.nblk_np = nblocks256(e->np);
   v.nblk_pair = e->nblk_pair;
+  v.pair_lanes = pair_lanes_for(e);
+  v.pair_pos64 = pair_pos64_for(e) ? 1 : 0;
   v.nblk_conv = e->nblk_conv;
   // UTotal's force scaling (sclene/scllen)/sclfor and the surface term's
   // force factor EpsiFourPi*(4 pi/3V) (Src/energy.F90:405-418, 1352-1362)
@@ -383,6 +434,9 @@ int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStre
     g.lds_pair = std::max(g.lds_pair, e->t_lds_bytes);
     g.pair_npt = std::max(g.pair_npt, e->npt);
     if (e->t_lds_mode < 2) g.pair_cf_lds = false;
+    if (e->nknot > 0) g.knots = true;
+    if (pair_pos64_for(e)) g.pair_any64 = true;
+    else g.pair_any32 = true;
     g.cell_blocks = std::max(g.cell_blocks, nblocks256(e->cg.ncells));
     g.kvec = std::max(g.kvec, e->nkvec);
     if (e->ewald == MFE_EWALD_SPME) {
@@ -591,6 +645,13 @@ int mfe_destroy(mfe_engine *e) {
 
 const char *mfe_last_error(const mfe_engine *e) { return e ? e->err.c_str() : "null engine"; }
 
+int mfe_set_net_charge(mfe_engine *e, int32_t on) {
+  if (!e) return MFE_E_BAD_PARAM;
+  e->net_charge = on != 0;
+  touch(e);
+  return MFE_OK;
+}
+
 int mfe_set_sync_limit(mfe_engine *e, int32_t polls) {
   if (!e) return MFE_E_BAD_PARAM;
   e->sync_spin = polls < 0 ? -1 : polls;
@@ -620,7 +681,7 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
   if (!e || !iptpn || !az || !massipt) return MFE_E_BAD_PARAM;
   if (np < 1 || np > e->max_np) return fail(e, MFE_E_BAD_PARAM, "np out of range (0, max_np]");
   if (npt < 1 || npt > MFE_MAX_TYPES) return fail(e, MFE_E_BAD_PARAM, "npt out of range");
-  if (np > (1 << 30)) return fail(e, MFE_E_BAD_PARAM, "np too large");
+  if (np > mfe::kMaxParticles) return fail(e, MFE_E_BAD_PARAM, "np > 2^25");
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
     std::vector<int> t(np);
@@ -641,6 +702,7 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
       dalloc(e->c_of, (size_t)np); dalloc(e->c_slot, (size_t)np);
       dalloc(e->c_sorted, (size_t)np); dalloc(e->c_pos, (size_t)np); dalloc(e->nn, (size_t)np);
       dalloc(e->ro_s, 3 * (size_t)np); dalloc(e->type_s, (size_t)np);
+      if (e->prec == MFE_PREC_MIXED) dalloc(e->pos_f, (size_t)np);
       dalloc(e->b_id, (size_t)np); dalloc(e->b_slot, (size_t)np);
       dalloc(e->rec_meta, (size_t)np);
       dfree(e->rec_frac);
@@ -716,12 +778,32 @@ int mfe_set_tables(mfe_engine *e, const double *ubuf, int32_t nbuf, const int32_
         for (int c = 11; c < mfe::kCoefStride; ++c) cf.push_back(0.0);
       }
     }
-    // FP32 copy for the mixed path; the free twelfth slot of each record
-    // carries the rounding of the interval constant w0, w0 - (float)w0, so
-    // that energy passes add w0 to the FP32 remainder polynomial in FP64
-    std::vector<float> cff(cf.begin(), cf.end());
-    for (size_t r = 0; r + mfe::kCoefStride <= cf.size(); r += mfe::kCoefStride)
-      cff[r + 11] = (float)(cf[r] - (double)cff[r]);
+    // FP32 copy for the mixed path.  Its kernel evaluates the polynomials
+    // in d' = r2 - (float)zlow, so each interval's two polynomials are
+    // re-expanded about the rounded bound (in d = d' - eps, eps = zlow -
+    // (float)zlow; in long double) before rounding: otherwise every pair of
+    // an interval would see the same argument shift, a bias that does not
+    // average out over the sum.  The free twelfth slot carries the rounding
+    // of the interval constant w0, w0 - (float)w0, so that energy passes add
+    // w0 to the FP32 remainder polynomial in FP64.
+    std::vector<float> cff(cf.size());
+    for (size_t k = 0; k < zl.size(); ++k) {
+      const size_t r = k * mfe::kCoefStride;
+      const long double eps = (long double)zl[k] - (long double)(float)zl[k];
+      auto shift = [&](int first, int deg) {
+        long double w[6];
+        for (int j = 0; j <= deg; ++j) w[j] = cf[r + first + j];
+        // synthetic division by (d' - eps) repeated: Taylor coefficients
+        // of p(d' - eps) in d'
+        for (int i = 0; i < deg; ++i)
+          for (int j = deg - 1; j >= i; --j) w[j] -= eps * w[j + 1];
+        for (int j = 0; j <= deg; ++j) cff[r + first + j] = (float)w[j];
+        return (double)w[0];
+      };
+      const double w0 = shift(0, 5);
+      shift(6, 4);
+      cff[r + 11] = (float)(w0 - (double)cff[r]);
+    }
     // bins uniform in s = 1/r2, where the tables' intervals are roughly
     // evenly spaced.  The kernel computes the bin in FP32 from r2 clamped
     // to [lo, inf) and clamps it into [0, nbin); each bin is built over
@@ -776,6 +858,16 @@ int mfe_set_tables(mfe_engine *e, const double *ubuf, int32_t nbuf, const int32_
     dalloc(e->t_bin_k, bin_k.size()); dalloc(e->t_rec, (size_t)P);
     upload(e->t_bin_k, bin_k.data(), bin_k.size(), e->stream);
     upload(e->t_rec, rec.data(), (size_t)P, e->stream);
+    std::vector<mfe::PairRecF> rec_f(P);
+    for (int p = 0; p < P; ++p) {
+      rec_f[p] = mfe::PairRecF{(float)rec[p].r2at, (float)rec[p].r2um, (float)rec[p].lo,
+                               rec[p].s_lo, rec[p].s_inv, rec[p].off, rec[p].nint, 0};
+    }
+    std::vector<float> zl_f(zl.begin(), zl.end());
+    dalloc(e->t_rec_f, (size_t)P);
+    dalloc(e->t_zlow_f, zl_f.size());
+    upload(e->t_rec_f, rec_f.data(), (size_t)P, e->stream);
+    upload(e->t_zlow_f, zl_f.data(), zl_f.size(), e->stream);
     e->t_nbin = nbin;
     // LDS staging of the pair kernel within its budget
     const size_t csz = e->prec == MFE_PREC_MIXED ? 4 : 8;
@@ -1731,7 +1823,7 @@ int mfe_get_pairs(mfe_engine *e, int32_t *nneighpn, int32_t *jpnlist, int32_t ma
       const int ip = sorted[t] + 1;
       auto &row = rows[ip - 1];
       for (int k = 0; k < nn[t]; ++k) {
-        const int jp = sorted[nbr[(size_t)k * np + t]] + 1;
+        const int jp = sorted[(unsigned)nbr[(size_t)k * np + t] & mfe::kNbrIdxMask] + 1;
         const int par = (ip + jp) % 2;
         // checkerboard ownership of SetVListMD (Src/nlist.F90:742-745)
         if (ip < jp && par == 0) continue;

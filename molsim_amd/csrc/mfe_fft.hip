@@ -20,10 +20,6 @@ namespace mfe {
 namespace {
 
 template <typename R>
-__device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b) {
-  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-template <typename R>
 __device__ __forceinline__ cplx<R> cadd(cplx<R> a, cplx<R> b) {
   return {a.x + b.x, a.y + b.y};
 }
@@ -34,6 +30,32 @@ __device__ __forceinline__ cplx<R> csub(cplx<R> a, cplx<R> b) {
 template <typename R>
 __device__ __forceinline__ cplx<R> conjc(cplx<R> a) {
   return {a.x, -a.y};
+}
+
+// v * w (v * conj(w) with cj, the inverse transforms).  FP32 forward: the
+// twiddle's residual enters the products through FMA, so the result is the
+// product with the exact twiddle, rounded, and carries no magnitude bias
+// (the reciprocal energy is a sum of squared forward outputs).  The inverse
+// transforms feed only the forces, whose tolerance is far above that bias.
+__device__ __forceinline__ cplx<float> twmul(cplx<float> v, const Twiddle<float> &w, bool cj) {
+  if (cj) return {v.x * w.x + v.y * w.y, v.y * w.x - v.x * w.y};
+  const float c1 = v.x * w.lx - v.y * w.ly, c2 = v.x * w.ly + v.y * w.lx;
+  return {__builtin_fmaf(v.x, w.x, __builtin_fmaf(-v.y, w.y, c1)),
+          __builtin_fmaf(v.x, w.y, __builtin_fmaf(v.y, w.x, c2))};
+}
+__device__ __forceinline__ cplx<double> twmul(cplx<double> v, const Twiddle<double> &w,
+                                              bool cj) {
+  const double wy = cj ? -w.y : w.y;
+  return {v.x * w.x - v.y * wy, v.x * wy + v.y * w.x};
+}
+
+// x * sqrt(1/2): FP32 forward with the constant's rounding residual added
+// by FMA (RN(sqrt(1/2)) is below sqrt(1/2))
+__device__ __forceinline__ float mul_sqrt_half(float x, bool inv) {
+  return inv ? x * 0.70710677f : __builtin_fmaf(x, 0.70710677f, x * 1.2101617e-08f);
+}
+__device__ __forceinline__ double mul_sqrt_half(double x, bool) {
+  return x * 0.70710678118654752440;
 }
 
 constexpr int kMaxRadix = 32;
@@ -61,8 +83,8 @@ __device__ __forceinline__ void batched_copy(int n, Load load, Store store) {
 
 // a twiddle table of n entries copied to LDS (read by every stage)
 template <typename R>
-__device__ __forceinline__ void stage_twiddles(cplx<R> *dst, const cplx<R> *__restrict__ src,
-                                               int n) {
+__device__ __forceinline__ void stage_twiddles(Twiddle<R> *dst,
+                                               const Twiddle<R> *__restrict__ src, int n) {
   for (int k = threadIdx.x; k < n; k += blockDim.x) dst[k] = src[k];
 }
 
@@ -70,7 +92,7 @@ __device__ __forceinline__ void stage_twiddles(cplx<R> *dst, const cplx<R> *__re
 // w = exp(-+2 pi i / RAD) taken from the length-n table tw (RAD divides n).
 template <typename R, int RAD>
 __device__ __forceinline__ void small_dft(cplx<R> *v, bool inv,
-                                          const cplx<R> *tw, int n) {
+                                          const Twiddle<R> *tw, int n) {
   if constexpr (RAD == 2) {
     cplx<R> a = v[0], b = v[1];
     v[0] = cadd(a, b);
@@ -87,15 +109,16 @@ __device__ __forceinline__ void small_dft(cplx<R> *v, bool inv,
   } else if constexpr (RAD == 8) {
     // radix 2 x 4: two 4-point DFTs of the even and odd samples, then the
     // odd half twiddled by w8^q (w8 = e^{-+i pi/4})
-    const R h = R(0.70710678118654752440);
     cplx<R> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
     small_dft<R, 4>(e, inv, tw, n);
     small_dft<R, 4>(o, inv, tw, n);
     // o[1] *= w8, o[2] *= w8^2 = -+i, o[3] *= w8^3
     const R sg = inv ? R(1) : R(-1);
-    cplx<R> o1 = {h * (o[1].x - sg * o[1].y), h * (o[1].y + sg * o[1].x)};
+    cplx<R> o1 = {mul_sqrt_half(o[1].x - sg * o[1].y, inv),
+                  mul_sqrt_half(o[1].y + sg * o[1].x, inv)};
     cplx<R> o2 = {-sg * o[2].y, sg * o[2].x};
-    cplx<R> o3 = {h * (-o[3].x - sg * o[3].y), h * (-o[3].y + sg * o[3].x)};
+    cplx<R> o3 = {mul_sqrt_half(-o[3].x - sg * o[3].y, inv),
+                  mul_sqrt_half(-o[3].y + sg * o[3].x, inv)};
     v[0] = cadd(e[0], o[0]);
     v[4] = csub(e[0], o[0]);
     v[1] = cadd(e[1], o1);
@@ -111,11 +134,7 @@ __device__ __forceinline__ void small_dft(cplx<R> *v, bool inv,
     for (int qq = 0; qq < RAD; ++qq) {
       cplx<R> acc = v[0];
 #pragma unroll
-      for (int j = 1; j < RAD; ++j) {
-        cplx<R> w = tw[((j * qq) % RAD) * step];
-        if (inv) w.y = -w.y;
-        acc = cadd(acc, cmul(v[j], w));
-      }
+      for (int j = 1; j < RAD; ++j) acc = cadd(acc, twmul(v[j], tw[((j * qq) % RAD) * step], inv));
       out[qq] = acc;
     }
 #pragma unroll
@@ -132,7 +151,7 @@ __device__ __forceinline__ void small_dft(cplx<R> *v, bool inv,
 template <typename R, int RAD>
 __device__ __forceinline__ void stockham_stage(const cplx<R> *a, cplx<R> *b,
                                                int n, int Ns, int batch,
-                                               int ld, const cplx<R> *tw,
+                                               int ld, const Twiddle<R> *tw,
                                                bool inv, int lg_nbf, int lg_ns) {
   const int nbf = n / RAD;
   const int total = nbf * batch;
@@ -156,11 +175,7 @@ __device__ __forceinline__ void stockham_stage(const cplx<R> *a, cplx<R> *b,
 #pragma unroll
     for (int q = 0; q < RAD; ++q) v[q] = in[j + q * nbf];
 #pragma unroll
-    for (int q = 1; q < RAD; ++q) {
-      cplx<R> w = tw[k * q * tstep];
-      if (inv) w.y = -w.y;
-      v[q] = cmul(v[q], w);
-    }
+    for (int q = 1; q < RAD; ++q) v[q] = twmul(v[q], tw[k * q * tstep], inv);
     small_dft<R, RAD>(v, inv, tw, n);
     const int base = jb * Ns * RAD + k;
 #pragma unroll
@@ -172,7 +187,7 @@ __device__ __forceinline__ void stockham_stage(const cplx<R> *a, cplx<R> *b,
 template <typename R>
 __device__ __forceinline__ void stockham_stage_generic(const cplx<R> *a, cplx<R> *b, int n,
                                        int Ns, int rad, int batch, int ld,
-                                       const cplx<R> *tw, bool inv) {
+                                       const Twiddle<R> *tw, bool inv) {
   const int nbf = n / rad;
   const int total = nbf * batch;
   const int tstep = n / (Ns * rad);
@@ -186,10 +201,8 @@ __device__ __forceinline__ void stockham_stage_generic(const cplx<R> *a, cplx<R>
     for (int qq = 0; qq < rad; ++qq) {
       cplx<R> acc = {R(0), R(0)};
       for (int q = 0; q < rad; ++q) {
-        cplx<R> x = in[j + q * nbf];
-        cplx<R> w = tw[((k * q * tstep) + ((q * qq) % rad) * step) % n];
-        if (inv) w.y = -w.y;
-        acc = cadd(acc, cmul(x, w));
+        acc = cadd(acc, twmul(in[j + q * nbf], tw[((k * q * tstep) + ((q * qq) % rad) * step) % n],
+                              inv));
       }
       out[base + qq * Ns] = acc;
     }
@@ -201,7 +214,7 @@ __device__ __forceinline__ void stockham_stage_generic(const cplx<R> *a, cplx<R>
 // the result in natural order.  All threads of the block take part.
 template <typename R>
 __device__ __forceinline__ cplx<R> *lds_fft(cplx<R> *a, cplx<R> *b, const FftPlan &p,
-                            int batch, int ld, const cplx<R> *tw, bool inv) {
+                            int batch, int ld, const Twiddle<R> *tw, bool inv) {
   const int n = p.n;
   int Ns = 1;
   const bool pow2 = (n & (n - 1)) == 0;
@@ -274,8 +287,8 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
   cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
   const int s1 = V.sg.s[0], nrows = V.sg.s[1] * V.sg.s[2], rows_per_blk = V.fx_rpb;
   const FftPlan &ph = V.plan_half;
-  const cplx<R> *__restrict__ tw_half = (const cplx<R> *)V.tw_half;
-  const cplx<R> *__restrict__ tw_full = (const cplx<R> *)V.tw[0];
+  const Twiddle<R> *__restrict__ tw_half = (const Twiddle<R> *)V.tw_half;
+  const Twiddle<R> *__restrict__ tw_full = (const Twiddle<R> *)V.tw[0];
   extern __shared__ char smem[];
   const int m = s1 / 2, hx = m + 1, ld = m + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
@@ -285,7 +298,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
   R *ar = reinterpret_cast<R *>(a);
   const R *__restrict__ tiles = (const R *)V.spread_tiles;
   const int s2 = V.sg.s[1];
-  cplx<R> *twh_s = b + rows_per_blk * ld, *twf_s = twh_s + m;
+  Twiddle<R> *twh_s = reinterpret_cast<Twiddle<R> *>(b + rows_per_blk * ld), *twf_s = twh_s + m;
   stage_twiddles(twh_s, tw_half, m);
   stage_twiddles(twf_s, tw_full, s1);
   batched_copy(
@@ -309,7 +322,7 @@ __global__ void __launch_bounds__(256) k_fft_x_fwd(const SysDev *__restrict__ sy
     cplx<R> ev = {(zk.x + zm.x) * R(0.5), (zk.y + zm.y) * R(0.5)};
     cplx<R> d = csub(zk, zm);
     cplx<R> od = {d.y * R(0.5), -d.x * R(0.5)};
-    cplx<R> X = cadd(ev, cmul(twf_s[k], od));
+    cplx<R> X = cadd(ev, twmul(od, twf_s[k], false));
     cm[(size_t)(row0 + r) * hx + k] = X;
   }
 }
@@ -327,15 +340,15 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
   R *__restrict__ real = (R *)V.mesh_r;
   const int s1 = V.sg.s[0], nrows = V.sg.s[1] * V.sg.s[2], rows_per_blk = V.fx_rpb;
   const FftPlan &ph = V.plan_half;
-  const cplx<R> *__restrict__ tw_half = (const cplx<R> *)V.tw_half;
-  const cplx<R> *__restrict__ tw_full = (const cplx<R> *)V.tw[0];
+  const Twiddle<R> *__restrict__ tw_half = (const Twiddle<R> *)V.tw_half;
+  const Twiddle<R> *__restrict__ tw_full = (const Twiddle<R> *)V.tw[0];
   extern __shared__ char smem[];
   const int m = s1 / 2, hx = m + 1, ld = m + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
   cplx<R> *b = a + rows_per_blk * ld;
   const int row0 = blk * rows_per_blk;
   const int nr = min(rows_per_blk, nrows - row0);
-  cplx<R> *twh_s = b + rows_per_blk * ld, *twf_s = twh_s + m;
+  Twiddle<R> *twh_s = reinterpret_cast<Twiddle<R> *>(b + rows_per_blk * ld), *twf_s = twh_s + m;
   stage_twiddles(twh_s, tw_half, m);
   stage_twiddles(twf_s, tw_full, s1);
   batched_copy(
@@ -351,7 +364,7 @@ __global__ void __launch_bounds__(256) k_fft_x_inv(const SysDev *__restrict__ sy
     cplx<R> xk = b[r * ld + k];
     cplx<R> xm = conjc(b[r * ld + (m - k)]);
     cplx<R> A = cadd(xk, xm), D = csub(xk, xm);
-    cplx<R> Dw = cmul(D, conjc(twf_s[k]));
+    cplx<R> Dw = twmul(D, twf_s[k], true);
     a[r * ld + k] = {A.x - Dw.y, A.y + Dw.x};
   }
   __syncthreads();
@@ -375,7 +388,7 @@ __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, i
   cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
   const int s2 = V.sg.s[1], hx = V.hx, T = V.fy_T;
   const FftPlan &py = V.plan[1];
-  const cplx<R> *__restrict__ tw = (const cplx<R> *)V.tw[1];
+  const Twiddle<R> *__restrict__ tw = (const Twiddle<R> *)V.tw[1];
   extern __shared__ char smem[];
   const int ld = s2 + 1;
   cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
@@ -383,7 +396,7 @@ __global__ void __launch_bounds__(256) k_fft_y(const SysDev *__restrict__ sys, i
   const int kx0 = bx * T;
   const int nt = min(T, hx - kx0);
   cplx<R> *base = cm + (size_t)z * s2 * hx + kx0;
-  cplx<R> *tw_s = b + T * ld;
+  Twiddle<R> *tw_s = reinterpret_cast<Twiddle<R> *>(b + T * ld);
   stage_twiddles(tw_s, tw, s2);
   // elements e = y T + c over the full tile width T; columns c >= nt (the
   // last tile) are skipped
@@ -422,7 +435,7 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   cplx<R> *__restrict__ cm = (cplx<R> *)V.mesh_c;
   const int s1 = V.sg.s[0], s2 = V.sg.s[1], s3 = V.sg.s[2], hx = V.hx, T = V.fz_T;
   const FftPlan &pz = V.plan[2];
-  const cplx<R> *__restrict__ tw = (const cplx<R> *)V.tw[2];
+  const Twiddle<R> *__restrict__ tw = (const Twiddle<R> *)V.tw[2];
   const SpmeAxes &ax = V.ax;
   double *__restrict__ part = V.part + V.off_conv;
   extern __shared__ char smem[];
@@ -434,7 +447,7 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   const int nt = min(T, hx - kx0);
   const size_t zstride = (size_t)s2 * hx;
   cplx<R> *base = cm + (size_t)y * hx + kx0;
-  cplx<R> *tw_s = b + T * ld;
+  Twiddle<R> *tw_s = reinterpret_cast<Twiddle<R> *>(b + T * ld);
   stage_twiddles(tw_s, tw, s3);
   batched_copy(
       T * s3,
@@ -525,20 +538,21 @@ void fft_geometry(int prec, const int s[3], int *fx_rpb, int *fx_nblk, int *fy_T
                   int *fz_T, int *fz_nbx, size_t *lds_fx, size_t *lds_fy, size_t *lds_fz) {
   const bool f = prec == MFE_PREC_MIXED;
   const size_t csz = f ? sizeof(cplx<float>) : sizeof(cplx<double>);
+  const size_t tsz = f ? sizeof(Twiddle<float>) : sizeof(Twiddle<double>);
   const int hx = s[0] / 2 + 1;
   const int rpb = f ? rows_per_block<float>(s[0]) : rows_per_block<double>(s[0]);
   *fx_rpb = rpb;
   *fx_nblk = (s[1] * s[2] + rpb - 1) / rpb;
   // two row buffers, then the half- and full-length twiddles
-  *lds_fx = (2 * (size_t)rpb * (s[0] / 2 + 1) + s[0] / 2 + s[0]) * csz;
+  *lds_fx = 2 * (size_t)rpb * (s[0] / 2 + 1) * csz + (size_t)(s[0] / 2 + s[0]) * tsz;
   const int Ty = tile_width(hx, f ? tile_target<float>(s[1]) : tile_target<double>(s[1]));
   *fy_T = Ty;
   *fy_nbx = (hx + Ty - 1) / Ty;
-  *lds_fy = (2 * (size_t)Ty * (s[1] + 1) + s[1]) * csz;
+  *lds_fy = 2 * (size_t)Ty * (s[1] + 1) * csz + (size_t)s[1] * tsz;
   const int Tz = tile_width(hx, f ? tile_target<float>(s[2]) : tile_target<double>(s[2]));
   *fz_T = Tz;
   *fz_nbx = (hx + Tz - 1) / Tz;
-  *lds_fz = (2 * (size_t)Tz * (s[2] + 1) + s[2]) * csz;
+  *lds_fz = 2 * (size_t)Tz * (s[2] + 1) * csz + (size_t)s[2] * tsz;
 }
 
 void fft_x_forward(const Group &g) {

@@ -112,7 +112,7 @@ __global__ void __launch_bounds__(kFmBlock) k_fm_pair(const SysDev *__restrict__
     }
     const int n = V.nn[tt];
     for (int k = lane; k < n; k += kFmLanes) {
-      const int pj = V.nbr[(size_t)k * np + tt];
+      const int pj = (int)((unsigned)V.nbr[(size_t)k * np + tt] & kNbrIdxMask);
       double dx = xi - V.ro_s[3 * pj], dy = yi - V.ro_s[3 * pj + 1], dz = zi - V.ro_s[3 * pj + 2];
       const double r2d = pbcr2(dx, dy, dz, b);
       if (V.nlsem == 0 ? (r2d > t.rcut2) : !(r2d < t.rcut2)) continue;

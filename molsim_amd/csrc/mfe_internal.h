@@ -62,17 +62,48 @@ struct cplx {
   R x, y;
 };
 
+// A twiddle factor.  FP64: the value.  FP32: its rounded value and the
+// rounding residual (w - (float)w), which the products add back by FMA:
+// rounded twiddles alone have |w| < 1 on average for power-of-two sizes
+// (cos(pi/4) rounds down, for one), and every transform would shrink its
+// output by ~2e-8 per axis -- a bias the reciprocal energy would keep.
+template <typename R>
+struct Twiddle;
+template <>
+struct Twiddle<double> {
+  double x, y;
+};
+template <>
+struct Twiddle<float> {
+  float x, y, lx, ly;
+};
+
 // Orthorhombic periodic box (Src/molaux.F90:326-335); dpbc == boxlen.
 struct Box {
   double L[3], L2[3], Li[3];
 };
 
 // Cell grid used to find Verlet-list partners: edge >= rcut + drnlist.
+// The mixed path's pair kernel reads positions relative to a reference
+// point of the particle's cell, o(c) = -L/2 + (c + 1/2) hf (hf the cell
+// edge rounded to FP32, so that neighbouring cells differ by exactly hf),
+// in FP32; each list entry carries the cell offset of its pair, one of
+// {-W, -hf, 0, hf, W} per axis, W = L - (n-1) hf for pairs across the
+// periodic boundary (off[axis][code + 2]).  Axes of one cell carry code 0
+// and take the minimum image in FP32.
 struct CellGeom {
   int n[3];          // cells per axis (1 when the axis has < 3 cells)
   double cinv[3];    // cells per unit length
   int ncells;
+  double hf[3];      // cell edge, an FP32 value
+  float off[3][5];   // pair offsets per axis and code
+  float L[3], L2[3]; // box edge and half edge in FP32
 };
+// list entries: cell-order partner position in the low 25 bits, the pair's
+// offset code (cx+2) + 5 (cy+2) + 25 (cz+2) in the high 7 bits
+constexpr unsigned kNbrIdxMask = (1u << 25) - 1;
+constexpr int kNbrCodeShift = 25;
+constexpr int kMaxParticles = 1 << 25;
 
 // Table intervals in device layout: for pair p, intervals k = 0..nint[p]-1
 // start at index off[p]; zlow[off+k] (FP64, decreasing in k) and the 11
@@ -83,6 +114,12 @@ struct alignas(16) PairRec {
   double lo, pad;     // lowest interval bound zlow(nint - 1)
   float s_lo, s_inv;  // bin of r2: (1/r2 - s_lo) * s_inv
   int off, nint;      // the pair's intervals in zlow / coef
+};
+// the same in FP32 for the mixed path's pair kernel
+struct alignas(16) PairRecF {
+  float r2at, r2um, lo, s_lo;
+  float s_inv;
+  int off, nint, pad;
 };
 constexpr int kSlowBin = 0x8000;
 // coefficients stored per interval (11 used, padded for 16-byte loads)
@@ -102,6 +139,8 @@ struct TableDev {
   // spans more intervals and the search walks from kb
   MFE_DPTR(const unsigned short) bin_k;  // [nptpt * nbin]
   MFE_DPTR(const PairRec) rec;           // [nptpt]
+  MFE_DPTR(const PairRecF) rec_f;        // [nptpt]
+  MFE_DPTR(const float) zlow_f;          // zlow in FP32
   int nbin;
   // what the pair kernel stages in LDS: 0 = the bins, 1 = also the
   // interval bounds, 2 = also the polynomial coefficients
@@ -218,12 +257,19 @@ struct SysDev {
   MFE_DPTR(int) c_pos;          // inverse of c_sorted: particle -> cell-order position
   MFE_DPTR(double) ro_s;
   MFE_DPTR(int) type_s;
+  // mixed path: (x, y, z) relative to the cell's reference point in FP32
+  // and the type in the fourth word, in cell order (null on FP64 engines)
+  MFE_DPTR(float4) pos_f;
   MFE_DPTR(int) nn;
   MFE_DPTR(int) nbr;
   MFE_DPTR(int) flags;        // [0] error bits, [1] rebuild flag, [2] rebuilds, [3] BD step
   // Ewald
   int ewald, lsurf;
   double alpha, efp, self_energy, vol;
+  // neutralising-background term of a net charge Q (opt-in, 0 when off):
+  // -pi Q^2 / (2 V alpha^2), its virial -3 times that, and dE/dq_i =
+  // -pi Q / (V alpha^2) (all before the EpsiFourPi factor)
+  double net_energy, net_dq;
   // SPME
   SpmeGeom sg;
   int hx, nbricks;
@@ -292,6 +338,9 @@ struct SysDev {
   MFE_DPTR(double) en;
   int off_pair, off_conv, off_bond, off_surf, off_std, off_move, off_skin;
   int nblk_np, nblk_pair, nblk_conv;
+  // mixed pair kernel: lanes per particle (1, 2, 4) and whether the box
+  // has an axis of fewer than three list cells (separations in FP64)
+  int pair_lanes, pair_pos64;
   double ffac, surf_fac;
 };
 
@@ -316,6 +365,10 @@ struct Group {
   int ncu = 0;
   int sync_spin = -1;       // polls before a barrier wait gives up (-1: default)
   bool pair_cf_lds = true;  // every replica's coefficients fit in LDS
+  bool knots = false;       // some replica carries the spline pair term
+  // replicas whose mixed pair kernel forms separations from cell-relative
+  // FP32 coordinates / in FP64 (boxes with an axis of < 3 list cells)
+  bool pair_any32 = false, pair_any64 = false;
 };
 
 }  // namespace mfe
@@ -335,6 +388,7 @@ struct mfe_engine {
   bool have_ewald = false, have_neighbor = false, have_positions = false;
   bool list_valid = false;
   double self_energy = 0.0;  // -alpha/sqrt(pi) * sum q^2 (UEwaldSelf)
+  bool net_charge = false;   // neutralising-background term on (opt-in)
 
   // particle state (device, xyz-interleaved FP64)
   double *ro = nullptr, *rod = nullptr, *rodd = nullptr;
@@ -359,6 +413,8 @@ struct mfe_engine {
   double *t_r2atat = nullptr, *t_r2umin = nullptr;
   unsigned short *t_bin_k = nullptr;
   mfe::PairRec *t_rec = nullptr;
+  mfe::PairRecF *t_rec_f = nullptr;
+  float *t_zlow_f = nullptr;
   int t_nbin = 0;
   int t_lds_mode = 0;
   size_t t_lds_bytes = 0;
@@ -375,6 +431,7 @@ struct mfe_engine {
   int *c_pos = nullptr;      // particle index -> cell-order position
   double *ro_s = nullptr;    // positions in cell order (refreshed per UTotal)
   int *type_s = nullptr;
+  float4 *pos_f = nullptr;   // mixed path: cell-relative FP32 positions and types
   int *nn = nullptr, *nbr = nullptr;  // by cell-order position
   int *flags = nullptr;      // [0] error bits, [1] rebuild flag, [2] rebuild count
   int ncells_alloc = 0;

@@ -148,6 +148,25 @@ __device__ __forceinline__ int cell_of(const double *r, const CellGeom &g,
   return c[0] + g.n[0] * (c[1] + g.n[1] * c[2]);
 }
 
+// The mixed path's FP32 position of a particle at x (FP64) in cell c,
+// relative to the cell's reference point -L/2 + (c + 1/2) hf per axis and
+// taken into [-L/2, L/2] (the particle may have crossed the box edge since
+// the list was built), with its type in the fourth word.
+__device__ __forceinline__ float4 rel_pos(const SysDev &V, const double *x, int c, int type) {
+  const CellGeom &g = V.cg;
+  const int cc[3] = {c % g.n[0], (c / g.n[0]) % g.n[1], c / (g.n[0] * g.n[1])};
+  float r[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double o = -V.box.L2[m] + (cc[m] + 0.5) * g.hf[m];
+    double d = x[m] - o;
+    if (d > V.box.L2[m]) d -= V.box.L[m];
+    else if (d < -V.box.L2[m]) d += V.box.L[m];
+    r[m] = (float)d;
+  }
+  return make_float4(r[0], r[1], r[2], __int_as_float(type));
+}
+
 // The rebuild's phases, each over threads t0, t0 + nt, ...: run as five
 // kernels (batches, explicit builds) or as the phases of k_rebuild.
 __device__ __forceinline__ void ph_cell_count(const SysDev &V, int t0, int nt) {
@@ -194,6 +213,10 @@ __device__ __forceinline__ void ph_sort_pos(const SysDev &V, int t0, int nt) {
     V.ro_s[3 * t + 1] = y;
     V.ro_s[3 * t + 2] = z;
     V.type_s[t] = ty;
+    if (V.pos_f) {
+      const double r[3] = {x, y, z};
+      V.pos_f[t] = rel_pos(V, r, V.c_of[i], ty);
+    }
   }
 }
 
@@ -232,10 +255,17 @@ __device__ __forceinline__ void ph_nlist_build(const SysDev &V, int t0, int nt) 
     int n = 0;
     for (int dz = -rz; dz <= rz; ++dz) {
       const int z = (cz + dz + g.n[2]) % g.n[2];
+      // the pair's offset code per axis (see CellGeom): -d inside the box,
+      // +-2 across its edge, 0 on an axis of one cell
+      const int kz = rz ? (cz + dz < 0 ? 2 : (cz + dz >= g.n[2] ? -2 : -dz)) : 0;
       for (int dy = -ry; dy <= ry; ++dy) {
         const int y = (cy + dy + g.n[1]) % g.n[1];
+        const int ky = ry ? (cy + dy < 0 ? 2 : (cy + dy >= g.n[1] ? -2 : -dy)) : 0;
         for (int dx = -rx; dx <= rx; ++dx) {
           const int x = (cx + dx + g.n[0]) % g.n[0];
+          const int kx = rx ? (cx + dx < 0 ? 2 : (cx + dx >= g.n[0] ? -2 : -dx)) : 0;
+          const unsigned code = (unsigned)((kx + 2) + 5 * (ky + 2) + 25 * (kz + 2))
+                                << kNbrCodeShift;
           const int cc = x + g.n[0] * (y + g.n[1] * z);
           for (int p = start[cc]; p < start[cc + 1]; ++p) {
             if (p == t) continue;
@@ -253,7 +283,7 @@ __device__ __forceinline__ void ph_nlist_build(const SysDev &V, int t0, int nt) 
             double ddx = xi - xj, ddy = yi - yj, ddz = zi - zj;
             const double r2 = pbcr2(ddx, ddy, ddz, b);
             if (r2 < rs2) {
-              if (n < maxnn) nbr[(size_t)n * np + t] = p;
+              if (n < maxnn) nbr[(size_t)n * np + t] = (int)((unsigned)p | code);
               ++n;
             }
           }
@@ -271,7 +301,12 @@ __device__ __forceinline__ void ph_nlist_build(const SysDev &V, int t0, int nt) 
       V.ro_s[3 * t] = xi;
       V.ro_s[3 * t + 1] = yi;
       V.ro_s[3 * t + 2] = zi;
-      V.type_s[t] = V.type[i];
+      const int ty = V.type[i];
+      V.type_s[t] = ty;
+      if (V.pos_f) {
+        const double r[3] = {xi, yi, zi};
+        V.pos_f[t] = rel_pos(V, r, c, ty);
+      }
     }
     if (drosum) {
       drosum[3 * i] = 0.0;
@@ -473,10 +508,10 @@ __device__ __forceinline__ void skin_two_blocks(const SysDev &V, int sb0) {
 // fixed order, and per block into the partials.
 // With skin (MD/BD steps) blocks from pgrid on take the skin test's block
 // candidates (two k_move blocks each) instead of a separate launch.
-template <bool MIXED, int NT, bool CF>
+template <int NT, bool CF>
 __global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(4)))
 k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
-  using CT = typename std::conditional<MIXED, float, double>::type;
+  using CT = double;
   const SysDev &V = sys[blockIdx.z];
   if (skin && (int)blockIdx.x >= pgrid) {
     skin_two_blocks(V, 2 * ((int)blockIdx.x - pgrid));
@@ -508,10 +543,7 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
   if (lds_zl)
     for (int k = threadIdx.x; k < nzl; k += blockDim.x) zl_s[k] = t.zlow[k];
   if (CF) {
-    for (int k = threadIdx.x; k < nzl * kCoefStride; k += blockDim.x) {
-      if constexpr (MIXED) cf_s[k] = t.coef_f[k];
-      else cf_s[k] = t.coef[k];
-    }
+    for (int k = threadIdx.x; k < nzl * kCoefStride; k += blockDim.x) cf_s[k] = t.coef[k];
   }
   for (int k = threadIdx.x; k < P; k += blockDim.x) rec_s[k] = t.rec[k];
   for (int k = threadIdx.x; k < npt * npt; k += blockDim.x) pidx_s[k] = t.pairidx[k];
@@ -520,26 +552,13 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
   __syncthreads();
   // the 12 stored coefficients of interval iv (16-byte loads)
   auto coefs = [&](int iv, CT *c) {
-    if constexpr (MIXED) {
-      const float4 *q = CF ? reinterpret_cast<const float4 *>(cf_s) + 3 * iv
-                           : reinterpret_cast<const float4 *>(t.coef_f) + 3 * iv;
+    const double2 *q = CF ? reinterpret_cast<const double2 *>(cf_s) + 6 * iv
+                          : reinterpret_cast<const double2 *>(t.coef) + 6 * iv;
 #pragma unroll
-      for (int h = 0; h < 3; ++h) {
-        const float4 v = q[h];
-        c[4 * h] = v.x;
-        c[4 * h + 1] = v.y;
-        c[4 * h + 2] = v.z;
-        c[4 * h + 3] = v.w;
-      }
-    } else {
-      const double2 *q = CF ? reinterpret_cast<const double2 *>(cf_s) + 6 * iv
-                            : reinterpret_cast<const double2 *>(t.coef) + 6 * iv;
-#pragma unroll
-      for (int h = 0; h < 6; ++h) {
-        const double2 v = q[h];
-        c[2 * h] = v.x;
-        c[2 * h + 1] = v.y;
-      }
+    for (int h = 0; h < 6; ++h) {
+      const double2 v = q[h];
+      c[2 * h] = v.x;
+      c[2 * h + 1] = v.y;
     }
   };
   auto zlow = [&](int idx) -> double {
@@ -556,7 +575,6 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
   for (int k = 0; k < NA; ++k) ej[k] = 0.0;
   double vir = 0.0;
   double fx, fy, fz;
-  KahanF fxf, fyf, fzf;
   int ti = 0;
   // pair index of this particle with partner type u in bits 8u..8u+7
   unsigned long long ptp = 0;
@@ -595,25 +613,8 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
     if (in) err |= (ov ? ERR_OVERLAP : 0) | (te ? ERR_TABLE : 0);
     if (!in || ov || te) return;
     const double d = r2 - zv;
-    double usum, fsum;
-    if (MIXED) {
-      const float df = (float)d;
-      const float fs = c[6] + df * (c[7] + df * (c[8] + df * (c[9] + df * c[10])));
-      fsum = fs;
-      if (energy) {
-        // the interval constant w0 in FP64 (its FP32 value plus the stored
-        // rounding in slot 11), the remainder polynomial in FP32: the
-        // constant's rounding is the same for every pair of the interval
-        // and would not average out over the sum
-        const float up = df * (c[1] + df * (c[2] + df * (c[3] + df * (c[4] + df * c[5]))));
-        usum = ((double)c[0] + (double)c[11]) + (double)up;
-      } else {
-        usum = 0.0;
-      }
-    } else {
-      fsum = c[6] + d * (c[7] + d * (c[8] + d * (c[9] + d * c[10])));
-      usum = c[0] + d * (c[1] + d * (c[2] + d * (c[3] + d * (c[4] + d * c[5]))));
-    }
+    double fsum = c[6] + d * (c[7] + d * (c[8] + d * (c[9] + d * c[10])));
+    double usum = c[0] + d * (c[1] + d * (c[2] + d * (c[3] + d * (c[4] + d * c[5]))));
     if (hc) {
       usum = 1e10;
       fsum = 1e10;
@@ -631,16 +632,9 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
         fsum -= dup / r;
       }
     }
-    if (MIXED) {
-      const float fs = (float)fsum;
-      fxf.add(fs * (float)dx);
-      fyf.add(fs * (float)dy);
-      fzf.add(fs * (float)dz);
-    } else {
-      fx += fsum * dx;
-      fy += fsum * dy;
-      fz += fsum * dz;
-    }
+    fx += fsum * dx;
+    fy += fsum * dy;
+    fz += fsum * dz;
     if (energy) {
 #pragma unroll
       for (int k2 = 0; k2 < NA; ++k2) ej[k2] += (k2 == (kRegE ? p : tj)) ? 0.5 * usum : 0.0;
@@ -658,9 +652,6 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
       for (int k = 0; k < NA; ++k) ej[k] = 0.0;
     }
     fx = fy = fz = 0.0;
-    fxf = KahanF();
-    fyf = KahanF();
-    fzf = KahanF();
     ti = 0;
     if (i < np) {
       const double xi = ro[3 * i], yi = ro[3 * i + 1], zi = ro[3 * i + 2];
@@ -678,7 +669,7 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
 #pragma unroll
         for (int g = 0; g < kPairBatch; ++g) {
           const int kk = k0 + g * kPairLanes;
-          const int jr = nbr[(size_t)max(0, min(kk, last)) * np + i];
+          const int jr = (int)((unsigned)nbr[(size_t)max(0, min(kk, last)) * np + i] & kNbrIdxMask);
           js[g] = kk < n ? jr : i;
         }
         double xs[kPairBatch], ys[kPairBatch], zs[kPairBatch];
@@ -693,11 +684,6 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
 #pragma unroll
         for (int g = 0; g < kPairBatch; ++g)
           if (k0 + g * kPairLanes < n) eval(xi - xs[g], yi - ys[g], zi - zs[g], ts[g]);
-      }
-      if (MIXED) {
-        fx = fxf.value();
-        fy = fyf.value();
-        fz = fzf.value();
       }
     }
     // the four lanes' partial forces: exchanges within the quad by DPP
@@ -743,6 +729,273 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
     }
     // the virial, then per-block partials, component-major (component k
     // of block b at k * nblk_pair + b), the wavefronts summed in order
+    double v = vir;
+    for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
+    if (l64 == 0) wacc[wv][P] = v;
+    __syncthreads();
+    double *out = V.part + V.off_pair + blk;
+    for (int k = threadIdx.x; k <= P; k += blockDim.x) {
+      double s2 = wacc[0][k];
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) s2 += wacc[w][k];
+      out[(size_t)k * nb] = s2;
+    }
+  }
+}
+
+// The mixed path's pair forces: the list walk, partition and reductions of
+// k_pair, in FP32.  Positions come from pos_f (relative to the particle's
+// cell reference point, 16 bytes with the type); a list entry's code names
+// the pair's cell offset, kept in LDS per code, so dx = x_i - x_j + offset
+// is formed in FP32 from values of at most a cell edge or so.  The interval
+// is chosen on FP32 bounds and the polynomials are evaluated in
+// d = r2 - (float)zlow; the FP32 coefficients were re-expanded about the
+// rounded bound, so that rounding shifts no interval's argument.  The
+// interval constant w0 enters in FP64 (its FP32 value plus the rounding
+// kept in slot 11); forces are Kahan-compensated FP32 sums, energies FP64
+// per partner type, flushed into the wavefront's LDS row per chunk.  KN:
+// some replica carries the additive spline term.  V.pair_lanes lanes share
+// a particle's list: 4 for dense systems, fewer where lists are short (a
+// lane then keeps its batch slots busy and the per-particle work is not
+// repeated on idle lanes); a replica's choice is its own, so it computes
+// the same alone and in a batch.  POS64: the replicas whose box has an
+// axis of fewer than three list cells, where cell-relative coordinates
+// would span half the box -- the pair's separation is formed in FP64 from
+// the cell-ordered positions and rounded once; the other replicas' blocks
+// return (a group launches both variants when it holds both kinds).
+template <int NT, bool CF, bool KN, bool POS64>
+__global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(KN || POS64 ? 4 : 5)))
+k_pair_f(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
+  const SysDev &V = sys[blockIdx.z];
+  if (skin && (int)blockIdx.x >= pgrid) {
+    skin_two_blocks(V, 2 * ((int)blockIdx.x - pgrid));
+    return;
+  }
+  const int blk = xcd_block(blockIdx.x, pgrid);
+  const int nb = V.nblk_pair;
+  if (blk >= nb || V.pair_pos64 != (POS64 ? 1 : 0)) return;
+  const float4 *__restrict__ pos = V.pos_f;
+  const int np = V.np;
+  const int *__restrict__ nn = V.nn;
+  const unsigned *__restrict__ nbr = reinterpret_cast<const unsigned *>((const int *)V.nbr);
+  const TableDev t = V.tab;
+  const int semantics = V.nlsem;
+  extern __shared__ __align__(16) unsigned char pair_lds[];
+  __shared__ PairRecF rec_s[MFE_MAX_PAIRS];
+  __shared__ int pidx_s[MFE_MAX_TYPES * MFE_MAX_TYPES];
+  __shared__ float4 off_s[125];
+  __shared__ double wacc[kPairBlock / 64][MFE_MAX_PAIRS + 1];
+  const int P = t.nptpt, nbin = t.nbin, nzl = t.nzl, npt = t.npt;
+  const size_t bb = pair_bin_bytes(P, nbin);
+  unsigned short *bin_s = reinterpret_cast<unsigned short *>(pair_lds);
+  float *zl_s = reinterpret_cast<float *>(pair_lds + bb);
+  float *cf_s = reinterpret_cast<float *>(pair_lds + bb + pair_zl_bytes(nzl));
+  for (int k = threadIdx.x; k < P * nbin; k += blockDim.x) bin_s[k] = t.bin_k[k];
+  for (int k = threadIdx.x; k < nzl; k += blockDim.x) zl_s[k] = t.zlow_f[k];
+  if (CF)
+    for (int k = threadIdx.x; k < nzl * kCoefStride; k += blockDim.x) cf_s[k] = t.coef_f[k];
+  for (int k = threadIdx.x; k < P; k += blockDim.x) rec_s[k] = t.rec_f[k];
+  for (int k = threadIdx.x; k < npt * npt; k += blockDim.x) pidx_s[k] = t.pairidx[k];
+  for (int k = threadIdx.x; k < 125; k += blockDim.x)
+    off_s[k] = make_float4(V.cg.off[0][k % 5], V.cg.off[1][(k / 5) % 5], V.cg.off[2][k / 25], 0.f);
+  const int l64 = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = l64; k <= P; k += 64) wacc[wv][k] = 0.0;
+  __syncthreads();
+  const float rc2 = (float)t.rcut2;
+  const float L0 = V.cg.L[0], L1 = V.cg.L[1], L2 = V.cg.L[2];
+  const float H0 = V.cg.L2[0], H1 = V.cg.L2[1], H2 = V.cg.L2[2];
+  // lanes per particle: 1, 2 or 4 (lsh its log2)
+  const int LANES = V.pair_lanes, lsh = LANES == 4 ? 2 : LANES - 1;
+  const int lane = threadIdx.x & (LANES - 1);
+  double ej[NT];
+  double vir = 0.0;
+  KahanF fxf, fyf, fzf;
+  int ti = 0;
+  unsigned long long ptp = 0;
+  int err = 0;
+  // the separation of a partner at relative position pj with code e
+  auto sep = [&](float xi, float yi, float zi, float4 pj, unsigned e, float &dx, float &dy,
+                 float &dz) {
+    const float4 o4 = off_s[e >> kNbrCodeShift];
+    dx = (xi - pj.x) + o4.x;
+    dy = (yi - pj.y) + o4.y;
+    dz = (zi - pj.z) + o4.z;
+    // axes of one cell: the minimum image
+    if (fabsf(dx) > H0) dx -= copysignf(L0, dx);
+    if (fabsf(dy) > H1) dy -= copysignf(L1, dy);
+    if (fabsf(dz) > H2) dz -= copysignf(L2, dz);
+  };
+  // one list entry: a partner of type tj at separation (dx, dy, dz)
+  auto eval = [&](float dx, float dy, float dz, int tj) {
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const bool in = semantics == 0 ? !(r2 > rc2) : (r2 < rc2);
+    const int p = (int)((ptp >> (8 * tj)) & 0xffu);
+    const PairRecF R = rec_s[p];
+    const float r2c = fmaxf(r2, R.lo);
+    const float sf = __builtin_amdgcn_rcpf(r2c);
+    const float xb = fminf(fmaxf((sf - R.s_lo) * R.s_inv, 0.0f), (float)(nbin - 1));
+    const int bk = bin_s[p * nbin + (int)xb];
+    const int o = R.off, n = R.nint, kb = bk & (kSlowBin - 1);
+    const float z0 = zl_s[o + kb], z1 = zl_s[o + min(kb + 1, n - 1)];
+    int iv;
+    float zv;
+    if (bk & kSlowBin) {
+      auto zl = [&](int k) -> float { return zl_s[o + k]; };
+      iv = find_interval(zl, n, r2c, kb);
+      zv = zl_s[o + iv];
+    } else {
+      const bool at = r2c >= z0;
+      iv = at ? kb : min(kb + 1, n - 1);
+      zv = at ? z0 : z1;
+    }
+    const float4 *q = CF ? reinterpret_cast<const float4 *>(cf_s) + 3 * (o + iv)
+                         : reinterpret_cast<const float4 *>((const float *)t.coef_f) +
+                               3 * (o + iv);
+    const float4 c0 = q[0], c1 = q[1], c2 = q[2];
+    const bool hc = r2 < R.r2at;
+    const bool ov = !hc && r2 < R.r2um;
+    const bool te = !hc && !ov && !(r2 >= R.lo);
+    if (in) err |= (ov ? ERR_OVERLAP : 0) | (te ? ERR_TABLE : 0);
+    if (!in || ov || te) return;
+    const float df = r2 - zv;
+    // c0 = (w0, w1, w2, w3), c1 = (w4, w5, f0, f1), c2 = (f2, f3, f4, w0 rounding)
+    float fs = c1.z + df * (c1.w + df * (c2.x + df * (c2.y + df * c2.z)));
+    double usum = 0.0;
+    if (energy) {
+      const float up = df * (c0.y + df * (c0.z + df * (c0.w + df * (c1.x + df * c1.y))));
+      usum = ((double)c0.x + (double)c2.w) + (double)up;
+    }
+    if (hc) {
+      usum = 1e10;
+      fs = 1e10f;
+    } else if (KN && V.nknot) {
+      const double r = sqrt((double)r2);
+      int ki;
+      double w[4], dw[4];
+      if (knot_weights(r, V.knot_r0, V.knot_dinv, V.nknot, ki, w, dw)) {
+        const double *kc = V.knot_c + (size_t)p * V.nknot + ki;
+        usum += kc[0] * w[0] + kc[1] * w[1] + kc[2] * w[2] + kc[3] * w[3];
+        fs -= (float)((kc[0] * dw[0] + kc[1] * dw[1] + kc[2] * dw[2] + kc[3] * dw[3]) *
+                      V.knot_dinv / r);
+      }
+    }
+    fxf.add(fs * dx);
+    fyf.add(fs * dy);
+    fzf.add(fs * dz);
+    if (energy) {
+#pragma unroll
+      for (int k2 = 0; k2 < NT; ++k2) ej[k2] += (k2 == tj) ? 0.5 * usum : 0.0;
+      vir -= 0.5 * (double)(fs * r2);
+    }
+  };
+  const int kChunk = kPairBlock >> lsh;
+  const int nch = (np + kChunk - 1) / kChunk;
+  const int ch0 = (int)((long long)blk * nch / nb), ch1 = (int)((long long)(blk + 1) * nch / nb);
+  const int maxrow = V.maxnn - 1;
+  for (int ch = ch0; ch < ch1; ++ch) {
+    const int i = ch * kChunk + (threadIdx.x >> lsh);
+#pragma unroll
+    for (int k = 0; k < NT; ++k) ej[k] = 0.0;
+    fxf = KahanF();
+    fyf = KahanF();
+    fzf = KahanF();
+    ti = 0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (i < np) {
+      // the particle, its list length and its first entries issued together
+      const float4 pi = POS64 ? make_float4(0.f, 0.f, 0.f, __int_as_float(V.type_s[i])) : pos[i];
+      const double xi = POS64 ? V.ro_s[3 * i] : 0.0, yi = POS64 ? V.ro_s[3 * i + 1] : 0.0,
+                   zi = POS64 ? V.ro_s[3 * i + 2] : 0.0;
+      const int n = nn[i];
+      unsigned es[kPairBatch];
+#pragma unroll
+      for (int g = 0; g < kPairBatch; ++g)
+        es[g] = nbr[(size_t)min(lane + g * LANES, maxrow) * np + i];
+      ti = __float_as_int(pi.w);
+      ptp = 0;
+      for (int u = 0; u < npt; ++u)
+        ptp |= (unsigned long long)pidx_s[ti * npt + u] << (8 * u);
+      for (int k0 = lane; k0 < n; k0 += kPairBatch * LANES) {
+        if (k0 != lane) {
+#pragma unroll
+          for (int g = 0; g < kPairBatch; ++g)
+            es[g] = nbr[(size_t)min(k0 + g * LANES, maxrow) * np + i];
+        }
+        if constexpr (POS64) {
+          double xs[kPairBatch], ys[kPairBatch], zs[kPairBatch];
+          int ts[kPairBatch];
+#pragma unroll
+          for (int g = 0; g < kPairBatch; ++g) {
+            const int j = k0 + g * LANES < n ? (int)(es[g] & kNbrIdxMask) : i;
+            xs[g] = V.ro_s[3 * j];
+            ys[g] = V.ro_s[3 * j + 1];
+            zs[g] = V.ro_s[3 * j + 2];
+            ts[g] = V.type_s[j];
+          }
+#pragma unroll
+          for (int g = 0; g < kPairBatch; ++g) {
+            if (k0 + g * LANES < n) {
+              double ddx = xi - xs[g], ddy = yi - ys[g], ddz = zi - zs[g];
+              pbcr2(ddx, ddy, ddz, V.box);
+              eval((float)ddx, (float)ddy, (float)ddz, ts[g]);
+            }
+          }
+        } else {
+          float4 ps[kPairBatch];
+#pragma unroll
+          for (int g = 0; g < kPairBatch; ++g) {
+            // entries past the list end read the particle itself
+            const int j = k0 + g * LANES < n ? (int)(es[g] & kNbrIdxMask) : i;
+            ps[g] = pos[j];
+          }
+#pragma unroll
+          for (int g = 0; g < kPairBatch; ++g) {
+            if (k0 + g * LANES < n) {
+              float dx, dy, dz;
+              sep(pi.x, pi.y, pi.z, ps[g], es[g], dx, dy, dz);
+              eval(dx, dy, dz, __float_as_int(ps[g].w));
+            }
+          }
+        }
+      }
+      fx = fxf.value();
+      fy = fyf.value();
+      fz = fzf.value();
+    }
+    // the lanes' partial forces: exchanges within pairs / quads by DPP
+    // (LANES is uniform over the block, so every lane takes part)
+    if (LANES >= 2) {
+      fx += quad_swap<kXor1>(fx);
+      fy += quad_swap<kXor1>(fy);
+      fz += quad_swap<kXor1>(fz);
+    }
+    if (LANES == 4) {
+      fx += quad_swap<kXor2>(fx);
+      fy += quad_swap<kXor2>(fy);
+      fz += quad_swap<kXor2>(fz);
+    }
+    if (i < np && lane == 0) {
+      const int io = V.c_sorted[i];
+      V.force[3 * io] = fx;
+      V.force[3 * io + 1] = fy;
+      V.force[3 * io + 2] = fz;
+    }
+    if (energy) {
+      // pair-type energies of this chunk: a shuffle tree in each
+      // wavefront, added to the wavefront's own LDS row
+      for (int k = 0; k < P; ++k) {
+        double v = 0.0;
+        if (i < np) {
+#pragma unroll
+          for (int u = 0; u < NT; ++u)
+            if (u < npt && pidx_s[ti * npt + u] == k) v += ej[u];
+        }
+        for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
+        if (l64 == 0) wacc[wv][k] += v;
+      }
+    }
+  }
+  if (err) atomicOr(&V.flags[0], err);
+  if (energy) {
     double v = vir;
     for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
     if (l64 == 0) wacc[wv][P] = v;
@@ -1282,7 +1535,7 @@ __global__ void __launch_bounds__(256) k_mc_pair(const SysDev *__restrict__ sys,
         yj = V.ro[3 * k + 1];
         zj = V.ro[3 * k + 2];
       } else {
-        const int j = V.nbr[(size_t)k * np + t];  // cell-ordered slot
+        const int j = (int)((unsigned)V.nbr[(size_t)k * np + t] & kNbrIdxMask);  // cell-ordered slot
         jp = V.c_sorted[j];
         tj = V.type_s[j];
         xj = V.ro_s[3 * j];
@@ -1675,7 +1928,10 @@ __device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin,
   vir += v[P + 4];
   double rec = 0.0;
   if (V.ewald != MFE_EWALD_NONE) {
-    urec += V.self_energy;
+    // self term; the neutralising background of a net charge scales as
+    // 1/V, so its virial (dE/dlambda under r -> lambda r) is -3 times it
+    urec += V.self_energy + V.net_energy;
+    virrec -= 3.0 * V.net_energy;
     if (V.lsurf) {
       const double fac = 2.0 * 3.1415926535897932 / (3.0 * V.vol);
       const double term = fac * (S[0] * S[0] + S[1] * S[1] + S[2] * S[2]);
@@ -1733,7 +1989,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(const SysDev *__restrict__ s
     double phi = 0.0;
     const int n = V.nn[tt];
     for (int k = 0; k < n; ++k) {
-      const int pj = V.nbr[(size_t)k * np + tt];
+      const int pj = (int)((unsigned)V.nbr[(size_t)k * np + tt] & kNbrIdxMask);
       double dx = xi - V.ro_s[3 * pj], dy = yi - V.ro_s[3 * pj + 1], dz = zi - V.ro_s[3 * pj + 2];
       const double r2 = pbcr2(dx, dy, dz, b);
       if (V.nlsem == 0 ? (r2 > t.rcut2) : !(r2 < t.rcut2)) continue;
@@ -1790,6 +2046,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_q(const SysDev *__restrict__
   if (V.ewald == MFE_EWALD_SPME) {
     d += V.phi_rec[i];
     d -= 2.0 * V.efp * (V.alpha / 1.7724538509055160273) * V.q[i];
+    d += V.efp * V.net_dq;
   }
   if (V.lsurf) {
     // efp (2 pi/3V) |sum q r|^2 -> efp (4 pi/3V) (sum q r) . r_i
@@ -1891,6 +2148,7 @@ __global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
     const double fac = 0.5 * (dt * dt);
     double x[3], d[3], s[3];
     const int cp = V.c_pos[i];
+    const int cell = V.pos_f ? V.c_of[i] : 0, ty = V.pos_f ? type[i] : 0;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       x[m] = ro[3 * i + m];
@@ -1931,6 +2189,7 @@ __global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
       V.ro_s[3 * cp + m] = x[m];
       drosum[3 * i + m] = s[m];
     }
+    if (V.pos_f) V.pos_f[cp] = rel_pos(V, x, cell, ty);
     r2 = s[0] * s[0];
     r2 = r2 + s[1] * s[1];
     r2 = r2 + s[2] * s[2];
@@ -1957,6 +2216,7 @@ __global__ void __launch_bounds__(kBlock) k_set_positions(const SysDev *__restri
     // every load before the first store
     double xn[3], xo[3], s[3];
     const int cp = V.c_pos[i];
+    const int cell = V.pos_f ? V.c_of[i] : 0, ty = V.pos_f ? V.type[i] : 0;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
       xn[m] = ro_new[3 * i + m];
@@ -1975,6 +2235,7 @@ __global__ void __launch_bounds__(kBlock) k_set_positions(const SysDev *__restri
       V.ro_s[3 * cp + m] = xn[m];
       V.drosum[3 * i + m] = s[m];
     }
+    if (V.pos_f) V.pos_f[cp] = rel_pos(V, xn, cell, ty);
     const double *ds = s;
     r2 = ds[0] * ds[0];
     r2 = r2 + ds[1] * ds[1];
@@ -2206,21 +2467,43 @@ void launch_pair(const Group &g, bool energy, bool sort_pos, bool skin) {
   const int pgrid = std::max(g.pair_blocks, 1);
   const dim3 gr = grid(pgrid + (skin ? (g.np_blocks + 1) / 2 : 0), g);
   const int e1 = energy ? 1 : 0, s1 = skin ? 1 : 0;
-#define MFE_PAIR(M, NT, CF)                                                                  \
-  hipLaunchKernelGGL((k_pair<M, NT, CF>), gr, dim3(kPairBlock), g.lds_pair, g.stream, g.d, e1, \
+#define MFE_PAIR(NT, CF)                                                                  \
+  hipLaunchKernelGGL((k_pair<NT, CF>), gr, dim3(kPairBlock), g.lds_pair, g.stream, g.d, e1, \
                      pgrid, s1)
-#define MFE_PAIR_CF(M, NT) \
-  do { if (g.pair_cf_lds) MFE_PAIR(M, NT, true); else MFE_PAIR(M, NT, false); } while (0)
+#define MFE_PAIR_CF(NT) \
+  do { if (g.pair_cf_lds) MFE_PAIR(NT, true); else MFE_PAIR(NT, false); } while (0)
+  // the skin blocks ride on the first of the (one or two) launches; the
+  // second has no blocks beyond pgrid
+  const dim3 gr0 = grid(pgrid, g);
+#define MFE_PAIRF(NT, CF, KN, P64, GR, SK)                                                 \
+  hipLaunchKernelGGL((k_pair_f<NT, CF, KN, P64>), GR, dim3(kPairBlock), g.lds_pair, g.stream, \
+                     g.d, e1, pgrid, SK)
+#define MFE_PAIRF_LN(NT, CF, KN)                                         \
+  do {                                                                   \
+    if (g.pair_any32) MFE_PAIRF(NT, CF, KN, false, gr, s1);              \
+    if (g.pair_any64) {                                                  \
+      if (g.pair_any32) MFE_PAIRF(NT, CF, KN, true, gr0, 0);             \
+      else MFE_PAIRF(NT, CF, KN, true, gr, s1);                          \
+    }                                                                    \
+  } while (0)
+#define MFE_PAIRF_KN(NT, CF) \
+  do { if (g.knots) MFE_PAIRF_LN(NT, CF, true); else MFE_PAIRF_LN(NT, CF, false); } while (0)
+#define MFE_PAIRF_CF(NT) \
+  do { if (g.pair_cf_lds) MFE_PAIRF_KN(NT, true); else MFE_PAIRF_KN(NT, false); } while (0)
   const int nt = g.pair_npt;
   if (g.prec == MFE_PREC_MIXED) {
-    if (nt <= 3) MFE_PAIR_CF(true, 3);
-    else if (nt <= 4) MFE_PAIR_CF(true, 4);
-    else MFE_PAIR_CF(true, 8);
+    if (nt <= 3) MFE_PAIRF_CF(3);
+    else if (nt <= 4) MFE_PAIRF_CF(4);
+    else MFE_PAIRF_CF(8);
   } else {
-    if (nt <= 3) MFE_PAIR_CF(false, 3);
-    else if (nt <= 4) MFE_PAIR_CF(false, 4);
-    else MFE_PAIR_CF(false, 8);
+    if (nt <= 3) MFE_PAIR_CF(3);
+    else if (nt <= 4) MFE_PAIR_CF(4);
+    else MFE_PAIR_CF(8);
   }
+#undef MFE_PAIRF_CF
+#undef MFE_PAIRF_KN
+#undef MFE_PAIRF_LN
+#undef MFE_PAIRF
 #undef MFE_PAIR_CF
 #undef MFE_PAIR
 }
@@ -2361,6 +2644,12 @@ void launch_skin(const Group &g) {
 // Bricks above 64 KB of LDS (meshes whose only divisors >= order-1 are
 // large, e.g. 2*23) need the per-kernel dynamic LDS limit raised.
 void mfe_allow_big_spread_lds(int prec, int bytes) {
+  // the attribute only ever grows (per precision): a smaller brick set up
+  // later must not take the room of an earlier, larger one
+  static int allowed[2] = {0, 0};
+  int &al = allowed[prec == MFE_PREC_MIXED ? 1 : 0];
+  if (bytes <= al) return;
+  al = bytes;
 #define MFE_ATTR(R, O)                                                              \
   (void)hipFuncSetAttribute((const void *)mfe::k_spread_ptr<R, O>(),                 \
                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);        \

@@ -86,7 +86,7 @@ SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_batch_step_md", "mfe_batch_step_bd", "mfe_batch_get_energies",
            "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint", "mfe_mc_delta_rec",
            "mfe_mc_delta_pair", "mfe_mc_accept", "mfe_ewald_parameters",
-           "mfe_set_sync_limit", "mfe_model_forces", "mfe_fm_set_frames", "mfe_fm_loss_grad"]
+           "mfe_set_sync_limit", "mfe_set_net_charge", "mfe_model_forces", "mfe_fm_set_frames", "mfe_fm_loss_grad"]
 
 _lib = None
 
@@ -107,6 +107,7 @@ def lib():
         L.mfe_last_error.argtypes = [C.c_void_p]
         L.mfe_last_error.restype = C.c_char_p
         L.mfe_set_sync_limit.argtypes = [C.c_void_p, C.c_int32]
+        L.mfe_set_net_charge.argtypes = [C.c_void_p, C.c_int32]
         L.mfe_model_forces.argtypes = [C.c_void_p, _dp]
         L.mfe_fm_set_frames.argtypes = [C.c_void_p, C.c_int32, _dp, _dp]
         L.mfe_fm_loss_grad.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
@@ -235,6 +236,21 @@ def build_tables(zat, sigat, epsat, *, rcut, epsi_four_pi, alpha=0.0,
 
 
 # ------------------------------------------------------------------ units
+def bd_factors(tstep, temp, dcoeff):
+    """BDStep's per-type factors (Src/bd.F90:136-139) with MOLSIM's default
+    units: fac1 = dt D/(R T) (A per unit of forceo), fac2 = sqrt(2 D dt)
+    (A); tstep in ps, temp in K, dcoeff in 1e-9 m^2/s.  Evaluated left to
+    right as the reference writes them."""
+    scllen, sclmas, scltem, scltim, scldif = 1.0e-10, 1.0e-03, 1.0, 1.0e-12, 1.0e-09
+    sclfor = scllen / scltim / scltim * sclmas
+    gas = 8.314472
+    d = np.asarray(dcoeff, dtype=np.float64)
+    f1 = np.array([tstep * scltim * v * scldif * sclfor / (gas * temp * scltem) / scllen
+                   for v in d])
+    f2 = np.array([np.sqrt(2.0 * v * scldif * tstep * scltim) / scllen for v in d])
+    return f1, f2
+
+
 def epsi0_four_pi() -> float:
     """Epsi0FourPi in kJ A/mol (Src/molsim.F90:805, constants Src/mol.F90:70-76)."""
     pi = 3.1415926535897932
@@ -416,6 +432,10 @@ class ForceEngine:
 
     def step_md(self, nsteps, tstep, flags=FORCES | ENERGY | VIRIAL):
         self._check(self.L.mfe_step_md(self.h, int(nsteps), float(tstep), flags), "mfe_step_md")
+
+    def set_net_charge(self, on=True):
+        """Neutralising-background term of a net charge (opt-in)."""
+        self._check(self.L.mfe_set_net_charge(self.h, 1 if on else 0), "mfe_set_net_charge")
 
     def set_sync_limit(self, polls):
         """Poll bound of the list rebuild's grid-barrier waits (< 0: default)."""

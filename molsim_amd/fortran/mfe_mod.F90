@@ -266,6 +266,36 @@ module mfe_mod
          real(c_double), intent(out) :: energy
       end function
 
+      ! force matching: model forces at the current positions, frames
+      ! ro(3,np,nframes) with targets ft(3,np,nframes), loss and gradient
+      integer(c_int) function mfe_model_forces(eng, force) bind(c, name='mfe_model_forces')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: force(*)
+      end function
+
+      integer(c_int) function mfe_fm_set_frames(eng, nframes, ro, ft) &
+            bind(c, name='mfe_fm_set_frames')
+         import :: c_int, c_int32_t, c_ptr, c_double
+         type(c_ptr), value :: eng
+         integer(c_int32_t), value :: nframes
+         real(c_double), intent(in) :: ro(*), ft(*)
+      end function
+
+      integer(c_int) function mfe_fm_loss_grad(eng, loss, dzat, dsigat, depsat, dknot) &
+            bind(c, name='mfe_fm_loss_grad')
+         import :: c_int, c_ptr, c_double
+         type(c_ptr), value :: eng
+         real(c_double), intent(out) :: loss, dzat(*), dsigat(*), depsat(*), dknot(*)
+      end function
+
+      ! opt-in neutralising-background term of a net charge
+      integer(c_int) function mfe_set_net_charge(eng, on) bind(c, name='mfe_set_net_charge')
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: eng
+         integer(c_int32_t), value :: on
+      end function
+
       ! DUTwoBodyEwaldRecSPM for a trial move of natm particles ianatm
       ! (1-based) to rtm(3,natm): du = change of u%rec
       integer(c_int) function mfe_mc_delta_rec(eng, natm, ianatm, rtm, reuse, du) &

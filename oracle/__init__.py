@@ -17,6 +17,10 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# the all-cores twin: the same source built with -fopenmp (particle loops,
+# FFT rows and spreading split over threads; summation order not the
+# reference's, so a timing baseline only)
+LIB_OMP_PATH = os.path.join(_HERE, "liboracle_omp.so")
 SRC_PATH = os.path.join(_HERE, "mfe_oracle.c")
 
 _dp = C.POINTER(C.c_double)
@@ -24,25 +28,47 @@ _ip = C.POINTER(C.c_int)
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with gcc (no FMA contraction, like the reference)."""
-    if force or not os.path.exists(LIB_PATH) or (
-            os.path.getmtime(LIB_PATH) < os.path.getmtime(SRC_PATH)):
-        subprocess.check_call([
-            "gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC",
-            "-shared", "-o", LIB_PATH, SRC_PATH, "-lm"])
+    """Compile the oracle with gcc (no FMA contraction, like the reference),
+    and its all-cores twin with -fopenmp."""
+    for path, extra in ((LIB_PATH, []), (LIB_OMP_PATH, ["-fopenmp"])):
+        if force or not os.path.exists(path) or (
+                os.path.getmtime(path) < os.path.getmtime(SRC_PATH)):
+            subprocess.check_call([
+                "gcc", "-O2", "-ffp-contract=off", "-fno-fast-math", "-fPIC"] + extra +
+                ["-shared", "-o", path, SRC_PATH, "-lm"])
     return LIB_PATH
 
 
 _lib = None
+_lib_serial = None
+_lib_omp = None
+
+
+def use_all_cores(on: bool) -> None:
+    """Route every binding through the serial oracle (default) or its
+    all-cores OpenMP twin."""
+    global _lib, _lib_serial, _lib_omp
+    if on:
+        if _lib_omp is None:
+            if not os.path.exists(LIB_OMP_PATH):
+                build()
+            _lib_omp = C.CDLL(LIB_OMP_PATH)
+            _declare(_lib_omp)
+        _lib = _lib_omp
+    else:
+        _lib = None
+        lib()
 
 
 def lib():
-    global _lib
+    global _lib, _lib_serial
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            build()
-        _lib = C.CDLL(LIB_PATH)
-        _declare(_lib)
+        if _lib_serial is None:
+            if not os.path.exists(LIB_PATH):
+                build()
+            _lib_serial = C.CDLL(LIB_PATH)
+            _declare(_lib_serial)
+        _lib = _lib_serial
     return _lib
 
 

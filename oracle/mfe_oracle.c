@@ -453,6 +453,53 @@ int orc_set_vlist_md_cells(const orc_box *b, int np, const double *ro,
     head[c] = i;
   }
   int rc = 0;
+#ifdef _OPENMP
+  free(cand);
+  cand = NULL;
+#pragma omp parallel reduction(max : rc)
+  {
+    int cap = 4096;
+    int *cd = malloc(sizeof(int) * (size_t)cap);
+#pragma omp for schedule(dynamic, 256)
+    for (int ip = 1; ip <= np; ++ip) {
+      int c = cellof[ip - 1];
+      int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
+      int ncand = 0;
+      int rx = nc[0] > 1 ? 1 : 0, ry = nc[1] > 1 ? 1 : 0, rz = nc[2] > 1 ? 1 : 0;
+      for (int dzc = -rz; dzc <= rz; ++dzc)
+        for (int dyc = -ry; dyc <= ry; ++dyc)
+          for (int dxc = -rx; dxc <= rx; ++dxc) {
+            int x = (cx + dxc + nc[0]) % nc[0], y = (cy + dyc + nc[1]) % nc[1],
+                z = (cz + dzc + nc[2]) % nc[2];
+            for (int j = head[x + nc[0] * (y + nc[1] * z)]; j >= 0; j = next[j]) {
+              int jp = j + 1;
+              double dx, dy, dz, r2;
+              if (!owns(ip, jp)) continue;
+              dx = ro[3 * (ip - 1)] - ro[3 * j];
+              dy = ro[3 * (ip - 1) + 1] - ro[3 * j + 1];
+              dz = ro[3 * (ip - 1) + 2] - ro[3 * j + 2];
+              orc_pbcr2(b, &dx, &dy, &dz, &r2);
+              if (r2 < rs2cut) {
+                if (ncand == cap) {
+                  cap *= 2;
+                  cd = realloc(cd, sizeof(int) * (size_t)cap);
+                }
+                cd[ncand++] = jp;
+              }
+            }
+          }
+      qsort(cd, (size_t)ncand, sizeof(int), cmp_int);
+      if (ncand > maxnneigh) {
+        rc = 4;
+        continue;
+      }
+      memcpy(&jpnlist[(size_t)(ip - 1) * maxnneigh], cd, sizeof(int) * (size_t)ncand);
+      nneighpn[ip - 1] = ncand;
+    }
+    free(cd);
+  }
+  np = 0; /* the serial loop below is skipped */
+#endif
   for (int ip = 1; ip <= np && !rc; ++ip) {
     int c = cellof[ip - 1];
     int cx = c % nc[0], cy = (c / nc[0]) % nc[1], cz = c / (nc[0] * nc[1]);
@@ -679,10 +726,76 @@ static int pair_type(int npt, int ipt, int jpt) {
   return (a - 1) * npt - (a - 1) * (a - 2) / 2 + (b - a) + 1;
 }
 
+#ifdef _OPENMP
+/* All-cores twin (built with -fopenmp only; the serial build keeps the
+   reference's summation order): particles split over threads, each with
+   its own force array, pair-type energies and virial, summed afterwards. */
+static int utwobody_a_omp(const orc_table_set *t, const orc_box *b, int np,
+                          const double *ro, const int *iptpn, int maxnneigh,
+                          const int *nneighpn, const int *jpnlist, double *force,
+                          double *u_twob, double *virial) {
+  int nptpt = t->npt * (t->npt + 1) / 2, err = 0;
+  double vir = 0.0;
+  for (int k = 0; k <= nptpt; ++k) u_twob[k] = 0.0;
+#pragma omp parallel reduction(+ : vir) reduction(max : err)
+  {
+    double *f = calloc(3 * (size_t)np, sizeof(double));
+    double ut[64] = {0};
+    double usum = 0.0, fsum = 0.0;
+#pragma omp for schedule(dynamic, 256)
+    for (int ip = 1; ip <= np; ++ip) {
+      int ipt = iptpn[ip - 1];
+      for (int jl = 0; jl < nneighpn[ip - 1]; ++jl) {
+        int jp = jpnlist[(size_t)(ip - 1) * maxnneigh + jl];
+        int iptjpt = pair_type(t->npt, ipt, iptpn[jp - 1]);
+        double dx = ro[3 * (ip - 1)] - ro[3 * (jp - 1)];
+        double dy = ro[3 * (ip - 1) + 1] - ro[3 * (jp - 1) + 1];
+        double dz = ro[3 * (ip - 1) + 2] - ro[3 * (jp - 1) + 2];
+        double r2;
+        orc_pbcr2(b, &dx, &dy, &dz, &r2);
+        if (r2 > t->rcut2) continue;
+        if (r2 < t->r2atat[iptjpt - 1]) {
+          usum = 1e10;
+        } else if (r2 < t->r2umin[iptjpt - 1]) {
+          err = 2;
+          continue;
+        } else if (orc_table_eval(t->ubuf, t->nbuf, t->iubuflow[iptjpt - 1], r2, &usum, &fsum)) {
+          err = 3;
+          continue;
+        }
+        ut[iptjpt] += usum;
+        f[3 * (ip - 1)] += fsum * dx;
+        f[3 * (ip - 1) + 1] += fsum * dy;
+        f[3 * (ip - 1) + 2] += fsum * dz;
+        f[3 * (jp - 1)] -= fsum * dx;
+        f[3 * (jp - 1) + 1] -= fsum * dy;
+        f[3 * (jp - 1) + 2] -= fsum * dz;
+        vir -= fsum * r2;
+      }
+    }
+#pragma omp critical
+    {
+      for (size_t k = 0; k < 3 * (size_t)np; ++k) force[k] += f[k];
+      for (int k = 1; k <= nptpt; ++k) u_twob[k] += ut[k];
+    }
+    free(f);
+  }
+  if (err) return err;
+  u_twob[0] = 0.0;
+  for (int k = 1; k <= nptpt; ++k) u_twob[0] += u_twob[k];
+  *virial += vir;
+  return 0;
+}
+#endif
+
 int orc_utwobody_a(const orc_table_set *t, const orc_box *b, int np,
                    const double *ro, const int *iptpn, int maxnneigh,
                    const int *nneighpn, const int *jpnlist, double *force,
                    double *u_twob, double *virial) {
+#ifdef _OPENMP
+  return utwobody_a_omp(t, b, np, ro, iptpn, maxnneigh, nneighpn, jpnlist, force, u_twob,
+                        virial);
+#endif
   int nptpt = t->npt * (t->npt + 1) / 2;
   double virtwob = 0.0, usum = 0.0, fsum = 0.0;
   for (int k = 0; k <= nptpt; ++k) u_twob[k] = 0.0;
@@ -821,57 +934,70 @@ static void fft1d(int n, double complex *data, int sign) {
 
 void orc_fft_r2c_3d(int s1, int s2, int s3, const double *in, double *out) {
   int h = s1 / 2 + 1, nmax = s1 > s2 ? (s1 > s3 ? s1 : s3) : (s2 > s3 ? s2 : s3);
-  double complex *row = malloc(sizeof(double complex) * (size_t)nmax);
   double complex *c = (double complex *)out;
-  for (int z = 0; z < s3; ++z)
-    for (int y = 0; y < s2; ++y) {
-      for (int x = 0; x < s1; ++x) row[x] = in[((size_t)z * s2 + y) * s1 + x];
-      fft1d(s1, row, -1);
-      for (int x = 0; x < h; ++x) c[((size_t)z * s2 + y) * h + x] = row[x];
-    }
-  for (int z = 0; z < s3; ++z)
-    for (int x = 0; x < h; ++x) {
-      for (int y = 0; y < s2; ++y) row[y] = c[((size_t)z * s2 + y) * h + x];
-      fft1d(s2, row, -1);
-      for (int y = 0; y < s2; ++y) c[((size_t)z * s2 + y) * h + x] = row[y];
-    }
-  for (int y = 0; y < s2; ++y)
-    for (int x = 0; x < h; ++x) {
-      for (int z = 0; z < s3; ++z) row[z] = c[((size_t)z * s2 + y) * h + x];
-      fft1d(s3, row, -1);
-      for (int z = 0; z < s3; ++z) c[((size_t)z * s2 + y) * h + x] = row[z];
-    }
-  free(row);
+  /* every row transform is independent (the all-cores twin splits them) */
+#pragma omp parallel
+  {
+    double complex *row = malloc(sizeof(double complex) * (size_t)nmax);
+#pragma omp for collapse(2)
+    for (int z = 0; z < s3; ++z)
+      for (int y = 0; y < s2; ++y) {
+        for (int x = 0; x < s1; ++x) row[x] = in[((size_t)z * s2 + y) * s1 + x];
+        fft1d(s1, row, -1);
+        for (int x = 0; x < h; ++x) c[((size_t)z * s2 + y) * h + x] = row[x];
+      }
+#pragma omp for collapse(2)
+    for (int z = 0; z < s3; ++z)
+      for (int x = 0; x < h; ++x) {
+        for (int y = 0; y < s2; ++y) row[y] = c[((size_t)z * s2 + y) * h + x];
+        fft1d(s2, row, -1);
+        for (int y = 0; y < s2; ++y) c[((size_t)z * s2 + y) * h + x] = row[y];
+      }
+#pragma omp for collapse(2)
+    for (int y = 0; y < s2; ++y)
+      for (int x = 0; x < h; ++x) {
+        for (int z = 0; z < s3; ++z) row[z] = c[((size_t)z * s2 + y) * h + x];
+        fft1d(s3, row, -1);
+        for (int z = 0; z < s3; ++z) c[((size_t)z * s2 + y) * h + x] = row[z];
+      }
+    free(row);
+  }
 }
 
 void orc_fft_c2r_3d(int s1, int s2, int s3, const double *in, double *out) {
   int h = s1 / 2 + 1, nmax = s1 > s2 ? (s1 > s3 ? s1 : s3) : (s2 > s3 ? s2 : s3);
   size_t nc = (size_t)h * s2 * s3;
   double complex *c = malloc(sizeof(double complex) * nc);
-  double complex *row = malloc(sizeof(double complex) * (size_t)nmax);
   memcpy(c, in, sizeof(double complex) * nc);
-  for (int y = 0; y < s2; ++y)
-    for (int x = 0; x < h; ++x) {
-      for (int z = 0; z < s3; ++z) row[z] = c[((size_t)z * s2 + y) * h + x];
-      fft1d(s3, row, +1);
-      for (int z = 0; z < s3; ++z) c[((size_t)z * s2 + y) * h + x] = row[z];
-    }
-  for (int z = 0; z < s3; ++z)
-    for (int x = 0; x < h; ++x) {
-      for (int y = 0; y < s2; ++y) row[y] = c[((size_t)z * s2 + y) * h + x];
-      fft1d(s2, row, +1);
-      for (int y = 0; y < s2; ++y) c[((size_t)z * s2 + y) * h + x] = row[y];
-    }
-  for (int z = 0; z < s3; ++z)
-    for (int y = 0; y < s2; ++y) {
-      const double complex *src = &c[((size_t)z * s2 + y) * h];
-      for (int x = 0; x < h; ++x) row[x] = src[x];
-      for (int x = h; x < s1; ++x) row[x] = conj(src[s1 - x]);
-      fft1d(s1, row, +1);
-      for (int x = 0; x < s1; ++x) out[((size_t)z * s2 + y) * s1 + x] = creal(row[x]);
-    }
+#pragma omp parallel
+  {
+    double complex *row = malloc(sizeof(double complex) * (size_t)nmax);
+#pragma omp for collapse(2)
+    for (int y = 0; y < s2; ++y)
+      for (int x = 0; x < h; ++x) {
+        for (int z = 0; z < s3; ++z) row[z] = c[((size_t)z * s2 + y) * h + x];
+        fft1d(s3, row, +1);
+        for (int z = 0; z < s3; ++z) c[((size_t)z * s2 + y) * h + x] = row[z];
+      }
+#pragma omp for collapse(2)
+    for (int z = 0; z < s3; ++z)
+      for (int x = 0; x < h; ++x) {
+        for (int y = 0; y < s2; ++y) row[y] = c[((size_t)z * s2 + y) * h + x];
+        fft1d(s2, row, +1);
+        for (int y = 0; y < s2; ++y) c[((size_t)z * s2 + y) * h + x] = row[y];
+      }
+#pragma omp for collapse(2)
+    for (int z = 0; z < s3; ++z)
+      for (int y = 0; y < s2; ++y) {
+        const double complex *src = &c[((size_t)z * s2 + y) * h];
+        for (int x = 0; x < h; ++x) row[x] = src[x];
+        for (int x = h; x < s1; ++x) row[x] = conj(src[s1 - x]);
+        fft1d(s1, row, +1);
+        for (int x = 0; x < s1; ++x) out[((size_t)z * s2 + y) * s1 + x] = creal(row[x]);
+      }
+    free(row);
+  }
   free(c);
-  free(row);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -953,7 +1079,42 @@ void orc_spme_rec(const orc_spme *p, int na, const double *r,
   double *spline = malloc(sizeof(double) * (size_t)order * 3 * (size_t)na);
   double *deriv = malloc(sizeof(double) * (size_t)order * 3 * (size_t)na);
   int *meshmax = malloc(sizeof(int) * 3 * (size_t)na);
-  /* MakeGQMesh (:1226-1245) */
+  /* MakeGQMesh (:1226-1245); the all-cores twin spreads into a mesh per
+     thread and adds them up afterwards */
+#ifdef _OPENMP
+#pragma omp parallel
+  {
+    double *Qt = calloc(M, sizeof(double));
+#pragma omp for
+    for (int ia = 0; ia < na; ++ia) {
+      for (int m = 0; m < 3; ++m) {
+        int km;
+        orc_cardinal_bspline(order, p->dkdr[m] * r[3 * ia + m], &km,
+                             &spline[((size_t)ia * 3 + m) * order],
+                             &deriv[((size_t)ia * 3 + m) * order]);
+        meshmax[3 * ia + m] = (km + p->s[m]) % p->s[m];
+      }
+      const double *sx = &spline[((size_t)ia * 3 + 0) * order];
+      const double *sy = &spline[((size_t)ia * 3 + 1) * order];
+      const double *sz = &spline[((size_t)ia * 3 + 2) * order];
+      for (int iz = 0; iz < order; ++iz) {
+        int nz = (s3 + meshmax[3 * ia + 2] - iz) % s3;
+        for (int iy = 0; iy < order; ++iy) {
+          int ny = (s2 + meshmax[3 * ia + 1] - iy) % s2;
+          double qyz = az[ia] * sz[iz] * sy[iy];
+          for (int ix = 0; ix < order; ++ix) {
+            int nx = (s1 + meshmax[3 * ia] - ix) % s1;
+            Qt[((size_t)nz * s2 + ny) * s1 + nx] += qyz * sx[ix];
+          }
+        }
+      }
+    }
+#pragma omp critical
+    for (size_t o = 0; o < M; ++o) Q[o] += Qt[o];
+    free(Qt);
+  }
+  if (0)
+#endif
   for (int ia = 0; ia < na; ++ia) {
     for (int m = 0; m < 3; ++m) {
       int km;
@@ -985,6 +1146,8 @@ void orc_spme_rec(const orc_spme *p, int na, const double *r,
   orc_fft_r2c_3d(s1, s2, s3, Q, FQ);
   double complex *F = (double complex *)FQ;
   double uloc = 0.0, virloc = *virrec;
+  /* distinct nz touch distinct z planes (nz and s3 - nz) */
+#pragma omp parallel for reduction(+ : uloc) reduction(- : virloc)
   for (int nz = 0; nz <= s3 / 2; ++nz) {
     int nza = (s3 - nz) % s3;
     for (int ny = 0; ny <= s2 / 2; ++ny) {
@@ -1013,6 +1176,7 @@ void orc_spme_rec(const orc_spme *p, int na, const double *r,
   *u_rec = uloc;
   *virrec = virloc;
   /* CalcProp (:1259-1285) */
+#pragma omp parallel for
   for (int ia = 0; ia < na; ++ia) {
     double esum[3] = {0, 0, 0};
     const double *sx = &spline[((size_t)ia * 3 + 0) * order];

@@ -195,6 +195,24 @@ def c1_cnf():
         shutil.rmtree(work, ignore_errors=True)
 
 
+def cnf_forces():
+    """UTotal of the reference at the positions its C1 run wrote to
+    tests/golden/c1_final.cnf (read with molsim_amd.cnf): force, forceo and
+    energies -> tests/golden/c1_final_utotal.npz."""
+    sys.path.insert(0, REPO)
+    from molsim_amd import cnf
+    c = cnf.read_cnf(os.path.join(HERE, "c1_final.cnf"))
+    work = tempfile.mkdtemp(prefix="mfe_golden_")
+    try:
+        d = case("c1", c1_text(), work, probe_pos=c.ro, md=False)
+        f = read_force(os.path.join(d, "c1.probe_f"), 3)
+        np.savez_compressed(os.path.join(HERE, "c1_final_utotal.npz"),
+                            **{"ref_" + k: v for k, v in f.items()})
+        print("cnf forces: u_tot", f["u_tot"])
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+
+
 def ewald_options():
     """EwaldSetup's parameter choice for iewaldopt 0-4 (std,
     Src/energy.F90:6644-6690, 7052-7124) on the C1 system: what the
@@ -303,6 +321,9 @@ def main():
         return
     if sys.argv[1:] == ["bd"]:
         bd_steps()
+        return
+    if sys.argv[1:] == ["cnf_forces"]:
+        cnf_forces()
         return
     if sys.argv[1:] == ["cnf"]:
         c1_cnf()

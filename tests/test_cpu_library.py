@@ -137,3 +137,15 @@ def test_ewald_parameters_match_reference(golden_dir):
                            uewaldtol=1e-3, ualpha=0.2)
     with pytest.raises(E.MfeError):
         E.ewald_parameters(3, q2sum=1.0, boxlenshort=50.0, epsi_four_pi=17.7, rcut=15.0)
+
+
+def test_host_bd_factors_equal_the_pinned_oracle():
+    """molsim_amd.engine.bd_factors (what a host passes to mfe_set_bd) is the
+    oracle's orc_bd_factors bitwise; the oracle's is pinned against the
+    reference's BDStep displacements (test_oracle_golden.py)."""
+    import oracle as O
+    for ts, temp, d in [(0.005, 298.0, [1.0, 2.0, 1.33]), (0.002, 300.0, [0.7]),
+                        (0.01, 350.0, [1.33, 2.03])]:
+        a = E.bd_factors(ts, temp, d)
+        b = O.bd_factors(ts, temp, d)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])

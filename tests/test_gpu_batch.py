@@ -199,3 +199,45 @@ def test_autotuner_plan_meets_tolerance():
     say("  errors:", ["%.2e" % e for e in errs])
     assert max(errs) <= 2 * tol
     assert all(A.MESH_MIN <= m <= A.MESH_MAX and m % 2 == 0 for m in plan.meshes)
+
+
+def test_engines_step_alone_after_their_batch_is_closed():
+    """A batch holds views of its engines, never their memory: after
+    mfe_batch_destroy each engine steps alone -- host-noise BD steps equal a
+    fresh engine's from the same positions bitwise, and device-noise steps
+    (a captured graph of the engine's own) run and stay in the box."""
+    reps = [S.c4_replica(r, mesh=MESHES[r], order=4) for r in range(3)]
+    engs = []
+    for r, s in enumerate(reps):
+        tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                             alpha=s.alpha)
+        e = S.configure(s, "mixed", tables=tab)
+        e.set_bd(FAC1, FAC2, 100 + r)
+        engs.append((e, tab))
+    say("case: engines after their batch is closed, 3 replicas, mixed")
+    b = E.Batch([e for e, _ in engs])
+    b.compute()
+    b.step_bd(5)
+    b.energies()
+    b.close()
+    rng = np.random.default_rng(31)
+    for (e, tab), s in zip(engs, reps):
+        ro, _, _ = e.download_state()
+        # the list built anew at these positions, as the fresh engine builds it
+        e.build_list()
+        e.utotal()
+        noise = rng.standard_normal((3, s.np, 3))
+        e.step_bd(FAC1, FAC2, noise)
+        ro_a, _, _ = e.download_state()
+        s2 = S.c4_replica(reps.index(s), mesh=s.mesh, order=4)
+        s2.ro = ro
+        fresh = S.configure(s2, "mixed", tables=tab)
+        fresh.utotal()
+        fresh.step_bd(FAC1, FAC2, noise)
+        ro_b, _, _ = fresh.download_state()
+        fresh.close()
+        assert np.array_equal(ro_a, ro_b)
+        e.step_bd(nsteps=5)
+        ro_c, _, _ = e.download_state()
+        assert np.isfinite(ro_c).all() and np.abs(ro_c).max() <= s.boxlen / 2
+        e.close()
