@@ -242,6 +242,18 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm,
    however far the move. */
 int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm,
                       const double *rtm, double *du_twob, int32_t *overlap);
+/* No reference counterpart: mfe_mc_delta_pair for callers that cannot
+   allocate device memory (code under HIP graph capture, tight MC serving
+   loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
+   results; it calls no hipMalloc or hipFree: the moved particles, the
+   results and the kernels' block partials live in a fixed device array
+   (about 38 KB per device, held until the process ends).  natm <= 1024
+   (MFE_E_BAD_PARAM beyond).  The engine's list must exist already
+   (mfe_build_list or any force call first; MFE_E_STATE otherwise).  All
+   calls on one device share that array: calls that overlap on one device
+   (from several host threads or streams) race. */
+int mfe_mc_delta_pair_noalloc(mfe_engine *e, int32_t natm, const int32_t *ianatm,
+                              const double *rtm, double *du_twob, int32_t *overlap);
 /* Monte Carlo accepted move: EwaldUpdateArray for 'spm'
    (Src/denergy.F90:1776-1819: QMesh = QMeshTM, urecold = urecnew) with the
    MC driver's position update ro(:,ianatm) = rtm and NList's skin

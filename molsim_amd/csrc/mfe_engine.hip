@@ -8,6 +8,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1291,6 +1292,102 @@ int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm, const 
                         e->nlkind == MFE_NL_CELL);
     double h[2];
     MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out + 2, sizeof(h), hipMemcpyDeviceToHost, e->stream));
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    *overlap = h[1] != 0.0 ? 1 : 0;
+    *du_twob = *overlap ? 0.0 : h[0];
+    return MFE_OK;
+  });
+}
+
+// ---------------------------------------------------------------------
+// The pair part of a trial move without any allocation: the moved
+// particles' indices and positions and the kernels' results and block
+// partials live in one file-scope device array, `scratch`, of
+// kNoAllocBytes.  Host code never passes the array's name to a kernel or a
+// copy (on the host it is a stand-in symbol, not a device address): its
+// address on each device is looked up once with hipGetSymbolAddress, with
+// that device current, and kept per device.  Every call on a device shares
+// that device's array, so calls that overlap on one device race; nothing
+// in it needs clearing (every kernel overwrites what it later reads).
+namespace {
+
+constexpr int kNoAllocMaxAtoms = 1024;
+// [ia: kNoAllocMaxAtoms int][rtm: 3 kNoAllocMaxAtoms double][out: 4 double]
+// [partials: 2 kMcPairBlocks double], each part 256-byte aligned
+constexpr size_t kNaIaOff = 0;
+constexpr size_t kNaRtmOff = 4096;
+constexpr size_t kNaOutOff = kNaRtmOff + 3 * 8 * kNoAllocMaxAtoms;
+constexpr size_t kNaPartOff = kNaOutOff + 256;
+constexpr size_t kNoAllocBytes = kNaPartOff + 2 * 8 * mfe::kMcPairBlocks;
+static_assert(kNaRtmOff >= sizeof(int) * kNoAllocMaxAtoms, "scratch layout");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// the device address of `scratch` on device dev (current), looked up once
+int scratch_on(int dev, void **ptr, std::string &err) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    err = "device ordinal out of range for the no-allocation entry point";
+    return MFE_E_BAD_PARAM;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (rc != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc);
+      return MFE_E_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return MFE_OK;
+}
+
+}  // namespace
+
+int mfe_mc_delta_pair_noalloc(mfe_engine *e, int32_t natm, const int32_t *ianatm,
+                              const double *rtm, double *du_twob, int32_t *overlap) {
+  if (!e || !ianatm || !rtm || !du_twob || !overlap || natm < 1) return MFE_E_BAD_PARAM;
+  if (natm > kNoAllocMaxAtoms)
+    return fail(e, MFE_E_BAD_PARAM, "natm > 1024, the no-allocation scratch limit");
+  int rc = check_ready(e);
+  if (rc) return rc;
+  if (natm > e->np) return fail(e, MFE_E_BAD_PARAM, "natm > np");
+  // the device view must exist already (creating it would allocate)
+  if (!e->d_sys || !e->list_valid)
+    return fail(e, MFE_E_STATE, "build the list first (mfe_build_list or mfe_compute)");
+  int ia[kNoAllocMaxAtoms];
+  for (int a = 0; a < natm; ++a) {
+    const int i = ianatm[a] - 1;
+    if (i < 0 || i >= e->np) return fail(e, MFE_E_BAD_PARAM, "ianatm out of range");
+    for (int b = 0; b < a; ++b)
+      if (ia[b] == i) return fail(e, MFE_E_BAD_PARAM, "ianatm repeats a particle");
+    ia[a] = i;
+    for (int m = 0; m < 3; ++m)
+      if (!std::isfinite(rtm[3 * a + m])) return fail(e, MFE_E_BAD_PARAM, "rtm not finite");
+  }
+  return guarded(e, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(e->device));
+    void *base = nullptr;
+    std::string err;
+    const int src = scratch_on(e->device, &base, err);
+    if (src) return fail(e, src, err);
+    unsigned char *sp = static_cast<unsigned char *>(base);
+    int *d_ia = reinterpret_cast<int *>(sp + kNaIaOff);
+    double *d_rtm = reinterpret_cast<double *>(sp + kNaRtmOff);
+    double *d_out = reinterpret_cast<double *>(sp + kNaOutOff);
+    double *d_part = reinterpret_cast<double *>(sp + kNaPartOff);
+    MFE_HIP_TRY(hipMemcpyAsync(d_ia, ia, natm * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    MFE_HIP_TRY(hipMemcpyAsync(d_rtm, rtm, 3 * (size_t)natm * sizeof(double),
+                               hipMemcpyHostToDevice, e->stream));
+    sync_view(e);
+    mfe::launch_mc_pair(e->grp, natm, d_ia, d_rtm, d_out, d_part, e->nlkind == MFE_NL_CELL);
+    double h[2];
+    MFE_HIP_TRY(hipMemcpyAsync(h, d_out, sizeof(h), hipMemcpyDeviceToHost, e->stream));
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     *overlap = h[1] != 0.0 ? 1 : 0;
     *du_twob = *overlap ? 0.0 : h[0];

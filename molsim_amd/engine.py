@@ -86,7 +86,8 @@ SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_batch_step_md", "mfe_batch_step_bd", "mfe_batch_get_energies",
            "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint", "mfe_mc_delta_rec",
            "mfe_mc_delta_pair", "mfe_mc_accept", "mfe_ewald_parameters",
-           "mfe_set_sync_limit", "mfe_set_net_charge", "mfe_model_forces", "mfe_fm_set_frames", "mfe_fm_loss_grad"]
+           "mfe_set_sync_limit", "mfe_set_net_charge", "mfe_model_forces", "mfe_fm_set_frames", "mfe_fm_loss_grad",
+           "mfe_mc_delta_pair_noalloc"]
 
 _lib = None
 
