@@ -392,8 +392,18 @@ int mfe_batch_compute(mfe_batch *b, uint32_t flags);
 /* nsteps of VelVer + NList for every replica, one graph launch per step */
 int mfe_batch_step_md(mfe_batch *b, int32_t nsteps, double tstep, uint32_t flags);
 /* nsteps of BDStep + NList with each replica's device noise stream and
-   factors (mfe_set_bd) */
+   factors (mfe_set_bd).  Mixed-precision SPME replicas without chains,
+   surface term or spline knots are advanced in one persistent launch: workgroups take replicas from a
+   queue (costliest first) and run all nsteps of a replica with workgroup
+   barriers only -- every stage of BDStep, UTotal and the gated SetVListMD.
+   A replica's result there depends on that replica alone.  Other batches
+   replay one captured graph per step. */
 int mfe_batch_step_bd(mfe_batch *b, int32_t nsteps, uint32_t flags);
+/* No reference counterpart: whether mfe_batch_step_bd may take the
+   persistent launch (on = 1, the default) or always replays the per-stage
+   graph (on = 0); and whether it would take it for this batch now. */
+int mfe_batch_set_persistent(mfe_batch *b, int32_t on);
+int32_t mfe_batch_uses_persistent(mfe_batch *b);
 /* energies of the last UTotal of every replica: out[nrep] */
 int mfe_batch_get_energies(mfe_batch *b, mfe_energy *out);
 

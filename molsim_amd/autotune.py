@@ -20,7 +20,11 @@ Both steps are measured on the device rather than modelled:
      fine mesh, over a ladder of alpha*h values; (alpha h)* is the largest
      value at which every sampled replica passes;
   2. timing -- the batched BD step of a sample of replicas with the meshes
-     each feasible (alpha, p) implies; the fastest wins.
+     each feasible (alpha, p) implies; the fastest wins;
+  3. the launch schedule -- the chosen batch timed whole in both of the
+     engine's schedules for BD steps: one persistent launch (a workgroup
+     per replica at a time, all stages inside) or one graph launch per
+     stage over all replicas; the faster is kept (Plan.persistent).
 """
 from __future__ import annotations
 
@@ -117,13 +121,16 @@ def calibrate(sample, alpha, order, tol, cache, precision="mixed"):
     return None, errs
 
 
-def time_bd(systems, steps, cache, precision="mixed", fac1=2.0e-7, fac2=0.0316):
-    """Seconds per batched BD step of `systems`."""
+def time_bd(systems, steps, cache, precision="mixed", fac1=2.0e-7, fac2=0.0316,
+            persistent=True):
+    """Seconds per batched BD step of `systems` (persistent: whether the
+    batch may take the one-launch schedule)."""
     engs = [S.configure(s, precision, tables=_tables(s, cache)) for s in systems]
     for r, e in enumerate(engs):
         e.set_bd(np.full(s_npt(systems[r]), fac1), np.full(s_npt(systems[r]), fac2), 77 + r)
     b = E.Batch(engs)
     try:
+        b.set_persistent(persistent)
         b.compute()
         b.step_bd(max(2, steps // 5))
         b.energies()
@@ -153,6 +160,9 @@ class Plan:
     error: float = 0.0   # measured force rel-RMS at the chosen alpha*h (sample max)
     met: bool = True     # False: no candidate reached the tolerance; the
                          # most accurate feasible one was taken
+    persistent: bool = True   # launch schedule: one persistent launch for all
+                              # replicas, else one graph launch per stage
+    schedule_sec: dict = dataclasses.field(default_factory=dict)
 
     def apply(self, systems):
         return [with_params(s, self.alpha, self.order, m) for s, m in zip(systems, self.meshes)]
@@ -199,4 +209,13 @@ def autotune(systems, tol=1e-4, alphas=ALPHAS, orders=ORDERS, n_sample=4, n_time
         best = Plan(a, p, ah, meshes, min(3.0 / a, 0.45 * min(s.boxlen for s in systems)),
                     time_bd(tm, time_steps, cache, precision), table, error=e, met=False)
     best.table = table
+    # the launch schedule, timed on the whole batch: the persistent launch
+    # (a workgroup per replica at a time) against the per-stage graph
+    if precision == "mixed" and len(systems) > 1:
+        full = best.apply(systems)
+        steps = max(3, time_steps // 4)
+        tp = time_bd(full, steps, cache, precision, persistent=True)
+        ts = time_bd(full, steps, cache, precision, persistent=False)
+        best.persistent = tp <= ts
+        best.schedule_sec = {"persistent": tp, "per_stage": ts}
     return best

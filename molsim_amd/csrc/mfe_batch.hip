@@ -31,6 +31,9 @@ struct mfe_batch {
   int *h_flags = nullptr;
   double *h_en = nullptr;
   int failed = -1;
+  // persistent BD: replica order (costliest first) and queue counter
+  bool persist = true;
+  int *d_order = nullptr, *d_queue = nullptr;
 };
 
 namespace {
@@ -88,6 +91,10 @@ int refresh(mfe_batch *b) {
   if (lds > 32 * 1024) mfe_allow_big_spread_lds(b->grp.prec, (int)lds);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   b->graph = nullptr;
+  // the persistent launch's order follows the replicas' sizes
+  if (b->d_order) (void)hipFree(b->d_order);
+  if (b->d_queue) (void)hipFree(b->d_queue);
+  b->d_order = b->d_queue = nullptr;
   return MFE_OK;
 }
 
@@ -188,6 +195,8 @@ int mfe_batch_destroy(mfe_batch *b) {
   if (b->d_sys) (void)hipFree(b->d_sys);
   if (b->d_flags) (void)hipFree(b->d_flags);
   if (b->d_en) (void)hipFree(b->d_en);
+  if (b->d_order) (void)hipFree(b->d_order);
+  if (b->d_queue) (void)hipFree(b->d_queue);
   if (b->h_flags) (void)hipHostFree(b->h_flags);
   if (b->h_en) (void)hipHostFree(b->h_en);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -217,6 +226,59 @@ int mfe_batch_compute(mfe_batch *b, uint32_t flags) {
     return check(b);
   });
 }
+
+// The persistent BD launch takes replicas on the mixed path with SPME and
+// no chains, surface term or spline knots, whose LDS needs fit a CU.
+namespace {
+
+bool persist_eligible(const mfe_batch *b, size_t *lds) {
+  if (!b->persist) return false;
+  const mfe::Group &g = b->grp;
+  if (g.prec != MFE_PREC_MIXED || g.ewald != MFE_EWALD_SPME || g.chains || g.surf ||
+      g.knots || g.order < 3 || g.order > 8)
+    return false;
+  // the pair tables, and one x row, y plane and z line of every replica's
+  // mesh with its twiddles, fit the launch's LDS
+  const size_t L = mfe::kPersistLds, c = sizeof(mfe::cplx<float>), t = sizeof(mfe::Twiddle<float>);
+  if (g.lds_pair > L) return false;
+  for (const mfe_engine *e : b->eng) {
+    const size_t s1 = e->s[0], s2 = e->s[1], s3 = e->s[2], hx = s1 / 2 + 1;
+    if (s1 * s2 * sizeof(unsigned long long) > L) return false;   // a spreading slab
+    if (2 * hx * c + (s1 / 2 + s1) * t > L) return false;
+    if (2 * hx * (s2 + 1) * c + s2 * t > L) return false;
+    if (2 * hx * (s3 + 1) * c + s3 * t > L) return false;
+  }
+  *lds = L;
+  return mfe::persist_allow_lds(g.order, L);
+}
+
+int run_persistent(mfe_batch *b, int32_t nsteps, uint32_t flags, size_t lds) {
+  const int R = (int)b->eng.size();
+  if (!b->d_order) {
+    // costliest replicas first, so the queue's tail holds the small ones
+    std::vector<std::pair<double, int>> cost(R);
+    for (int r = 0; r < R; ++r) {
+      const mfe_engine *e = b->eng[r];
+      const double m = (double)e->s[0] * e->s[1] * e->s[2];
+      cost[r] = {-(8.0 * e->np + m), r};
+    }
+    std::stable_sort(cost.begin(), cost.end());
+    std::vector<int> ord(R);
+    for (int r = 0; r < R; ++r) ord[r] = cost[r].second;
+    MFE_HIP_TRY(hipMalloc((void **)&b->d_order, R * sizeof(int)));
+    MFE_HIP_TRY(hipMalloc((void **)&b->d_queue, sizeof(int)));
+    MFE_HIP_TRY(hipMemcpyAsync(b->d_order, ord.data(), R * sizeof(int), hipMemcpyHostToDevice,
+                               b->stream));
+  }
+  const int ncu = std::max(b->eng[0]->ncu, 1);
+  const int blocks = std::min(R, ncu);
+  const int energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) != 0 ? 1 : 0;
+  mfe::launch_bd_persistent(b->grp, b->d_order, R, nsteps, energy, b->d_queue, blocks, lds);
+  MFE_HIP_TRY(hipGetLastError());
+  return check(b);
+}
+
+}  // namespace
 
 static int run_graph(mfe_batch *b, int kind, int32_t nsteps, double tstep, uint32_t flags) {
   if (!b->graph || b->graph_kind != kind || b->graph_flags != flags ||
@@ -257,8 +319,30 @@ int mfe_batch_step_bd(mfe_batch *b, int32_t nsteps, uint32_t flags) {
       if (!b->eng[r]->have_bd) return bfail(b, MFE_E_STATE, "BD factors not set (mfe_set_bd)", (int)r);
     int rc = prepare(b);
     if (rc) return rc;
+    size_t lds = 0;
+    if (nsteps > 0 && persist_eligible(b, &lds)) return run_persistent(b, nsteps, flags, lds);
     return run_graph(b, 2, nsteps, 0.0, flags);
   });
+}
+
+int mfe_batch_set_persistent(mfe_batch *b, int32_t on) {
+  if (!b) return MFE_E_BAD_PARAM;
+  b->persist = on != 0;
+  return MFE_OK;
+}
+
+int32_t mfe_batch_uses_persistent(mfe_batch *b) {
+  if (!b) return 0;
+  size_t lds = 0;
+  int32_t yes = 0;
+  const int rc = bguarded(b, [&]() -> int {
+    MFE_HIP_TRY(hipSetDevice(b->device));
+    int rc2 = refresh(b);
+    if (rc2) return rc2;
+    yes = persist_eligible(b, &lds) ? 1 : 0;
+    return MFE_OK;
+  });
+  return rc == MFE_OK ? yes : 0;
 }
 
 int mfe_batch_get_energies(mfe_batch *b, mfe_energy *out) {

@@ -590,6 +590,15 @@ void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, 
 void launch_skin(const Group &g);
 void launch_set_positions(const Group &g, const double *ro_new);
 void launch_collect(const Group &g, int *flags, double *en, int nen);
+// persistent batched BD (mfe_kernels.hip): nsteps BDSteps of every replica
+// in one launch, a workgroup per replica at a time, replicas taken from a
+// queue in the given order
+constexpr int kPersistThreads = 1024;
+// dynamic LDS of the persistent launch: tables and FFT tiles
+constexpr size_t kPersistLds = 144 * 1024;
+void launch_bd_persistent(const Group &g, const int *order, int nrep, int nsteps, int energy,
+                          int *queue, int blocks, size_t lds);
+bool persist_allow_lds(int order, size_t bytes);
 void launch_fill_int(int *p, int n, int v, hipStream_t s);
 void launch_fill_double(double *p, int n, double v, hipStream_t s);
 

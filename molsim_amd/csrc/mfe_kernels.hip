@@ -17,6 +17,7 @@
 #include <type_traits>
 
 #include "mfe_device.h"
+#include "mfe_fft_device.h"
 #include "mfe_internal.h"
 
 namespace mfe {
@@ -742,17 +743,90 @@ k_pair(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
   }
 }
 
+// The mixed path's tables as a pair evaluation sees them (in LDS, or the
+// coefficients in global memory when they do not fit): bins, FP32 interval
+// bounds, coefficient records, per-pair records, and the list semantics.
+struct PairTabF {
+  const unsigned short *bin;
+  const float *zl;
+  const float *cf;
+  const PairRecF *rec;
+  int nbin, semantics;
+  float rc2;
+};
+
+// One FP32 pair term at separation (dx, dy, dz), pair type p: false when
+// the pair is outside the cut-off or an error (overlap below r2umin, r2
+// below the table: bits added to err); else the force factor fs (force =
+// fs * d), the energy usum (FP64; with energy) and r2.  The interval is
+// chosen on FP32 bounds and the polynomials are evaluated in d = r2 -
+// (float)zlow; the FP32 coefficients were re-expanded about the rounded
+// bound, so that rounding shifts no interval's argument.  The interval
+// constant w0 enters in FP64 (its FP32 value plus the rounding kept in
+// slot 11).  KN: the additive spline term (when the replica carries one).
+template <bool KN>
+__device__ __forceinline__ bool pair_eval_f(const SysDev &V, const PairTabF &T, float dx, float dy,
+                                            float dz, int p, int energy, float &fs,
+                                            double &usum, float &r2, int &err) {
+  r2 = dx * dx + dy * dy + dz * dz;
+  const bool in = T.semantics == 0 ? !(r2 > T.rc2) : (r2 < T.rc2);
+  const PairRecF R = T.rec[p];
+  const float r2c = fmaxf(r2, R.lo);
+  const float sf = __builtin_amdgcn_rcpf(r2c);
+  const float xb = fminf(fmaxf((sf - R.s_lo) * R.s_inv, 0.0f), (float)(T.nbin - 1));
+  const int bk = T.bin[p * T.nbin + (int)xb];
+  const int o = R.off, n = R.nint, kb = bk & (kSlowBin - 1);
+  const float z0 = T.zl[o + kb], z1 = T.zl[o + min(kb + 1, n - 1)];
+  int iv;
+  float zv;
+  if (bk & kSlowBin) {
+    auto zl = [&](int k) -> float { return T.zl[o + k]; };
+    iv = find_interval(zl, n, r2c, kb);
+    zv = T.zl[o + iv];
+  } else {
+    const bool at = r2c >= z0;
+    iv = at ? kb : min(kb + 1, n - 1);
+    zv = at ? z0 : z1;
+  }
+  const float4 *q = reinterpret_cast<const float4 *>(T.cf) + 3 * (o + iv);
+  const float4 c0 = q[0], c1 = q[1], c2 = q[2];
+  const bool hc = r2 < R.r2at;
+  const bool ov = !hc && r2 < R.r2um;
+  const bool te = !hc && !ov && !(r2 >= R.lo);
+  if (in) err |= (ov ? ERR_OVERLAP : 0) | (te ? ERR_TABLE : 0);
+  if (!in || ov || te) return false;
+  const float df = r2 - zv;
+  // c0 = (w0, w1, w2, w3), c1 = (w4, w5, f0, f1), c2 = (f2, f3, f4, w0 rounding)
+  fs = c1.z + df * (c1.w + df * (c2.x + df * (c2.y + df * c2.z)));
+  usum = 0.0;
+  if (energy) {
+    const float up = df * (c0.y + df * (c0.z + df * (c0.w + df * (c1.x + df * c1.y))));
+    usum = ((double)c0.x + (double)c2.w) + (double)up;
+  }
+  if (hc) {
+    usum = 1e10;
+    fs = 1e10f;
+  } else if (KN && V.nknot) {
+    const double r = sqrt((double)r2);
+    int ki;
+    double w[4], dw[4];
+    if (knot_weights(r, V.knot_r0, V.knot_dinv, V.nknot, ki, w, dw)) {
+      const double *kc = V.knot_c + (size_t)p * V.nknot + ki;
+      usum += kc[0] * w[0] + kc[1] * w[1] + kc[2] * w[2] + kc[3] * w[3];
+      fs -= (float)((kc[0] * dw[0] + kc[1] * dw[1] + kc[2] * dw[2] + kc[3] * dw[3]) *
+                    V.knot_dinv / r);
+    }
+  }
+  return true;
+}
+
 // The mixed path's pair forces: the list walk, partition and reductions of
-// k_pair, in FP32.  Positions come from pos_f (relative to the particle's
-// cell reference point, 16 bytes with the type); a list entry's code names
-// the pair's cell offset, kept in LDS per code, so dx = x_i - x_j + offset
-// is formed in FP32 from values of at most a cell edge or so.  The interval
-// is chosen on FP32 bounds and the polynomials are evaluated in
-// d = r2 - (float)zlow; the FP32 coefficients were re-expanded about the
-// rounded bound, so that rounding shifts no interval's argument.  The
-// interval constant w0 enters in FP64 (its FP32 value plus the rounding
-// kept in slot 11); forces are Kahan-compensated FP32 sums, energies FP64
-// per partner type, flushed into the wavefront's LDS row per chunk.  KN:
+// k_pair, in FP32 (pair_eval_f).  Positions come from pos_f (relative to
+// the particle's cell reference point, 16 bytes with the type); a list
+// entry's code names the pair's cell offset, kept in LDS per code, so
+// dx = x_i - x_j + offset is formed in FP32 from values of at most a cell
+// edge or so.  Forces are Kahan-compensated FP32 sums, energies FP64 per
+// partner type, flushed into the wavefront's LDS row per chunk.  KN:
 // some replica carries the additive spline term.  V.pair_lanes lanes share
 // a particle's list: 4 for dense systems, fewer where lists are short (a
 // lane then keeps its batch slots busy and the per-particle work is not
@@ -824,60 +898,13 @@ k_pair_f(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
     if (fabsf(dy) > H1) dy -= copysignf(L1, dy);
     if (fabsf(dz) > H2) dz -= copysignf(L2, dz);
   };
+  const PairTabF T{bin_s, zl_s, CF ? cf_s : (const float *)t.coef_f, rec_s, nbin, semantics, rc2};
   // one list entry: a partner of type tj at separation (dx, dy, dz)
   auto eval = [&](float dx, float dy, float dz, int tj) {
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const bool in = semantics == 0 ? !(r2 > rc2) : (r2 < rc2);
     const int p = (int)((ptp >> (8 * tj)) & 0xffu);
-    const PairRecF R = rec_s[p];
-    const float r2c = fmaxf(r2, R.lo);
-    const float sf = __builtin_amdgcn_rcpf(r2c);
-    const float xb = fminf(fmaxf((sf - R.s_lo) * R.s_inv, 0.0f), (float)(nbin - 1));
-    const int bk = bin_s[p * nbin + (int)xb];
-    const int o = R.off, n = R.nint, kb = bk & (kSlowBin - 1);
-    const float z0 = zl_s[o + kb], z1 = zl_s[o + min(kb + 1, n - 1)];
-    int iv;
-    float zv;
-    if (bk & kSlowBin) {
-      auto zl = [&](int k) -> float { return zl_s[o + k]; };
-      iv = find_interval(zl, n, r2c, kb);
-      zv = zl_s[o + iv];
-    } else {
-      const bool at = r2c >= z0;
-      iv = at ? kb : min(kb + 1, n - 1);
-      zv = at ? z0 : z1;
-    }
-    const float4 *q = CF ? reinterpret_cast<const float4 *>(cf_s) + 3 * (o + iv)
-                         : reinterpret_cast<const float4 *>((const float *)t.coef_f) +
-                               3 * (o + iv);
-    const float4 c0 = q[0], c1 = q[1], c2 = q[2];
-    const bool hc = r2 < R.r2at;
-    const bool ov = !hc && r2 < R.r2um;
-    const bool te = !hc && !ov && !(r2 >= R.lo);
-    if (in) err |= (ov ? ERR_OVERLAP : 0) | (te ? ERR_TABLE : 0);
-    if (!in || ov || te) return;
-    const float df = r2 - zv;
-    // c0 = (w0, w1, w2, w3), c1 = (w4, w5, f0, f1), c2 = (f2, f3, f4, w0 rounding)
-    float fs = c1.z + df * (c1.w + df * (c2.x + df * (c2.y + df * c2.z)));
-    double usum = 0.0;
-    if (energy) {
-      const float up = df * (c0.y + df * (c0.z + df * (c0.w + df * (c1.x + df * c1.y))));
-      usum = ((double)c0.x + (double)c2.w) + (double)up;
-    }
-    if (hc) {
-      usum = 1e10;
-      fs = 1e10f;
-    } else if (KN && V.nknot) {
-      const double r = sqrt((double)r2);
-      int ki;
-      double w[4], dw[4];
-      if (knot_weights(r, V.knot_r0, V.knot_dinv, V.nknot, ki, w, dw)) {
-        const double *kc = V.knot_c + (size_t)p * V.nknot + ki;
-        usum += kc[0] * w[0] + kc[1] * w[1] + kc[2] * w[2] + kc[3] * w[3];
-        fs -= (float)((kc[0] * dw[0] + kc[1] * dw[1] + kc[2] * dw[2] + kc[3] * dw[3]) *
-                      V.knot_dinv / r);
-      }
-    }
+    float fs, r2;
+    double usum;
+    if (!pair_eval_f<KN>(V, T, dx, dy, dz, p, energy, fs, usum, r2, err)) return;
     fxf.add(fs * dx);
     fyf.add(fs * dy);
     fzf.add(fs * dz);
@@ -2367,6 +2394,575 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_finalize(const SysDev
   }
 }
 
+// ------------------------------------------------- persistent batched BD
+// Many small replicas advanced by BDStep in one launch: a workgroup takes
+// a replica from a queue (the costliest first), runs all nsteps steps of
+// it -- move with the counter-based noise, NList's skin test, FP32 pair
+// forces, SPME (fixed-point spreading into the replica's own mesh, the five
+// FFT passes tile by tile, Green function, gather), the finalize and the
+// reductions, the gated list rebuild -- with workgroup barriers only, then
+// takes the next replica.  The arithmetic is the mixed path's: the same
+// pair evaluation (pair_eval_f), B-splines, FFT passes and reductions in
+// FP64.  Each replica's result depends on that replica alone (integer
+// spreading sums, fixed summation orders), so a batch computes exactly
+// what the same kernel computes for each replica on its own.
+// Eligible replicas: mixed precision, SPME, no chains, surface term or
+// spline knots.
+
+struct PersistArgs {
+  const SysDev *sys;
+  const int *order;                   // replica indices, costliest first
+  int nrep, nsteps, energy;
+  int *queue;                         // dequeue counter, zero at the launch
+  int lds;                            // dynamic LDS bytes (tables, slabs, FFT tiles)
+};
+
+// inclusive prefix maximum over the block of one value per thread (wave
+// shuffles, then the earlier waves' maxima from LDS); tot gets the block's
+// maximum.  wm: one slot per wave.
+__device__ __forceinline__ double block_prefix_max(double v, double *wm, double &excl,
+                                                   double &tot) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double inc = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const double y = __shfl_up(inc, off, 64);
+    if (lane >= off) inc = fmax(inc, y);
+  }
+  double e = __shfl_up(inc, 1, 64);
+  if (lane == 0) e = 0.0;
+  if (lane == 63) wm[wave] = inc;
+  __syncthreads();
+  double pw = 0.0, t = 0.0;
+  for (int k = 0; k < nw; ++k) {
+    if (k < wave) pw = fmax(pw, wm[k]);
+    t = fmax(t, wm[k]);
+  }
+  __syncthreads();
+  excl = fmax(pw, e);
+  tot = t;
+  return inc;
+}
+
+// exclusive prefix sum of cnt[0..n) into start[0..n], start[n] = total
+__device__ __forceinline__ void block_scan_counts(const int *cnt, int *start, int n, int *wsum) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int carry = 0;
+  for (int base = 0; base < n; base += blockDim.x) {
+    const int k = base + threadIdx.x;
+    const int v = k < n ? cnt[k] : 0;
+    int inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int pw = 0, t = 0;
+    for (int w = 0; w < nw; ++w) {
+      if (w < wave) pw += wsum[w];
+      t += wsum[w];
+    }
+    if (k < n) start[k] = carry + pw + inc - v;
+    carry += t;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) start[n] = carry;
+}
+
+// BDStep for particle i at step n (k_move's arithmetic, device noise):
+// position, its cell-ordered copies, NList's displacement sum; returns
+// |sum|^2 (also stored in r2disp)
+__device__ __forceinline__ double bd_move_particle(const SysDev &V, int i, unsigned n) {
+#pragma clang fp contract(off)
+  const Box b = V.box;
+  double x[3], d[3], sd[3], gn[3];
+  const int cp = V.c_pos[i];
+  const int cell = V.c_of[i], ty = V.type[i];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    x[m] = V.ro[3 * i + m];
+    sd[m] = V.drosum[3 * i + m];
+  }
+  gauss3(V.seed, (unsigned)i, n, gn);
+  const double f1 = V.bd_fac1[ty], f2 = V.bd_fac2[ty];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) d[m] = f1 * V.forceo[3 * i + m] + f2 * gn[m];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    double xm = x[m] + d[m];
+    if (fabs(xm) > b.L2[m]) xm = xm - b.L[m] * round(xm * b.Li[m]);
+    x[m] = xm;
+    sd[m] = sd[m] + d[m];
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    V.ro[3 * i + m] = x[m];
+    V.ro_s[3 * cp + m] = x[m];
+    V.drosum[3 * i + m] = sd[m];
+  }
+  V.pos_f[cp] = rel_pos(V, x, cell, ty);
+  double r2 = sd[0] * sd[0];
+  r2 = r2 + sd[1] * sd[1];
+  r2 = r2 + sd[2] * sd[2];
+  V.r2disp[i] = r2;
+  return r2;
+}
+
+// The stages of the persistent step, each a function of its own (not
+// inlined) so that register allocation works stage by stage.
+#define MFE_STAGE __device__ __attribute__((noinline))
+
+// BDStep for the replica's particles, then NList's skin test
+// (Src/nlist.F90:597-621): the largest displacement and the largest one
+// that was not a running maximum, in particle order.  Returns the rebuild
+// decision (every thread).
+MFE_STAGE int p_move_skin(const SysDev &V, unsigned stepctr, double *wm, int *flag_s) {
+  const int tid = threadIdx.x, NT = blockDim.x, np = V.np;
+  double run = 0.0, cand = 0.0;
+  for (int base = 0; base < np; base += NT) {
+    const int i = base + tid;
+    const double v = i < np ? bd_move_particle(V, i, stepctr) : 0.0;
+    double excl, tot;
+    block_prefix_max(v, wm, excl, tot);
+    const double before = fmax(run, excl);
+    if (i < np && !(v > before)) cand = fmax(cand, v);
+    run = fmax(run, tot);
+  }
+  const double c = block_max(cand, wm);
+  if (tid == 0) *flag_s = (sqrt(run) + sqrt(c) > V.drnlist) ? 1 : 0;
+  __syncthreads();
+  return *flag_s;
+}
+
+// FP32 pair forces in cell order, one lane per particle (pair_eval_f);
+// per-wavefront pair-type energies in wred[wave][0..P), virial in [P].
+// NTY bounds the number of particle types (energies per partner type).
+template <int NTY>
+MFE_STAGE void p_pair(const SysDev &V, int energy, char *smem, double (*wred)[40],
+                      PairRecF *rec_s, int *pidx_s, float4 *off_s) {
+  const int tid = threadIdx.x, NT = blockDim.x, np = V.np;
+  const int lane = tid & 63, wave = tid >> 6;
+  const TableDev t = V.tab;
+  const int P = t.nptpt, npt = t.npt, nbin = t.nbin, nzl = t.nzl;
+  const size_t bb = pair_bin_bytes(P, nbin);
+  unsigned short *bin_s = reinterpret_cast<unsigned short *>(smem);
+  float *zl_s = reinterpret_cast<float *>(smem + bb);
+  float *cf_s = reinterpret_cast<float *>(smem + bb + pair_zl_bytes(nzl));
+  const bool cf_lds = t.lds_mode >= 2;
+  for (int k = tid; k < P * nbin; k += NT) bin_s[k] = t.bin_k[k];
+  for (int k = tid; k < nzl; k += NT) zl_s[k] = t.zlow_f[k];
+  if (cf_lds)
+    for (int k = tid; k < nzl * kCoefStride; k += NT) cf_s[k] = t.coef_f[k];
+  for (int k = tid; k < P; k += NT) rec_s[k] = t.rec_f[k];
+  for (int k = tid; k < npt * npt; k += NT) pidx_s[k] = t.pairidx[k];
+  for (int k = tid; k < 125; k += NT)
+    off_s[k] = make_float4(V.cg.off[0][k % 5], V.cg.off[1][(k / 5) % 5], V.cg.off[2][k / 25], 0.f);
+  for (int k = lane; k <= P; k += 64) wred[wave][k] = 0.0;
+  __syncthreads();
+  const PairTabF T{bin_s, zl_s, cf_lds ? cf_s : (const float *)t.coef_f, rec_s, nbin, V.nlsem,
+                   (float)t.rcut2};
+  const float L0 = V.cg.L[0], L1 = V.cg.L[1], L2 = V.cg.L[2];
+  const float H0 = V.cg.L2[0], H1 = V.cg.L2[1], H2 = V.cg.L2[2];
+  const unsigned *__restrict__ nbr = reinterpret_cast<const unsigned *>((const int *)V.nbr);
+  const int maxrow = V.maxnn - 1;
+  const bool pos64 = V.pair_pos64 != 0;
+  double vir = 0.0;
+  int err = 0;
+  for (int base = 0; base < np; base += NT) {
+    const int tt = base + tid;
+    double ej[NTY];
+#pragma unroll
+    for (int k = 0; k < NTY; ++k) ej[k] = 0.0;
+    int ti = 0;
+    if (tt < np) {
+      // boxes with an axis of fewer than three list cells: separations in
+      // FP64 from the cell-ordered positions (as k_pair_f's POS64 variant)
+      const float4 pi = pos64 ? make_float4(0.f, 0.f, 0.f, __int_as_float(V.type_s[tt]))
+                              : V.pos_f[tt];
+      const double xi = pos64 ? V.ro_s[3 * tt] : 0.0, yi = pos64 ? V.ro_s[3 * tt + 1] : 0.0,
+                   zi = pos64 ? V.ro_s[3 * tt + 2] : 0.0;
+      const int n = V.nn[tt];
+      ti = __float_as_int(pi.w);
+      unsigned long long ptp = 0;
+      for (int u = 0; u < npt; ++u) ptp |= (unsigned long long)pidx_s[ti * npt + u] << (8 * u);
+      KahanF fxf, fyf, fzf;
+      // the indices of the next batch are loaded while this one is
+      // evaluated (one memory round trip per batch on the critical path)
+      unsigned en[kPairBatch];
+#pragma unroll
+      for (int q = 0; q < kPairBatch; ++q) en[q] = nbr[(size_t)min(q, maxrow) * np + tt];
+      for (int k0 = 0; k0 < n; k0 += kPairBatch) {
+        unsigned es[kPairBatch];
+        float4 ps[kPairBatch];
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) es[q] = en[q];
+        if (pos64) {
+#pragma unroll
+          for (int q = 0; q < kPairBatch; ++q) {
+            const int j = k0 + q < n ? (int)(es[q] & kNbrIdxMask) : tt;
+            double ddx = xi - V.ro_s[3 * j], ddy = yi - V.ro_s[3 * j + 1],
+                   ddz = zi - V.ro_s[3 * j + 2];
+            pbcr2(ddx, ddy, ddz, V.box);
+            ps[q] = make_float4((float)ddx, (float)ddy, (float)ddz, __int_as_float(V.type_s[j]));
+          }
+        } else {
+#pragma unroll
+          for (int q = 0; q < kPairBatch; ++q)
+            ps[q] = V.pos_f[k0 + q < n ? (int)(es[q] & kNbrIdxMask) : tt];
+        }
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q)
+          en[q] = nbr[(size_t)min(k0 + kPairBatch + q, maxrow) * np + tt];
+#pragma unroll
+        for (int q = 0; q < kPairBatch; ++q) {
+          if (k0 + q >= n) break;
+          float dx, dy, dz;
+          if (pos64) {
+            dx = ps[q].x;
+            dy = ps[q].y;
+            dz = ps[q].z;
+          } else {
+            const float4 o4 = off_s[es[q] >> kNbrCodeShift];
+            dx = (pi.x - ps[q].x) + o4.x;
+            dy = (pi.y - ps[q].y) + o4.y;
+            dz = (pi.z - ps[q].z) + o4.z;
+            if (fabsf(dx) > H0) dx -= copysignf(L0, dx);
+            if (fabsf(dy) > H1) dy -= copysignf(L1, dy);
+            if (fabsf(dz) > H2) dz -= copysignf(L2, dz);
+          }
+          const int tj = __float_as_int(ps[q].w);
+          const int p = (int)((ptp >> (8 * tj)) & 0xffu);
+          float fs, r2;
+          double usum;
+          if (!pair_eval_f<false>(V, T, dx, dy, dz, p, energy, fs, usum, r2, err)) continue;
+          fxf.add(fs * dx);
+          fyf.add(fs * dy);
+          fzf.add(fs * dz);
+          if (energy) {
+#pragma unroll
+            for (int k2 = 0; k2 < NTY; ++k2) ej[k2] += (k2 == tj) ? 0.5 * usum : 0.0;
+            vir -= 0.5 * (double)(fs * r2);
+          }
+        }
+      }
+      const int io = V.c_sorted[tt];
+      V.force[3 * io] = fxf.value();
+      V.force[3 * io + 1] = fyf.value();
+      V.force[3 * io + 2] = fzf.value();
+    }
+    if (energy) {
+      for (int k = 0; k < P; ++k) {
+        double v = 0.0;
+        if (tt < np) {
+#pragma unroll
+          for (int u = 0; u < NTY; ++u)
+            if (u < npt && pidx_s[ti * npt + u] == k) v += ej[u];
+        }
+        for (int w = 32; w > 0; w >>= 1) v += __shfl_down(v, w, 64);
+        if (lane == 0) wred[wave][k] += v;
+      }
+    }
+  }
+  if (err) atomicOr(&V.flags[0], err);
+  for (int w = 32; w > 0; w >>= 1) vir += __shfl_down(vir, w, 64);
+  if (lane == 0) wred[wave][P] = vir;
+  __syncthreads();
+}
+
+// spreading, slab by slab of z planes in LDS: each slab's points are
+// accumulated in exact fixed point (LDS 64-bit adds) from every particle
+// whose stencil reaches it -- the points m, m-1, ..., m-order+1 from
+// meshmax m per axis (Src/energy.F90:1223-1246), each product q wz wy wx
+// in FP32 as k_spread forms it -- and then stored, rounded to FP32, into
+// the real mesh (the x pass reads it there)
+template <int ORDER>
+MFE_STAGE void p_spread(const SysDev &V, char *smem, int lds) {
+  typedef SpreadAcc<float>::T Acc;
+  const int tid = threadIdx.x, NT = blockDim.x, np = V.np;
+  const SpmeGeom g = V.sg;
+  const int s1 = g.s[0], s2 = g.s[1], s3 = g.s[2];
+  const int plane = s1 * s2;
+  const int nzs = max(1, min(s3, lds / (plane * (int)sizeof(Acc))));
+  Acc *slab = reinterpret_cast<Acc *>(smem);
+  float *__restrict__ mesh = (float *)V.mesh_r;
+  for (int z0 = 0; z0 < s3; z0 += nzs) {
+    const int nz = min(nzs, s3 - z0);
+    for (int k = tid; k < nz * plane; k += NT) slab[k] = Acc(0);
+    __syncthreads();
+    for (int i = tid; i < np; i += NT) {
+      int mm[3];
+      double fr[3];
+      mesh_coords(V.ro + 3 * i, g, mm, fr);
+      // stencil planes mm2, mm2-1, ... (mod s3) inside the slab
+      bool any = false;
+#pragma unroll
+      for (int iz = 0; iz < ORDER; ++iz) {
+        const int z = mm[2] - iz < 0 ? mm[2] - iz + s3 : mm[2] - iz;
+        any |= (unsigned)(z - z0) < (unsigned)nz;
+      }
+      if (!any) continue;
+      float wx[ORDER], wy[ORDER], wz[ORDER];
+      bspline<float, ORDER>((float)fr[0], wx, nullptr);
+      bspline<float, ORDER>((float)fr[1], wy, nullptr);
+      bspline<float, ORDER>((float)fr[2], wz, nullptr);
+      const float q = (float)V.q[i];
+#pragma unroll
+      for (int iz = 0; iz < ORDER; ++iz) {
+        const int z = mm[2] - iz < 0 ? mm[2] - iz + s3 : mm[2] - iz;
+        if ((unsigned)(z - z0) >= (unsigned)nz) continue;
+        const float qz = q * wz[iz];
+#pragma unroll
+        for (int iy = 0; iy < ORDER; ++iy) {
+          const int y = mm[1] - iy < 0 ? mm[1] - iy + s2 : mm[1] - iy;
+          const float qyz = qz * wy[iy];
+          Acc *row = slab + ((z - z0) * s2 + y) * s1;
+#pragma unroll
+          for (int ix = 0; ix < ORDER; ++ix) {
+            const int x = mm[0] - ix < 0 ? mm[0] - ix + s1 : mm[0] - ix;
+            SpreadAcc<float>::add(&row[x], qyz * wx[ix]);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    float *out = mesh + (size_t)z0 * plane;
+    for (int k = tid; k < nz * plane; k += NT) out[k] = SpreadAcc<float>::get(slab[k]);
+    __syncthreads();
+  }
+}
+
+// the five FFT passes of the replica's mesh (one function per pass), each
+// over as many rows, planes or lines as the lds bytes of LDS hold; the
+// convolution's energy and virial terms of this thread into esum, vsum
+constexpr int kCplxF = (int)sizeof(cplx<float>), kTwF = (int)sizeof(Twiddle<float>);
+__device__ __forceinline__ int persist_rows(const SysDev &V, int lds) {
+  const int m = V.sg.s[0] / 2;
+  return max(1, (lds - (m + V.sg.s[0]) * kTwF) / (2 * (m + 1) * kCplxF));
+}
+__device__ __forceinline__ int persist_planes(const SysDev &V, int lds) {
+  return max(1, (lds - V.sg.s[1] * kTwF) / (2 * V.hx * (V.sg.s[1] + 1) * kCplxF));
+}
+__device__ __forceinline__ int persist_lines(const SysDev &V, int lds) {
+  return max(1, (lds - V.sg.s[2] * kTwF) / (2 * V.hx * (V.sg.s[2] + 1) * kCplxF));
+}
+MFE_STAGE void p_fftx_fwd(const SysDev &V, char *smem, int lds) {
+  const int s1 = V.sg.s[0], s2 = V.sg.s[1], nrows = V.sg.s[1] * V.sg.s[2];
+  const int cap = persist_rows(V, lds);
+  const float *__restrict__ mesh = (const float *)V.mesh_r;
+  for (int row0 = 0; row0 < nrows; row0 += cap) {
+    fftx_fwd_rows<float>(V, row0, min(cap, nrows - row0), cap, smem, [&](int x, int y, int z) {
+      return mesh[((size_t)z * s2 + y) * s1 + x];
+    });
+    __syncthreads();
+  }
+}
+MFE_STAGE void p_ffty(const SysDev &V, int inv, char *smem, int lds) {
+  const int s3 = V.sg.s[2], cap = persist_planes(V, lds);
+  for (int z0 = 0; z0 < s3; z0 += cap) {
+    ffty_planes<float>(V, z0, min(cap, s3 - z0), inv, smem);
+    __syncthreads();
+  }
+}
+MFE_STAGE void p_fftconv(const SysDev &V, int energy, char *smem, int lds, double *esum,
+                         double *vsum) {
+  const int s2 = V.sg.s[1], cap = persist_lines(V, lds);
+  double e = 0.0, v = 0.0;
+  for (int y0 = 0; y0 < s2; y0 += cap) {
+    fftconv_lines<float>(V, y0, min(cap, s2 - y0), energy, smem, e, v);
+    __syncthreads();
+  }
+  *esum = e;
+  *vsum = v;
+}
+MFE_STAGE void p_fftx_inv(const SysDev &V, char *smem, int lds) {
+  const int nrows = V.sg.s[1] * V.sg.s[2], cap = persist_rows(V, lds);
+  for (int row0 = 0; row0 < nrows; row0 += cap) {
+    fftx_inv_rows<float>(V, row0, min(cap, nrows - row0), cap, smem);
+    __syncthreads();
+  }
+}
+
+// gather (CalcProp, Src/energy.F90:1259-1285) in particle order, then
+// BDStep's forceo (fin_apply, UTotal's scaling)
+template <int ORDER>
+MFE_STAGE void p_gather(const SysDev &V, char *smem, int lds) {
+  const int tid = threadIdx.x, NT = blockDim.x, np = V.np;
+  const SpmeGeom g = V.sg;
+  const int s1 = g.s[0], s2 = g.s[1], s3 = g.s[2];
+  // particle i's field from the convolved mesh read through rd
+  auto one = [&](int i, auto rd) {
+    int mm[3];
+    double fr[3];
+    mesh_coords(V.ro + 3 * i, g, mm, fr);
+    float v[3][ORDER], dv[3][ORDER];
+    bspline<float, ORDER>((float)fr[0], v[0], dv[0]);
+    bspline<float, ORDER>((float)fr[1], v[1], dv[1]);
+    bspline<float, ORDER>((float)fr[2], v[2], dv[2]);
+    Acc3<float> ac;
+#pragma unroll
+    for (int iz = 0; iz < ORDER; ++iz) {
+      const int z = mm[2] - iz < 0 ? mm[2] - iz + s3 : mm[2] - iz;
+      const float vz = v[2][iz], dz = dv[2][iz];
+#pragma unroll
+      for (int iy = 0; iy < ORDER; ++iy) {
+        const int y = mm[1] - iy < 0 ? mm[1] - iy + s2 : mm[1] - iy;
+        const int row = (z * s2 + y) * s1;
+        float sx = 0.f, sdx = 0.f;
+#pragma unroll
+        for (int ix = 0; ix < ORDER; ++ix) {
+          const int x = mm[0] - ix < 0 ? mm[0] - ix + s1 : mm[0] - ix;
+          const float qv = rd(row + x);
+          sx += v[0][ix] * qv;
+          sdx += dv[0][ix] * qv;
+        }
+        ac.add(0, sdx * (v[1][iy] * vz));
+        ac.add(1, sx * (dv[1][iy] * vz));
+        ac.add(2, sx * (v[1][iy] * dz));
+      }
+    }
+    const double qd = (double)(float)V.q[i];
+    const double fld[3] = {-ac.get(0) * g.dkdr[0] * V.efp, -ac.get(1) * g.dkdr[1] * V.efp,
+                           -ac.get(2) * g.dkdr[2] * V.efp};
+    double ft[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) ft[m] = V.force[3 * i + m] + qd * fld[m];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) V.force[3 * i + m] = ft[m];
+    FinState st;
+    fin_load(V, i, 0, st);
+    const double es[3] = {0.0, 0.0, 0.0};
+    fin_apply(V, i, ft, st, 0, 0.0, es);
+  };
+  const int M = s1 * s2 * s3;
+  const float *__restrict__ mesh = (const float *)V.mesh_r;
+  if ((size_t)M * sizeof(float) <= (size_t)lds) {
+    // the convolved mesh staged in LDS (one pass over it), every stencil
+    // read from there
+    float *m_s = reinterpret_cast<float *>(smem);
+    for (int k = tid; k < M; k += NT) m_s[k] = mesh[k];
+    __syncthreads();
+    for (int i = tid; i < np; i += NT) one(i, [&](int k) { return m_s[k]; });
+    __syncthreads();
+  } else {
+    for (int i = tid; i < np; i += NT) one(i, [&](int k) { return mesh[k]; });
+  }
+}
+
+// the gated list rebuild (the phases of k_rebuild with workgroup barriers)
+MFE_STAGE void p_rebuild(const SysDev &V, int *wi) {
+  const int tid = threadIdx.x, NT = blockDim.x;
+  const int nc = V.cg.ncells;
+  for (int k = tid; k < nc; k += NT) V.c_count[k] = 0;
+  __syncthreads();
+  ph_cell_count(V, tid, NT);
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  block_scan_counts(V.c_count, V.c_start, nc, wi);
+  __syncthreads();
+  ph_cell_scatter(V, tid, NT);
+  __syncthreads();
+  ph_cell_sort(V, tid, NT);
+  __syncthreads();
+  ph_sort_pos(V, tid, NT);
+  __syncthreads();
+  ph_nlist_build<true>(V, tid, NT);
+  if (tid == 0) V.flags[2] += 1;
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+#undef MFE_STAGE
+
+template <int ORDER>
+__device__ void persist_replica(const SysDev &V, int nsteps, int energy, char *smem, int lds,
+                                double (*wred)[40], double *wm, int *wi) {
+  const int tid = threadIdx.x, nw = blockDim.x >> 6;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int P = V.tab.nptpt;
+  __shared__ PairRecF rec_s[MFE_MAX_PAIRS];
+  __shared__ int pidx_s[MFE_MAX_TYPES * MFE_MAX_TYPES];
+  __shared__ float4 off_s[125];
+  __shared__ int flag_s;
+  unsigned stepctr = (unsigned)V.flags[3];
+  for (int step = 0; step < nsteps; ++step) {
+    const int rebuild = p_move_skin(V, stepctr, wm, &flag_s);
+    const int npt = V.tab.npt;
+    if (npt <= 2) p_pair<2>(V, energy, smem, wred, rec_s, pidx_s, off_s);
+    else if (npt <= 4) p_pair<4>(V, energy, smem, wred, rec_s, pidx_s, off_s);
+    else p_pair<MFE_MAX_TYPES>(V, energy, smem, wred, rec_s, pidx_s, off_s);
+    p_spread<ORDER>(V, smem, lds);
+    double esum = 0.0, vsum = 0.0;
+    p_fftx_fwd(V, smem, lds);
+    p_ffty(V, 0, smem, lds);
+    p_fftconv(V, energy, smem, lds, &esum, &vsum);
+    p_ffty(V, 1, smem, lds);
+    p_fftx_inv(V, smem, lds);
+    p_gather<ORDER>(V, smem, lds);
+    // ---- reductions (ph_reduce's energy terms)
+    for (int w = 32; w > 0; w >>= 1) {
+      esum += __shfl_down(esum, w, 64);
+      vsum += __shfl_down(vsum, w, 64);
+    }
+    if (lane == 0) {
+      wred[wave][P + 1] = esum;
+      wred[wave][P + 2] = vsum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int *flags = V.flags;
+      double *en = V.en;
+      flags[3] = (int)(stepctr + 1u);
+      for (int m = 0; m < 3; ++m) en[5 + m] = 0.0;
+      if (energy) {
+        // each component's wavefront partials in wavefront order
+        auto total = [&](int k) {
+          double sacc = wred[0][k];
+          for (int w = 1; w < nw; ++w) sacc += wred[w][k];
+          return sacc;
+        };
+        double twob0 = 0.0;
+        for (int k = 0; k < P; ++k) {
+          const double u = total(k);
+          en[8 + k] = u;
+          twob0 += u;
+        }
+        const double urec = total(P + 1) + V.self_energy + V.net_energy;
+        const double virrec = total(P + 2) - 3.0 * V.net_energy;
+        const double rec = V.efp * urec;
+        en[0] = twob0 + rec;
+        en[1] = twob0;
+        en[2] = rec;
+        en[3] = 0.0;
+        en[4] = total(P) + V.efp * virrec;
+        en[kEnAngle] = 0.0;
+      }
+      flags[1] = 0;
+    }
+    ++stepctr;
+    __syncthreads();
+    // ---- the gated list rebuild (SetVListMD after NList's test)
+    if (rebuild) p_rebuild(V, wi);
+  }
+}
+
+template <int ORDER>
+__global__ void __launch_bounds__(kPersistThreads) __attribute__((amdgpu_waves_per_eu(4)))
+k_bd_persistent(PersistArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ int r_s;
+  __shared__ double wred[kPersistThreads / 64][40];
+  __shared__ double wm[kPersistThreads / 64];
+  __shared__ int wi[kPersistThreads / 64];
+  for (;;) {
+    if (threadIdx.x == 0) r_s = atomicAdd(a.queue, 1);
+    __syncthreads();
+    const int rr = r_s;
+    __syncthreads();
+    if (rr >= a.nrep) return;
+    const int r = a.order[rr];
+    persist_replica<ORDER>(a.sys[r], a.nsteps, a.energy, smem, a.lds, wred, wm, wi);
+  }
+}
+
 inline int nblocks(int n) { return (n + kBlock - 1) / kBlock; }
 
 inline dim3 grid(int x, const Group &g, int y = 1) { return dim3(x > 0 ? x : 1, y, g.R); }
@@ -2615,6 +3211,45 @@ __global__ void k_collect(const SysDev *__restrict__ sys, int *fl, double *en, i
   const SysDev &V = sys[blockIdx.z];
   for (int k = threadIdx.x; k < 4; k += blockDim.x) fl[4 * blockIdx.z + k] = V.flags[k];
   for (int k = threadIdx.x; k < nen; k += blockDim.x) en[(size_t)nen * blockIdx.z + k] = V.en[k];
+}
+
+// The persistent batched BD launch: blocks workgroups of kPersistThreads
+// taking replicas from the queue (zeroed by the fill launch before it).
+void launch_bd_persistent(const Group &g, const int *order, int nrep, int nsteps, int energy,
+                          int *queue, int blocks, size_t lds) {
+  PersistArgs a{g.d, order, nrep, nsteps, energy, queue, (int)lds};
+  launch_fill_int(queue, 1, 0, g.stream);
+  switch (g.order) {
+#define MFE_PERSIST(O)                                                                      \
+  case O:                                                                                  \
+    hipLaunchKernelGGL(k_bd_persistent<O>, dim3(blocks), dim3(kPersistThreads), lds, g.stream, \
+                       a);                                                                 \
+    break;
+    MFE_PERSIST(3) MFE_PERSIST(4) MFE_PERSIST(5) MFE_PERSIST(6) MFE_PERSIST(7) MFE_PERSIST(8)
+#undef MFE_PERSIST
+  }
+}
+
+// dynamic LDS above the default 64 KB for the persistent launch (the
+// attribute only ever grows)
+bool persist_allow_lds(int order, size_t bytes) {
+  static size_t allowed[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (order < 3 || order > 8) return false;
+  if (bytes <= 64 * 1024 || bytes <= allowed[order]) return true;
+  const void *f = nullptr;
+  switch (order) {
+    case 3: f = (const void *)&k_bd_persistent<3>; break;
+    case 4: f = (const void *)&k_bd_persistent<4>; break;
+    case 5: f = (const void *)&k_bd_persistent<5>; break;
+    case 6: f = (const void *)&k_bd_persistent<6>; break;
+    case 7: f = (const void *)&k_bd_persistent<7>; break;
+    default: f = (const void *)&k_bd_persistent<8>; break;
+  }
+  if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) !=
+      hipSuccess)
+    return false;
+  allowed[order] = bytes;
+  return true;
 }
 
 void launch_collect(const Group &g, int *fl, double *en, int nen) {

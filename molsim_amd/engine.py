@@ -84,6 +84,7 @@ SYMBOLS = ["mfe_abi_version", "mfe_pot_defaults", "mfe_build_tables_1_6_12",
            "mfe_batch_create", "mfe_batch_destroy", "mfe_batch_last_error",
            "mfe_batch_failed_replica", "mfe_batch_build_list", "mfe_batch_compute",
            "mfe_batch_step_md", "mfe_batch_step_bd", "mfe_batch_get_energies",
+           "mfe_batch_set_persistent", "mfe_batch_uses_persistent",
            "mfe_set_pair_params", "mfe_set_knots", "mfe_adjoint", "mfe_mc_delta_rec",
            "mfe_mc_delta_pair", "mfe_mc_accept", "mfe_ewald_parameters",
            "mfe_set_sync_limit", "mfe_set_net_charge", "mfe_model_forces", "mfe_fm_set_frames", "mfe_fm_loss_grad",
@@ -147,6 +148,9 @@ def lib():
         L.mfe_batch_compute.argtypes = [C.c_void_p, C.c_uint32]
         L.mfe_batch_step_md.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_uint32]
         L.mfe_batch_step_bd.argtypes = [C.c_void_p, C.c_int32, C.c_uint32]
+        L.mfe_batch_set_persistent.argtypes = [C.c_void_p, C.c_int32]
+        L.mfe_batch_uses_persistent.argtypes = [C.c_void_p]
+        L.mfe_batch_uses_persistent.restype = C.c_int32
         L.mfe_batch_get_energies.argtypes = [C.c_void_p, C.POINTER(EnergyStruct)]
         L.mfe_set_pair_params.argtypes = [C.c_void_p, C.POINTER(PotSpec)]
         L.mfe_set_knots.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double, _dp]
@@ -615,6 +619,15 @@ class Batch:
 
     def step_bd(self, nsteps, flags=FORCES | ENERGY | VIRIAL):
         self._check(self.L.mfe_batch_step_bd(self.h, int(nsteps), flags), "mfe_batch_step_bd")
+
+    def set_persistent(self, on=True):
+        """Let step_bd take the persistent launch where eligible (default)."""
+        self._check(self.L.mfe_batch_set_persistent(self.h, 1 if on else 0),
+                    "mfe_batch_set_persistent")
+
+    def uses_persistent(self):
+        """Whether step_bd takes the persistent launch for this batch."""
+        return bool(self.L.mfe_batch_uses_persistent(self.h))
 
     def energies(self):
         arr = (EnergyStruct * len(self.engines))()

@@ -388,6 +388,19 @@ module mfe_mod
          integer(c_int32_t), value :: nsteps, flags
       end function
 
+      integer(c_int) function mfe_batch_set_persistent(batch, on) &
+            bind(c, name='mfe_batch_set_persistent')
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int32_t), value :: on
+      end function
+
+      integer(c_int32_t) function mfe_batch_uses_persistent(batch) &
+            bind(c, name='mfe_batch_uses_persistent')
+         import :: c_int32_t, c_ptr
+         type(c_ptr), value :: batch
+      end function
+
       integer(c_int) function mfe_batch_get_energies(batch, energies) &
             bind(c, name='mfe_batch_get_energies')
          import :: c_int, c_ptr, mfe_energies

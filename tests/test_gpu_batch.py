@@ -129,6 +129,8 @@ def test_batch_equals_single_mixed_bitwise(c4_tab):
         e2.set_bd(FAC1, FAC2, 1000 + r)
     for e in single:
         e.step_bd(nsteps=15)
+    # the per-stage launches (the persistent launch has its own pins)
+    b.set_persistent(False)
     b.step_bd(15)
     ens = b.energies()
     for r, (e1, e2) in enumerate(zip(single, inbatch)):

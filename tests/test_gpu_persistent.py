@@ -1,0 +1,143 @@
+"""The persistent batched BD launch (no reference counterpart): every
+replica of a batch advanced by BDStep (Src/bd.F90:120-155) + UTotal +
+NList in one launch, a workgroup per replica at a time.
+
+Pinned three ways:
+- a replica's result depends on that replica alone: a batch equals the
+  same replicas each run as a batch of one, bitwise (positions, forces,
+  energies, rebuild counts);
+- the per-stage launches of the same batch give the same trajectory to
+  FP32 rounding (the two paths differ only in summation groupings of the
+  spreading and the gather);
+- after the steps, each replica's energies and forces equal the oracle's
+  UTotal at the replica's positions within the mixed path's bounds.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from molsim_amd import engine as E
+from molsim_amd import systems as S
+
+from helpers import efp, oracle_for, rel_rms
+
+pytestmark = pytest.mark.gpu
+
+FAC1 = np.array([2.0e-7, 2.0e-7])
+FAC2 = np.array([0.0316, 0.0316])
+# meshes of mixed radix, including 46 = 2 x 23 (generic prime radix)
+MESHES = [24, 32, 40, 46, 36, 30, 16, 48]
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def tab():
+    s = S.c4_replica(0)
+    return E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                          alpha=s.alpha)
+
+
+def engines(sysl, tab, seed0=500):
+    out = []
+    for r, s in enumerate(sysl):
+        t = tab if s.rcut == 12.0 else E.build_tables(
+            s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(), alpha=s.alpha)
+        e = S.configure(s, "mixed", tables=t)
+        e.set_bd(FAC1, FAC2, seed0 + r)
+        out.append(e)
+    return out
+
+
+def replicas(n):
+    out = [S.c4_replica(r, mesh=MESHES[r % len(MESHES)], order=4) for r in range(n)]
+    # one replica with a cut-off so long that its box holds two list cells
+    # per axis (the FP64-separation variant of the pair stage)
+    s = out[-1]
+    out[-1] = dataclasses.replace(s, rcut=0.45 * s.boxlen, alpha=3.0 / (0.45 * s.boxlen))
+    return out
+
+
+def run(engs, nsteps, persistent=True):
+    b = E.Batch(engs)
+    b.set_persistent(persistent)
+    b.compute()
+    used = b.uses_persistent()
+    b.step_bd(nsteps)
+    ens = b.energies()
+    b.close()
+    return used, ens
+
+
+def test_batch_equals_each_replica_alone_bitwise(tab):
+    sysl = replicas(8)
+    say("case: persistent BD, 8 replicas, N =", [s.np for s in sysl], "mesh =",
+        [s.mesh for s in sysl])
+    batch = engines(sysl, tab)
+    # enough steps for NList's test to fire (sqrt(2 D dt) = 0.032 A per step)
+    used, ens = run(batch, 80)
+    assert used
+    for r, s in enumerate(sysl):
+        say(f"  replica {r}: alone")
+        alone = engines([s], tab, seed0=500 + r)
+        _, en1 = run(alone, 80)
+        ro_a, _, _ = alone[0].download_state()
+        ro_b, _, _ = batch[r].download_state()
+        assert np.array_equal(ro_a, ro_b), f"replica {r} positions"
+        assert np.array_equal(alone[0].download_forces(), batch[r].download_forces())
+        assert en1[0].tot == ens[r].tot and en1[0].rebuilds == ens[r].rebuilds
+        alone[0].close()
+    say("  rebuilds:", [en.rebuilds for en in ens])
+    assert sum(en.rebuilds for en in ens) > 0
+    for e in batch:
+        e.close()
+
+
+def test_persistent_matches_per_stage_launches(tab):
+    sysl = replicas(6)
+    say("case: persistent vs per-stage BD, 6 replicas, 80 steps")
+    a = engines(sysl, tab)
+    b = engines(sysl, tab)
+    used_a, ens_a = run(a, 80, True)
+    used_b, ens_b = run(b, 80, False)
+    assert used_a and not used_b
+    for r in range(len(sysl)):
+        ro_a, _, _ = a[r].download_state()
+        ro_b, _, _ = b[r].download_state()
+        d = np.abs(ro_a - ro_b)
+        d = np.minimum(d, sysl[r].boxlen - d)
+        fr = rel_rms(a[r].download_forces(), b[r].download_forces())
+        de = abs(ens_a[r].tot - ens_b[r].tot) / abs(ens_b[r].tot)
+        say(f"  replica {r}: max |dro| {d.max():.2e} A, force rel-RMS {fr:.2e}, energy {de:.2e},"
+            f" rebuilds {ens_a[r].rebuilds}/{ens_b[r].rebuilds}")
+        assert d.max() <= 1e-7
+        assert fr <= 1e-5
+        assert de <= 1e-6
+        assert ens_a[r].rebuilds == ens_b[r].rebuilds
+    for e in a + b:
+        e.close()
+
+
+def test_persistent_energies_match_oracle(tab):
+    sysl = replicas(4)
+    say("case: persistent BD then oracle UTotal at the final positions, 4 replicas")
+    engs = engines(sysl, tab)
+    used, ens = run(engs, 8)
+    assert used
+    for r, s in enumerate(sysl):
+        ro, _, _ = engs[r].download_state()
+        t = tab if s.rcut == 12.0 else E.build_tables(
+            s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(), alpha=s.alpha)
+        orc = oracle_for(s, t)
+        orc.set_list(ro)
+        fo, eo = orc.utotal(ro)
+        f = engs[r].download_forces()
+        fr = rel_rms(f, fo)
+        say(f"  replica {r}: force rel-RMS {fr:.2e}, tot {ens[r].tot:.9e} vs {eo['tot']:.9e}")
+        assert fr <= 1e-5
+        assert abs(ens[r].tot - eo["tot"]) <= 1e-6 * abs(eo["tot"])
+        assert abs(ens[r].twob0 - eo["twob0"]) <= 1e-6 * abs(eo["twob0"])
+        engs[r].close()
