@@ -31,9 +31,12 @@ struct mfe_batch {
   int *h_flags = nullptr;
   double *h_en = nullptr;
   int failed = -1;
-  // persistent BD: replica order (costliest first) and queue counter
+  // persistent BD: replica order (costliest first), queue counter, and
+  // the captured graph of the launch (queue reset + kernel) for pgraph_key
   bool persist = true;
   int *d_order = nullptr, *d_queue = nullptr;
+  hipGraphExec_t pgraph = nullptr;
+  long long pgraph_key = -1;
 };
 
 namespace {
@@ -92,6 +95,9 @@ int refresh(mfe_batch *b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   b->graph = nullptr;
   // the persistent launch's order follows the replicas' sizes
+  if (b->pgraph) (void)hipGraphExecDestroy(b->pgraph);
+  b->pgraph = nullptr;
+  b->pgraph_key = -1;
   if (b->d_order) (void)hipFree(b->d_order);
   if (b->d_queue) (void)hipFree(b->d_queue);
   b->d_order = b->d_queue = nullptr;
@@ -195,6 +201,7 @@ int mfe_batch_destroy(mfe_batch *b) {
   if (b->d_sys) (void)hipFree(b->d_sys);
   if (b->d_flags) (void)hipFree(b->d_flags);
   if (b->d_en) (void)hipFree(b->d_en);
+  if (b->pgraph) (void)hipGraphExecDestroy(b->pgraph);
   if (b->d_order) (void)hipFree(b->d_order);
   if (b->d_queue) (void)hipFree(b->d_queue);
   if (b->h_flags) (void)hipHostFree(b->h_flags);
@@ -273,8 +280,21 @@ int run_persistent(mfe_batch *b, int32_t nsteps, uint32_t flags, size_t lds) {
   const int ncu = std::max(b->eng[0]->ncu, 1);
   const int blocks = std::min(R, ncu);
   const int energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) != 0 ? 1 : 0;
-  mfe::launch_bd_persistent(b->grp, b->d_order, R, nsteps, energy, b->d_queue, blocks, lds);
-  MFE_HIP_TRY(hipGetLastError());
+  // the queue reset and the launch, captured once per (nsteps, energy)
+  const long long key = 2LL * nsteps + energy;
+  if (!b->pgraph || b->pgraph_key != key) {
+    if (b->pgraph) (void)hipGraphExecDestroy(b->pgraph);
+    b->pgraph = nullptr;
+    hipGraph_t gph;
+    MFE_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    mfe::launch_bd_persistent(b->grp, b->d_order, R, nsteps, energy, b->d_queue, blocks, lds);
+    MFE_HIP_TRY(hipStreamEndCapture(b->stream, &gph));
+    hipError_t ie = hipGraphInstantiate(&b->pgraph, gph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gph);
+    MFE_HIP_TRY(ie);
+    b->pgraph_key = key;
+  }
+  MFE_HIP_TRY(hipGraphLaunch(b->pgraph, b->stream));
   return check(b);
 }
 

@@ -2443,8 +2443,9 @@ __device__ __forceinline__ double block_prefix_max(double v, double *wm, double 
   return inc;
 }
 
-// exclusive prefix sum of cnt[0..n) into start[0..n], start[n] = total
-__device__ __forceinline__ void block_scan_counts(const int *cnt, int *start, int n, int *wsum) {
+// exclusive prefix sum of cnt[0..n) into start[0..n], start[n] = total;
+// the counts are left zero, as every rebuild expects them (ph_scan_offset)
+__device__ __forceinline__ void block_scan_counts(int *cnt, int *start, int n, int *wsum) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   int carry = 0;
   for (int base = 0; base < n; base += blockDim.x) {
@@ -2462,7 +2463,10 @@ __device__ __forceinline__ void block_scan_counts(const int *cnt, int *start, in
       if (w < wave) pw += wsum[w];
       t += wsum[w];
     }
-    if (k < n) start[k] = carry + pw + inc - v;
+    if (k < n) {
+      start[k] = carry + pw + inc - v;
+      cnt[k] = 0;
+    }
     carry += t;
     __syncthreads();
   }

@@ -62,11 +62,15 @@ def replicas(n):
 
 
 def run(engs, nsteps, persistent=True):
+    """nsteps BD steps in two calls of nsteps/2: the second replays the
+    first call's captured launch, after the host read the energies."""
     b = E.Batch(engs)
     b.set_persistent(persistent)
     b.compute()
     used = b.uses_persistent()
-    b.step_bd(nsteps)
+    b.step_bd(nsteps // 2)
+    b.energies()
+    b.step_bd(nsteps - nsteps // 2)
     ens = b.energies()
     b.close()
     return used, ens
@@ -141,3 +145,31 @@ def test_persistent_energies_match_oracle(tab):
         assert abs(ens[r].tot - eo["tot"]) <= 1e-6 * abs(eo["tot"])
         assert abs(ens[r].twob0 - eo["twob0"]) <= 1e-6 * abs(eo["twob0"])
         engs[r].close()
+
+
+def test_engines_continue_alone_after_persistent_steps(tab):
+    """The state a persistent launch leaves is the per-stage path's: after
+    persistent steps with list rebuilds, each engine steps on alone (its
+    own rebuilds run on the cell counts the persistent rebuild left) and
+    still matches the oracle's UTotal at its final positions."""
+    sysl = replicas(3)
+    say("case: engines continue alone after 80 persistent steps, 3 replicas")
+    engs = engines(sysl, tab)
+    used, ens = run(engs, 80)
+    assert used and sum(en.rebuilds for en in ens) > 0
+    for r, s in enumerate(sysl):
+        e = engs[r]
+        e.step_bd(nsteps=80)
+        en = e.energy()
+        ro, _, _ = e.download_state()
+        t = tab if s.rcut == 12.0 else E.build_tables(
+            s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(), alpha=s.alpha)
+        orc = oracle_for(s, t)
+        orc.set_list(ro)
+        fo, eo = orc.utotal(ro)
+        fr = rel_rms(e.download_forces(), fo)
+        say(f"  replica {r}: rebuilds {ens[r].rebuilds} -> {en.rebuilds}, force rel-RMS {fr:.2e}")
+        assert en.rebuilds > ens[r].rebuilds
+        assert fr <= 1e-5
+        assert abs(en.tot - eo["tot"]) <= 1e-6 * abs(eo["tot"])
+        e.close()
