@@ -371,8 +371,9 @@ int mfe_fm_loss_grad(mfe_engine *e, double *loss, double *dL_dzat,
 /* ---- batched replicas ------------------------------------------------- */
 /* No reference counterpart: MOLSIM runs one system per process.  A batch
    drives nrep engines, each set up as above (own box, particles, tables,
-   list, mesh), as one system: every stage of UTotal, VelVer and BDStep is
-   one launch over all replicas.  The replicas must share device,
+   list, mesh), as one system: every stage of UTotal and VelVer is one
+   launch over all replicas; BD steps take one persistent launch where the
+   replicas allow it (mfe_batch_step_bd).  The replicas must share device,
    precision, Ewald kind and SPME order; box, N, mesh, tables and cutoffs
    are per replica.  Each replica computes exactly what its engine computes
    alone.  Results stay in the engines (mfe_download_forces,
