@@ -126,16 +126,19 @@ def test_reference_cnf_drives_the_engine(prec, ftol, etol):
     eng.close()
 
 
-def test_utotal_cell_list_surface_vs_reference(golden):
+@pytest.mark.parametrize("prec,ftol,etol", [("fp64", 2e-6, 1e-7), ("mixed", 1e-5, 1e-7)])
+def test_utotal_cell_list_surface_vs_reference(golden, prec, ftol, etol):
     zc = golden["c1_cell_surf"]
     s = c1_from_golden(golden["c1_std"])
     s.ro = zc["probe_pos"].copy()
     s.lsurf = True
-    say("case: UTotal cell list + surface c1 golden")
-    eng = S.configure(s, "fp64", nlist="cell")
+    say("case: UTotal cell list + surface c1 golden", prec)
+    eng = S.configure(s, prec, nlist="cell")
     f, en = eng.utotal()
-    assert rel_rms(f, zc["ref_forceo"]) <= 2e-6
-    assert abs(en.tot - zc["ref_u_tot"]) <= 1e-7 * abs(zc["ref_u_tot"])
+    say(f"  force rel-RMS {rel_rms(f, zc['ref_forceo']):.2e}, energy "
+        f"{abs(en.tot - zc['ref_u_tot']) / abs(zc['ref_u_tot']):.2e}")
+    assert rel_rms(f, zc["ref_forceo"]) <= ftol
+    assert abs(en.tot - zc["ref_u_tot"]) <= etol * abs(zc["ref_u_tot"])
     eng.close()
 
 
