@@ -19,6 +19,7 @@ struct mfe_batch {
   std::vector<unsigned> ver;
   int device = 0;
   hipStream_t stream = nullptr;
+  mfe::Side side;
   std::string err;
   mfe::SysDev *d_sys = nullptr;
   mfe::Group grp{};
@@ -90,6 +91,7 @@ int refresh(mfe_batch *b) {
   std::string err;
   int rc = mfe_group_build(b->eng.data(), R, b->d_sys, b->stream, b->grp, err);
   if (rc) return bfail(b, rc, err);
+  b->grp.side = b->side;
   const size_t lds = std::max(b->grp.lds_spread, b->grp.lds_gather);
   if (lds > 32 * 1024) mfe_allow_big_spread_lds(b->grp.prec, (int)lds);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
@@ -175,6 +177,7 @@ int mfe_batch_create(mfe_engine *const *engines, int32_t nrep, mfe_batch **out) 
         return bfail(b, MFE_E_BAD_PARAM, "replicas on different devices", r);
     MFE_HIP_TRY(hipSetDevice(b->device));
     MFE_HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    MFE_HIP_TRY(mfe::side_create(b->side));
     MFE_HIP_TRY(hipMalloc((void **)&b->d_flags, 4 * (size_t)nrep * sizeof(int)));
     MFE_HIP_TRY(hipMalloc((void **)&b->d_en, (size_t)kNen * nrep * sizeof(double)));
     MFE_HIP_TRY(hipHostMalloc((void **)&b->h_flags, 4 * (size_t)nrep * sizeof(int)));
@@ -206,6 +209,8 @@ int mfe_batch_destroy(mfe_batch *b) {
   if (b->d_queue) (void)hipFree(b->d_queue);
   if (b->h_flags) (void)hipHostFree(b->h_flags);
   if (b->h_en) (void)hipHostFree(b->h_en);
+  if (b->side.stream) (void)hipStreamSynchronize(b->side.stream);
+  mfe::side_destroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return MFE_OK;

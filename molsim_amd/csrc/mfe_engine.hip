@@ -407,6 +407,25 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   v.surf_fac = e->efp * (2.0 * (2.0 * kPi / (3.0 * v.vol)));
 }
 
+namespace mfe {
+
+hipError_t side_create(Side &s) {
+  hipError_t rc = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+  if (rc == hipSuccess) rc = hipEventCreateWithFlags(&s.fork, hipEventDisableTiming);
+  if (rc == hipSuccess) rc = hipEventCreateWithFlags(&s.join, hipEventDisableTiming);
+  if (rc != hipSuccess) side_destroy(s);
+  return rc;
+}
+
+void side_destroy(Side &s) {
+  if (s.fork) (void)hipEventDestroy(s.fork);
+  if (s.join) (void)hipEventDestroy(s.join);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+  s = Side{};
+}
+
+}  // namespace mfe
+
 // the launch group of a set of engines (maxima of the per-replica grids)
 int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStream_t st,
                     mfe::Group &g, std::string &err) {
@@ -478,6 +497,7 @@ void sync_view(mfe_engine *e) {
   std::string err;
   mfe_engine *one[1] = {e};
   mfe_group_build(one, 1, e->d_sys, e->stream, e->grp, err);
+  e->grp.side = e->side;
   e->grp.rebuild_bar = e->rebuild_bar;
   e->grp.rebuild_sum = e->rebuild_sum;
   e->grp.ncu = e->ncu;
@@ -528,8 +548,28 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
                        bool bd) {
   const bool energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) != 0;
   // inside MD/BD steps (skin) k_move has already updated the cell-ordered
-  // copy, and the pair launch takes the skin test's block candidates
-  mfe::launch_pair(g, energy, !skin, skin);
+  // copy, and the pair launch takes the skin test's block candidates.
+  // With SPME the pair forces and the reciprocal stages up to the inverse
+  // x pass share no data: the pair launch runs on the side stream,
+  // alongside them, and is joined before the gather adds the reciprocal
+  // forces to the pair forces.  A pair launch alone fills a small part of
+  // the GPU (a lane per particle at low density).
+  const bool fork = g.ewald == MFE_EWALD_SPME && g.side.stream;
+  int first = SP_BIN;
+  if (fork) {
+    if (!skin) mfe::launch_sort_pos(g);
+    MFE_HIP_TRY(hipEventRecord(g.side.fork, g.stream));
+    MFE_HIP_TRY(hipStreamWaitEvent(g.side.stream, g.side.fork, 0));
+    mfe::Group gp = g;
+    gp.stream = g.side.stream;
+    mfe::launch_pair(gp, energy, false, skin);
+    MFE_HIP_TRY(hipEventRecord(g.side.join, g.side.stream));
+    mfe::launch_spme(g, energy, SP_BIN, SP_IFFTX);
+    MFE_HIP_TRY(hipStreamWaitEvent(g.stream, g.side.join, 0));
+    first = SP_GATHER;
+  } else {
+    mfe::launch_pair(g, energy, !skin, skin);
+  }
   // with SPME as the last force stage and no VelVer kick, the gather
   // launch also finalizes each particle (forceo only) and reduces the
   // partials.  With the kick the finalize phase reads and writes velocities
@@ -538,10 +578,10 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
   const bool gather_fin = g.ewald == MFE_EWALD_SPME && !g.chains && !g.surf && mode == 0;
   if (gather_fin) {
     const mfe::FinArgs fa{energy ? 1 : 0, skin ? 1 : 0, bd ? 1 : 0, mode, dt};
-    mfe::launch_spme(g, energy, SP_BIN, SP_GATHER, mfe::GM_FORCE, &fa);
+    mfe::launch_spme(g, energy, first, SP_GATHER, mfe::GM_FORCE, &fa);
     return;
   }
-  if (g.ewald == MFE_EWALD_SPME) mfe::launch_spme(g, energy, SP_BIN, SP_GATHER);
+  if (g.ewald == MFE_EWALD_SPME) mfe::launch_spme(g, energy, first, SP_GATHER);
   else if (g.ewald == MFE_EWALD_STD) mfe::launch_std_ewald(g);
   if (g.chains) mfe::launch_bonds(g);
   if (g.surf) mfe::launch_surf(g);
@@ -605,6 +645,7 @@ int mfe_create(const mfe_config *cfg, mfe_engine **out) {
     e->max_np = cfg->max_np;
     MFE_HIP_TRY(hipSetDevice(e->device));
     MFE_HIP_TRY(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    MFE_HIP_TRY(mfe::side_create(e->side));
     hipDeviceProp_t prop;
     MFE_HIP_TRY(hipGetDeviceProperties(&prop, e->device));
     e->ncu = prop.multiProcessorCount;
@@ -626,6 +667,7 @@ int mfe_create(const mfe_config *cfg, mfe_engine **out) {
   });
   if (rc != MFE_OK) {
     free_all(e);
+    mfe::side_destroy(e->side);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     return rc;
@@ -638,7 +680,9 @@ int mfe_destroy(mfe_engine *e) {
   if (!e) return MFE_OK;
   (void)hipSetDevice(e->device);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->side.stream) (void)hipStreamSynchronize(e->side.stream);
   free_all(e);
+  mfe::side_destroy(e->side);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
   return MFE_OK;

@@ -346,10 +346,22 @@ struct SysDev {
 
 // Replicas launched together: same device, precision, Ewald kind and
 // SPME order (a template parameter of the stencil kernels).
+// A second stream and two events: a force evaluation forks the pair
+// forces off the main stream and joins them before the stage that adds to
+// them (under stream capture: two branches of the graph)
+struct Side {
+  hipStream_t stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
+hipError_t side_create(Side &s);
+void side_destroy(Side &s);
+
 struct Group {
   int R = 0;
   const SysDev *d = nullptr;  // device array [R]
   hipStream_t stream = nullptr;
+  Side side;                  // no stream: every launch on the main stream
   int prec = 0, ewald = 0, order = 0;
   bool chains = false, surf = false, cells = false;
   // maxima over the replicas of each launch's per-replica block counts
@@ -378,6 +390,7 @@ struct mfe_engine {
   int device = 0;
   int prec = MFE_PREC_MIXED;
   hipStream_t stream = nullptr;
+  mfe::Side side;  // the pair forces' branch of a force evaluation
   std::string err;
   int max_np = 0;
 
