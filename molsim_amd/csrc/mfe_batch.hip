@@ -23,7 +23,7 @@ struct mfe_batch {
   std::string err;
   mfe::SysDev *d_sys = nullptr;
   mfe::Group grp{};
-  hipGraphExec_t graph = nullptr;
+  mfe::StepGraphs graph;
   int graph_kind = 0;  // 1 = MD, 2 = BD
   double graph_dt = -1.0;
   unsigned graph_flags = 0;
@@ -94,8 +94,7 @@ int refresh(mfe_batch *b) {
   b->grp.side = b->side;
   const size_t lds = std::max(b->grp.lds_spread, b->grp.lds_gather);
   if (lds > 32 * 1024) mfe_allow_big_spread_lds(b->grp.prec, (int)lds);
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  b->graph = nullptr;
+  b->graph.clear();
   // the persistent launch's order follows the replicas' sizes
   if (b->pgraph) (void)hipGraphExecDestroy(b->pgraph);
   b->pgraph = nullptr;
@@ -200,7 +199,7 @@ int mfe_batch_destroy(mfe_batch *b) {
   if (!b) return MFE_OK;
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph.clear();
   if (b->d_sys) (void)hipFree(b->d_sys);
   if (b->d_flags) (void)hipFree(b->d_flags);
   if (b->d_en) (void)hipFree(b->d_en);
@@ -306,23 +305,16 @@ int run_persistent(mfe_batch *b, int32_t nsteps, uint32_t flags, size_t lds) {
 }  // namespace
 
 static int run_graph(mfe_batch *b, int kind, int32_t nsteps, double tstep, uint32_t flags) {
-  if (!b->graph || b->graph_kind != kind || b->graph_flags != flags ||
-      (kind == 1 && b->graph_dt != tstep)) {
-    if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    b->graph = nullptr;
-    hipGraph_t g;
-    MFE_HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    if (kind == 1) mfe_launch_md_step(b->grp, tstep, flags);
-    else mfe_launch_bd_step(b->grp, 2, flags);
-    MFE_HIP_TRY(hipStreamEndCapture(b->stream, &g));
-    hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    MFE_HIP_TRY(ie);
+  if (b->graph_kind != kind || b->graph_flags != flags || (kind == 1 && b->graph_dt != tstep)) {
+    b->graph.clear();
     b->graph_kind = kind;
     b->graph_dt = tstep;
     b->graph_flags = flags;
   }
-  for (int s = 0; s < nsteps; ++s) MFE_HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+  MFE_HIP_TRY(b->graph.run(b->stream, nsteps, [&]() {
+    if (kind == 1) mfe_launch_md_step(b->grp, tstep, flags);
+    else mfe_launch_bd_step(b->grp, 2, flags);
+  }));
   return check(b);
 }
 

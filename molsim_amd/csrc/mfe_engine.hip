@@ -137,9 +137,9 @@ int smallest_divisor_at_least(int s, int lo) {
 }
 
 void free_all(mfe_engine *e) {
-  if (e->md_graph) (void)hipGraphExecDestroy(e->md_graph);
-  e->md_graph = nullptr;
+  e->md_graph.clear();
   dfree(e->ro); dfree(e->rod); dfree(e->rodd); dfree(e->force); dfree(e->forceo);
+  dfree(e->force_rec);
   dfree(e->drosum); dfree(e->r2disp); dfree(e->type); dfree(e->q); dfree(e->massip);
   dfree(e->t_zlow); dfree(e->t_coef); dfree(e->t_coef_f); dfree(e->t_off);
   dfree(e->t_nint); dfree(e->t_pairidx); dfree(e->t_r2atat); dfree(e->t_r2umin);
@@ -296,6 +296,7 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v) {
   v.nlsem = e->nlkind == MFE_NL_CELL ? 1 : 0;
   v.box = e->box;
   MFE_SETP(v.ro, e->ro); MFE_SETP(v.rod, e->rod); MFE_SETP(v.rodd, e->rodd); MFE_SETP(v.force, e->force); MFE_SETP(v.forceo, e->forceo);
+  MFE_SETP(v.force_rec, e->force_rec);
   MFE_SETP(v.drosum, e->drosum); MFE_SETP(v.r2disp, e->r2disp); MFE_SETP(v.type, e->type); MFE_SETP(v.q, e->q);
   MFE_SETP(v.massip, e->massip);
   mfe::TableDev &t = v.tab;
@@ -481,6 +482,11 @@ int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStre
       g.lds_gather = std::max(g.lds_gather, gext);
     }
   }
+  // the fused pair + gather launch: its block holds both kernels' static
+  // LDS (~6.5 KB) and the larger dynamic need, within 64 KB
+  g.fuse_pg = g.prec == MFE_PREC_MIXED && g.ewald == MFE_EWALD_SPME && !g.chains && !g.surf &&
+              !g.knots && !g.pair_any64 && g.pair_cf_lds &&
+              std::max(g.lds_pair, g.lds_gather) <= mfe::kPairLdsBudget;
   return MFE_OK;
 }
 
@@ -554,6 +560,15 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
   // alongside them, and is joined before the gather adds the reciprocal
   // forces to the pair forces.  A pair launch alone fills a small part of
   // the GPU (a lane per particle at low density).
+  if (g.fuse_pg) {
+    // the pair forces ride in the gather's launch (no stream fork); the
+    // finalize adds the two force arrays
+    if (!skin) mfe::launch_sort_pos(g);
+    mfe::launch_spme(g, energy, SP_BIN, SP_IFFTX);
+    mfe::launch_pair_gather(g, energy, skin);
+    mfe::launch_reduce_finalize(g, flags, skin, bd, mode, dt, true);
+    return;
+  }
   const bool fork = g.ewald == MFE_EWALD_SPME && g.side.stream;
   int first = SP_BIN;
   if (fork) {
@@ -741,6 +756,7 @@ int mfe_set_particles(mfe_engine *e, int32_t np, int32_t npt, const int32_t *ipt
     if (np != e->np) {
       dalloc(e->ro, 3 * (size_t)np); dalloc(e->rod, 3 * (size_t)np);
       dalloc(e->rodd, 3 * (size_t)np); dalloc(e->force, 3 * (size_t)np);
+      dalloc(e->force_rec, 3 * (size_t)np);
       dalloc(e->forceo, 3 * (size_t)np); dalloc(e->drosum, 3 * (size_t)np);
       dalloc(e->r2disp, (size_t)np); dalloc(e->type, (size_t)np);
       dalloc(e->q, (size_t)np); dalloc(e->massip, (size_t)np);
@@ -1573,23 +1589,16 @@ int mfe_step_md(mfe_engine *e, int32_t nsteps, double tstep, uint32_t flags) {
     // one step is captured once into a graph and replayed; the rebuild
     // kernels read the device-side skin-test flag, so the graph is static
     sync_view(e);
-    if (e->graph_dirty || !e->md_graph || e->md_graph_kind != 1 || e->md_graph_dt != tstep ||
+    if (e->graph_dirty || e->md_graph_kind != 1 || e->md_graph_dt != tstep ||
         e->md_graph_flags != flags) {
-      if (e->md_graph) (void)hipGraphExecDestroy(e->md_graph);
-      e->md_graph = nullptr;
-      hipGraph_t g;
-      MFE_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-      mfe_launch_md_step(e->grp, tstep, flags);
-      MFE_HIP_TRY(hipStreamEndCapture(e->stream, &g));
-      hipError_t ie = hipGraphInstantiate(&e->md_graph, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      MFE_HIP_TRY(ie);
+      e->md_graph.clear();
       e->md_graph_kind = 1;
       e->md_graph_dt = tstep;
       e->md_graph_flags = flags;
       e->graph_dirty = false;
     }
-    for (int s = 0; s < nsteps; ++s) MFE_HIP_TRY(hipGraphLaunch(e->md_graph, e->stream));
+    MFE_HIP_TRY(e->md_graph.run(e->stream, nsteps,
+                                [&]() { mfe_launch_md_step(e->grp, tstep, flags); }));
     fetch_energy(e);
     return read_flags(e);
   });
@@ -1647,21 +1656,14 @@ int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1, const double 
       }
     } else {
       // counter-based deviates on the device: the step is a static graph
-      if (e->graph_dirty || !e->md_graph || e->md_graph_kind != 2 || e->md_graph_flags != flags) {
-        if (e->md_graph) (void)hipGraphExecDestroy(e->md_graph);
-        e->md_graph = nullptr;
-        hipGraph_t g;
-        MFE_HIP_TRY(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        mfe_launch_bd_step(e->grp, 2, flags);
-        MFE_HIP_TRY(hipStreamEndCapture(e->stream, &g));
-        hipError_t ie = hipGraphInstantiate(&e->md_graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        MFE_HIP_TRY(ie);
+      if (e->graph_dirty || e->md_graph_kind != 2 || e->md_graph_flags != flags) {
+        e->md_graph.clear();
         e->md_graph_kind = 2;
         e->md_graph_flags = flags;
         e->graph_dirty = false;
       }
-      for (int s2 = 0; s2 < nsteps; ++s2) MFE_HIP_TRY(hipGraphLaunch(e->md_graph, e->stream));
+      MFE_HIP_TRY(e->md_graph.run(e->stream, nsteps,
+                                  [&]() { mfe_launch_bd_step(e->grp, 2, flags); }));
     }
     fetch_energy(e);
     return read_flags(e);

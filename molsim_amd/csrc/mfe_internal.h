@@ -239,6 +239,7 @@ struct SysDev {
   MFE_DPTR(double) rodd;
   MFE_DPTR(double) force;
   MFE_DPTR(double) forceo;
+  MFE_DPTR(double) force_rec;  // reciprocal forces apart (fused pair + gather)
   MFE_DPTR(double) drosum;
   MFE_DPTR(double) r2disp;
   MFE_DPTR(int) type;
@@ -357,6 +358,47 @@ struct Side {
 hipError_t side_create(Side &s);
 void side_destroy(Side &s);
 
+// Step graphs: one step, and kGraphSteps steps captured back to back.  A
+// run of nsteps replays the long graph nsteps / kGraphSteps times and the
+// short one for the rest, so a graph launch's fixed cost (several us
+// between replays) falls on every kGraphSteps-th step only.  The steps are
+// static (the rebuild is gated on the device), so both graphs compute the
+// same as launching the steps one by one.
+constexpr int kGraphSteps = 8;
+struct StepGraphs {
+  hipGraphExec_t one = nullptr, many = nullptr;
+  void clear() {
+    if (one) (void)hipGraphExecDestroy(one);
+    if (many) (void)hipGraphExecDestroy(many);
+    one = many = nullptr;
+  }
+  // launch_step() enqueues one step on stream s (captured on first use)
+  template <class F>
+  hipError_t run(hipStream_t s, int nsteps, F &&launch_step) {
+    const int nmany = nsteps / kGraphSteps, rest = nsteps % kGraphSteps;
+    hipError_t rc = hipSuccess;
+    if (nmany > 0 && !many) rc = capture(s, kGraphSteps, launch_step, &many);
+    if (rc == hipSuccess && rest > 0 && !one) rc = capture(s, 1, launch_step, &one);
+    for (int k = 0; rc == hipSuccess && k < nmany; ++k) rc = hipGraphLaunch(many, s);
+    for (int k = 0; rc == hipSuccess && k < rest; ++k) rc = hipGraphLaunch(one, s);
+    return rc;
+  }
+
+ private:
+  template <class F>
+  static hipError_t capture(hipStream_t s, int n, F &launch_step, hipGraphExec_t *out) {
+    hipError_t rc = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (rc != hipSuccess) return rc;
+    for (int k = 0; k < n; ++k) launch_step();
+    hipGraph_t g = nullptr;
+    rc = hipStreamEndCapture(s, &g);
+    if (rc != hipSuccess) return rc;
+    rc = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    return rc;
+  }
+};
+
 struct Group {
   int R = 0;
   const SysDev *d = nullptr;  // device array [R]
@@ -381,6 +423,10 @@ struct Group {
   // replicas whose mixed pair kernel forms separations from cell-relative
   // FP32 coordinates / in FP64 (boxes with an axis of < 3 list cells)
   bool pair_any32 = false, pair_any64 = false;
+  // the pair forces and the gather in one launch (launch_pair_gather):
+  // mixed SPME without bonds, surface term, spline knots or FP64
+  // separations, coefficients in LDS, both LDS needs within the block limit
+  bool fuse_pg = false;
 };
 
 }  // namespace mfe
@@ -411,6 +457,7 @@ struct mfe_engine {
   int sync_spin = -1;               // k_rebuild's barrier poll bound (-1: default)
   double *force = nullptr;   // UTotal's force(3,na) before unit scaling
   double *forceo = nullptr;  // force * fac
+  double *force_rec = nullptr;  // the reciprocal forces of a fused pair + gather launch
   double *drosum = nullptr, *r2disp = nullptr;
   int *type = nullptr;       // 0-based
   double *q = nullptr;
@@ -541,7 +588,7 @@ struct mfe_engine {
   unsigned version = 1;      // bumped whenever the view changes
 
   // MD bookkeeping
-  hipGraphExec_t md_graph = nullptr;
+  mfe::StepGraphs md_graph;
   double md_graph_dt = -1.0;
   unsigned md_graph_flags = 0;
   int md_graph_kind = 0;     // 1 = MD, 2 = BD
@@ -565,7 +612,7 @@ struct FinArgs {
   double dt;
 };
 // what the gather does beside the forces (k_gather's MODE)
-enum { GM_FORCE = 0, GM_POT = 1, GM_FIELD = 2, GM_POTONLY = 3 };
+enum { GM_FORCE = 0, GM_POT = 1, GM_FIELD = 2, GM_POTONLY = 3, GM_FORCE_REC = 4 };
 void launch_spme(const Group &g, bool energy, int first_stage, int last_stage,
                  int gather_mode = GM_FORCE, const FinArgs *fin = nullptr);
 // adjoint passes: real-space pair sums, per-particle dE/dq, column sums
@@ -576,8 +623,11 @@ void launch_surf(const Group &g);
 void launch_reduce(const Group &g, unsigned flags, bool skin, bool bd);
 void launch_finalize(const Group &g, int mode, double dt);
 // reduce and finalize in one launch (only without the surface term)
+// split: the reciprocal forces are in force_rec (after launch_pair_gather)
 void launch_reduce_finalize(const Group &g, unsigned flags, bool skin, bool bd, int mode,
-                            double dt);
+                            double dt, bool split = false);
+// the pair forces and the gather (into force_rec) in one launch
+void launch_pair_gather(const Group &g, bool energy, bool skin);
 void launch_move(const Group &g, double dt, int bd);
 // the cell-ordered positions refreshed from the particle-ordered ones
 void launch_sort_pos(const Group &g);

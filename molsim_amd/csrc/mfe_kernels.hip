@@ -836,15 +836,17 @@ __device__ __forceinline__ bool pair_eval_f(const SysDev &V, const PairTabF &T, 
 // would span half the box -- the pair's separation is formed in FP64 from
 // the cell-ordered positions and rounded once; the other replicas' blocks
 // return (a group launches both variants when it holds both kinds).
+// The body takes its block index bxi (the launch's own, or its place in a
+// launch that also carries other work).
 template <int NT, bool CF, bool KN, bool POS64>
-__global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(KN || POS64 ? 4 : 5)))
-k_pair_f(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
+__device__ __forceinline__ void pair_f_body(const SysDev *__restrict__ sys, int energy, int pgrid,
+                                            int skin, int bxi) {
   const SysDev &V = sys[blockIdx.z];
-  if (skin && (int)blockIdx.x >= pgrid) {
-    skin_two_blocks(V, 2 * ((int)blockIdx.x - pgrid));
+  if (skin && bxi >= pgrid) {
+    skin_two_blocks(V, 2 * (bxi - pgrid));
     return;
   }
-  const int blk = xcd_block(blockIdx.x, pgrid);
+  const int blk = xcd_block(bxi, pgrid);
   const int nb = V.nblk_pair;
   if (blk >= nb || V.pair_pos64 != (POS64 ? 1 : 0)) return;
   const float4 *__restrict__ pos = V.pos_f;
@@ -1034,6 +1036,12 @@ k_pair_f(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
       out[(size_t)k * nb] = s2;
     }
   }
+}
+
+template <int NT, bool CF, bool KN, bool POS64>
+__global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(KN || POS64 ? 4 : 5)))
+k_pair_f(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
+  pair_f_body<NT, CF, KN, POS64>(sys, energy, pgrid, skin, (int)blockIdx.x);
 }
 
 // ------------------------------------------------------------------ SPME
@@ -1262,15 +1270,16 @@ __device__ __forceinline__ void ph_reduce(const SysDev &V, int energy, int skin,
 // FIN: also the finalize phase of each particle (its owner lane, after the
 // force) and, in block nbricks, the reductions, whose inputs come from
 // earlier launches.
+// GM_FORCE_REC writes the reciprocal force alone into force_rec (the
+// finalize adds it to the pair force).  The body takes its block index bxi.
 template <typename R, int ORDER, int MODE, bool FIN>
-__global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restrict__ sys,
-                                                         FinArgs fa) {
+__device__ __forceinline__ void gather_body(const SysDev *__restrict__ sys, FinArgs fa, int bxi) {
   const SysDev &V = sys[blockIdx.z];
-  if (FIN && (int)blockIdx.x == V.nbricks) {
+  if (FIN && bxi == V.nbricks) {
     ph_reduce(V, fa.energy, fa.skin, fa.bd);
     return;
   }
-  if ((int)blockIdx.x >= V.nbricks) return;
+  if (bxi >= V.nbricks) return;
   const R *__restrict__ frac = (const R *)V.rec_frac;
   const int2 *__restrict__ meta = V.rec_meta;
   const int *__restrict__ bstart = V.b_start;
@@ -1280,9 +1289,9 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
   double *__restrict__ force = V.force;
   extern __shared__ char smem[];
   R *tile = reinterpret_cast<R *>(smem);
-  const int bx = blockIdx.x % g.nb[0];
-  const int by = (blockIdx.x / g.nb[0]) % g.nb[1];
-  const int bz = blockIdx.x / (g.nb[0] * g.nb[1]);
+  const int bx = bxi % g.nb[0];
+  const int by = (bxi / g.nb[0]) % g.nb[1];
+  const int bz = bxi / (g.nb[0] * g.nb[1]);
   const int o0 = bx * g.B[0], o1 = by * g.B[1], o2 = bz * g.B[2];
   const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
   const int nstage = E0 * E1 * E2;
@@ -1292,7 +1301,7 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
   constexpr int GL = sizeof(R) == 4 ? 2 : 8;
   constexpr int kGStep = kGatherLanes / GL;
   constexpr int kPlanes = (ORDER + GL - 1) / GL;
-  const int first = bstart[blockIdx.x], n = bstart[blockIdx.x + 1] - first;
+  const int first = bstart[bxi], n = bstart[bxi + 1] - first;
   const int l = threadIdx.x % GL;
   int p = threadIdx.x / GL;
   // the first particle's record is in flight during the staging
@@ -1340,7 +1349,7 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
     const int idx = md.y;
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     FinState st;
-    if (owner && MODE != GM_POTONLY) {
+    if (owner && MODE != GM_POTONLY && MODE != GM_FORCE_REC) {
       f0 = force[3 * idx];
       f1 = force[3 * idx + 1];
       f2 = force[3 * idx + 2];
@@ -1418,11 +1427,19 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
       }
     }
     if (owner) {
+      // q fld rounded before the pair force is added (no contraction): the
+      // same sum as the finalize forms from force and force_rec
+#pragma clang fp contract(off)
       const double qd = (double)f.q;
       const double fld[3] = {-ex * g.dkdr[0] * efp, -ey * g.dkdr[1] * efp,
                              -ez * g.dkdr[2] * efp};
-      const double ft[3] = {f0 + qd * fld[0], f1 + qd * fld[1], f2 + qd * fld[2]};
-      if (MODE != GM_POTONLY) {
+      const double fr[3] = {qd * fld[0], qd * fld[1], qd * fld[2]};
+      const double ft[3] = {f0 + fr[0], f1 + fr[1], f2 + fr[2]};
+      if (MODE == GM_FORCE_REC) {
+        V.force_rec[3 * idx] = fr[0];
+        V.force_rec[3 * idx + 1] = fr[1];
+        V.force_rec[3 * idx + 2] = fr[2];
+      } else if (MODE != GM_POTONLY) {
         force[3 * idx] = ft[0];
         force[3 * idx + 1] = ft[1];
         force[3 * idx + 2] = ft[2];
@@ -1442,6 +1459,31 @@ __global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restric
     f = fn;
     md = mn;
   }
+}
+
+template <typename R, int ORDER, int MODE, bool FIN>
+__global__ void __launch_bounds__(kGatherLanes) k_gather(const SysDev *__restrict__ sys,
+                                                         FinArgs fa) {
+  gather_body<R, ORDER, MODE, FIN>(sys, fa, (int)blockIdx.x);
+}
+
+// The mixed path's pair forces and the reciprocal forces in one launch:
+// blocks [0, npb) run k_pair_f's blocks (pair partition, then the skin
+// blocks; npb a multiple of eight, so the gather blocks keep their
+// placement on the XCDs), block npb + b the gather of brick b, which
+// writes force_rec.  The two share no data; the pair waves, a lane per
+// particle at low density, fill what the gather leaves idle.
+static_assert(kPairBlock == kGatherLanes, "the fused launch runs both bodies in one block size");
+
+template <int NT, int ORDER>
+__global__ void __launch_bounds__(kPairBlock) __attribute__((amdgpu_waves_per_eu(5)))
+k_pair_gather(const SysDev *__restrict__ sys, int energy, int pgrid, int skin, int npb) {
+  const int b = (int)blockIdx.x;
+  if (b < npb) {
+    pair_f_body<NT, true, false, false>(sys, energy, pgrid, skin, b);
+    return;
+  }
+  gather_body<float, ORDER, GM_FORCE_REC, false>(sys, FinArgs{0, 0, 0, 0, 0.0}, b - npb);
 }
 
 // an accepted trial move written into a copy of the positions
@@ -2350,12 +2392,20 @@ __device__ __forceinline__ void fin_apply(const SysDev &V, int i, const double f
   }
 }
 
-// every load before the first store
+// every load before the first store; split: the reciprocal forces are in
+// force_rec (the fused pair + gather launch), added here
 __device__ __forceinline__ void ph_finalize(const SysDev &V, int i, int mode, double dt,
-                                            const double es[3]) {
+                                            const double es[3], bool split = false) {
   double f[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) f[m] = V.force[3 * i + m];
+  if (split) {
+    double r[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) r[m] = V.force_rec[3 * i + m];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) f[m] = f[m] + r[m];
+  }
   FinState st;
   fin_load(V, i, mode, st);
   fin_apply(V, i, f, st, mode, dt, es);
@@ -2380,7 +2430,8 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const SysDev *__restrict__ 
 // kReduceThreads particles each (F of the replica).
 __global__ void __launch_bounds__(kReduceThreads) k_reduce_finalize(const SysDev *__restrict__ sys,
                                                                     int energy, int skin, int bd,
-                                                                    int mode, double dt) {
+                                                                    int mode, double dt,
+                                                                    int split) {
   const SysDev &V = sys[blockIdx.z];
   const int F = (V.np + kReduceThreads - 1) / kReduceThreads;
   if (blockIdx.x == 0) {
@@ -2390,7 +2441,7 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_finalize(const SysDev
   if ((int)blockIdx.x <= F) {
     const int i = ((int)blockIdx.x - 1) * kReduceThreads + threadIdx.x;
     const double es[3] = {0.0, 0.0, 0.0};
-    if (i < V.np) ph_finalize(V, i, mode, dt, es);
+    if (i < V.np) ph_finalize(V, i, mode, dt, es, split != 0);
   }
 }
 
@@ -3194,11 +3245,35 @@ void launch_reduce(const Group &g, unsigned flags, bool skin, bool bd) {
 }
 
 void launch_reduce_finalize(const Group &g, unsigned flags, bool skin, bool bd, int mode,
-                            double dt) {
+                            double dt, bool split) {
   const int energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
   const int F = (g.np_blocks * kBlock + kReduceThreads - 1) / kReduceThreads;
   hipLaunchKernelGGL(k_reduce_finalize, grid(F + 1, g), dim3(kReduceThreads), 0, g.stream, g.d,
-                     energy, skin ? 1 : 0, bd ? 1 : 0, mode, dt);
+                     energy, skin ? 1 : 0, bd ? 1 : 0, mode, dt, split ? 1 : 0);
+}
+
+// the fused pair + gather launch (Group::fuse_pg)
+void launch_pair_gather(const Group &g, bool energy, bool skin) {
+  const int pgrid = std::max(g.pair_blocks, 1);
+  const int npb = (pgrid + (skin ? (g.np_blocks + 1) / 2 : 0) + 7) / 8 * 8;
+  const dim3 gr = grid(npb + g.bricks, g);
+  const size_t lds = std::max(g.lds_pair, g.lds_gather);
+  const int e1 = energy ? 1 : 0, s1 = skin ? 1 : 0;
+  const int nt = g.pair_npt <= 3 ? 3 : (g.pair_npt <= 4 ? 4 : 8);
+#define MFE_PG(NT, O)                                                                      \
+  hipLaunchKernelGGL((k_pair_gather<NT, O>), gr, dim3(kPairBlock), lds, g.stream, g.d, e1, \
+                     pgrid, s1, npb)
+#define MFE_PG_NT(O)                \
+  case O:                           \
+    if (nt == 3) MFE_PG(3, O);      \
+    else if (nt == 4) MFE_PG(4, O); \
+    else MFE_PG(8, O);              \
+    break;
+  switch (g.order) {
+    MFE_PG_NT(3) MFE_PG_NT(4) MFE_PG_NT(5) MFE_PG_NT(6) MFE_PG_NT(7) MFE_PG_NT(8)
+  }
+#undef MFE_PG_NT
+#undef MFE_PG
 }
 
 void launch_finalize(const Group &g, int mode, double dt) {
