@@ -531,7 +531,7 @@ int mfe_map_flag_error(mfe_engine *e, int err) {
     // the barrier state is undefined after a timed-out wait, and the cell
     // counts may hold the failed rebuild's increments: both are cleared
     // before the list is built again from scratch
-    mfe::launch_fill_int((int *)e->rebuild_bar, 2, 0, e->stream);
+    mfe::launch_fill_int((int *)e->rebuild_bar, mfe::kBarWords, 0, e->stream);
     if (e->c_count) mfe::launch_fill_int(e->c_count, e->ncells_alloc, 0, e->stream);
     MFE_HIP_TRY(hipStreamSynchronize(e->stream));
     e->list_valid = false;
@@ -664,9 +664,9 @@ int mfe_create(const mfe_config *cfg, mfe_engine **out) {
     hipDeviceProp_t prop;
     MFE_HIP_TRY(hipGetDeviceProperties(&prop, e->device));
     e->ncu = prop.multiProcessorCount;
-    dalloc(e->rebuild_bar, 2);
+    dalloc(e->rebuild_bar, mfe::kBarWords);
     dalloc(e->rebuild_sum, (size_t)e->ncu);
-    mfe::launch_fill_int((int *)e->rebuild_bar, 2, 0, e->stream);
+    mfe::launch_fill_int((int *)e->rebuild_bar, mfe::kBarWords, 0, e->stream);
     dalloc(e->flags, 4);
     dalloc(e->en, mfe::kEnLen);
     MFE_HIP_TRY(hipHostMalloc((void **)&e->h_en, mfe::kEnLen * sizeof(double)));

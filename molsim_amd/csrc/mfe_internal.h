@@ -153,6 +153,13 @@ struct TableDev {
 // r2 bins per pair, the total over pairs bounded by kTableBins
 constexpr int kTableBins = 4096;
 constexpr int kTableBinsPerPair = 256;
+// words of k_rebuild's grid barrier: kBarGroups group counters, then the
+// top counter and the generation, each on a 128-byte line of its own
+constexpr int kBarGroups = 8;
+constexpr int kBarStride = 32;
+constexpr int kBarTop = kBarGroups * kBarStride;
+constexpr int kBarGen = kBarTop + kBarStride;
+constexpr int kBarWords = kBarGen + kBarStride;
 // threads per block of the pair kernel and its dynamic LDS budget
 constexpr int kPairBlock = 512;
 // blocks of the persistent pair kernel (two per CU)
@@ -451,7 +458,7 @@ struct mfe_engine {
 
   // particle state (device, xyz-interleaved FP64)
   double *ro = nullptr, *rod = nullptr, *rodd = nullptr;
-  unsigned *rebuild_bar = nullptr;  // grid barrier of k_rebuild [2]
+  unsigned *rebuild_bar = nullptr;  // grid barrier of k_rebuild [kBarWords]
   int *rebuild_sum = nullptr;       // its per-block scan totals [ncu]
   int ncu = 0;                      // compute units of the device
   int sync_spin = -1;               // k_rebuild's barrier poll bound (-1: default)

@@ -262,29 +262,59 @@ __device__ __forceinline__ void ph_nlist_build(const SysDev &V, int t0, int nt) 
       for (int dy = -ry; dy <= ry; ++dy) {
         const int y = (cy + dy + g.n[1]) % g.n[1];
         const int ky = ry ? (cy + dy < 0 ? 2 : (cy + dy >= g.n[1] ? -2 : -dy)) : 0;
-        for (int dx = -rx; dx <= rx; ++dx) {
-          const int x = (cx + dx + g.n[0]) % g.n[0];
-          const int kx = rx ? (cx + dx < 0 ? 2 : (cx + dx >= g.n[0] ? -2 : -dx)) : 0;
-          const unsigned code = (unsigned)((kx + 2) + 5 * (ky + 2) + 25 * (kz + 2))
-                                << kNbrCodeShift;
-          const int cc = x + g.n[0] * (y + g.n[1] * z);
-          for (int p = start[cc]; p < start[cc + 1]; ++p) {
-            if (p == t) continue;
-            double xj, yj, zj;
+        // the row's cells x-1, x, x+1 (one on an axis of one cell): their
+        // bounds loaded together, then their particles taken as one
+        // sequence in the same order (cell by cell, p ascending), four
+        // positions in flight at a time
+        int cs[3] = {0, 0, 0}, cn[3] = {0, 0, 0};
+        unsigned cd[3] = {0u, 0u, 0u};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int dx = k - 1;
+          if (rx || dx == 0) {
+            const int x = (cx + dx + g.n[0]) % g.n[0];
+            const int kx = rx ? (cx + dx < 0 ? 2 : (cx + dx >= g.n[0] ? -2 : -dx)) : 0;
+            cd[k] = (unsigned)((kx + 2) + 5 * (ky + 2) + 25 * (kz + 2)) << kNbrCodeShift;
+            const int cc = x + g.n[0] * (y + g.n[1] * z);
+            cs[k] = start[cc];
+            cn[k] = start[cc + 1];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) cn[k] -= cs[k];
+        const int n01 = cn[0] + cn[1], tot = n01 + cn[2];
+        for (int q0 = 0; q0 < tot; q0 += 4) {
+          int pp[4], kk[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int q = q0 + u;
+            const int k = q < cn[0] ? 0 : (q < n01 ? 1 : 2);
+            kk[u] = k;
+            // past the row's end: the particle itself (skipped below)
+            pp[u] = q < tot ? cs[k] + q - (k == 0 ? 0 : (k == 1 ? cn[0] : n01)) : t;
+          }
+          double xj[4], yj[4], zj[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
             if (POS_S) {
-              xj = ros[3 * p];
-              yj = ros[3 * p + 1];
-              zj = ros[3 * p + 2];
+              xj[u] = ros[3 * pp[u]];
+              yj[u] = ros[3 * pp[u] + 1];
+              zj[u] = ros[3 * pp[u] + 2];
             } else {
-              const int j = sorted[p];
-              xj = ro[3 * j];
-              yj = ro[3 * j + 1];
-              zj = ro[3 * j + 2];
+              const int j = sorted[pp[u]];
+              xj[u] = ro[3 * j];
+              yj[u] = ro[3 * j + 1];
+              zj[u] = ro[3 * j + 2];
             }
-            double ddx = xi - xj, ddy = yi - yj, ddz = zi - zj;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            if (pp[u] == t) continue;
+            double ddx = xi - xj[u], ddy = yi - yj[u], ddz = zi - zj[u];
             const double r2 = pbcr2(ddx, ddy, ddz, b);
             if (r2 < rs2) {
-              if (n < maxnn) nbr[(size_t)n * np + t] = (int)((unsigned)p | code);
+              const unsigned code = kk[u] == 0 ? cd[0] : (kk[u] == 1 ? cd[1] : cd[2]);
+              if (n < maxnn) nbr[(size_t)n * np + t] = (int)((unsigned)pp[u] | code);
               ++n;
             }
           }
@@ -346,13 +376,19 @@ __global__ void __launch_bounds__(kBlock) k_nlist_build(const SysDev *__restrict
 #undef MFE_GATE
 
 // Grid-wide barrier of k_rebuild, whose blocks are all resident (one per
-// CU): arrival counter bar[0] and generation bar[1].  Each wave waits for
-// its stores (in L2), the block meets at s_barrier; thread 0 then arrives with an
-// acquire-release increment (its release writes the XCD's L2 back, so
-// the other XCDs see the block's data), the last block to arrive resets
-// the counter and advances the generation (release), the others poll it
-// (relaxed) and take one acquire fence, which invalidates the CU's stale
-// lines for the whole block.  The wait is bounded: after `spin` polls a
+// CU), in two levels: the blocks arrive on kBarGroups group counters
+// (block b on counter b % kBarGroups, each counter on a line of its own,
+// so arrivals on different counters do not queue behind one another), the
+// last block of each group on the top counter, and the last of those
+// advances the generation.  Each wave waits for its stores (in L2), the
+// block meets at s_barrier; thread 0 then arrives with acquire-release
+// increments (a release writes the XCD's L2 back, so the other XCDs see
+// the block's data; a group's last block has acquired its group's data
+// before it releases on the top counter), the last arrival at each level
+// resets that level's counter, the final one advances the generation
+// (release), the others poll it (relaxed) and take one acquire fence,
+// which invalidates the CU's stale lines for the whole block.  The words
+// are laid out as kBarWords in mfe_internal.h.  The wait is bounded: after `spin` polls a
 // block raises ERR_SYNC and leaves.  A block goes on to the next phase
 // only when every block has arrived (so the data of the finished phase is
 // complete) and no block has given up; otherwise it returns false, the
@@ -368,15 +404,25 @@ __device__ __forceinline__ bool grid_sync(unsigned *bar, int *flags, int spin) {
   __syncthreads();
   if (threadIdx.x == 0) {
     int ok = 1;
-    const unsigned gen = __hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned arrived =
-        __hip_atomic_fetch_add(&bar[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-    if (arrived == gridDim.x) {
-      __hip_atomic_store(&bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&bar[1], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    } else {
+    const unsigned nblk = gridDim.x;
+    const unsigned ng = nblk < (unsigned)kBarGroups ? nblk : (unsigned)kBarGroups;
+    const unsigned grp = blockIdx.x % ng;
+    const unsigned ingrp = (nblk - 1u - grp) / ng + 1u;  // blocks of this group
+    unsigned *gc = bar + grp * kBarStride;
+    unsigned *gen_p = bar + kBarGen, *top = bar + kBarTop;
+    const unsigned gen = __hip_atomic_load(gen_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool final_block = false;
+    if (__hip_atomic_fetch_add(gc, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u == ingrp) {
+      __hip_atomic_store(gc, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (__hip_atomic_fetch_add(top, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u == ng) {
+        __hip_atomic_store(top, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(gen_p, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        final_block = true;
+      }
+    }
+    if (!final_block) {
       int spins = 0;
-      while (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+      while (__hip_atomic_load(gen_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gen) {
         if (++spins > spin) {
           atomicOr(&flags[0], ERR_SYNC);
           ok = 0;
