@@ -2635,11 +2635,25 @@ MFE_STAGE int p_move_skin(const SysDev &V, unsigned stepctr, double *wm, int *fl
   return *flag_s;
 }
 
+// Where the persistent pair stage keeps the replica's cell-relative
+// positions: after the tables in LDS (offset po) when they fit the launch's
+// dynamic LDS, and the separations are formed in FP32; else (false) they
+// are read from global memory.
+__device__ __forceinline__ bool pair_pos_in_lds(const SysDev &V, int lds, size_t &po) {
+  const TableDev &t = V.tab;
+  size_t tb = pair_bin_bytes(t.nptpt, t.nbin) + pair_zl_bytes(t.nzl);
+  if (t.lds_mode >= 2) tb += (size_t)t.nzl * kCoefStride * sizeof(float);
+  po = (tb + 15) / 16 * 16;
+  return V.pair_pos64 == 0 && po + (size_t)V.np * sizeof(float4) <= (size_t)lds;
+}
+
 // FP32 pair forces in cell order, one lane per particle (pair_eval_f);
 // per-wavefront pair-type energies in wred[wave][0..P), virial in [P].
 // NTY bounds the number of particle types (energies per partner type).
-template <int NTY>
-MFE_STAGE void p_pair(const SysDev &V, int energy, char *smem, double (*wred)[40],
+// POSL: the partners' positions are read from an LDS copy (pair_pos_in_lds),
+// a few hundred cycles instead of an L2 round trip per batch of partners.
+template <int NTY, bool POSL>
+MFE_STAGE void p_pair(const SysDev &V, int energy, char *smem, int lds, double (*wred)[40],
                       PairRecF *rec_s, int *pidx_s, float4 *off_s) {
   const int tid = threadIdx.x, NT = blockDim.x, np = V.np;
   const int lane = tid & 63, wave = tid >> 6;
@@ -2659,6 +2673,14 @@ MFE_STAGE void p_pair(const SysDev &V, int energy, char *smem, double (*wred)[40
   for (int k = tid; k < 125; k += NT)
     off_s[k] = make_float4(V.cg.off[0][k % 5], V.cg.off[1][(k / 5) % 5], V.cg.off[2][k / 25], 0.f);
   for (int k = lane; k <= P; k += 64) wred[wave][k] = 0.0;
+  const float4 *__restrict__ pos = V.pos_f;
+  if constexpr (POSL) {
+    size_t po = 0;
+    (void)pair_pos_in_lds(V, lds, po);
+    float4 *pos_s = reinterpret_cast<float4 *>(smem + po);
+    for (int k = tid; k < np; k += NT) pos_s[k] = V.pos_f[k];
+    pos = pos_s;
+  }
   __syncthreads();
   const PairTabF T{bin_s, zl_s, cf_lds ? cf_s : (const float *)t.coef_f, rec_s, nbin, V.nlsem,
                    (float)t.rcut2};
@@ -2679,7 +2701,7 @@ MFE_STAGE void p_pair(const SysDev &V, int energy, char *smem, double (*wred)[40
       // boxes with an axis of fewer than three list cells: separations in
       // FP64 from the cell-ordered positions (as k_pair_f's POS64 variant)
       const float4 pi = pos64 ? make_float4(0.f, 0.f, 0.f, __int_as_float(V.type_s[tt]))
-                              : V.pos_f[tt];
+                              : pos[tt];
       const double xi = pos64 ? V.ro_s[3 * tt] : 0.0, yi = pos64 ? V.ro_s[3 * tt + 1] : 0.0,
                    zi = pos64 ? V.ro_s[3 * tt + 2] : 0.0;
       const int n = V.nn[tt];
@@ -2687,16 +2709,23 @@ MFE_STAGE void p_pair(const SysDev &V, int energy, char *smem, double (*wred)[40
       unsigned long long ptp = 0;
       for (int u = 0; u < npt; ++u) ptp |= (unsigned long long)pidx_s[ti * npt + u] << (8 * u);
       KahanF fxf, fyf, fzf;
-      // the indices of the next batch are loaded while this one is
-      // evaluated (one memory round trip per batch on the critical path)
-      unsigned en[kPairBatch];
+      // the indices of the next two batches are in flight while this one
+      // is evaluated (lists of tens of partners: the list loads, not the
+      // arithmetic, would otherwise set the pace)
+      unsigned en[kPairBatch], en2[kPairBatch];
 #pragma unroll
       for (int q = 0; q < kPairBatch; ++q) en[q] = nbr[(size_t)min(q, maxrow) * np + tt];
+#pragma unroll
+      for (int q = 0; q < kPairBatch; ++q)
+        en2[q] = nbr[(size_t)min(kPairBatch + q, maxrow) * np + tt];
       for (int k0 = 0; k0 < n; k0 += kPairBatch) {
         unsigned es[kPairBatch];
         float4 ps[kPairBatch];
 #pragma unroll
-        for (int q = 0; q < kPairBatch; ++q) es[q] = en[q];
+        for (int q = 0; q < kPairBatch; ++q) {
+          es[q] = en[q];
+          en[q] = en2[q];
+        }
         if (pos64) {
 #pragma unroll
           for (int q = 0; q < kPairBatch; ++q) {
@@ -2709,11 +2738,11 @@ MFE_STAGE void p_pair(const SysDev &V, int energy, char *smem, double (*wred)[40
         } else {
 #pragma unroll
           for (int q = 0; q < kPairBatch; ++q)
-            ps[q] = V.pos_f[k0 + q < n ? (int)(es[q] & kNbrIdxMask) : tt];
+            ps[q] = pos[k0 + q < n ? (int)(es[q] & kNbrIdxMask) : tt];
         }
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q)
-          en[q] = nbr[(size_t)min(k0 + kPairBatch + q, maxrow) * np + tt];
+          en2[q] = nbr[(size_t)min(k0 + 2 * kPairBatch + q, maxrow) * np + tt];
 #pragma unroll
         for (int q = 0; q < kPairBatch; ++q) {
           if (k0 + q >= n) break;
@@ -2987,9 +3016,16 @@ __device__ void persist_replica(const SysDev &V, int nsteps, int energy, char *s
   for (int step = 0; step < nsteps; ++step) {
     const int rebuild = p_move_skin(V, stepctr, wm, &flag_s);
     const int npt = V.tab.npt;
-    if (npt <= 2) p_pair<2>(V, energy, smem, wred, rec_s, pidx_s, off_s);
-    else if (npt <= 4) p_pair<4>(V, energy, smem, wred, rec_s, pidx_s, off_s);
-    else p_pair<MFE_MAX_TYPES>(V, energy, smem, wred, rec_s, pidx_s, off_s);
+    size_t po = 0;
+    if (pair_pos_in_lds(V, lds, po)) {
+      if (npt <= 2) p_pair<2, true>(V, energy, smem, lds, wred, rec_s, pidx_s, off_s);
+      else if (npt <= 4) p_pair<4, true>(V, energy, smem, lds, wred, rec_s, pidx_s, off_s);
+      else p_pair<MFE_MAX_TYPES, true>(V, energy, smem, lds, wred, rec_s, pidx_s, off_s);
+    } else {
+      if (npt <= 2) p_pair<2, false>(V, energy, smem, lds, wred, rec_s, pidx_s, off_s);
+      else if (npt <= 4) p_pair<4, false>(V, energy, smem, lds, wred, rec_s, pidx_s, off_s);
+      else p_pair<MFE_MAX_TYPES, false>(V, energy, smem, lds, wred, rec_s, pidx_s, off_s);
+    }
     p_spread<ORDER>(V, smem, lds);
     double esum = 0.0, vsum = 0.0;
     p_fftx_fwd(V, smem, lds);
