@@ -280,7 +280,9 @@ int mfe_step_md(mfe_engine *e, int32_t nsteps, double tstep, uint32_t flags);
    stream (see mfe_set_bd) instead of GauRandom's serial LCG
    (Src/mollib.F90:2739-2766), so replicas draw independent, reproducible
    noise without a host round trip.  The reference's first BD step uses
-   the forces of the preceding UTotal; call mfe_compute first. */
+   the forces of the preceding UTotal: when the engine's forces are not
+   those of its current positions (no mfe_compute or step since the
+   positions or the setup changed), the call evaluates them first. */
 int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1,
                 const double *fac2, const double *noise, uint32_t flags);
 /* BDStep's per-type factors (fac1 = dt D/kT in force units, fac2 =
@@ -398,7 +400,8 @@ int mfe_batch_step_md(mfe_batch *b, int32_t nsteps, double tstep, uint32_t flags
    queue (costliest first) and run all nsteps of a replica with workgroup
    barriers only -- every stage of BDStep, UTotal and the gated SetVListMD.
    A replica's result there depends on that replica alone.  Other batches
-   replay one captured graph per step. */
+   replay one captured graph per step.  As in mfe_step_bd, replicas whose
+   forces are not those of their current positions are evaluated first. */
 int mfe_batch_step_bd(mfe_batch *b, int32_t nsteps, uint32_t flags);
 /* No reference counterpart: whether mfe_batch_step_bd may take the
    persistent launch (on = 1, the default) or always replays the per-stage

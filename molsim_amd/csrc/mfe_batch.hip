@@ -234,7 +234,9 @@ int mfe_batch_compute(mfe_batch *b, uint32_t flags) {
     int rc = prepare(b);
     if (rc) return rc;
     mfe_launch_utotal(b->grp, flags | MFE_ENERGY | MFE_VIRIAL, 0, 0.0, false, false);
-    return check(b);
+    rc = check(b);
+    for (mfe_engine *e : b->eng) e->forces_current = rc == MFE_OK;
+    return rc;
   });
 }
 
@@ -324,7 +326,9 @@ int mfe_batch_step_md(mfe_batch *b, int32_t nsteps, double tstep, uint32_t flags
   return bguarded(b, [&]() -> int {
     int rc = prepare(b);
     if (rc) return rc;
-    return run_graph(b, 1, nsteps, tstep, flags);
+    rc = run_graph(b, 1, nsteps, tstep, flags);
+    for (mfe_engine *e : b->eng) e->forces_current = rc == MFE_OK;
+    return rc;
   });
 }
 
@@ -336,9 +340,20 @@ int mfe_batch_step_bd(mfe_batch *b, int32_t nsteps, uint32_t flags) {
       if (!b->eng[r]->have_bd) return bfail(b, MFE_E_STATE, "BD factors not set (mfe_set_bd)", (int)r);
     int rc = prepare(b);
     if (rc) return rc;
+    bool current = true;
+    for (mfe_engine *e : b->eng) current = current && e->forces_current;
+    if (!current) {
+      // the first move takes the forces at the current positions (the
+      // reference's UTotal before its BD loop)
+      mfe_launch_utotal(b->grp, 0u, 0, 0.0, false, false);
+      rc = check(b);
+      if (rc) return rc;
+    }
     size_t lds = 0;
-    if (nsteps > 0 && persist_eligible(b, &lds)) return run_persistent(b, nsteps, flags, lds);
-    return run_graph(b, 2, nsteps, 0.0, flags);
+    if (nsteps > 0 && persist_eligible(b, &lds)) rc = run_persistent(b, nsteps, flags, lds);
+    else rc = run_graph(b, 2, nsteps, 0.0, flags);
+    for (mfe_engine *e : b->eng) e->forces_current = rc == MFE_OK;
+    return rc;
   });
 }
 

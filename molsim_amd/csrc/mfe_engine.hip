@@ -504,6 +504,7 @@ void sync_view(mfe_engine *e) {
   mfe_engine *one[1] = {e};
   mfe_group_build(one, 1, e->d_sys, e->stream, e->grp, err);
   e->grp.side = e->side;
+  e->grp.flags = e->flags;
   e->grp.rebuild_bar = e->rebuild_bar;
   e->grp.rebuild_sum = e->rebuild_sum;
   e->grp.ncu = e->ncu;
@@ -517,6 +518,7 @@ void touch(mfe_engine *e) {
   e->graph_dirty = true;
   e->sys_dirty = true;
   e->mc_cached = false;
+  e->forces_current = false;
   ++e->version;
 }
 
@@ -1190,6 +1192,7 @@ int mfe_get_angle_energy(mfe_engine *e, double *u_angle) {
 
 int mfe_upload_positions(mfe_engine *e, const double *ro) {
   if (!e || !ro) return MFE_E_BAD_PARAM;
+  e->forces_current = false;
   e->mc_cached = false;
   if (!e->have_particles || !e->have_box) return fail(e, MFE_E_STATE, "box and particles first");
   for (int i = 0; i < 3 * e->np; ++i) {
@@ -1289,6 +1292,7 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo, mfe_energy *energ
     fetch_energy(e);
     int r = read_flags(e);
     if (r) return r;
+    e->forces_current = true;
     if (energy) energy_out(e, energy);
     return MFE_OK;
   });
@@ -1525,6 +1529,7 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
 // reuse_current spreads nothing; otherwise the next trial spreads anew.
 int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const double *rtm) {
   if (!e || !ianatm || !rtm || natm < 1) return MFE_E_BAD_PARAM;
+  e->forces_current = false;
   std::vector<int> ia;
   int rc = mc_prepare(e, natm, ianatm, rtm, ia);
   if (rc) return rc;
@@ -1600,7 +1605,9 @@ int mfe_step_md(mfe_engine *e, int32_t nsteps, double tstep, uint32_t flags) {
     MFE_HIP_TRY(e->md_graph.run(e->stream, nsteps,
                                 [&]() { mfe_launch_md_step(e->grp, tstep, flags); }));
     fetch_energy(e);
-    return read_flags(e);
+    const int r = read_flags(e);
+    e->forces_current = r == MFE_OK;
+    return r;
   });
 }
 
@@ -1648,6 +1655,11 @@ int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1, const double 
       ++e->version;
     }
     sync_view(e);
+    if (!e->forces_current) {
+      // the first move takes the forces at the current positions (the
+      // reference's UTotal before its BD loop)
+      mfe_launch_utotal(e->grp, 0u, 0, 0.0, false, false);
+    }
     if (noise) {
       // BDStep with the host's deviates (one upload per step)
       for (int s2 = 0; s2 < nsteps; ++s2) {
@@ -1666,7 +1678,9 @@ int mfe_step_bd(mfe_engine *e, int32_t nsteps, const double *fac1, const double 
                                   [&]() { mfe_launch_bd_step(e->grp, 2, flags); }));
     }
     fetch_energy(e);
-    return read_flags(e);
+    const int r = read_flags(e);
+    e->forces_current = r == MFE_OK;
+    return r;
   });
 }
 
@@ -1721,6 +1735,7 @@ int mfe_set_knots(mfe_engine *e, int32_t nknot, double r_lo, double r_hi, const 
 int mfe_adjoint(mfe_engine *e, double *dE_dq, double *dE_dzat, double *dE_dsigat,
                 double *dE_depsat, double *dE_dknot, double *dE_dro, double *energy) {
   if (!e) return MFE_E_BAD_PARAM;
+  e->forces_current = false;
   int rc = check_ready(e);
   if (rc) return rc;
   if (!e->have_ewald) return fail(e, MFE_E_STATE, "mfe_set_ewald not called");
@@ -1841,6 +1856,7 @@ int fm_check(mfe_engine *e) {
 
 int mfe_model_forces(mfe_engine *e, double *force) {
   if (!e || !force) return MFE_E_BAD_PARAM;
+  e->forces_current = false;
   int rc = fm_check(e);
   if (rc) return rc;
   if (!e->list_valid) {
@@ -1866,6 +1882,7 @@ int mfe_model_forces(mfe_engine *e, double *force) {
 
 int mfe_fm_set_frames(mfe_engine *e, int32_t nframes, const double *ro, const double *f_target) {
   if (!e || nframes < 1 || !ro || !f_target) return MFE_E_BAD_PARAM;
+  e->forces_current = false;
   if (!e->have_particles || !e->have_box) return fail(e, MFE_E_STATE, "box and particles first");
   const size_t n3 = 3 * (size_t)e->np;
   for (size_t k = 0; k < n3 * nframes; ++k) {
@@ -1892,6 +1909,7 @@ int mfe_fm_set_frames(mfe_engine *e, int32_t nframes, const double *ro, const do
 int mfe_fm_loss_grad(mfe_engine *e, double *loss, double *dL_dzat, double *dL_dsigat,
                      double *dL_depsat, double *dL_dknot) {
   if (!e) return MFE_E_BAD_PARAM;
+  e->forces_current = false;
   int rc = fm_check(e);
   if (rc) return rc;
   if (e->fm_nframes < 1 || e->fm_frame_np != e->np)
@@ -1987,6 +2005,7 @@ int mfe_get_pairs(mfe_engine *e, int32_t *nneighpn, int32_t *jpnlist, int32_t ma
 
 int mfe_time_stage(mfe_engine *e, int32_t stage, int32_t nrep, double *ms) {
   if (!e || !ms || nrep < 1 || stage < 0 || stage >= MFE_NSTAGES) return MFE_E_BAD_PARAM;
+  e->forces_current = false;
   int rc = check_ready(e);
   if (rc) return rc;
   if (!e->list_valid) {

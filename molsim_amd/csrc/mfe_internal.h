@@ -420,7 +420,9 @@ struct Group {
   size_t lds_spread = 0, lds_gather = 0, lds_fx = 0, lds_fy = 0, lds_fz = 0;
   size_t lds_pair = 0;
   int pair_npt = 0;         // most types of any replica
-  // fused list rebuild (one engine): grid barrier state and CU count
+  // fused list rebuild (one engine): its flags, grid barrier state and CU
+  // count
+  int *flags = nullptr;
   unsigned *rebuild_bar = nullptr;
   int *rebuild_sum = nullptr;
   int ncu = 0;
@@ -453,6 +455,9 @@ struct mfe_engine {
   bool have_box = false, have_particles = false, have_tables = false;
   bool have_ewald = false, have_neighbor = false, have_positions = false;
   bool list_valid = false;
+  // forceo holds the forces at the current positions (a UTotal or a step
+  // since the last change): a BD step's first move needs them
+  bool forces_current = false;
   double self_energy = 0.0;  // -alpha/sqrt(pi) * sum q^2 (UEwaldSelf)
   bool net_charge = false;   // neutralising-background term on (opt-in)
 

@@ -447,10 +447,13 @@ __device__ __forceinline__ bool grid_sync(unsigned *bar, int *flags, int spin) {
 // until the host rebuilds it); otherwise the phases run in order,
 // separated by grid barriers.  Launched with one 1024-thread block per
 // CU, so all blocks are resident.
-__global__ void __launch_bounds__(1024) k_rebuild(const SysDev *__restrict__ sys, unsigned *bar,
+// gate: the engine's flags (V.flags), passed directly so that the skin
+// decision is one load away rather than behind the view's.
+__global__ void __launch_bounds__(1024) k_rebuild(const SysDev *__restrict__ sys,
+                                                  const int *__restrict__ gate, unsigned *bar,
                                                   int *bsum, int spin) {
+  if (gate[1] == 0 || (gate[0] & ERR_SYNC)) return;
   const SysDev &V = sys[0];
-  if (V.flags[1] == 0 || (V.flags[0] & ERR_SYNC)) return;
   const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   ph_cell_count(V, t0, nt);
   if (!grid_sync(bar, V.flags, spin)) return;
@@ -3134,9 +3137,9 @@ void launch_fill_double(double *p, int n, double v, hipStream_t s) {
 }
 
 void launch_nlist_rebuild(const Group &g, bool gated) {
-  if (gated && g.R == 1 && g.rebuild_bar && g.ncu > 0) {
-    hipLaunchKernelGGL(k_rebuild, dim3(g.ncu), dim3(1024), 0, g.stream, g.d, g.rebuild_bar,
-                       g.rebuild_sum, g.sync_spin >= 0 ? g.sync_spin : kGridSpin);
+  if (gated && g.R == 1 && g.rebuild_bar && g.flags && g.ncu > 0) {
+    hipLaunchKernelGGL(k_rebuild, dim3(g.ncu), dim3(1024), 0, g.stream, g.d, g.flags,
+                       g.rebuild_bar, g.rebuild_sum, g.sync_spin >= 0 ? g.sync_spin : kGridSpin);
     return;
   }
   // gated kernels run on small grids (grid-stride loops): when the skin
