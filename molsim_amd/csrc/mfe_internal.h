@@ -396,8 +396,14 @@ struct StepGraphs {
   static hipError_t capture(hipStream_t s, int n, F &launch_step, hipGraphExec_t *out) {
     hipError_t rc = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (rc != hipSuccess) return rc;
-    for (int k = 0; k < n; ++k) launch_step();
     hipGraph_t g = nullptr;
+    try {
+      for (int k = 0; k < n; ++k) launch_step();
+    } catch (...) {
+      // leave the stream out of capture mode before the error goes on
+      if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+      throw;
+    }
     rc = hipStreamEndCapture(s, &g);
     if (rc != hipSuccess) return rc;
     rc = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
