@@ -214,16 +214,21 @@ int mfe_compute(mfe_engine *e, uint32_t flags, double *forceo,
    (1-based) move from their current positions to rtm(3,natm), in the
    units of mfe_energy.rec (EpsiFourPi included; the self term does not
    change).  Nothing of the engine's state changes (the next force call
-   spreads the mesh again).  The engine keeps the current configuration's
-   mesh and E_rec, as the reference keeps QMesh and urecold: reuse_current
-   = 1 starts from them and skips spreading and transforming the current
-   configuration (the caller states that nothing was accepted since, other
-   than through mfe_mc_accept); the
-   engine still spreads anew after any upload, step or setup call, and on
-   its first call.  FP64
-   engines give the difference to ~1e-12 of |E_rec|; mixed ones subtract
-   two FP32-mesh energies and only reach ~1e-6 of |E_rec|.  SPME only
-   (MFE_E_STATE otherwise). */
+   spreads the mesh again).  The reference forms the trial mesh and takes
+   its energy minus urecold; the transform is linear, so the engine keeps
+   the spectrum F of the current configuration's mesh (as the reference
+   keeps QMesh) and adds the transform D of the moved stencils, a product
+   of one-dimensional sums per particle: the change is the sum over the
+   spectrum of the convolution's weights times 2 Re(conj(F) D) + |D|^2 --
+   one pass, no transform per trial, no difference of two large energies.
+   reuse_current = 1 starts from the kept spectrum and skips spreading and
+   transforming the current configuration (the caller states that nothing
+   was accepted since, other than through mfe_mc_accept); the engine still
+   spreads anew after any upload, step or setup call, and on its first
+   call.  The change agrees with the difference of the two energies to
+   ~1e-11 of |E_rec| on FP64 engines and ~1e-6 of itself (a few 1e-6 kT
+   for one-particle moves) on mixed ones.  SPME only (MFE_E_STATE
+   otherwise). */
 int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm,
                      const double *rtm, int32_t reuse_current, double *du_rec);
 /* Monte Carlo trial move, pair part: UTwoBodyANew - UTwoBodyAOld
@@ -260,11 +265,11 @@ int mfe_mc_delta_pair_noalloc(mfe_engine *e, int32_t natm, const int32_t *ianatm
    bookkeeping (the list is rebuilt on the device only when the
    reference's skin test fires, as after mfe_upload_positions).  rtm must
    lie inside the box.  On an FP64 engine whose last mfe_mc_delta_rec was
-   this same move, its trial mesh and E_rec become the current ones, so the
-   next trial spreads nothing; otherwise (any other move, or a mixed
-   engine, whose FP32 tiles would keep the exchange's rounding) the next
-   trial spreads anew.  reuse_current = 1 is valid after this call in both
-   cases. */
+   this same move, its change D is added to the kept spectrum, which then
+   is the current one, so the next trial spreads nothing; otherwise (any
+   other move, or a mixed engine, whose FP32 spectrum would keep the
+   addition's rounding) the next trial spreads anew.  reuse_current = 1 is
+   valid after this call in both cases. */
 int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm,
                   const double *rtm);
 /* nsteps of MDDriver(iSimulationStep) + NList(iSimulationStep): VelVer

@@ -152,7 +152,14 @@ void free_all(mfe_engine *e) {
   dfree(e->b_id); dfree(e->b_slot); dfree(e->rec_frac); dfree(e->rec_meta);
   dfree(e->spread_tiles);
   dfree(e->mc_ia); dfree(e->mc_rtm); dfree(e->mc_out);
-  if (e->mc_tiles) (void)hipFree(e->mc_tiles);
+  if (e->mc_spec) (void)hipFree(e->mc_spec);
+  e->mc_spec = nullptr;
+  e->mc_spec_bytes = 0;
+  dfree(e->mc_fac);
+  dfree(e->mc_cnt);
+  if (e->h_mc) (void)hipHostFree(e->h_mc);
+  e->h_mc = nullptr;
+  e->h_mc_bytes = 0;
   dfree(e->ax_g); dfree(e->ax_kk);
   for (auto &t : e->tw) dfree(t);
   dfree(e->tw_half);
@@ -466,6 +473,7 @@ int mfe_group_build(mfe_engine *const *eng, int R, const mfe::SysDev *d, hipStre
       mfe::fft_geometry(e->prec, e->s, &fx_rpb, &fx_nblk, &fy_T, &fy_nbx, &fz_T, &fz_nbx,
                         &lfx, &lfy, &lfz);
       g.bricks = std::max(g.bricks, e->nbricks);
+      for (int m = 0; m < 3; ++m) g.mc_s[m] = std::max(g.mc_s[m], e->s[m]);
       g.mesh_blocks = std::max(g.mesh_blocks, (e->s[0] * e->s[1] * e->s[2] + 255) / 256);
       g.fx_blocks = std::max(g.fx_blocks, fx_nblk);
       g.fy_x = std::max(g.fy_x, fy_nbx);
@@ -1329,8 +1337,63 @@ void mc_buffers(mfe_engine *e, int natm) {
     dalloc(e->mc_rtm, 3 * (size_t)natm);
     e->mc_cap = natm;
   }
-  // [E_rec current, E_rec trial, du_twob, overlap, pair-delta block partials]
+  // [reciprocal change, 0, du_twob, overlap, block partials]
   if (!e->mc_out) dalloc(e->mc_out, 4 + 2 * (size_t)mfe::kMcPairBlocks);
+  // pinned staging: 4 results, then the indices, then the positions
+  const size_t bytes = 32 + ((size_t)natm * sizeof(int) + 7) / 8 * 8 + 24 * (size_t)natm;
+  if (e->h_mc_bytes < bytes) {
+    if (e->h_mc) MFE_HIP_TRY(hipHostFree(e->h_mc));
+    e->h_mc = nullptr;
+    e->h_mc_bytes = 0;
+    MFE_HIP_TRY(hipHostMalloc((void **)&e->h_mc, bytes));
+    e->h_mc_bytes = bytes;
+  }
+}
+
+// the move's indices and positions to the device through the pinned
+// staging (asynchronous copies; every MC call ends with a synchronization
+// before the staging is written again)
+void mc_upload(mfe_engine *e, const std::vector<int> &ia, const double *rtm) {
+  const size_t natm = ia.size();
+  int *hi = reinterpret_cast<int *>(e->h_mc + 32);
+  double *hr = reinterpret_cast<double *>(e->h_mc + 32 + (natm * sizeof(int) + 7) / 8 * 8);
+  std::memcpy(hi, ia.data(), natm * sizeof(int));
+  std::memcpy(hr, rtm, 3 * natm * sizeof(double));
+  upload(e->mc_ia, hi, natm, e->stream);
+  upload(e->mc_rtm, hr, 3 * natm, e->stream);
+}
+
+// n results from mc_out + off, through the pinned staging (synchronizes)
+void mc_results(mfe_engine *e, int off, int n, double *out) {
+  double *h = reinterpret_cast<double *>(e->h_mc);
+  MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out + off, n * sizeof(double), hipMemcpyDeviceToHost,
+                             e->stream));
+  MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+  for (int k = 0; k < n; ++k) out[k] = h[k];
+}
+
+// the kept spectrum (the complex mesh's size) and the stencil factors of
+// natm moved particles; a new spectrum buffer is not a valid one
+void mc_spectrum_buffers(mfe_engine *e, int natm) {
+  const size_t csz = e->prec == MFE_PREC_MIXED ? 8 : 16;
+  const size_t bytes = (size_t)e->hx * e->s[1] * e->s[2] * csz;
+  if (e->mc_spec_bytes != bytes) {
+    if (e->mc_spec) MFE_HIP_TRY(hipFree(e->mc_spec));
+    e->mc_spec = nullptr;
+    e->mc_spec_bytes = 0;
+    e->mc_cached = false;
+    MFE_HIP_TRY(hipMalloc(&e->mc_spec, bytes));
+    e->mc_spec_bytes = bytes;
+  }
+  const size_t nfac = 2 * 2 * (size_t)natm * (e->s[0] + e->s[1] + e->s[2]);
+  if (e->mc_fac_cap < nfac) {
+    dalloc(e->mc_fac, nfac);
+    e->mc_fac_cap = nfac;
+  }
+  if (!e->mc_cnt) {
+    dalloc(e->mc_cnt, 1);
+    mfe::launch_fill_int(e->mc_cnt, 1, 0, e->stream);
+  }
 }
 
 }  // namespace
@@ -1349,14 +1412,12 @@ int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm, const 
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
     mc_buffers(e, natm);
-    upload(e->mc_ia, ia.data(), (size_t)natm, e->stream);
-    upload(e->mc_rtm, rtm, 3 * (size_t)natm, e->stream);
+    mc_upload(e, ia, rtm);
     sync_view(e);
     mfe::launch_mc_pair(e->grp, natm, e->mc_ia, e->mc_rtm, e->mc_out + 2, e->mc_out + 4,
                         e->nlkind == MFE_NL_CELL);
     double h[2];
-    MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out + 2, sizeof(h), hipMemcpyDeviceToHost, e->stream));
-    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    mc_results(e, 2, 2, h);
     *overlap = h[1] != 0.0 ? 1 : 0;
     *du_twob = *overlap ? 0.0 : h[0];
     return MFE_OK;
@@ -1479,43 +1540,28 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
     mc_buffers(e, natm);
-    if (e->mc_tiles_bytes != e->spread_tiles_bytes) {
-      if (e->mc_tiles) MFE_HIP_TRY(hipFree(e->mc_tiles));
-      e->mc_tiles = nullptr;
-      e->mc_tiles_bytes = 0;
-      e->mc_cached = false;
-      MFE_HIP_TRY(hipMalloc(&e->mc_tiles, e->spread_tiles_bytes));
-      e->mc_tiles_bytes = e->spread_tiles_bytes;
-    }
-    upload(e->mc_ia, ia.data(), (size_t)natm, e->stream);
-    upload(e->mc_rtm, rtm, 3 * (size_t)natm, e->stream);
+    mc_spectrum_buffers(e, natm);
+    mc_upload(e, ia, rtm);
     sync_view(e);
     const mfe::Group &G = e->grp;
-    if (reuse_current && e->mc_cached) {
-      // the current configuration's tiles; its E_rec stays in mc_out[0]
-      MFE_HIP_TRY(hipMemcpyAsync(e->spread_tiles, e->mc_tiles, e->mc_tiles_bytes,
-                                 hipMemcpyDeviceToDevice, e->stream));
-    } else {
-      // current mesh: binning reads the cell-ordered positions
+    if (!(reuse_current && e->mc_cached)) {
+      // the current configuration's spectrum: its mesh (binning reads the
+      // cell-ordered positions), forward along x, y and z, kept
       e->mc_cached = false;
       mfe::launch_sort_pos(G);
-      mfe::launch_spme(G, true, SP_BIN, SP_SPREAD);
-      MFE_HIP_TRY(hipMemcpyAsync(e->mc_tiles, e->spread_tiles, e->mc_tiles_bytes,
-                                 hipMemcpyDeviceToDevice, e->stream));
-      mfe::launch_spme(G, true, SP_FFTX, SP_CONV);
-      mfe::launch_rec_sum(G, e->mc_out);
+      mfe::launch_spme(G, true, SP_BIN, SP_FFTY);
+      mfe::fft_z_forward(G, e->mc_spec);
     }
-    // trial mesh
-    mfe::launch_mc_stencils(G, natm, e->mc_ia, e->mc_rtm);
-    mfe::launch_spme(G, true, SP_FFTX, SP_CONV);
-    mfe::launch_rec_sum(G, e->mc_out + 1);
-    double h[2];
-    MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out, sizeof(h), hipMemcpyDeviceToHost, e->stream));
-    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    // the trial's stencil factors, then the energy change over the spectrum
+    mfe::launch_mc_kfac(G, natm, e->mc_ia, e->mc_rtm, e->mc_fac);
+    mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, false, e->mc_out + 4, e->mc_out,
+                          e->mc_cnt);
+    double h = 0.0;
+    mc_results(e, 0, 1, &h);
     e->mc_cached = true;
     e->mc_last_ia = ia;
     e->mc_last_rtm.assign(rtm, rtm + 3 * (size_t)natm);
-    *du_rec = e->efp * (h[1] - h[0]);
+    *du_rec = e->efp * h;
     return MFE_OK;
   });
 }
@@ -1539,11 +1585,10 @@ int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const doub
         return fail(e, MFE_E_BAD_PARAM, "rtm outside the box [-L/2, L/2]");
   if (!e->list_valid || !e->have_positions || !e->have_neighbor)
     return fail(e, MFE_E_STATE, "positions and neighbour list first");
-  // FP64 only: mixed tiles are FP32, and exchanging stencils in them
-  // would leave FP32 rounding that a fresh spread does not have
+  // FP64 only: the mixed spectrum is FP32, and adding the change to it
+  // would leave FP32 rounding that a fresh transform does not have
   const bool keep = e->prec == MFE_PREC_FP64 && e->mc_cached && e->have_ewald &&
-                    e->ewald == MFE_EWALD_SPME &&
-                    e->mc_tiles_bytes == e->spread_tiles_bytes && e->mc_last_ia == ia &&
+                    e->ewald == MFE_EWALD_SPME && e->mc_spec && e->mc_last_ia == ia &&
                     e->mc_last_rtm.size() == 3 * (size_t)natm &&
                     std::equal(e->mc_last_rtm.begin(), e->mc_last_rtm.end(), rtm);
   e->mc_cached = false;
@@ -1552,16 +1597,13 @@ int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const doub
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
     mc_buffers(e, natm);
-    upload(e->mc_ia, ia.data(), (size_t)natm, e->stream);
-    upload(e->mc_rtm, rtm, 3 * (size_t)natm, e->stream);
+    mc_upload(e, ia, rtm);
     sync_view(e);
     const mfe::Group &G = e->grp;
     if (keep) {
-      // the moved stencils exchanged in the kept tiles themselves (old
-      // stencils read from ro, which still holds the old positions)
-      mfe::launch_mc_stencils(G, natm, e->mc_ia, e->mc_rtm, e->mc_tiles);
-      MFE_HIP_TRY(hipMemcpyAsync(e->mc_out, e->mc_out + 1, sizeof(double),
-                                 hipMemcpyDeviceToDevice, e->stream));
+      // the trial's change added to the kept spectrum (its stencil factors
+      // are those of the last trial, this move)
+      mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, true, nullptr, nullptr, nullptr);
     }
     // the accepted positions, then NList(iSimulationStep) as after an upload
     if (!e->ro_stage) dalloc(e->ro_stage, 3 * (size_t)e->max_np);

@@ -130,6 +130,17 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   }
 }
 
+// the z forward transform into out (the MC trial moves' kept spectrum)
+template <typename R>
+__global__ void __launch_bounds__(256) k_fft_z_fwd(const SysDev *__restrict__ sys, void *out) {
+  const SysDev &V = sys[blockIdx.z];
+  const int L = xcd_block(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+  const int bx = L % gridDim.x, y = L / gridDim.x;
+  if (bx >= V.fz_nbx || y >= V.sg.s[1]) return;
+  extern __shared__ char smem[];
+  fftz_fwd_tile<R>(V, bx, y, (cplx<R> *)out, smem);
+}
+
 inline int tile_width(int hx, int target) {
   const int ntiles = (hx + target - 1) / target;
   return (hx + ntiles - 1) / ntiles;
@@ -193,6 +204,13 @@ void fft_y(const Group &g, int inverse) {
     hipLaunchKernelGGL(k_fft_y<float>, gr, dim3(256), g.lds_fy, g.stream, g.d, inverse);
   else
     hipLaunchKernelGGL(k_fft_y<double>, gr, dim3(256), g.lds_fy, g.stream, g.d, inverse);
+}
+void fft_z_forward(const Group &g, void *out) {
+  const dim3 gr(g.fz_x, g.fz_y, g.R);
+  if (g.prec == MFE_PREC_MIXED)
+    hipLaunchKernelGGL(k_fft_z_fwd<float>, gr, dim3(256), g.lds_fz, g.stream, g.d, out);
+  else
+    hipLaunchKernelGGL(k_fft_z_fwd<double>, gr, dim3(256), g.lds_fz, g.stream, g.d, out);
 }
 void fft_conv(const Group &g, bool energy) {
   const dim3 gr(g.fz_x, g.fz_y, g.R);

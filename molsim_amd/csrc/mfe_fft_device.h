@@ -425,6 +425,44 @@ __device__ __forceinline__ void fftconv_tile(const SysDev &V, int bx, int y, int
   }
 }
 
+// The z forward transform alone (no Green function), for the MC trial
+// moves' kept spectrum: tile of kx columns bx * fz_T, ... of line y, read
+// from the complex mesh and written, transformed, to out (same layout).
+template <typename R>
+__device__ __forceinline__ void fftz_fwd_tile(const SysDev &V, int bx, int y, cplx<R> *out,
+                                              char *smem) {
+  const cplx<R> *__restrict__ cm = (const cplx<R> *)V.mesh_c;
+  const int s2 = V.sg.s[1], s3 = V.sg.s[2], hx = V.hx, T = V.fz_T;
+  const FftPlan &pz = V.plan[2];
+  const Twiddle<R> *__restrict__ tw = (const Twiddle<R> *)V.tw[2];
+  const int ld = s3 + 1;
+  cplx<R> *a = reinterpret_cast<cplx<R> *>(smem);
+  cplx<R> *b = a + T * ld;
+  const int kx0 = bx * T;
+  const int nt = min(T, hx - kx0);
+  const size_t zstride = (size_t)s2 * hx;
+  const cplx<R> *base = cm + (size_t)y * hx + kx0;
+  cplx<R> *obase = out + (size_t)y * hx + kx0;
+  Twiddle<R> *tw_s = reinterpret_cast<Twiddle<R> *>(b + T * ld);
+  stage_twiddles(tw_s, tw, s3);
+  batched_copy(
+      T * s3,
+      [&](int e) {
+        const int z = fdiv(e, V.f_fzT), c = e - z * T;
+        return c < nt ? base[z * zstride + c] : cplx<R>{R(0), R(0)};
+      },
+      [&](int e, cplx<R> v) {
+        const int z = fdiv(e, V.f_fzT), c = e - z * T;
+        if (c < nt) a[c * ld + z] = v;
+      });
+  __syncthreads();
+  const cplx<R> *f = lds_fft(a, b, pz, nt, ld, tw_s, false);
+  for (int e = threadIdx.x; e < T * s3; e += blockDim.x) {
+    const int z = fdiv(e, V.f_fzT), c = e - z * T;
+    if (c < nt) obase[z * zstride + c] = f[c * ld + z];
+  }
+}
+
 // ------------------------------------------- whole planes and lines (LDS)
 // The same transforms over larger LDS tiles, for a workgroup that runs all
 // passes of one replica: every kx column of planes z0 .. z0 + nz - 1 along

@@ -438,6 +438,7 @@ struct Group {
   // replicas whose mixed pair kernel forms separations from cell-relative
   // FP32 coordinates / in FP64 (boxes with an axis of < 3 list cells)
   bool pair_any32 = false, pair_any64 = false;
+  int mc_s[3] = {0, 0, 0};  // mesh of a one-replica group (MC trial moves)
   // the pair forces and the gather in one launch (launch_pair_gather):
   // mixed SPME without bonds, surface term, spline knots or FP64
   // separations, coefficients in LDS, both LDS needs within the block limit
@@ -531,8 +532,17 @@ struct mfe_engine {
   int mc_cap = 0;
   // the current configuration's spreading tiles and (in mc_out[0]) its
   // E_rec, kept for callers whose positions have not changed since
-  void *mc_tiles = nullptr;
-  size_t mc_tiles_bytes = 0;
+  // MC trial moves: the kept spectrum (forward transform of the current
+  // configuration's mesh, valid while mc_cached) and the stencil factors
+  void *mc_spec = nullptr;
+  size_t mc_spec_bytes = 0;
+  double *mc_fac = nullptr;
+  size_t mc_fac_cap = 0;
+  int *mc_cnt = nullptr;  // k_mc_kdelta's done counter (zero between calls)
+  // pinned host staging of a move (indices, positions) and its results, so
+  // the copies of an MC call run asynchronously on the stream
+  char *h_mc = nullptr;
+  size_t h_mc_bytes = 0;
   bool mc_cached = false;
   // the move of the last mfe_mc_delta_rec (0-based indices, positions):
   // mfe_mc_accept of the same move keeps its trial mesh and E_rec
@@ -649,17 +659,21 @@ void launch_pair_gather(const Group &g, bool energy, bool skin);
 void launch_move(const Group &g, double dt, int bd);
 // the cell-ordered positions refreshed from the particle-ordered ones
 void launch_sort_pos(const Group &g);
-// Monte Carlo trial move: the natm stencils at the current positions of
-// particles ia[] taken out of the spreading tiles and those at rtm[] added
 // ro_stage[ia[a]] = rtm[a] for a < natm (a trial move written into a copy
 // of the positions)
 void launch_mc_place(const Group &g, int natm, const int *ia, const double *rtm,
                      double *ro_stage);
-// tiles: nullptr = the spreading tiles, else a buffer of their layout
-void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm,
-                        void *tiles = nullptr);
-// the reciprocal energy of the last convolution pass (its block partials)
-void launch_rec_sum(const Group &g, double *out);
+// Monte Carlo trial move, reciprocal part (k_mc_kfac, k_mc_kdelta): the
+// stencil factors of the natm moved particles (current and trial
+// positions) into fac[natm][2][s1 + s2 + s3] (double2), then one pass over
+// the kept spectrum spec: the energy change (without EpsiFourPi) into
+// out[0] over part[kMcPairBlocks] and the zeroed done counter cnt, or
+// (apply) spec += D.  One replica.
+void launch_mc_kfac(const Group &g, int natm, const int *ia, const double *rtm, void *fac);
+void launch_mc_kdelta(const Group &g, int natm, const void *fac, void *spec, bool apply,
+                      double *part, double *out, int *cnt);
+// the z forward transform of the complex mesh into out (no Green function)
+void fft_z_forward(const Group &g, void *out);
 // Monte Carlo trial move, pair part: out[0] the two-body energy change,
 // out[1] 1 when a trial pair overlaps
 // all: every particle a candidate partner (cell-list semantics), over

@@ -1546,52 +1546,135 @@ __global__ void __launch_bounds__(256) k_mc_place(int natm, const int *__restric
 }
 
 // ------------------------------------------------------- MC trial move
-// DUTwoBodyEwaldRecSPM (Src/denergy.F90:1413-1495): the generalized charge
-// mesh of a trial move is the current one less the moved particles' old
-// stencils plus their new ones (DUMakeGQMesh, :1432-1460).  The current
-// mesh is held as the spreading tiles, and each stencil point lies in the
-// extended tile of the brick of its particle's meshmax, the tile the
-// spreading adds it to.  One block: the stencils one after another (old,
-// then new, per moved particle), the points of one stencil over the
-// threads, so every add happens in a fixed order.  Values as the
-// spreading forms them: fraction and charge rounded to R, then
-// q w_z, times w_y, times w_x.
+// DUTwoBodyEwaldRecSPM (Src/denergy.F90:1413-1495) takes the reciprocal
+// energy of the trial mesh -- the current one less the moved particles' old
+// stencils plus their new ones (DUMakeGQMesh, :1432-1460) -- minus the
+// current energy.  The transform is linear, so the trial spectrum is the
+// current one, F, plus D(k), the transform of the stencil difference: per
+// moved particle q (prod_m c'_m(k_m) - prod_m c_m(k_m)), each factor a
+// one-dimensional sum over the particle's stencil along axis m, c(k) =
+// sum_j w_j exp(-2 pi i k (p - j) / s) (weight w_j at mesh point p - j from
+// meshmax p), the forward transform's sign.  The energy change is then
+// sum_k wt(k) G(k) (2 Re(conj(F) D) + |D|^2) with the convolution pass's
+// weights wt and Green function G: one pass over the kept spectrum, no
+// transform per trial, and no difference of two large energies.  Weights
+// and charge are rounded as the spreading rounds them (the fraction and q
+// to R, the B-spline in R), so D takes out the stencil the mesh holds up
+// to the rounding of the spreading's products.
+//
+// The factor tables, fac[(a * 2 + n) * (s1 + s2 + s3) + off_m + k] for
+// moved particle a (n = 0 its current position, 1 the trial one), axis m
+// and wavenumber k, the charge folded into the x factor.
 template <typename R, int ORDER>
-__global__ void __launch_bounds__(256) k_mc_stencils(const SysDev *__restrict__ sys, int natm,
-                                                    const int *__restrict__ ia,
-                                                    const double *__restrict__ rtm,
-                                                    void *tiles_in) {
+__global__ void __launch_bounds__(256) k_mc_kfac(const SysDev *__restrict__ sys, int natm,
+                                                 const int *__restrict__ ia,
+                                                 const double *__restrict__ rtm,
+                                                 double2 *__restrict__ fac) {
   const SysDev &V = sys[blockIdx.z];
   const SpmeGeom g = V.sg;
-  // the spreading tiles, or the given ones of the same layout
-  R *__restrict__ tiles = (R *)(tiles_in ? tiles_in : (void *)V.spread_tiles);
-  const int E0 = g.B[0] + ORDER - 1, E1 = g.B[1] + ORDER - 1, E2 = g.B[2] + ORDER - 1;
-  const size_t ntile = (size_t)E0 * E1 * E2;
-  constexpr int NP = ORDER * ORDER * ORDER;
-  __shared__ R w[3][ORDER];
-  for (int a = 0; a < natm; ++a) {
+  const int s0 = g.s[0], s1 = g.s[1], s2 = g.s[2], ns = s0 + s1 + s2;
+  const int total = natm * 2 * ns;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+    const int row = t / ns, k0 = t - row * ns;
+    const int a = row >> 1, trial = row & 1;
+    const int m = k0 < s0 ? 0 : (k0 < s0 + s1 ? 1 : 2);
+    const int k = k0 - (m == 0 ? 0 : (m == 1 ? s0 : s0 + s1));
     const int i = ia[a];
-    const R q = (R)V.q[i];
-    for (int sgn = 0; sgn < 2; ++sgn) {
-      const double *r = sgn ? rtm + 3 * a : V.ro + 3 * i;
-      int mm[3];
-      double fr[3];
-      mesh_coords(r, g, mm, fr);
-      if (threadIdx.x < 3) bspline<R, ORDER>((R)fr[threadIdx.x], w[threadIdx.x], nullptr);
-      __syncthreads();
-      const int b0 = mm[0] / g.B[0], b1 = mm[1] / g.B[1], b2 = mm[2] / g.B[2];
-      const int lx = mm[0] - b0 * g.B[0] + ORDER - 1;
-      const int ly = mm[1] - b1 * g.B[1] + ORDER - 1;
-      const int lz = mm[2] - b2 * g.B[2] + ORDER - 1;
-      R *tile = tiles + (size_t)(b0 + g.nb[0] * (b1 + g.nb[1] * b2)) * ntile;
-      for (int k = threadIdx.x; k < NP; k += blockDim.x) {
-        const int j = k % ORDER, jy = (k / ORDER) % ORDER, iz = k / (ORDER * ORDER);
-        const R v = ((q * w[2][iz]) * w[1][jy]) * w[0][j];
-        R &t = tile[((size_t)(lz - iz) * E1 + (ly - jy)) * E0 + (lx - j)];
-        t = sgn ? t + v : t - v;
-      }
-      __syncthreads();
+    int mm[3];
+    double fr[3];
+    mesh_coords(trial ? rtm + 3 * a : V.ro + 3 * i, g, mm, fr);
+    R w[ORDER];
+    bspline<R, ORDER>((R)fr[m], w, nullptr);
+    const int sm = g.s[m];
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int j = 0; j < ORDER; ++j) {
+      int pt = (mm[m] - j) % sm;
+      pt += pt < 0 ? sm : 0;
+      // the phase k pt / s reduced exactly
+      const int ph = (int)(((long long)k * pt) % sm);
+      double sn, cs;
+      sincospi(-2.0 * (double)ph / (double)sm, &sn, &cs);
+      re += (double)w[j] * cs;
+      im += (double)w[j] * sn;
     }
+    if (m == 0) {
+      const double q = (double)(R)V.q[i];
+      re *= q;
+      im *= q;
+    }
+    fac[t] = make_double2(re, im);
+  }
+}
+
+// One pass over the kept spectrum F (the complex mesh's layout, kx
+// fastest, fewer than 2^31 points): apply == 1 adds D to F (an accepted
+// move); apply == 0 sums the energy change, each block its part into
+// part[blockIdx.x], and the last block to finish (an acquire-release done
+// counter, cnt, left zero) adds the parts in a fixed order into out[0].
+// The grid is fixed, so each thread's points and the sums' order are too.
+// One replica (an engine's MC move): cnt is the engine's.
+template <typename R>
+__global__ void __launch_bounds__(256) k_mc_kdelta(const SysDev *__restrict__ sys, int natm,
+                                                   const double2 *__restrict__ fac, void *spec,
+                                                   int apply, double *__restrict__ part,
+                                                   double *__restrict__ out, int *cnt) {
+  const SysDev &V = sys[0];
+  const SpmeGeom g = V.sg;
+  const SpmeAxes &ax = V.ax;
+  const int s0 = g.s[0], s1 = g.s[1], s2 = g.s[2], hx = V.hx, ns = s0 + s1 + s2;
+  cplx<R> *__restrict__ F = (cplx<R> *)spec;
+  const unsigned n = (unsigned)hx * s1 * s2;
+  double acc = 0.0;
+  for (unsigned e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+    const unsigned r = e / (unsigned)hx;
+    const int kx = (int)(e - r * (unsigned)hx);
+    const unsigned kzu = r / (unsigned)s1;
+    const int ky = (int)(r - kzu * (unsigned)s1), kz = (int)kzu;
+    double dr = 0.0, di = 0.0;
+    for (int a = 0; a < natm; ++a) {
+#pragma unroll
+      for (int trial = 0; trial < 2; ++trial) {
+        const double2 *f = fac + (size_t)(2 * a + trial) * ns;
+        const double2 cx = f[kx], cy = f[s0 + ky], cz = f[s0 + s1 + kz];
+        const double pr = cx.x * cy.x - cx.y * cy.y, pi = cx.x * cy.y + cx.y * cy.x;
+        const double tr = pr * cz.x - pi * cz.y, ti = pr * cz.y + pi * cz.x;
+        dr += trial ? tr : -tr;
+        di += trial ? ti : -ti;
+      }
+    }
+    const cplx<R> v = F[e];
+    if (apply) {
+      F[e] = {(R)((double)v.x + dr), (R)((double)v.y + di)};
+    } else {
+      const int nx = kx, ny = min(ky, s1 - ky), nz = min(kz, s2 - kz);
+      const double m2 = ax.kk[0][nx] + ax.kk[1][ny] + ax.kk[2][nz];
+      if (m2 > 0.0) {
+        const double mf = ax.fac0 * ax.g[0][nx] * ax.g[1][ny] * ax.g[2][nz] / m2;
+        const double wt = (nx == 0 || 2 * nx == s0) ? 0.5 : 1.0;
+        acc += wt * mf * (2.0 * ((double)v.x * dr + (double)v.y * di) + (dr * dr + di * di));
+      }
+    }
+  }
+  if (apply) return;
+  __shared__ double sh[256 / 64];
+  __shared__ int last;
+  const double t = block_sum(acc, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = t;
+    // the part is released before the count
+    last = __hip_atomic_fetch_add(cnt, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+           (int)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  double v = 0.0;
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) v += part[b];
+  const double tot = block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    out[0] = tot;
+    __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -1710,18 +1793,6 @@ __global__ void __launch_bounds__(64) k_mc_pair_sum(const double *__restrict__ p
     out[0] = s;
     out[1] = o;
   }
-}
-
-// the reciprocal energy sum of the last convolution pass: its block
-// partials in a fixed order (one wavefront, shuffle tree)
-__global__ void __launch_bounds__(64) k_rec_sum(const SysDev *__restrict__ sys,
-                                                double *__restrict__ out) {
-  const SysDev &V = sys[blockIdx.z];
-  const double *src = V.part + V.off_conv;
-  double acc = 0.0;
-  for (int b = threadIdx.x; b < V.nblk_conv; b += 64) acc += src[b];
-  for (int w = 32; w > 0; w >>= 1) acc += __shfl_down(acc, w, 64);
-  if (threadIdx.x == 0) out[blockIdx.z] = acc;
 }
 
 // --------------------------------------------------------- standard Ewald
@@ -3164,12 +3235,13 @@ void launch_mc_place(const Group &g, int natm, const int *ia, const double *rtm,
                      ro_stage);
 }
 
-void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *rtm,
-                        void *tiles) {
-#define MFE_MC(R, O)                                                                       \
-  case O:                                                                                  \
-    hipLaunchKernelGGL((k_mc_stencils<R, O>), grid(1, g), dim3(256), 0, g.stream, g.d, natm, \
-                       ia, rtm, tiles);                                                    \
+void launch_mc_kfac(const Group &g, int natm, const int *ia, const double *rtm, void *fac) {
+  const int total = natm * 2 * (g.mc_s[0] + g.mc_s[1] + g.mc_s[2]);
+  const dim3 gr(std::max(1, std::min((total + 255) / 256, 1024)), 1, g.R);
+#define MFE_MC(R, O)                                                                         \
+  case O:                                                                                    \
+    hipLaunchKernelGGL((k_mc_kfac<R, O>), gr, dim3(256), 0, g.stream, g.d, natm, ia, rtm,     \
+                       (double2 *)fac);                                                      \
     break;
   if (g.prec == MFE_PREC_MIXED) {
     switch (g.order) { MFE_MC(float, 3) MFE_MC(float, 4) MFE_MC(float, 5) MFE_MC(float, 6)
@@ -3179,6 +3251,17 @@ void launch_mc_stencils(const Group &g, int natm, const int *ia, const double *r
                        MFE_MC(double, 7) MFE_MC(double, 8) }
   }
 #undef MFE_MC
+}
+
+void launch_mc_kdelta(const Group &g, int natm, const void *fac, void *spec, bool apply,
+                      double *part, double *out, int *cnt) {
+  const dim3 gr(kMcPairBlocks);
+  if (g.prec == MFE_PREC_MIXED)
+    hipLaunchKernelGGL(k_mc_kdelta<float>, gr, dim3(256), 0, g.stream, g.d, natm,
+                       (const double2 *)fac, spec, apply ? 1 : 0, part, out, cnt);
+  else
+    hipLaunchKernelGGL(k_mc_kdelta<double>, gr, dim3(256), 0, g.stream, g.d, natm,
+                       (const double2 *)fac, spec, apply ? 1 : 0, part, out, cnt);
 }
 
 void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out,
@@ -3191,10 +3274,6 @@ void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, 
   hipLaunchKernelGGL(k_mc_pair, dim3(kMcPairBlocks), dim3(256), 0, g.stream, g.d, natm, ia, rtm,
                      out, part);
   hipLaunchKernelGGL(k_mc_pair_sum, dim3(1), dim3(64), 0, g.stream, part, kMcPairBlocks, out);
-}
-
-void launch_rec_sum(const Group &g, double *out) {
-  hipLaunchKernelGGL(k_rec_sum, grid(1, g), dim3(64), 0, g.stream, g.d, out);
 }
 
 void launch_pair(const Group &g, bool energy, bool sort_pos, bool skin) {

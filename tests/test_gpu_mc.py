@@ -1,11 +1,13 @@
 """Monte Carlo trial moves: the SPME reciprocal-energy change of moving
 some particles (DUTwoBodyEwaldRecSPM, Src/denergy.F90:1413-1495) against
 the oracle's reciprocal energy (UEwaldRecSPM restated) of both
-configurations.
+configurations.  The engine forms the change from the kept spectrum of the
+current mesh and the transform of the moved stencils (no difference of two
+energies), so its error follows the change, not |E_rec|.
 
-Tolerances: FP64 |du - du_oracle| <= 1e-9 |E_rec| (each energy agrees to
-~1e-11 relative); mixed <= 1e-5 |E_rec| (two FP32-mesh energies are
-subtracted) and, what a Metropolis decision sees, <= 0.02 kT at 298 K.
+Tolerances: FP64 |du - du_oracle| <= 1e-9 |E_rec|; mixed <= 1e-5 |E_rec|
+and, what a Metropolis decision sees, <= 1e-4 kT at 298 K (measured
+<= 6e-6 kT).
 """
 import numpy as np
 import pytest
@@ -47,7 +49,7 @@ def test_mc_delta_rec_vs_oracle(prec, tol):
               f"({abs(du - du0) / KT_298:.1e} kT)", flush=True)
         assert abs(du - du0) <= tol * abs(e0)
         if prec == "mixed":
-            assert abs(du - du0) <= 0.02 * KT_298
+            assert abs(du - du0) <= 1e-4 * KT_298
     # a particle moved across the box edge (the mesh is periodic)
     rtm = s.ro[[9]].copy()
     rtm[0, 0] += 0.5 * s.boxlen
@@ -83,10 +85,10 @@ def test_mc_delta_rec_rejects_bad_moves():
 
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("mixed", 0.0)])
 def test_mc_reuse_current_mesh(prec, tol):
-    """Trials that start from the kept mesh and energy (reuse_current) give
-    the energy changes of full evaluations, and the engine spreads anew
-    once its positions change (bitwise on the mixed path, whose spreading
-    does not depend on the order of its adds)."""
+    """Trials that start from the kept spectrum (reuse_current) give the
+    energy changes of trials that transform the current mesh anew, and the
+    engine transforms anew once its positions change (bitwise on the mixed
+    path, whose spreading does not depend on the order of its adds)."""
     s = S.c2()
     print("case: MC trials from the kept mesh c2", prec, flush=True)
     eng = S.configure(s, prec)
@@ -211,10 +213,10 @@ def test_mc_accept_keeps_the_trial_mesh(prec):
     """EwaldUpdateArray (Src/denergy.F90:1776-1819) with the accepted
     positions: after mc_accept of the move just tried, the next trial with
     reuse_current equals a fresh spread of the accepted configuration
-    uploaded whole (FP64, from the kept trial mesh: <= 1e-12 |E_rec|; mixed,
-    which spreads anew after an accept: bitwise) and the oracle's energy
-    change; positions, forces and energies after the accepts equal those of
-    the uploaded configuration."""
+    uploaded whole (FP64, from the kept spectrum with the accepted change
+    added: <= 1e-12 |E_rec|; mixed, which transforms anew after an accept:
+    bitwise) and the oracle's energy change; positions, forces and energies
+    after the accepts equal those of the uploaded configuration."""
     s = S.c2()
     print("case: MC accept c2", prec, flush=True)
     eng = S.configure(s, prec)
@@ -242,6 +244,7 @@ def test_mc_accept_keeps_the_trial_mesh(prec):
               f"fresh {d_ref:.12e} oracle {d_orc:.12e}", flush=True)
         if prec == "mixed":
             assert d_keep == d_ref
+            assert abs(d_keep - d_orc) <= 1e-4 * KT_298
         else:
             assert abs(d_keep - d_ref) <= 1e-12 * abs(e0)
         assert abs(d_keep - d_orc) <= tol * abs(e0)
