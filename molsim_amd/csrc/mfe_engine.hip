@@ -156,7 +156,6 @@ void free_all(mfe_engine *e) {
   e->mc_spec = nullptr;
   e->mc_spec_bytes = 0;
   dfree(e->mc_fac);
-  dfree(e->mc_cnt);
   if (e->h_mc) (void)hipHostFree(e->h_mc);
   e->h_mc = nullptr;
   e->h_mc_bytes = 0;
@@ -1390,10 +1389,6 @@ void mc_spectrum_buffers(mfe_engine *e, int natm) {
     dalloc(e->mc_fac, nfac);
     e->mc_fac_cap = nfac;
   }
-  if (!e->mc_cnt) {
-    dalloc(e->mc_cnt, 1);
-    mfe::launch_fill_int(e->mc_cnt, 1, 0, e->stream);
-  }
 }
 
 }  // namespace
@@ -1554,8 +1549,7 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
     }
     // the trial's stencil factors, then the energy change over the spectrum
     mfe::launch_mc_kfac(G, natm, e->mc_ia, e->mc_rtm, e->mc_fac);
-    mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, false, e->mc_out + 4, e->mc_out,
-                          e->mc_cnt);
+    mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, false, e->mc_out + 4, e->mc_out);
     double h = 0.0;
     mc_results(e, 0, 1, &h);
     e->mc_cached = true;
@@ -1603,7 +1597,7 @@ int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const doub
     if (keep) {
       // the trial's change added to the kept spectrum (its stencil factors
       // are those of the last trial, this move)
-      mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, true, nullptr, nullptr, nullptr);
+      mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, true, nullptr, nullptr);
     }
     // the accepted positions, then NList(iSimulationStep) as after an upload
     if (!e->ro_stage) dalloc(e->ro_stage, 3 * (size_t)e->max_np);

@@ -538,7 +538,6 @@ struct mfe_engine {
   size_t mc_spec_bytes = 0;
   double *mc_fac = nullptr;
   size_t mc_fac_cap = 0;
-  int *mc_cnt = nullptr;  // k_mc_kdelta's done counter (zero between calls)
   // pinned host staging of a move (indices, positions) and its results, so
   // the copies of an MC call run asynchronously on the stream
   char *h_mc = nullptr;
@@ -667,11 +666,10 @@ void launch_mc_place(const Group &g, int natm, const int *ia, const double *rtm,
 // stencil factors of the natm moved particles (current and trial
 // positions) into fac[natm][2][s1 + s2 + s3] (double2), then one pass over
 // the kept spectrum spec: the energy change (without EpsiFourPi) into
-// out[0] over part[kMcPairBlocks] and the zeroed done counter cnt, or
-// (apply) spec += D.  One replica.
+// out[0] over part[2 * kMcPairBlocks], or (apply) spec += D.  One replica.
 void launch_mc_kfac(const Group &g, int natm, const int *ia, const double *rtm, void *fac);
 void launch_mc_kdelta(const Group &g, int natm, const void *fac, void *spec, bool apply,
-                      double *part, double *out, int *cnt);
+                      double *part, double *out);
 // the z forward transform of the complex mesh into out (no Green function)
 void fft_z_forward(const Group &g, void *out);
 // Monte Carlo trial move, pair part: out[0] the two-body energy change,

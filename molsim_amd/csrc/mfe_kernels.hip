@@ -1609,16 +1609,14 @@ __global__ void __launch_bounds__(256) k_mc_kfac(const SysDev *__restrict__ sys,
 
 // One pass over the kept spectrum F (the complex mesh's layout, kx
 // fastest, fewer than 2^31 points): apply == 1 adds D to F (an accepted
-// move); apply == 0 sums the energy change, each block its part into
-// part[blockIdx.x], and the last block to finish (an acquire-release done
-// counter, cnt, left zero) adds the parts in a fixed order into out[0].
-// The grid is fixed, so each thread's points and the sums' order are too.
-// One replica (an engine's MC move): cnt is the engine's.
+// move); apply == 0 writes this block's part of the energy change to
+// part[2 * blockIdx.x] (and 0 to part[2 * blockIdx.x + 1]), summed by
+// k_mc_pair_sum in a fixed order.  The grid is fixed, so each thread's
+// points and the sums' order are too.  One replica (an engine's MC move).
 template <typename R>
 __global__ void __launch_bounds__(256) k_mc_kdelta(const SysDev *__restrict__ sys, int natm,
                                                    const double2 *__restrict__ fac, void *spec,
-                                                   int apply, double *__restrict__ part,
-                                                   double *__restrict__ out, int *cnt) {
+                                                   int apply, double *__restrict__ part) {
   const SysDev &V = sys[0];
   const SpmeGeom g = V.sg;
   const SpmeAxes &ax = V.ax;
@@ -1658,23 +1656,10 @@ __global__ void __launch_bounds__(256) k_mc_kdelta(const SysDev *__restrict__ sy
   }
   if (apply) return;
   __shared__ double sh[256 / 64];
-  __shared__ int last;
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) {
-    part[blockIdx.x] = t;
-    // the part is released before the count
-    last = __hip_atomic_fetch_add(cnt, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
-           (int)gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  double v = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) v += part[b];
-  const double tot = block_sum(v, sh);
-  if (threadIdx.x == 0) {
-    out[0] = tot;
-    __hip_atomic_store(cnt, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    part[2 * blockIdx.x] = t;
+    part[2 * blockIdx.x + 1] = 0.0;
   }
 }
 
@@ -3254,14 +3239,16 @@ void launch_mc_kfac(const Group &g, int natm, const int *ia, const double *rtm, 
 }
 
 void launch_mc_kdelta(const Group &g, int natm, const void *fac, void *spec, bool apply,
-                      double *part, double *out, int *cnt) {
+                      double *part, double *out) {
   const dim3 gr(kMcPairBlocks);
   if (g.prec == MFE_PREC_MIXED)
     hipLaunchKernelGGL(k_mc_kdelta<float>, gr, dim3(256), 0, g.stream, g.d, natm,
-                       (const double2 *)fac, spec, apply ? 1 : 0, part, out, cnt);
+                       (const double2 *)fac, spec, apply ? 1 : 0, part);
   else
     hipLaunchKernelGGL(k_mc_kdelta<double>, gr, dim3(256), 0, g.stream, g.d, natm,
-                       (const double2 *)fac, spec, apply ? 1 : 0, part, out, cnt);
+                       (const double2 *)fac, spec, apply ? 1 : 0, part);
+  if (!apply)
+    hipLaunchKernelGGL(k_mc_pair_sum, dim3(1), dim3(64), 0, g.stream, part, kMcPairBlocks, out);
 }
 
 void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out,
