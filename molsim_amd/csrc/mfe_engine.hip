@@ -557,10 +557,24 @@ int mfe_map_flag_error(mfe_engine *e, int err) {
 
 void mfe_sync_view(mfe_engine *e) { sync_view(e); }
 
+namespace {
+
+// the finalize launch; with rebuild (MD/BD steps) the gated list rebuild
+// rides in it where the group allows: true when it did
+bool finalize_launch(const mfe::Group &g, unsigned flags, bool skin, bool bd, int mode,
+                     double dt, bool split, bool rebuild) {
+  if (skin && rebuild && mfe::launch_finalize_rebuild(g, flags, bd, mode, dt, split))
+    return true;
+  mfe::launch_reduce_finalize(g, flags, skin, bd, mode, dt, split);
+  return false;
+}
+
+}  // namespace
+
 // UTotal's device sequence; mode 0 = forces only, 1 = with VelVer kick;
 // skin: also take the NList skin decision (MD/BD steps)
-void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt, bool skin,
-                       bool bd) {
+bool mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt, bool skin,
+                       bool bd, bool rebuild) {
   const bool energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) != 0;
   // inside MD/BD steps (skin) k_move has already updated the cell-ordered
   // copy, and the pair launch takes the skin test's block candidates.
@@ -575,8 +589,7 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
     if (!skin) mfe::launch_sort_pos(g);
     mfe::launch_spme(g, energy, SP_BIN, SP_IFFTX);
     mfe::launch_pair_gather(g, energy, skin);
-    mfe::launch_reduce_finalize(g, flags, skin, bd, mode, dt, true);
-    return;
+    return finalize_launch(g, flags, skin, bd, mode, dt, true, rebuild);
   }
   const bool fork = g.ewald == MFE_EWALD_SPME && g.side.stream;
   int first = SP_BIN;
@@ -603,7 +616,7 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
   if (gather_fin) {
     const mfe::FinArgs fa{energy ? 1 : 0, skin ? 1 : 0, bd ? 1 : 0, mode, dt};
     mfe::launch_spme(g, energy, first, SP_GATHER, mfe::GM_FORCE, &fa);
-    return;
+    return false;
   }
   if (g.ewald == MFE_EWALD_SPME) mfe::launch_spme(g, energy, first, SP_GATHER);
   else if (g.ewald == MFE_EWALD_STD) mfe::launch_std_ewald(g);
@@ -613,23 +626,21 @@ void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt,
     // the surface field of the reduction enters the forces
     mfe::launch_reduce(g, flags, skin, bd);
     mfe::launch_finalize(g, mode, dt);
-  } else {
-    mfe::launch_reduce_finalize(g, flags, skin, bd, mode, dt);
+    return false;
   }
+  return finalize_launch(g, flags, skin, bd, mode, dt, false, rebuild);
 }
 
 // one VelVer step with the NList skin test and gated rebuild
 void mfe_launch_md_step(const mfe::Group &g, double dt, unsigned flags) {
   mfe::launch_move(g, dt, 0);
-  mfe_launch_utotal(g, flags, 1, dt, true, false);
-  mfe::launch_nlist_rebuild(g, true);
+  if (!mfe_launch_utotal(g, flags, 1, dt, true, false, true)) mfe::launch_nlist_rebuild(g, true);
 }
 
 // one BDStep (noise: 1 = host-supplied buffer, 2 = counter-based stream)
 void mfe_launch_bd_step(const mfe::Group &g, int noise, unsigned flags) {
   mfe::launch_move(g, 0.0, noise);
-  mfe_launch_utotal(g, flags, 0, 0.0, true, true);
-  mfe::launch_nlist_rebuild(g, true);
+  if (!mfe_launch_utotal(g, flags, 0, 0.0, true, true, true)) mfe::launch_nlist_rebuild(g, true);
 }
 
 namespace {

@@ -653,6 +653,11 @@ void launch_finalize(const Group &g, int mode, double dt);
 // split: the reciprocal forces are in force_rec (after launch_pair_gather)
 void launch_reduce_finalize(const Group &g, unsigned flags, bool skin, bool bd, int mode,
                             double dt, bool split = false);
+// launch_reduce_finalize (with the skin test) followed by the gated
+// rebuild, in one launch; false (nothing launched) when the group cannot
+// take it (several replicas, or no grid barrier state)
+bool launch_finalize_rebuild(const Group &g, unsigned flags, bool bd, int mode, double dt,
+                             bool split);
 // the pair forces and the gather (into force_rec) in one launch
 void launch_pair_gather(const Group &g, bool energy, bool skin);
 void launch_move(const Group &g, double dt, int bd);
@@ -738,8 +743,10 @@ void mfe_fill_view(const mfe_engine *e, mfe::SysDev &v);
 void mfe_sync_view(mfe_engine *e);   // throws HipError
 int mfe_map_flag_error(mfe_engine *e, int bits);
 void mfe_allow_big_spread_lds(int prec, int bytes);
-void mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt, bool skin,
-                       bool bd);
+// rebuild (with skin): the gated list rebuild may ride in the last launch;
+// true when it did (else the caller launches it)
+bool mfe_launch_utotal(const mfe::Group &g, unsigned flags, int mode, double dt, bool skin,
+                       bool bd, bool rebuild = false);
 void mfe_launch_md_step(const mfe::Group &g, double dt, unsigned flags);
 void mfe_launch_bd_step(const mfe::Group &g, int noise, unsigned flags);
 

@@ -441,19 +441,11 @@ __device__ __forceinline__ bool grid_sync(unsigned *bar, int *flags, int spin) {
   return ok_s != 0;
 }
 
-// The gated list rebuild of one engine as one kernel: it returns at once
-// when the skin test did not fire (one launch per step instead of five),
-// or after a failed rebuild earlier in the same call (the list is void
-// until the host rebuilds it); otherwise the phases run in order,
-// separated by grid barriers.  Launched with one 1024-thread block per
-// CU, so all blocks are resident.
-// gate: the engine's flags (V.flags), passed directly so that the skin
-// decision is one load away rather than behind the view's.
-__global__ void __launch_bounds__(1024) k_rebuild(const SysDev *__restrict__ sys,
-                                                  const int *__restrict__ gate, unsigned *bar,
-                                                  int *bsum, int spin) {
-  if (gate[1] == 0 || (gate[0] & ERR_SYNC)) return;
-  const SysDev &V = sys[0];
+// The list rebuild's phases in order, separated by grid barriers, run by
+// every block of a launch whose blocks are all resident (one 1024-thread
+// block per CU).  A block leaves as soon as a barrier gives up.
+__device__ __forceinline__ void rebuild_phases(const SysDev &V, unsigned *bar, int *bsum,
+                                               int spin) {
   const int t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
   ph_cell_count(V, t0, nt);
   if (!grid_sync(bar, V.flags, spin)) return;
@@ -469,6 +461,20 @@ __global__ void __launch_bounds__(1024) k_rebuild(const SysDev *__restrict__ sys
   if (!grid_sync(bar, V.flags, spin)) return;
   ph_nlist_build<true>(V, t0, nt);
   if (t0 == 0) V.flags[2] += 1;
+}
+
+// The gated list rebuild of one engine as one kernel: it returns at once
+// when the skin test did not fire (one launch per step instead of five),
+// or after a failed rebuild earlier in the same call (the list is void
+// until the host rebuilds it); otherwise the phases run.  Launched with
+// one 1024-thread block per CU, so all blocks are resident.
+// gate: the engine's flags (V.flags), passed directly so that the skin
+// decision is one load away rather than behind the view's.
+__global__ void __launch_bounds__(1024) k_rebuild(const SysDev *__restrict__ sys,
+                                                  const int *__restrict__ gate, unsigned *bar,
+                                                  int *bsum, int spin) {
+  if (gate[1] == 0 || (gate[0] & ERR_SYNC)) return;
+  rebuild_phases(sys[0], bar, bsum, spin);
 }
 
 // positions and types in cell order for the pair kernel (every UTotal)
@@ -2550,6 +2556,61 @@ __global__ void __launch_bounds__(kReduceThreads) k_reduce_finalize(const SysDev
   }
 }
 
+// The last launch of an MD or BD step on one engine: k_reduce_finalize's
+// work (block 0 reduces, the other blocks finalize the particles, grid-
+// stride) and then, in the same launch, the gated list rebuild.  Every
+// block takes the skin decision itself, ph_reduce's test on the same block
+// maxima (Src/nlist.F90:601-608), so no block waits for another to learn
+// it, and the launch ends there when the test did not fire.  The rebuild
+// reads the positions and writes the cell, list and cell-ordered arrays;
+// the finalize reads and writes forces, velocities and accelerations only,
+// so the rebuild's first phase needs no barrier after it.  One
+// 1024-thread block per CU, all resident, as the rebuild's grid barriers
+// require.
+__global__ void __launch_bounds__(kReduceThreads) k_finalize_rebuild(
+    const SysDev *__restrict__ sys, int energy, int bd, int mode, double dt, int split,
+    unsigned *bar, int *bsum, int spin) {
+  const SysDev &V = sys[0];
+  const int nb = V.nblk_np;
+  const double *pmove = V.part + V.off_move, *pcand = V.part + V.off_skin;
+  // this thread's share of the two maxima, loaded before the finalize so
+  // that the loads overlap it
+  double m1 = 0.0, m2 = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    m1 = fmax(m1, pmove[b]);
+    m2 = fmax(m2, pcand[b]);
+  }
+  if (blockIdx.x == 0) {
+    ph_reduce(V, energy, 1, bd);
+  } else {
+    const double es[3] = {0.0, 0.0, 0.0};
+    const int stride = ((int)gridDim.x - 1) * kReduceThreads;
+    for (int i = ((int)blockIdx.x - 1) * kReduceThreads + (int)threadIdx.x; i < V.np;
+         i += stride)
+      ph_finalize(V, i, mode, dt, es, split != 0);
+  }
+  __shared__ double mx_s[2][kReduceThreads / 64];
+  for (int w = 32; w > 0; w >>= 1) {
+    m1 = fmax(m1, __shfl_xor(m1, w, 64));
+    m2 = fmax(m2, __shfl_xor(m2, w, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    mx_s[0][threadIdx.x >> 6] = m1;
+    mx_s[1][threadIdx.x >> 6] = m2;
+  }
+  __syncthreads();
+  double M1 = 0.0, M2 = 0.0;
+  for (int w = 0; w < kReduceThreads / 64; ++w) {
+    M1 = fmax(M1, mx_s[0][w]);
+    M2 = fmax(M2, mx_s[1][w]);
+  }
+  if (!(sqrt(M1) + sqrt(M2) > V.drnlist)) return;
+  // after a failed rebuild earlier in the call the list stays void
+  if (__hip_atomic_load(&V.flags[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & ERR_SYNC)
+    return;
+  rebuild_phases(V, bar, bsum, spin);
+}
+
 // ------------------------------------------------- persistent batched BD
 // Many small replicas advanced by BDStep in one launch: a workgroup takes
 // a replica from a queue (the costliest first), runs all nsteps steps of
@@ -3401,6 +3462,16 @@ void launch_reduce_finalize(const Group &g, unsigned flags, bool skin, bool bd, 
   const int F = (g.np_blocks * kBlock + kReduceThreads - 1) / kReduceThreads;
   hipLaunchKernelGGL(k_reduce_finalize, grid(F + 1, g), dim3(kReduceThreads), 0, g.stream, g.d,
                      energy, skin ? 1 : 0, bd ? 1 : 0, mode, dt, split ? 1 : 0);
+}
+
+bool launch_finalize_rebuild(const Group &g, unsigned flags, bool bd, int mode, double dt,
+                             bool split) {
+  if (!(g.R == 1 && g.rebuild_bar && g.flags && g.ncu > 1)) return false;
+  const int energy = (flags & (MFE_ENERGY | MFE_VIRIAL)) ? 1 : 0;
+  hipLaunchKernelGGL(k_finalize_rebuild, dim3(g.ncu), dim3(kReduceThreads), 0, g.stream, g.d,
+                     energy, bd ? 1 : 0, mode, dt, split ? 1 : 0, g.rebuild_bar, g.rebuild_sum,
+                     g.sync_spin >= 0 ? g.sync_spin : kGridSpin);
+  return true;
 }
 
 // the fused pair + gather launch (Group::fuse_pg)
