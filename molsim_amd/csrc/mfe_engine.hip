@@ -550,6 +550,12 @@ int mfe_map_flag_error(mfe_engine *e, int err) {
     e->list_valid = false;
     return fail(e, MFE_E_NEIGH_CAPACITY, "nneighpn > maxnneigh: raise maxnneigh");
   }
+  if (err & mfe::ERR_BIN) {
+    // the brick counts are cleared before the next binning
+    if (e->b_count) mfe::launch_fill_int(e->b_count, e->nbricks, 0, e->stream);
+    MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+    return fail(e, MFE_E_HIP, "SPME binning: brick counts do not add up to the particles");
+  }
   if (err & mfe::ERR_OVERLAP) return fail(e, MFE_E_OVERLAP_R2UMIN, "r2 < r2umin(iptjpt)");
   if (err & mfe::ERR_TABLE) return fail(e, MFE_E_TABLE_RANGE, "ibuf > nbuf");
   return MFE_OK;

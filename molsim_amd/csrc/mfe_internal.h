@@ -55,6 +55,7 @@ enum : int {
   ERR_TABLE = 2,      // no table interval for r2
   ERR_CAPACITY = 4,   // neighbour capacity exceeded
   ERR_SYNC = 8,       // a grid barrier of the fused list rebuild timed out
+  ERR_BIN = 16,       // SPME brick counts that do not add up to the particles
 };
 
 template <typename R>

@@ -1112,6 +1112,12 @@ k_pair_f(const SysDev *__restrict__ sys, int energy, int pgrid, int skin) {
 // one global atomic per particle.
 constexpr int kCountThreads = 1024;
 constexpr int kCountBins = 8192;
+// Up to this many bricks every k_spme_scatter block scans the counts itself
+// (no done counter and no last-block scan in k_spme_count), k_spread clears
+// each brick's count after the scatter has used it, and a binning that is
+// not followed by spreading clears them with k_spme_clear.
+constexpr int kScatterScan = 2048;
+static_assert(kScatterScan % kBlock == 0, "the scatter's scan gives each thread whole runs");
 
 __global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
@@ -1152,6 +1158,7 @@ __global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__re
       bslot[t] = hist[id] + rank;
     }
   }
+  if (V.nbricks <= kScatterScan) return;
   // The last block to finish scans the counts into b_start and clears them
   // (no separate scan launch).  Every count atomic of this block has
   // returned -- its value was used -- before the block barrier, so the
@@ -1172,8 +1179,44 @@ __global__ void __launch_bounds__(kCountThreads) k_spme_count(const SysDev *__re
 // particle records in brick order: fraction (x, y, z) and charge, plus the
 // packed meshmax (10 bits per axis) and the particle index; one thread per
 // place t of the cell order
+// Exclusive scan of the brick counts into LDS (st[k], st[n] = the total),
+// by one block of kBlock threads, each taking a contiguous run of bricks.
+__device__ __forceinline__ void block_scan_counts(const int *__restrict__ count, int *st,
+                                                  int n) {
+  constexpr int kPer = kScatterScan / kBlock;
+  __shared__ int wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lo = threadIdx.x * kPer;
+  int v[kPer];
+  int x = 0;
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    v[u] = lo + u < n ? count[lo + u] : 0;
+    x += v[u];
+  }
+  int inc = x;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += y;
+  }
+  if (lane == 63) wsum[wave] = inc;
+  __syncthreads();
+  int run = inc - x, tot = 0;
+  for (int w = 0; w < kBlock / 64; ++w) {
+    run += w < wave ? wsum[w] : 0;
+    tot += wsum[w];
+  }
+#pragma unroll
+  for (int u = 0; u < kPer; ++u) {
+    if (lo + u < n) st[lo + u] = run;
+    run += v[u];
+  }
+  if (threadIdx.x == 0) st[n] = tot;
+  __syncthreads();
+}
+
 template <typename R>
-__global__ void k_spme_scatter(const SysDev *__restrict__ sys) {
+__global__ void __launch_bounds__(kBlock) k_spme_scatter(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
   const double *__restrict__ ro_s = V.ro_s;
   const double *__restrict__ q = V.q;
@@ -1181,18 +1224,39 @@ __global__ void k_spme_scatter(const SysDev *__restrict__ sys) {
   const int *bid = V.b_id, *bslot = V.b_slot, *bstart = V.b_start;
   R *frac = (R *)V.rec_frac;
   int2 *meta = V.rec_meta;
+  const int nbk = V.nbricks;
+  __shared__ int st[kScatterScan + 1];
+  if (nbk <= kScatterScan) {
+    // the brick starts from the counts k_spme_count left (every block the
+    // same scan); block 0 stores them for the spreading and the gather
+    block_scan_counts(V.b_count, st, nbk);
+    if (st[nbk] != V.np) {
+      if (threadIdx.x == 0) atomicOr(&V.flags[0], ERR_BIN);
+      return;
+    }
+    if (blockIdx.x == 0)
+      for (int k = threadIdx.x; k <= nbk; k += blockDim.x) V.b_start[k] = st[k];
+  }
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= V.np) return;
   const int i = V.c_sorted[t];
   int mm[3];
   double fr[3];
   mesh_coords(ro_s + 3 * t, g, mm, fr);
-  const int pos = bstart[bid[t]] + bslot[t];
+  const int b = bid[t];
+  const int pos = (nbk <= kScatterScan ? st[b] : bstart[b]) + bslot[t];
   frac[4 * pos] = (R)fr[0];
   frac[4 * pos + 1] = (R)fr[1];
   frac[4 * pos + 2] = (R)fr[2];
   frac[4 * pos + 3] = (R)q[i];
   meta[pos] = make_int2(mm[0] | (mm[1] << 10) | (mm[2] << 20), i);
+}
+
+// the brick counts cleared after a binning that no spreading follows
+__global__ void k_spme_clear(const SysDev *__restrict__ sys) {
+  const SysDev &V = sys[blockIdx.z];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (V.nbricks <= kScatterScan && k < V.nbricks) V.b_count[k] = 0;
 }
 
 // Spreading, each particle once and no global atomics: one block per brick
@@ -1215,6 +1279,8 @@ template <typename R, int ORDER>
 __global__ void __launch_bounds__(kSpreadThreads) k_spread(const SysDev *__restrict__ sys) {
   const SysDev &V = sys[blockIdx.z];
   if ((int)blockIdx.x >= V.nbricks) return;
+  // the scatter has used this brick's count: cleared for the next binning
+  if (V.nbricks <= kScatterScan && threadIdx.x == 0) V.b_count[blockIdx.x] = 0;
   const R *__restrict__ frac = (const R *)V.rec_frac;
   const int2 *__restrict__ meta = V.rec_meta;
   const int *__restrict__ bstart = V.b_start;
@@ -3383,6 +3449,9 @@ static void spme_stages(const Group &g, bool energy, int first, int last, int gm
                            dim3(kCountThreads), 0, g.stream, g.d);
         hipLaunchKernelGGL(k_spme_scatter<R>, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream,
                            g.d);
+        if (last == SP_BIN)
+          hipLaunchKernelGGL(k_spme_clear, grid((g.bricks + kBlock - 1) / kBlock, g), dim3(kBlock),
+                             0, g.stream, g.d);
         break;
       case SP_SPREAD:
         switch (g.order) {
