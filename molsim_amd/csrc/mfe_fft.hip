@@ -148,10 +148,13 @@ inline int tile_width(int hx, int target) {
 
 template <typename R>
 int rows_per_block(int s1) {
-  // two LDS buffers of rows*(s1/2+1) complex values, about 32 KB in all
+  // two LDS buffers of rows*(s1/2+1) complex values, about 32 KB in all and
+  // at most 16 rows; at FP64 about 48 KB and at most 32 rows (23 rows of 128
+  // points measured faster there than 15, while FP32 is fastest at 16)
+  constexpr bool f64 = sizeof(R) == 8;
   const int per_row = 2 * (s1 / 2 + 1) * (int)sizeof(cplx<R>);
-  int rows = (32 * 1024) / per_row;
-  return rows < 1 ? 1 : (rows > 16 ? 16 : rows);
+  const int rows = ((f64 ? 48 : 32) * 1024) / per_row, cap = f64 ? 32 : 16;
+  return rows < 1 ? 1 : (rows > cap ? cap : rows);
 }
 
 template <typename R>
@@ -169,7 +172,11 @@ void fft_geometry(int prec, const int s[3], int *fx_rpb, int *fx_nblk, int *fy_T
   const size_t csz = f ? sizeof(cplx<float>) : sizeof(cplx<double>);
   const size_t tsz = f ? sizeof(Twiddle<float>) : sizeof(Twiddle<double>);
   const int hx = s[0] / 2 + 1;
-  const int rpb = f ? rows_per_block<float>(s[0]) : rows_per_block<double>(s[0]);
+  int rpb = f ? rows_per_block<float>(s[0]) : rows_per_block<double>(s[0]);
+  // the rows and the twiddles within 64 KB of LDS
+  while (rpb > 1 && 2 * (size_t)rpb * (s[0] / 2 + 1) * csz + (size_t)(s[0] / 2 + s[0]) * tsz >
+                        64 * 1024)
+    --rpb;
   *fx_rpb = rpb;
   *fx_nblk = (s[1] * s[2] + rpb - 1) / rpb;
   // two row buffers, then the half- and full-length twiddles
