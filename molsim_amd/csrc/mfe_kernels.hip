@@ -1225,6 +1225,19 @@ __global__ void __launch_bounds__(kBlock) k_spme_scatter(const SysDev *__restric
   R *frac = (R *)V.rec_frac;
   int2 *meta = V.rec_meta;
   const int nbk = V.nbricks;
+  // the particle's own loads first, so that they are in flight during the
+  // scan of the counts
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool act = t < V.np;
+  int i = 0, b = 0, slot = 0, mm[3] = {0, 0, 0};
+  double fr[3] = {0.0, 0.0, 0.0}, qi = 0.0;
+  if (act) {
+    i = V.c_sorted[t];
+    b = bid[t];
+    slot = bslot[t];
+    mesh_coords(ro_s + 3 * t, g, mm, fr);
+    qi = q[i];
+  }
   __shared__ int st[kScatterScan + 1];
   if (nbk <= kScatterScan) {
     // the brick starts from the counts k_spme_count left (every block the
@@ -1237,18 +1250,12 @@ __global__ void __launch_bounds__(kBlock) k_spme_scatter(const SysDev *__restric
     if (blockIdx.x == 0)
       for (int k = threadIdx.x; k <= nbk; k += blockDim.x) V.b_start[k] = st[k];
   }
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= V.np) return;
-  const int i = V.c_sorted[t];
-  int mm[3];
-  double fr[3];
-  mesh_coords(ro_s + 3 * t, g, mm, fr);
-  const int b = bid[t];
-  const int pos = (nbk <= kScatterScan ? st[b] : bstart[b]) + bslot[t];
+  if (!act) return;
+  const int pos = (nbk <= kScatterScan ? st[b] : bstart[b]) + slot;
   frac[4 * pos] = (R)fr[0];
   frac[4 * pos + 1] = (R)fr[1];
   frac[4 * pos + 2] = (R)fr[2];
-  frac[4 * pos + 3] = (R)q[i];
+  frac[4 * pos + 3] = (R)qi;
   meta[pos] = make_int2(mm[0] | (mm[1] << 10) | (mm[2] << 20), i);
 }
 
@@ -2370,9 +2377,13 @@ __global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
                                                  double dt, int bd) {
 #pragma clang fp contract(off)
   const SysDev &V = sys[blockIdx.z];
+  if ((int)blockIdx.x >= V.nblk_np) return;
   // after a failed list rebuild (a grid barrier gave up) the state stays
-  // where it was for the rest of the call, which then reports the error
-  if ((int)blockIdx.x >= V.nblk_np || (V.flags[0] & ERR_SYNC)) return;
+  // where it was for the rest of the call, which then reports the error:
+  // the flag is read alongside the particle's loads and holds back only
+  // the stores (the block maximum it still writes feeds a skin decision
+  // that no rebuild acts on after the failure)
+  const bool halt = (V.flags[0] & ERR_SYNC) != 0;
   double *__restrict__ ro = V.ro;
   const double *__restrict__ rod = V.rod;
   const double *__restrict__ rodd = V.rodd;
@@ -2429,17 +2440,19 @@ __global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
       x[m] = xm;
       s[m] = s[m] + d[m];
     }
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      ro[3 * i + m] = x[m];
-      V.ro_s[3 * cp + m] = x[m];
-      drosum[3 * i + m] = s[m];
-    }
-    if (V.pos_f) V.pos_f[cp] = rel_pos(V, x, cell, ty);
     r2 = s[0] * s[0];
     r2 = r2 + s[1] * s[1];
     r2 = r2 + s[2] * s[2];
-    r2disp[i] = r2;
+    if (!halt) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        ro[3 * i + m] = x[m];
+        V.ro_s[3 * cp + m] = x[m];
+        drosum[3 * i + m] = s[m];
+      }
+      if (V.pos_f) V.pos_f[cp] = rel_pos(V, x, cell, ty);
+      r2disp[i] = r2;
+    }
   }
   const double m = block_max(r2, sh);
   if (threadIdx.x == 0) part_max[blockIdx.x] = m;
