@@ -62,3 +62,29 @@ def test_edge_systems_vs_oracle(build, prec, ftol, etol):
     assert abs(en.rec - eo["rec"]) <= etol * abs(eo["rec"])
     assert abs(en.tot - eo["tot"]) <= etol * abs(eo["tot"])
     eng.close()
+
+
+@pytest.mark.parametrize("prec", ["mixed", "fp64"])
+def test_binning_alone_then_utotal(prec):
+    """Binnings that no spreading follows (the stage timings run the count
+    and scatter kernels alone, which then clear the brick counts
+    themselves), and spreadings repeated on one binning, leave the next
+    UTotal as a fresh engine computes it: bitwise on the mixed path, to
+    rounding on the FP64 path (its spreading adds FP64 in LDS in a
+    run-dependent order).  Brick counts left over would make the scatter
+    report MFE_E_HIP."""
+    s = S.c4_replica(7)
+    print("case: bin-only stages, then UTotal, electrolyte N", s.np, prec, flush=True)
+    a, b = S.configure(s, prec), S.configure(s, prec)
+    for st, n in (("bin", 3), ("spread", 2), ("bin", 2), ("fft_x", 1), ("bin", 1)):
+        a.time_stage(st, nrep=n)
+    fa, ea = a.utotal()
+    fb, eb = b.utotal()
+    d = float(np.max(np.abs(fa - fb)) / np.max(np.abs(fb)))
+    print(f"  max |dF|/max|F| {d:.2e}, E {ea.tot:.15e} vs {eb.tot:.15e}", flush=True)
+    if prec == "mixed":
+        assert np.array_equal(fa, fb) and ea.tot == eb.tot
+    else:
+        assert d <= 1e-12 and abs(ea.tot - eb.tot) <= 1e-13 * abs(eb.tot)
+    a.close()
+    b.close()
