@@ -1,7 +1,8 @@
 """ctypes bindings of the FP64 oracle (oracle/mfe_oracle.c).
 
 Test infrastructure: imported only by tests/, the smoke check and the CPU
-baseline leg of bench.py, as the checker.  The product never imports it.
+baseline leg of bench.py, to check results against.  The product never
+imports it.
 
 Every wrapper mirrors one reference routine (cited in mfe_oracle.c) and uses
 MOLSIM's conventions: xyz-interleaved positions (ro(3,np)), 1-based particle

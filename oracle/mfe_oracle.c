@@ -1,8 +1,8 @@
 /*
  * mfe_oracle.c -- FP64 CPU restatement of MOLSIM's nonbonded force path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker): used by tests/, the smoke check
- * and bench.py's cpu_baseline leg; never by the product.  Build with
+ * TEST INFRASTRUCTURE ONLY (results are checked against it): used by tests/,
+ * the smoke check and bench.py's cpu_baseline leg; never by the product.  Build with
  * gcc -O2 -ffp-contract=off so every product of the reference's FP64
  * expressions is rounded separately, as the reference's x86-64 build does.
  *

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (molsim_amd/, include/)
  * links, loads or calls this code; only tests/, the smoke check and the
- * CPU-baseline leg of bench.py use it, as the checker.
+ * CPU-baseline leg of bench.py use it, to check results against.
  *
  * Every function follows one reference routine loop for loop, in the same
  * arithmetic order, and cites it as Src/<file>:<lines>.  Conventions:
