@@ -1366,25 +1366,39 @@ void mc_buffers(mfe_engine *e, int natm) {
   }
 }
 
-// the move's indices and positions to the device through the pinned
-// staging (asynchronous copies; every MC call ends with a synchronization
-// before the staging is written again)
-void mc_upload(mfe_engine *e, const std::vector<int> &ia, const double *rtm) {
+// the move's indices and positions in the pinned staging (device-
+// accessible host memory: a kernel that reads them a few times takes them
+// from there directly)
+void mc_stage(mfe_engine *e, const std::vector<int> &ia, const double *rtm, int **hi_out,
+              double **hr_out) {
   const size_t natm = ia.size();
   int *hi = reinterpret_cast<int *>(e->h_mc + 32);
   double *hr = reinterpret_cast<double *>(e->h_mc + 32 + (natm * sizeof(int) + 7) / 8 * 8);
   std::memcpy(hi, ia.data(), natm * sizeof(int));
   std::memcpy(hr, rtm, 3 * natm * sizeof(double));
-  upload(e->mc_ia, hi, natm, e->stream);
-  upload(e->mc_rtm, hr, 3 * natm, e->stream);
+  *hi_out = hi;
+  *hr_out = hr;
 }
 
-// n results from mc_out + off, through the pinned staging (synchronizes)
-void mc_results(mfe_engine *e, int off, int n, double *out) {
-  double *h = reinterpret_cast<double *>(e->h_mc);
-  MFE_HIP_TRY(hipMemcpyAsync(h, e->mc_out + off, n * sizeof(double), hipMemcpyDeviceToHost,
-                             e->stream));
+// the move's indices and positions to the device through the pinned
+// staging (asynchronous copies; every MC call ends with a synchronization
+// before the staging is written again), for kernels that read them often
+void mc_upload(mfe_engine *e, const std::vector<int> &ia, const double *rtm) {
+  int *hi;
+  double *hr;
+  mc_stage(e, ia, rtm, &hi, &hr);
+  upload(e->mc_ia, hi, ia.size(), e->stream);
+  upload(e->mc_rtm, hr, 3 * ia.size(), e->stream);
+}
+
+// the results slots of the pinned staging, which the last kernel of a
+// trial writes directly
+double *mc_result_slots(mfe_engine *e) { return reinterpret_cast<double *>(e->h_mc); }
+
+// n results from the staging's result slots (waits for the stream)
+void mc_results(mfe_engine *e, int n, double *out) {
   MFE_HIP_TRY(hipStreamSynchronize(e->stream));
+  const double *h = mc_result_slots(e);
   for (int k = 0; k < n; ++k) out[k] = h[k];
 }
 
@@ -1426,10 +1440,10 @@ int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm, const 
     mc_buffers(e, natm);
     mc_upload(e, ia, rtm);
     sync_view(e);
-    mfe::launch_mc_pair(e->grp, natm, e->mc_ia, e->mc_rtm, e->mc_out + 2, e->mc_out + 4,
+    mfe::launch_mc_pair(e->grp, natm, e->mc_ia, e->mc_rtm, mc_result_slots(e), e->mc_out + 4,
                         e->nlkind == MFE_NL_CELL);
     double h[2];
-    mc_results(e, 2, 2, h);
+    mc_results(e, 2, h);
     *overlap = h[1] != 0.0 ? 1 : 0;
     *du_twob = *overlap ? 0.0 : h[0];
     return MFE_OK;
@@ -1553,7 +1567,9 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
     MFE_HIP_TRY(hipSetDevice(e->device));
     mc_buffers(e, natm);
     mc_spectrum_buffers(e, natm);
-    mc_upload(e, ia, rtm);
+    int *hi;
+    double *hr;
+    mc_stage(e, ia, rtm, &hi, &hr);
     sync_view(e);
     const mfe::Group &G = e->grp;
     if (!(reuse_current && e->mc_cached)) {
@@ -1565,10 +1581,11 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
       mfe::fft_z_forward(G, e->mc_spec);
     }
     // the trial's stencil factors, then the energy change over the spectrum
-    mfe::launch_mc_kfac(G, natm, e->mc_ia, e->mc_rtm, e->mc_fac);
-    mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, false, e->mc_out + 4, e->mc_out);
+    mfe::launch_mc_kfac(G, natm, hi, hr, e->mc_fac);
+    mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, false, e->mc_out + 4,
+                          mc_result_slots(e));
     double h = 0.0;
-    mc_results(e, 0, 1, &h);
+    mc_results(e, 1, &h);
     e->mc_cached = true;
     e->mc_last_ia = ia;
     e->mc_last_rtm.assign(rtm, rtm + 3 * (size_t)natm);
