@@ -1438,9 +1438,21 @@ int mfe_mc_delta_pair(mfe_engine *e, int32_t natm, const int32_t *ianatm, const 
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
     mc_buffers(e, natm);
-    mc_upload(e, ia, rtm);
+    // small moves straight from the pinned staging (the kernel stages them
+    // in LDS), larger ones through device copies
+    int *hi;
+    double *hr;
+    mc_stage(e, ia, rtm, &hi, &hr);
+    const int *mia = hi;
+    const double *mrtm = hr;
+    if (natm > mfe::kMcStage) {
+      upload(e->mc_ia, hi, (size_t)natm, e->stream);
+      upload(e->mc_rtm, hr, 3 * (size_t)natm, e->stream);
+      mia = e->mc_ia;
+      mrtm = e->mc_rtm;
+    }
     sync_view(e);
-    mfe::launch_mc_pair(e->grp, natm, e->mc_ia, e->mc_rtm, mc_result_slots(e), e->mc_out + 4,
+    mfe::launch_mc_pair(e->grp, natm, mia, mrtm, mc_result_slots(e), e->mc_out + 4,
                         e->nlkind == MFE_NL_CELL);
     double h[2];
     mc_results(e, 2, h);

@@ -684,6 +684,9 @@ void fft_z_forward(const Group &g, void *out);
 // kMcPairBlocks blocks whose partials part[2 * kMcPairBlocks] are summed
 // in a fixed order; else the Verlet-list partners in one block
 constexpr int kMcPairBlocks = 512;
+// moves of up to this many particles: k_mc_pair stages them in LDS and may
+// read them from host memory
+constexpr int kMcStage = 64;
 void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, double *out,
                     double *part, bool all);
 void launch_skin(const Group &g);

@@ -1771,11 +1771,24 @@ __device__ __forceinline__ double mc_pair_u(const SysDev &V, int ti, int tj, dou
 }
 
 __global__ void __launch_bounds__(256) k_mc_pair(const SysDev *__restrict__ sys, int natm,
-                                                const int *__restrict__ ia,
-                                                const double *__restrict__ rtm,
+                                                const int *__restrict__ ia_in,
+                                                const double *__restrict__ rtm_in,
                                                 double *__restrict__ out,
                                                 double *__restrict__ part) {
   const SysDev &V = sys[blockIdx.z];
+  // up to kMcStage moved particles are copied into LDS once per block, so
+  // the move may come from host memory (the pinned staging) and the loops
+  // below read it from LDS
+  __shared__ int ia_s[kMcStage];
+  __shared__ double rtm_s[3 * kMcStage];
+  const bool staged = natm <= kMcStage;
+  if (staged) {
+    for (int k = threadIdx.x; k < natm; k += blockDim.x) ia_s[k] = ia_in[k];
+    for (int k = threadIdx.x; k < 3 * natm; k += blockDim.x) rtm_s[k] = rtm_in[k];
+  }
+  __syncthreads();
+  const int *ia = staged ? ia_s : ia_in;
+  const double *rtm = staged ? rtm_s : rtm_in;
   const int np = V.np;
   const int stride = gridDim.x * blockDim.x;
   // cell-list semantics (UTwoBodyANew/OldCellList, Src/denergy.F90:728-928):
