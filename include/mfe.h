@@ -264,12 +264,14 @@ int mfe_mc_delta_pair_noalloc(mfe_engine *e, int32_t natm, const int32_t *ianatm
    MC driver's position update ro(:,ianatm) = rtm and NList's skin
    bookkeeping (the list is rebuilt on the device only when the
    reference's skin test fires, as after mfe_upload_positions).  rtm must
-   lie inside the box.  On an FP64 engine whose last mfe_mc_delta_rec was
-   this same move, its change D is added to the kept spectrum, which then
-   is the current one, so the next trial spreads nothing; otherwise (any
-   other move, or a mixed engine, whose FP32 spectrum would keep the
-   addition's rounding) the next trial spreads anew.  reuse_current = 1 is
-   valid after this call in both cases. */
+   lie inside the box.  When the last mfe_mc_delta_rec was this same move,
+   its change D is added to the kept spectrum (FP64 on both precision
+   paths), which then is the current one, so the next trial spreads
+   nothing; after any other move the next trial spreads anew.
+   reuse_current = 1 is valid after this call in both cases.  On a mixed
+   engine the kept spectrum then differs from a fresh FP32 transform of the
+   accepted configuration by that transform's rounding (the next trial's
+   change agrees to ~1e-6 kT for one-particle moves). */
 int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm,
                   const double *rtm);
 /* nsteps of MDDriver(iSimulationStep) + NList(iSimulationStep): VelVer

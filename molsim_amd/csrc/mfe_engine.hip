@@ -1405,8 +1405,8 @@ void mc_results(mfe_engine *e, int n, double *out) {
 // the kept spectrum (the complex mesh's size) and the stencil factors of
 // natm moved particles; a new spectrum buffer is not a valid one
 void mc_spectrum_buffers(mfe_engine *e, int natm) {
-  const size_t csz = e->prec == MFE_PREC_MIXED ? 8 : 16;
-  const size_t bytes = (size_t)e->hx * e->s[1] * e->s[2] * csz;
+  // FP64 values on both precision paths
+  const size_t bytes = (size_t)e->hx * e->s[1] * e->s[2] * 16;
   if (e->mc_spec_bytes != bytes) {
     if (e->mc_spec) MFE_HIP_TRY(hipFree(e->mc_spec));
     e->mc_spec = nullptr;
@@ -1608,11 +1608,11 @@ int mfe_mc_delta_rec(mfe_engine *e, int32_t natm, const int32_t *ianatm, const d
 
 // EwaldUpdateArray (Src/denergy.F90:1776-1819, 'spm': QMesh = QMeshTM,
 // urecold = urecnew) with the MC driver's position update of an accepted
-// move (ro(:,ianatm) = rtm) and NList's skin bookkeeping.  On an FP64
-// engine whose last mfe_mc_delta_rec was this move, its trial mesh (the
-// kept tiles with the moved stencils exchanged again, before the positions
-// change) and its E_rec become the current ones, so the next trial with
-// reuse_current spreads nothing; otherwise the next trial spreads anew.
+// move (ro(:,ianatm) = rtm) and NList's skin bookkeeping.  When the last
+// mfe_mc_delta_rec was this move, its change D (from the stored stencil
+// factors) is added to the kept FP64 spectrum, which is then the current
+// one, so the next trial with reuse_current transforms nothing; otherwise
+// the next trial transforms anew.
 int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const double *rtm) {
   if (!e || !ianatm || !rtm || natm < 1) return MFE_E_BAD_PARAM;
   e->forces_current = false;
@@ -1625,9 +1625,9 @@ int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const doub
         return fail(e, MFE_E_BAD_PARAM, "rtm outside the box [-L/2, L/2]");
   if (!e->list_valid || !e->have_positions || !e->have_neighbor)
     return fail(e, MFE_E_STATE, "positions and neighbour list first");
-  // FP64 only: the mixed spectrum is FP32, and adding the change to it
-  // would leave FP32 rounding that a fresh transform does not have
-  const bool keep = e->prec == MFE_PREC_FP64 && e->mc_cached && e->have_ewald &&
+  // the kept spectrum is FP64 on both paths: the accepted change adds to
+  // it without FP32 rounding
+  const bool keep = e->mc_cached && e->have_ewald &&
                     e->ewald == MFE_EWALD_SPME && e->mc_spec && e->mc_last_ia == ia &&
                     e->mc_last_rtm.size() == 3 * (size_t)natm &&
                     std::equal(e->mc_last_rtm.begin(), e->mc_last_rtm.end(), rtm);

@@ -130,7 +130,9 @@ __global__ void __launch_bounds__(256) k_fft_conv(const SysDev *__restrict__ sys
   }
 }
 
-// the z forward transform into out (the MC trial moves' kept spectrum)
+// the z forward transform into out (the MC trial moves' kept spectrum,
+// FP64 on both precision paths, so that accepted changes add to it without
+// FP32 rounding)
 template <typename R>
 __global__ void __launch_bounds__(256) k_fft_z_fwd(const SysDev *__restrict__ sys, void *out) {
   const SysDev &V = sys[blockIdx.z];
@@ -138,7 +140,7 @@ __global__ void __launch_bounds__(256) k_fft_z_fwd(const SysDev *__restrict__ sy
   const int bx = L % gridDim.x, y = L / gridDim.x;
   if (bx >= V.fz_nbx || y >= V.sg.s[1]) return;
   extern __shared__ char smem[];
-  fftz_fwd_tile<R>(V, bx, y, (cplx<R> *)out, smem);
+  fftz_fwd_tile<R, double>(V, bx, y, (cplx<double> *)out, smem);
 }
 
 inline int tile_width(int hx, int target) {

@@ -427,9 +427,10 @@ __device__ __forceinline__ void fftconv_tile(const SysDev &V, int bx, int y, int
 
 // The z forward transform alone (no Green function), for the MC trial
 // moves' kept spectrum: tile of kx columns bx * fz_T, ... of line y, read
-// from the complex mesh and written, transformed, to out (same layout).
-template <typename R>
-__device__ __forceinline__ void fftz_fwd_tile(const SysDev &V, int bx, int y, cplx<R> *out,
+// from the complex mesh and written, transformed, to out (same layout) as
+// values of type O.
+template <typename R, typename O>
+__device__ __forceinline__ void fftz_fwd_tile(const SysDev &V, int bx, int y, cplx<O> *out,
                                               char *smem) {
   const cplx<R> *__restrict__ cm = (const cplx<R> *)V.mesh_c;
   const int s2 = V.sg.s[1], s3 = V.sg.s[2], hx = V.hx, T = V.fz_T;
@@ -442,7 +443,7 @@ __device__ __forceinline__ void fftz_fwd_tile(const SysDev &V, int bx, int y, cp
   const int nt = min(T, hx - kx0);
   const size_t zstride = (size_t)s2 * hx;
   const cplx<R> *base = cm + (size_t)y * hx + kx0;
-  cplx<R> *obase = out + (size_t)y * hx + kx0;
+  cplx<O> *obase = out + (size_t)y * hx + kx0;
   Twiddle<R> *tw_s = reinterpret_cast<Twiddle<R> *>(b + T * ld);
   stage_twiddles(tw_s, tw, s3);
   batched_copy(
@@ -459,7 +460,10 @@ __device__ __forceinline__ void fftz_fwd_tile(const SysDev &V, int bx, int y, cp
   const cplx<R> *f = lds_fft(a, b, pz, nt, ld, tw_s, false);
   for (int e = threadIdx.x; e < T * s3; e += blockDim.x) {
     const int z = fdiv(e, V.f_fzT), c = e - z * T;
-    if (c < nt) obase[z * zstride + c] = f[c * ld + z];
+    if (c < nt) {
+      const cplx<R> v = f[c * ld + z];
+      obase[z * zstride + c] = cplx<O>{(O)v.x, (O)v.y};
+    }
   }
 }
 

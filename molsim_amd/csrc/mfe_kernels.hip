@@ -3393,13 +3393,10 @@ void launch_mc_kfac(const Group &g, int natm, const int *ia, const double *rtm, 
 
 void launch_mc_kdelta(const Group &g, int natm, const void *fac, void *spec, bool apply,
                       double *part, double *out) {
+  // the kept spectrum is FP64 on both precision paths
   const dim3 gr(kMcPairBlocks);
-  if (g.prec == MFE_PREC_MIXED)
-    hipLaunchKernelGGL(k_mc_kdelta<float>, gr, dim3(256), 0, g.stream, g.d, natm,
-                       (const double2 *)fac, spec, apply ? 1 : 0, part);
-  else
-    hipLaunchKernelGGL(k_mc_kdelta<double>, gr, dim3(256), 0, g.stream, g.d, natm,
-                       (const double2 *)fac, spec, apply ? 1 : 0, part);
+  hipLaunchKernelGGL(k_mc_kdelta<double>, gr, dim3(256), 0, g.stream, g.d, natm,
+                     (const double2 *)fac, spec, apply ? 1 : 0, part);
   if (!apply)
     hipLaunchKernelGGL(k_mc_pair_sum, dim3(1), dim3(64), 0, g.stream, part, kMcPairBlocks, out);
 }

@@ -212,11 +212,12 @@ def _free_move(eng, s, ro, rng, natm, step):
 def test_mc_accept_keeps_the_trial_mesh(prec):
     """EwaldUpdateArray (Src/denergy.F90:1776-1819) with the accepted
     positions: after mc_accept of the move just tried, the next trial with
-    reuse_current equals a fresh spread of the accepted configuration
-    uploaded whole (FP64, from the kept spectrum with the accepted change
-    added: <= 1e-12 |E_rec|; mixed, which transforms anew after an accept:
-    bitwise) and the oracle's energy change; positions, forces and energies
-    after the accepts equal those of the uploaded configuration."""
+    reuse_current, from the kept FP64 spectrum with the accepted change
+    added, equals a fresh spread of the accepted configuration uploaded
+    whole (FP64: <= 1e-12 |E_rec|; mixed, whose kept spectrum carries the
+    first FP32 transform's rounding where a fresh one carries its own:
+    <= 1e-4 kT) and the oracle's energy change; positions, forces and
+    energies after the accepts equal those of the uploaded configuration."""
     s = S.c2()
     print("case: MC accept c2", prec, flush=True)
     eng = S.configure(s, prec)
@@ -243,7 +244,7 @@ def test_mc_accept_keeps_the_trial_mesh(prec):
         print(f"  accept {it} natm {natm}: next trial kept {d_keep:.12e} "
               f"fresh {d_ref:.12e} oracle {d_orc:.12e}", flush=True)
         if prec == "mixed":
-            assert d_keep == d_ref
+            assert abs(d_keep - d_ref) <= 1e-4 * KT_298
             assert abs(d_keep - d_orc) <= 1e-4 * KT_298
         else:
             assert abs(d_keep - d_ref) <= 1e-12 * abs(e0)
