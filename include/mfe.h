@@ -271,7 +271,10 @@ int mfe_mc_delta_pair_noalloc(mfe_engine *e, int32_t natm, const int32_t *ianatm
    reuse_current = 1 is valid after this call in both cases.  On a mixed
    engine the kept spectrum then differs from a fresh FP32 transform of the
    accepted configuration by that transform's rounding (the next trial's
-   change agrees to ~1e-6 kT for one-particle moves). */
+   change agrees to ~1e-6 kT for one-particle moves), and each accepted
+   change adds the rounding of its FP32 stencil factors: a long run of
+   accepts without an upload or step can refresh the spectrum by passing
+   reuse_current = 0 now and then. */
 int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm,
                   const double *rtm);
 /* nsteps of MDDriver(iSimulationStep) + NList(iSimulationStep): VelVer
