@@ -1637,7 +1637,13 @@ int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const doub
   return guarded(e, [&]() -> int {
     MFE_HIP_TRY(hipSetDevice(e->device));
     mc_buffers(e, natm);
-    mc_upload(e, ia, rtm);
+    // small moves straight from the pinned staging, larger ones through
+    // device copies
+    const bool small = natm <= mfe::kMcStage;
+    int *hi = nullptr;
+    double *hr = nullptr;
+    if (small) mc_stage(e, ia, rtm, &hi, &hr);
+    else mc_upload(e, ia, rtm);
     sync_view(e);
     const mfe::Group &G = e->grp;
     if (keep) {
@@ -1646,11 +1652,15 @@ int mfe_mc_accept(mfe_engine *e, int32_t natm, const int32_t *ianatm, const doub
       mfe::launch_mc_kdelta(G, natm, e->mc_fac, e->mc_spec, true, nullptr, nullptr);
     }
     // the accepted positions, then NList(iSimulationStep) as after an upload
-    if (!e->ro_stage) dalloc(e->ro_stage, 3 * (size_t)e->max_np);
-    MFE_HIP_TRY(hipMemcpyAsync(e->ro_stage, e->ro, 3 * (size_t)e->np * sizeof(double),
-                               hipMemcpyDeviceToDevice, e->stream));
-    mfe::launch_mc_place(G, natm, e->mc_ia, e->mc_rtm, e->ro_stage);
-    mfe::launch_set_positions(G, e->ro_stage);
+    if (small) {
+      mfe::launch_set_positions_moved(G, natm, hi, hr);
+    } else {
+      if (!e->ro_stage) dalloc(e->ro_stage, 3 * (size_t)e->max_np);
+      MFE_HIP_TRY(hipMemcpyAsync(e->ro_stage, e->ro, 3 * (size_t)e->np * sizeof(double),
+                                 hipMemcpyDeviceToDevice, e->stream));
+      mfe::launch_mc_place(G, natm, e->mc_ia, e->mc_rtm, e->ro_stage);
+      mfe::launch_set_positions(G, e->ro_stage);
+    }
     mfe::launch_skin(G);
     mfe::launch_reduce(G, 0u, true, false);
     mfe::launch_nlist_rebuild(G, true);

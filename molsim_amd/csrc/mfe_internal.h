@@ -691,6 +691,9 @@ void launch_mc_pair(const Group &g, int natm, const int *ia, const double *rtm, 
                     double *part, bool all);
 void launch_skin(const Group &g);
 void launch_set_positions(const Group &g, const double *ro_new);
+// the same with the current positions, the natm <= kMcStage particles ia
+// moved to rtm (device or pinned host memory)
+void launch_set_positions_moved(const Group &g, int natm, const int *ia, const double *rtm);
 void launch_collect(const Group &g, int *flags, double *en, int nen);
 // persistent batched BD (mfe_kernels.hip): nsteps BDSteps of every replica
 // in one launch, a workgroup per replica at a time, replicas taken from a

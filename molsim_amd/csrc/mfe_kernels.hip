@@ -2471,16 +2471,28 @@ __global__ void __launch_bounds__(kBlock) k_move(const SysDev *__restrict__ sys,
   if (threadIdx.x == 0) part_max[blockIdx.x] = m;
 }
 
-// New positions from the host with a valid list: NList's displacement
-// bookkeeping (Src/nlist.F90:601-603) on the step since the last upload,
-// taken as the minimum image of new - old (one shift per axis, as PBCr2),
-// so the skin test that follows decides whether the list is rebuilt.
+// New positions with a valid list: NList's displacement bookkeeping
+// (Src/nlist.F90:601-603) on the step since the last upload, taken as the
+// minimum image of new - old (one shift per axis, as PBCr2), so the skin
+// test that follows decides whether the list is rebuilt.  The new
+// positions are ro_new (a host upload), or, with ro_new null, the current
+// ones with the natm <= kMcStage particles ia moved to rtm (an accepted MC
+// move, staged in LDS, so it may come from host memory).
 __global__ void __launch_bounds__(kBlock) k_set_positions(const SysDev *__restrict__ sys,
-                                                          const double *__restrict__ ro_new) {
+                                                          const double *__restrict__ ro_new,
+                                                          int natm, const int *__restrict__ ia,
+                                                          const double *__restrict__ rtm) {
 #pragma clang fp contract(off)
   const SysDev &V = sys[blockIdx.z];
   if ((int)blockIdx.x >= V.nblk_np) return;
   __shared__ double sh[kBlock];
+  __shared__ int ia_s[kMcStage];
+  __shared__ double rtm_s[3 * kMcStage];
+  if (!ro_new) {
+    for (int k = threadIdx.x; k < natm; k += blockDim.x) ia_s[k] = ia[k];
+    for (int k = threadIdx.x; k < 3 * natm; k += blockDim.x) rtm_s[k] = rtm[k];
+    __syncthreads();
+  }
   const Box b = V.box;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double r2 = 0.0;
@@ -2491,9 +2503,20 @@ __global__ void __launch_bounds__(kBlock) k_set_positions(const SysDev *__restri
     const int cell = V.pos_f ? V.c_of[i] : 0, ty = V.pos_f ? V.type[i] : 0;
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
-      xn[m] = ro_new[3 * i + m];
       xo[m] = V.ro[3 * i + m];
       s[m] = V.drosum[3 * i + m];
+    }
+    if (ro_new) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) xn[m] = ro_new[3 * i + m];
+    } else {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) xn[m] = xo[m];
+      for (int a = 0; a < natm; ++a)
+        if (ia_s[a] == i) {
+#pragma unroll
+          for (int m = 0; m < 3; ++m) xn[m] = rtm_s[3 * a + m];
+        }
     }
 #pragma unroll
     for (int m = 0; m < 3; ++m) {
@@ -3660,7 +3683,12 @@ void launch_adjoint(const Group &g, int ncol, double *out) {
 
 void launch_set_positions(const Group &g, const double *ro_new) {
   hipLaunchKernelGGL(k_set_positions, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d,
-                     ro_new);
+                     ro_new, 0, (const int *)nullptr, (const double *)nullptr);
+}
+
+void launch_set_positions_moved(const Group &g, int natm, const int *ia, const double *rtm) {
+  hipLaunchKernelGGL(k_set_positions, grid(g.np_blocks, g), dim3(kBlock), 0, g.stream, g.d,
+                     (const double *)nullptr, natm, ia, rtm);
 }
 
 void launch_skin(const Group &g) {
