@@ -273,7 +273,7 @@ def main():
                          "FETCH_SIZE/WRITE_SIZE counters of a profiling run)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it",
+        print("bench.py: the benchmark runs on one GPU; --gpus above 1 is not supported",
               file=sys.stderr)
         sys.exit(2)
 
