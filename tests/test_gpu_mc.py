@@ -275,3 +275,50 @@ def test_mc_accept_keeps_the_trial_mesh(prec):
         eng.mc_accept([1], np.array([[0.6 * s.boxlen, 0.0, 0.0]]))
     eng.close()
     ref.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_mc_large_moves(prec):
+    """Moves of more than 64 particles, which the pair delta and the accept
+    take through device copies of the move (smaller ones are staged in LDS
+    straight from host memory): the pair delta against the oracle's
+    two-body energies (<= 1e-10 of |u_twob|), and the accepted
+    configuration equal to the same positions uploaded whole (bitwise on
+    the mixed path, to 1e-12 on the FP64 path)."""
+    from helpers import oracle_for
+    s = S.c2()
+    natm = 70
+    print("case: MC moves of", natm, "particles c2", prec, flush=True)
+    tab = E.build_tables(s.zat, s.sigat, s.epsat, rcut=s.rcut, epsi_four_pi=efp(),
+                         alpha=s.alpha)
+    eng = S.configure(s, prec, tables=tab)
+    ref = S.configure(s, prec, tables=tab)
+    eng.utotal()
+    ref.utotal()
+    orc = oracle_for(s, tab)
+    orc.set_list(s.ro)
+    _, eo = orc.utotal(s.ro)
+    u0 = eo["twob0"]
+    rng = np.random.default_rng(31)
+    ia = rng.choice(s.np, natm, replace=False)
+    rtm = _wrap(s.ro[ia] + rng.uniform(-0.3, 0.3, (natm, 3)), s.boxlen)
+    du, ov = eng.mc_delta_pair(ia + 1, rtm)
+    ro1 = s.ro.copy()
+    ro1[ia] = rtm
+    _, e1 = orc.utotal(ro1)
+    print(f"  du {du:.10e} oracle {e1['twob0'] - u0:.10e} overlap {ov}", flush=True)
+    assert not ov
+    assert abs(du - (e1["twob0"] - u0)) <= 1e-10 * abs(u0)
+    eng.mc_delta_rec(ia + 1, rtm)
+    eng.mc_accept(ia + 1, rtm)
+    ref.upload_positions(ro1)
+    assert np.array_equal(eng.download_state()[0].reshape(-1, 3), ro1)
+    f1, en1 = eng.utotal()
+    f2, en2 = ref.utotal()
+    if prec == "mixed":
+        assert np.array_equal(f1, f2) and en1.tot == en2.tot
+    else:
+        assert rel_rms(f1, f2) <= 1e-12
+        assert abs(en1.tot - en2.tot) <= 1e-12 * abs(en2.tot)
+    eng.close()
+    ref.close()
